@@ -1,0 +1,75 @@
+"""Builds libhns.so (HIP, gfx950) in-tree, and the CPU oracle used by the tests.
+
+    python -m hnanosolver_amd.build            # library + oracle
+    python -m hnanosolver_amd.build --lib      # library only
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "libhns.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+SOURCES = ["topology.hip", "kernels_step.hip", "kernels_pressure.hip", "solvers.hip", "api.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("HNS_OFFLOAD_ARCH", "gfx950")
+# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in device code (build flags)
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
+          "-I", os.path.join(ROOT, "include")]
+
+
+def _run(cmd, cwd=None):
+    r = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (r.returncode, " ".join(cmd), r.stdout))
+    return r.stdout
+
+
+def build_lib(verbose: bool = False) -> str:
+    objdir = os.path.join(PKG, "_build")
+    os.makedirs(objdir, exist_ok=True)
+
+    def compile_one(src):
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        out = _run([HIPCC, *COMMON, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        if verbose and out.strip():
+            print(out)
+        return obj
+
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(compile_one, SOURCES))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> str:
+    _run(["make", "-s", "-C", ORACLE_DIR])
+    return os.path.join(ORACLE_DIR, "liboracle_hns.so")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true")
+    ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("-v", action="store_true")
+    a = ap.parse_args(argv)
+    both = not (a.lib or a.oracle)
+    if a.lib or both:
+        print("built", build_lib(a.v))
+    if a.oracle or both:
+        print("built", build_oracle())
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,177 @@
+// Device-side topology and sampling helpers for the gfx950 kernels of the sparse step.
+//
+// Layout in HBM (all leaf-dense, 512 voxels per 8^3 leaf, leaves in voxelsToGrid order):
+//   scalar field      float[N], voxel (leaf l, local x,y,z) at 512*l + (x<<6 | y<<3 | z)
+//   velocity          float[3N] SoA planes: x at [0,N), y at [N,2N), z at [2N,3N)
+//   pressure/div (RBGS) "colour-split": colour c=(x+y+z)&1 plane at [c*N/2, (c+1)*N/2),
+//                     inside it leaf l, local (x,y,z) at 256*l + 32*x + 4*y + (z>>1)
+//   topology          int4 origin per leaf; 32-int neighbour row per leaf (27 used,
+//                     entry (dx+1)*9+(dy+1)*3+(dz+1), -1 absent); open-addressing leaf hash.
+// A voxel lookup replaces NanoVDB's per-thread root->leaf walk (Stencils.hpp:51-71) with one
+// LDS read of the neighbour row (|offset| <= 1 leaf) or a hash probe (farther).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hns {
+
+constexpr int32_t kLeafVox = 512;
+constexpr int32_t kHalfLeaf = 256;
+constexpr int32_t kNbrStride = 32;
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr int32_t kLeafBias = 1 << 20;  // leaf coordinates in [-2^20, 2^20) -> voxels in [-2^23, 2^23)
+
+struct DevTopo {
+  int32_t L;
+  uint32_t hmask;
+  const int4* __restrict__ origins;
+  const int32_t* __restrict__ nbr;
+  const unsigned long long* __restrict__ hkeys;
+  const int32_t* __restrict__ hvals;
+};
+
+__host__ __device__ inline bool leaf_in_range(int32_t lx, int32_t ly, int32_t lz) {
+  return (uint32_t)(lx + kLeafBias) < (1u << 21) && (uint32_t)(ly + kLeafBias) < (1u << 21) &&
+         (uint32_t)(lz + kLeafBias) < (1u << 21);
+}
+__host__ __device__ inline uint64_t leaf_key(int32_t lx, int32_t ly, int32_t lz) {
+  return ((uint64_t)(uint32_t)(lx + kLeafBias) << 42) | ((uint64_t)(uint32_t)(ly + kLeafBias) << 21) |
+         (uint64_t)(uint32_t)(lz + kLeafBias);
+}
+__host__ __device__ inline uint32_t hash_mix(uint64_t k) {  // MurmurHash3 fmix64
+  k ^= k >> 33;
+  k *= 0xff51afd7ed558ccdULL;
+  k ^= k >> 33;
+  k *= 0xc4ceb9fe1a85ec53ULL;
+  k ^= k >> 33;
+  return (uint32_t)k;
+}
+
+__device__ inline int32_t hash_find(const DevTopo& T, int32_t lx, int32_t ly, int32_t lz) {
+  if (!leaf_in_range(lx, ly, lz)) return -1;
+  const uint64_t k = leaf_key(lx, ly, lz);
+  uint32_t h = hash_mix(k) & T.hmask;
+  // load factor <= 1/2, so an empty slot ends every probe sequence
+  for (;;) {
+    const uint64_t kk = T.hkeys[h];
+    if (kk == k) return T.hvals[h];
+    if (kk == kEmptyKey) return -1;
+    h = (h + 1) & T.hmask;
+  }
+}
+
+// Workgroup b is dispatched to XCD b % 8. Give each XCD a contiguous run of leaves so that a
+// leaf and its +-z/+-y neighbours (adjacent in voxelsToGrid order) share an L2.
+__device__ __forceinline__ int32_t leaf_of_block() {
+  const uint32_t b = blockIdx.x, G = gridDim.x;
+  return (int32_t)((b & 7u) * (G >> 3) + (b >> 3));
+}
+inline unsigned grid_for_leaves(int32_t L) { return 8u * (unsigned)((L + 7) / 8); }
+
+// float -> int round-down (Stencils.hpp:28-30, __float2int_rd); far/NaN positions are clamped
+// to a coordinate outside every domain so they read as misses.
+__device__ __forceinline__ int32_t floor_i(float v) {
+  float f = floorf(v);
+  f = fminf(fmaxf(f, -1073741824.0f), 1073741824.0f);
+  return (int32_t)f;
+}
+
+__device__ __forceinline__ int32_t local_off(int32_t gx, int32_t gy, int32_t gz) {
+  return ((gx & 7) << 6) | ((gy & 7) << 3) | (gz & 7);
+}
+__device__ __forceinline__ int32_t cs_local(int32_t x, int32_t y, int32_t z) {  // x,y,z in [0,8)
+  return (x << 5) | (y << 2) | (z >> 1);
+}
+
+// Per-workgroup view of one leaf and its 26 neighbours.
+struct LeafView {
+  const int32_t* nb;  // 27 neighbour leaf ids in LDS
+  int32_t ox, oy, oz;
+  DevTopo T;
+
+  __device__ __forceinline__ int32_t leaf_at(int32_t gx, int32_t gy, int32_t gz) const {
+    const int32_t rx = (gx - ox) >> 3, ry = (gy - oy) >> 3, rz = (gz - oz) >> 3;
+    if ((uint32_t)(rx + 1) < 3u && (uint32_t)(ry + 1) < 3u && (uint32_t)(rz + 1) < 3u)
+      return nb[(rx + 1) * 9 + (ry + 1) * 3 + (rz + 1)];
+    return hash_find(T, gx >> 3, gy >> 3, gz >> 3);
+  }
+  // leaf-dense index of a voxel, -1 when its leaf is absent
+  __device__ __forceinline__ int32_t index(int32_t gx, int32_t gy, int32_t gz) const {
+    const int32_t leaf = leaf_at(gx, gy, gz);
+    return leaf < 0 ? -1 : leaf * kLeafVox + local_off(gx, gy, gz);
+  }
+  // colour-split index of a voxel, -1 when absent
+  __device__ __forceinline__ int32_t cs_index(int32_t gx, int32_t gy, int32_t gz, int32_t half) const {
+    const int32_t leaf = leaf_at(gx, gy, gz);
+    if (leaf < 0) return -1;
+    const int32_t x = gx & 7, y = gy & 7, z = gz & 7;
+    return ((x + y + z) & 1) * half + leaf * kHalfLeaf + cs_local(x, y, z);
+  }
+};
+
+// Loads the leaf's neighbour row into LDS and returns its view. Call before any early exit
+// that is not uniform across the workgroup; ends with a barrier.
+__device__ __forceinline__ LeafView load_leaf(const DevTopo& T, int32_t l, int32_t* s_nb) {
+  if (threadIdx.x < 27) s_nb[threadIdx.x] = T.nbr[kNbrStride * l + threadIdx.x];
+  const int4 o = T.origins[l];
+  __syncthreads();
+  LeafView v;
+  v.nb = s_nb;
+  v.ox = o.x;
+  v.oy = o.y;
+  v.oz = o.z;
+  v.T = T;
+  return v;
+}
+
+struct Vel {
+  const float* __restrict__ x;
+  const float* __restrict__ y;
+  const float* __restrict__ z;
+};
+
+__device__ __forceinline__ float3 load_vel(const Vel& u, int32_t idx) {
+  return idx < 0 ? make_float3(0.f, 0.f, 0.f) : make_float3(u.x[idx], u.y[idx], u.z[idx]);
+}
+__device__ __forceinline__ float load_s(const float* __restrict__ f, int32_t idx) { return idx < 0 ? 0.f : f[idx]; }
+
+// TrilinearSampler::sample on Vec3f, device branch (Stencils.hpp:118-153): corners with
+// misses as 0, lerp fmaf(w, b - a, a), z then y then x.
+__device__ inline float3 trilinear_vec(const LeafView& V, const Vel& u, float px, float py, float pz) {
+  const int32_t i = floor_i(px), j = floor_i(py), k = floor_i(pz);
+  const float tx = px - (float)i, ty = py - (float)j, tz = pz - (float)k;
+  float3 c[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) c[a][b][d] = load_vel(u, V.index(i + a, j + b, k + d));
+  auto lerp = [](float3 va, float3 vb, float w) {
+    return make_float3(fmaf(w, vb.x - va.x, va.x), fmaf(w, vb.y - va.y, va.y), fmaf(w, vb.z - va.z, va.z));
+  };
+  const float3 z0 = lerp(c[0][0][0], c[0][0][1], tz), z1 = lerp(c[0][1][0], c[0][1][1], tz);
+  const float3 z2 = lerp(c[1][0][0], c[1][0][1], tz), z3 = lerp(c[1][1][0], c[1][1][1], tz);
+  const float3 y0 = lerp(z0, z1, ty), y1 = lerp(z2, z3, ty);
+  return lerp(y0, y1, tx);
+}
+
+// TrilinearSampler::sample on float (Stencils.hpp:140: a + w*(b - a)).
+__device__ inline float trilinear_s(const LeafView& V, const float* __restrict__ f, float px, float py, float pz) {
+  const int32_t i = floor_i(px), j = floor_i(py), k = floor_i(pz);
+  const float tx = px - (float)i, ty = py - (float)j, tz = pz - (float)k;
+  float c[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) c[a][b][d] = load_s(f, V.index(i + a, j + b, k + d));
+  auto lerp = [](float va, float vb, float w) { return va + w * (vb - va); };
+  const float z0 = lerp(c[0][0][0], c[0][0][1], tz), z1 = lerp(c[0][1][0], c[0][1][1], tz);
+  const float z2 = lerp(c[1][0][0], c[1][0][1], tz), z3 = lerp(c[1][1][0], c[1][1][1], tz);
+  const float y0 = lerp(z0, z1, ty), y1 = lerp(z2, z3, ty);
+  return lerp(y0, y1, tx);
+}
+
+}  // namespace hns

@@ -1,0 +1,133 @@
+// Host-side internals shared by the translation units of libhns.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/hns.h"
+#include "hns_device.hpp"
+
+namespace hns {
+
+// Error carried to the C ABI boundary, where it becomes a status code + message.
+struct Error : std::runtime_error {
+  hns_status code;
+  Error(hns_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(hns_status c, const std::string& m) { throw Error(c, m); }
+
+inline void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+    fail(HNS_ERR_OUT_OF_MEMORY, std::string(what) + ": " + hipGetErrorString(e));
+  if (e == hipErrorNoDevice || e == hipErrorInvalidDevice)
+    fail(HNS_ERR_NO_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+  fail(HNS_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HNS_HIP(call) ::hns::hip_check((call), #call)
+
+// Device buffer with RAII.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  explicit DBuf(size_t count) { alloc(count); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DBuf() { reset(); }
+  void alloc(size_t count) {
+    reset();
+    if (count == 0) return;
+    HNS_HIP(hipMalloc(reinterpret_cast<void**>(&p), count * sizeof(T)));
+    n = count;
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// Device-resident leaf topology (the role of NanoVDB's IndexGrid in the reference).
+struct TopoStore {
+  int device = 0;
+  size_t N = 0;
+  int32_t L = 0;
+  float voxelSize = 0.f;
+  DBuf<int4> origins;
+  DBuf<int32_t> nbr;
+  DBuf<unsigned long long> hkeys;
+  DBuf<int32_t> hvals;
+  uint32_t hmask = 0;
+  DevTopo dev() const {
+    DevTopo t;
+    t.L = L;
+    t.hmask = hmask;
+    t.origins = origins.p;
+    t.nbr = nbr.p;
+    t.hkeys = hkeys.p;
+    t.hvals = hvals.p;
+    return t;
+  }
+};
+
+// Builds the topology from host coords (3N int32). Throws Error on invalid input.
+std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float voxelSize, hipStream_t s);
+void leaf_order_host(const int32_t* origins, size_t L, uint64_t* perm);
+
+// ---- kernel launchers (kernels_step.hip) ------------------------------------------------
+void launch_advect_velocity(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, const float* sdf,
+                            hipStream_t s);
+struct FieldSet {
+  static constexpr int kMax = 8;
+  const float* in[kMax];
+  float* out[kMax];
+  int32_t m;
+};
+void launch_advect_scalars(const DevTopo& T, const float* u, const FieldSet& f, float dt, float inv_dx,
+                           const float* sdf, hipStream_t s);
+void launch_advect_scalar(const DevTopo& T, const float* u, const float* in, float* out, float dt, float inv_dx,
+                          const float* sdf, hipStream_t s);
+void launch_vorticity_forces(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, float scale,
+                             int32_t F, bool vort, const float* src_vel, float gravity, hipStream_t s);
+struct CombustionArgs {
+  float* fuel;
+  float* waste;
+  float* temp;
+  float* flame;
+  float* density;              // only written when src_density is set
+  const float* src_fuel;       // nullable; applied when the field existed before this substep
+  const float* src_temp;
+  const float* src_density;
+  float dt, temp_gain, expansion, ambient, buoyancy;
+};
+// mode 0: divergence only; mode 1: divergence + sources + combustion + buoyancy
+void launch_divergence(const DevTopo& T, const float* u, float* div, float inv_dx, bool colour_split,
+                       const CombustionArgs* comb, float* u_out, hipStream_t s);
+void launch_combustion_buoyancy(size_t N, const CombustionArgs& c, float* div, const float* u, float* u_out,
+                                hipStream_t s);
+void launch_subtract_gradient(const DevTopo& T, const float* u, const float* p_cs, float* out, float inv_dx,
+                              const float* sdf, hipStream_t s);
+void launch_enforce_collision(const DevTopo& T, float* u, const float* sdf, float voxelSize, hipStream_t s);
+void launch_to_colour_split(const DevTopo& T, const float* in, float* out, bool inverse, hipStream_t s);
+void launch_aos_soa(size_t n, const float* in, float* out, bool to_soa, hipStream_t s);
+
+// ---- pressure kernels (kernels_pressure.hip) -------------------------------------------
+// One colour sweep; first != 0 treats p as zero before the sweep (iteration 0 from p = 0).
+void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,
+                 hipStream_t s);
+// Writes per-workgroup partial sums (2 doubles per leaf) then reduces into out[0..1].
+void launch_residual(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* partial,
+                     double* out, hipStream_t s);
+
+}  // namespace hns

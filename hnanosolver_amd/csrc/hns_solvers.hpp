@@ -1,0 +1,111 @@
+// Pressure-solve driver and per-kernel event timing of a context.
+#pragma once
+#include <vector>
+
+#include "hns_internal.hpp"
+
+namespace hns {
+
+// HIP-event timing of kernel classes (HNS_K_*), enabled per context for measurement runs.
+struct Timing {
+  bool enabled = false;
+  double total_ms[HNS_K_COUNT] = {};
+  int64_t launches[HNS_K_COUNT] = {};
+
+  void begin(int k, hipStream_t s) {
+    if (!enabled) return;
+    cur_ = k;
+    a_ = event();
+    HNS_HIP(hipEventRecord(a_, s));
+  }
+  void end(hipStream_t s) {
+    if (!enabled) return;
+    hipEvent_t b = event();
+    HNS_HIP(hipEventRecord(b, s));
+    pending_.push_back(Rec{cur_, a_, b});
+  }
+  // call after the stream has been synchronised
+  void collect() {
+    for (const Rec& r : pending_) {
+      float ms = 0.f;
+      HNS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      total_ms[r.k] += ms;
+      launches[r.k] += 1;
+    }
+    pending_.clear();
+    used_ = 0;
+  }
+  void reset() {
+    for (int k = 0; k < HNS_K_COUNT; ++k) {
+      total_ms[k] = 0.0;
+      launches[k] = 0;
+    }
+  }
+  ~Timing() {
+    for (hipEvent_t e : pool_) (void)hipEventDestroy(e);
+  }
+
+ private:
+  struct Rec {
+    int k;
+    hipEvent_t a, b;
+  };
+  hipEvent_t event() {
+    if (used_ == pool_.size()) {
+      hipEvent_t e;
+      HNS_HIP(hipEventCreate(&e));
+      pool_.push_back(e);
+    }
+    return pool_[used_++];
+  }
+  std::vector<hipEvent_t> pool_;
+  size_t used_ = 0;
+  std::vector<Rec> pending_;
+  int cur_ = 0;
+  hipEvent_t a_ = nullptr;
+};
+
+struct SolveRequest {
+  int solver = HNS_SOLVER_RBGS_FIXED;
+  float tol = 1e-5f;
+  int max_iters = 100000;
+  int check_every = 0;
+  float omega = 1.f;
+  float dx = 1.f;
+  bool want_residual = false;
+};
+struct SolveResult {
+  int iterations = 0;
+  double rel_residual = -1.0;
+  double ms = 0.0;
+};
+
+struct CgState;
+
+// Pressure solves on colour-split p and div (p starts from 0 in every solve, HNanoSolver.cu:364).
+class Solvers {
+ public:
+  Solvers();
+  ~Solvers();
+  Solvers(const Solvers&) = delete;
+  Solvers& operator=(const Solvers&) = delete;
+  void init(int32_t L, size_t N);
+  SolveResult solve(const DevTopo& T, const float* div, float* p, int K, const SolveRequest& r, Timing& tm,
+                    hipStream_t s);
+  // ||div - L p|| / ||div|| (0 when div == 0); synchronises s.
+  double rel_residual(const DevTopo& T, const float* div, const float* p, float dx, Timing& tm, hipStream_t s);
+
+ private:
+  int32_t L_ = 0;
+  size_t N_ = 0;
+  DBuf<double> partial_, red_;
+  double* h_red_ = nullptr;
+  hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  CgState* cg_ = nullptr;
+  void rbgs_iters(const DevTopo& T, const float* div, float* p, int first_it, int n, const SolveRequest& r, Timing& tm,
+                  hipStream_t s);
+  SolveResult solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
+                       hipStream_t s);
+};
+
+}  // namespace hns

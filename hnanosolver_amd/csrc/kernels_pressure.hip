@@ -1,0 +1,156 @@
+// Pressure kernels: red-black Gauss-Seidel SOR sweeps (redBlackGaussSeidelUpdate,
+// src/Cuda/Kernel.cu:459-489) and the Poisson residual (new; compute_residual is declared but
+// never defined in the reference, src/Cuda/Kernels.cuh:59-60).
+//
+// Pressure and divergence are stored colour-split: all colour-0 voxels ((x+y+z)&1 == 0) of
+// every leaf, then all colour-1 voxels; inside a colour, leaf l's 256 voxels are contiguous at
+// 256*l + 32*x + 4*y + (z>>1). A colour sweep therefore launches 256 lanes per leaf (no idle
+// half-wave as in the reference's all-N launch with early exit), reads its own colour's div and
+// p and writes p as fully coalesced 1 KiB runs, and gathers the other colour's 6 neighbours
+// from the same leaf or a face-neighbour leaf. Per sweep and voxel of the domain that is
+// 2 B div + 2 B p + 2 B p written + 2 B of the other colour = 8 B; 16 B per iteration.
+#include "hns_internal.hpp"
+
+namespace hns {
+namespace {
+
+constexpr int kNbXm = 4, kNbXp = 22, kNbYm = 10, kNbYp = 16, kNbZm = 12, kNbZp = 14;  // (dx+1)*9+(dy+1)*3+(dz+1)
+
+template <int COLOUR, bool FIRST>
+__global__ __launch_bounds__(256) void k_rbgs(DevTopo T, const float* __restrict__ div, float* __restrict__ p,
+                                              float dx2, float omega) {
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const int32_t t = threadIdx.x;
+  const int32_t x = t >> 5, y = (t >> 2) & 7, zh = t & 3;
+  const int32_t z = (zh << 1) | ((x + y + COLOUR) & 1);
+  const int32_t half = T.L * kHalfLeaf;
+  const float* __restrict__ po = p + (1 - COLOUR) * half + 0;
+  float* __restrict__ pc = p + COLOUR * half;
+  const int32_t base = l * kHalfLeaf;
+  float s = 0.f;
+  if (!(FIRST && COLOUR == 0)) {
+    const int32_t* __restrict__ nr = T.nbr + kNbrStride * l;
+    float pxp, pxm, pyp, pym, pzp, pzm;
+    if (x < 7) pxp = po[base + cs_local(x + 1, y, z)];
+    else { const int32_t nl = nr[kNbXp]; pxp = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(0, y, z)]; }
+    if (x > 0) pxm = po[base + cs_local(x - 1, y, z)];
+    else { const int32_t nl = nr[kNbXm]; pxm = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(7, y, z)]; }
+    if (y < 7) pyp = po[base + cs_local(x, y + 1, z)];
+    else { const int32_t nl = nr[kNbYp]; pyp = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, 0, z)]; }
+    if (y > 0) pym = po[base + cs_local(x, y - 1, z)];
+    else { const int32_t nl = nr[kNbYm]; pym = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, 7, z)]; }
+    if (z < 7) pzp = po[base + cs_local(x, y, z + 1)];
+    else { const int32_t nl = nr[kNbZp]; pzp = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, y, 0)]; }
+    if (z > 0) pzm = po[base + cs_local(x, y, z - 1)];
+    else { const int32_t nl = nr[kNbZm]; pzm = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, y, 7)]; }
+    s = pxp + pxm + pyp + pym + pzp + pzm;
+  }
+  const float dv = div[COLOUR * half + base + t];
+  const float pOld = FIRST ? 0.f : pc[base + t];
+  const float pGS = (s - dv * dx2) * 0.166666667f;
+  pc[base + t] = pOld + omega * (pGS - pOld);
+}
+
+// Residual r = div - (sum6 p_nbr - 6 p) / dx^2 in FP64; per-leaf partial sums of r^2 and div^2.
+__global__ __launch_bounds__(512) void k_residual(DevTopo T, const float* __restrict__ p,
+                                                  const float* __restrict__ div, double inv_dx2,
+                                                  double* __restrict__ partial) {
+  __shared__ int32_t s_nb[27];
+  __shared__ double s_red[2][8];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const LeafView V = load_leaf(T, l, s_nb);
+  const int32_t half = T.L * kHalfLeaf;
+  const int32_t n = threadIdx.x;
+  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
+  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
+  const int32_t own = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
+  const double sum = (double)load_s(p, V.cs_index(x + 1, y, z, half)) + (double)load_s(p, V.cs_index(x - 1, y, z, half)) +
+                     (double)load_s(p, V.cs_index(x, y + 1, z, half)) + (double)load_s(p, V.cs_index(x, y - 1, z, half)) +
+                     (double)load_s(p, V.cs_index(x, y, z + 1, half)) + (double)load_s(p, V.cs_index(x, y, z - 1, half));
+  const double d = (double)div[own];
+  const double r = d - (sum - 6.0 * (double)p[own]) * inv_dx2;
+  double a = r * r, b = d * d;
+  // wave64 reduction with DPP/permute shuffles, then across the 8 waves through LDS
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int w = n >> 6;
+  if ((n & 63) == 0) {
+    s_red[0][w] = a;
+    s_red[1][w] = b;
+  }
+  __syncthreads();
+  if (n == 0) {
+    double ta = 0.0, tb = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      ta += s_red[0][i];
+      tb += s_red[1][i];
+    }
+    partial[2 * l] = ta;
+    partial[2 * l + 1] = tb;
+  }
+}
+
+// Deterministic final reduction: one workgroup, fixed summation order.
+__global__ __launch_bounds__(1024) void k_reduce_pairs(const double* __restrict__ partial, int32_t L,
+                                                       double* __restrict__ out) {
+  __shared__ double s_red[2][16];
+  double a = 0.0, b = 0.0;
+  for (int32_t i = threadIdx.x; i < L; i += blockDim.x) {
+    a += partial[2 * i];
+    b += partial[2 * i + 1];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_red[0][w] = a;
+    s_red[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double ta = 0.0, tb = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) {
+      ta += s_red[0][i];
+      tb += s_red[1][i];
+    }
+    out[0] = ta;
+    out[1] = tb;
+  }
+}
+
+}  // namespace
+
+void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,
+                 hipStream_t s) {
+  const unsigned g = grid_for_leaves(T.L);
+  const float dx2 = dx * dx;
+  if (colour == 0) {
+    if (first) k_rbgs<0, true><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+    else k_rbgs<0, false><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+  } else {
+    if (first) k_rbgs<1, true><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+    else k_rbgs<1, false><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+  }
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_residual(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* partial,
+                     double* out, hipStream_t s) {
+  const double inv_dx2 = 1.0 / ((double)dx * (double)dx);
+  k_residual<<<grid_for_leaves(T.L), 512, 0, s>>>(T, p_cs, div_cs, inv_dx2, partial);
+  HNS_HIP(hipGetLastError());
+  k_reduce_pairs<<<1, 1024, 0, s>>>(partial, T.L, out);
+  HNS_HIP(hipGetLastError());
+}
+
+}  // namespace hns

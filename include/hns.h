@@ -1,0 +1,264 @@
+/*
+ * hns.h — C ABI of the MI355X (gfx950) sparse smoke/fire step.
+ *
+ * Drop-in boundary for the launch functions HNanoSolver's SOP nodes call. Every entry point
+ * names the reference interface it replaces (paths relative to the reference repository).
+ * The reference exports `extern "C"` functions that take C++ references
+ * (HNS::GridIndexedData&, nanovdb::GridHandle&, std::vector&) and throw std::runtime_error;
+ * here every argument is a plain pointer or size and every call returns an hns_status.
+ * include/hns.hpp restores the reference's C++ signatures and exceptions on top of this.
+ *
+ * Data model (src/Utils/GridData.hpp:16-166, src/Utils/GridBuilder.hpp:164-174):
+ *   - a payload of N voxels, N = 512 x leaves ("leaf-dense"): coords[3*i .. 3*i+2] is the
+ *     (x, y, z) index-space coordinate of voxel i; each leaf's 512 voxels are consecutive in
+ *     offset order (x<<6 | y<<3 | z, local to the 8^3 leaf) and leaves come in the order
+ *     NanoVDB's voxelsToGrid sorts them (the order OpenVDB's LeafManager gives);
+ *   - named value blocks, each N floats (HNS_FLOAT) or N x 3 floats AoS (HNS_VEC3F).
+ * Device state lives in HBM for the lifetime of a context; nothing is re-uploaded per step.
+ *
+ * Threading: one host thread per context at a time. Every call that launches work ends with
+ * a stream synchronisation, as the reference's do (src/Cuda/HNanoSolver.cu:135, :220, :455, :494).
+ */
+#ifndef HNS_H_
+#define HNS_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HNS_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------------------- */
+typedef int32_t hns_status;
+#define HNS_OK 0
+#define HNS_ERR_INVALID_ARGUMENT (-1) /* bad size, pointer or parameter */
+#define HNS_ERR_EMPTY (-2)            /* N == 0 (HNanoSolver.cu:61) */
+#define HNS_ERR_NULL_CONTEXT (-3)     /* null context handle */
+#define HNS_ERR_TOPOLOGY (-4)         /* coords are not leaf-dense or do not match the topology */
+#define HNS_ERR_LEAF_ORDER (-5)       /* leaves are not in voxelsToGrid order (see hns_leaf_order) */
+#define HNS_ERR_NO_VELOCITY (-6)      /* payload has no Vec3f block (HNanoSolver.cu:101) */
+#define HNS_ERR_HIP (-7)              /* a HIP runtime call failed (CUDA_CHECK, Utils.cuh:10-18) */
+#define HNS_ERR_OUT_OF_MEMORY (-8)    /* device allocation failed (Utils.cuh:287-296) */
+#define HNS_ERR_NOT_FOUND (-9)        /* a named block is missing */
+#define HNS_ERR_NO_DEVICE (-10)       /* no HIP device is visible */
+#define HNS_ERR_NOT_CONVERGED (-11)   /* a to-tolerance solve hit max_iters (results still written) */
+
+/* Message of the last failing call on this thread (never null). */
+const char* hns_last_error(void);
+/* HNS_ABI_VERSION of the loaded library. */
+int32_t hns_abi_version(void);
+
+/* ---- payload (HNS::GridIndexedData as plain data) ----------------------------------- */
+#define HNS_FLOAT 1
+#define HNS_VEC3F 3
+typedef struct hns_value_block {
+  const char* name; /* e.g. "density", "velocity", "src_density", "collision_sdf" */
+  int32_t type;     /* HNS_FLOAT or HNS_VEC3F */
+  void* values;     /* N or 3N floats, caller-owned */
+} hns_value_block;
+
+typedef struct hns_grid_data {
+  size_t size;              /* N voxels (GridIndexedData::size, GridData.hpp:111) */
+  const int32_t* coords;    /* 3N int32, (x,y,z) per voxel (pCoords, GridData.hpp:91) */
+  size_t num_blocks;
+  hns_value_block* blocks;  /* insertion order, as GridData.hpp:73 keeps them */
+} hns_grid_data;
+
+/* CombustionParams, src/Cuda/Kernels.cuh:5-24 (and the identical copy in
+ * src/SOP/HNanoSolver/SOP_HNanoSolver.hpp:23-43): 72 bytes, same field order. */
+typedef struct hns_params {
+  float expansionRate;
+  float temperatureRelease;
+  float buoyancyStrength;
+  float ambientTemp;
+  float vorticityScale;
+  float factorScale;
+  float disturbanceEnable;
+  float disturbanceStrength;
+  float disturbanceSwirl;
+  float disturbanceThreshold;
+  float disturbanceGain;
+  float disturbanceFrequency;
+  int32_t substeps;
+  float gravity;
+  float maskDensityMin;
+  float maskDensityMax;
+  int32_t downloadVel;
+  int32_t simReset;
+} hns_params;
+
+/* ---- topology (replaces nanovdb::GridHandle<nanovdb::cuda::DeviceBuffer>) ------------ */
+typedef struct hns_topology hns_topology;
+
+/* CreateIndexGrid(GridIndexedData&, GridHandle<cuda::DeviceBuffer>&, float voxelSize),
+ * src/Cuda/HNanoSolver.cu:498-510. Builds the device-resident leaf topology (leaf origins,
+ * 27-entry neighbour table per leaf, leaf hash) from data->coords. Validates that the payload
+ * is leaf-dense and in voxelsToGrid order (HNS_ERR_TOPOLOGY / HNS_ERR_LEAF_ORDER otherwise).
+ * *out is replaced (an existing handle is released first). */
+hns_status hns_create_index_grid(const hns_grid_data* data, float voxel_size, hns_topology** out);
+/* Releases a topology handle (contexts built from it keep their own reference). */
+hns_status hns_topology_destroy(hns_topology** topo);
+/* Number of voxels / leaves described by a handle. */
+hns_status hns_topology_info(const hns_topology* topo, size_t* n_voxels, size_t* n_leaves);
+/* Copies the device neighbour table (27 int32 per leaf, -1 = absent; entry (dx+1)*9+(dy+1)*3+(dz+1)). */
+hns_status hns_topology_neighbours(const hns_topology* topo, int32_t* out, size_t capacity);
+
+/* Host helper: permutation of L leaf origins (3L int32) into voxelsToGrid order
+ * (nanovdb/tools/cuda/PointsToGrid.cuh:596-651): perm[r] = input index of the r-th leaf. */
+hns_status hns_leaf_order(const int32_t* origins, size_t n_leaves, uint64_t* perm);
+
+/* ---- simulation context (SimContext, src/Cuda/HNanoSolver.cu:27-48) ------------------ */
+typedef struct hns_context hns_context;
+
+/* HNS_EnsureContext(void** ctx, GridIndexedData&, const GridHandle&, float voxelSize,
+ * bool hasCollision), src/Cuda/HNanoSolver.cu:58-155. Creates the context when *ctx is null or
+ * when N or the voxel size changed (uploading velocity, every float block except
+ * "collision_sdf" and "src_*", and the collision SDF); otherwise keeps the device state and only
+ * refreshes the topology and the collision SDF. */
+hns_status hns_ensure_context(hns_context** ctx, const hns_grid_data* data, const hns_topology* topo,
+                              float voxel_size, int32_t has_collision);
+
+/* HNS_UploadSources(void*, GridIndexedData&), HNanoSolver.cu:158-221: replaces the per-cook
+ * sources ("src_vel" Vec3 or "src_vel_x/y/z", and every "src_*" float block). */
+hns_status hns_upload_sources(hns_context* ctx, const hns_grid_data* data);
+
+/* HNS_Advance(void*, int iteration, float dt, const CombustionParams&), HNanoSolver.cu:223-456.
+ * `iteration` red-black Gauss-Seidel iterations per pressure solve, as the reference. */
+hns_status hns_advance(hns_context* ctx, int32_t iteration, float dt, const hns_params* params);
+
+/* Pressure-solver selection for hns_advance_ex (no reference counterpart: the reference always
+ * runs a fixed iteration count, HNanoSolver.cu:366-373). */
+#define HNS_SOLVER_RBGS_FIXED 0  /* the reference: `iteration` RBGS iterations */
+#define HNS_SOLVER_RBGS_TOL 1    /* RBGS until ||div - L p|| / ||div|| <= tol (or max_iters) */
+#define HNS_SOLVER_CG 2          /* conjugate gradients on the same operator, to tol */
+#define HNS_SOLVER_PCG_LEARNED 3 /* flexible PCG with the learned preconditioner, to tol */
+typedef struct hns_solve_options {
+  uint32_t struct_size;  /* sizeof(hns_solve_options) */
+  int32_t solver;        /* HNS_SOLVER_* */
+  float tol;             /* relative residual target */
+  int32_t max_iters;     /* cap for the to-tolerance solvers */
+  int32_t check_every;   /* residual check period for HNS_SOLVER_RBGS_TOL (0 = default) */
+} hns_solve_options;
+typedef struct hns_solve_stats {
+  uint32_t struct_size;  /* sizeof(hns_solve_stats) */
+  int32_t iterations;    /* iterations of the last pressure solve */
+  double rel_residual;   /* ||div - L p||_2 / ||div||_2 after the last solve (FP64 sums) */
+  double solve_ms;       /* device time of the last solve */
+} hns_solve_stats;
+hns_status hns_advance_ex(hns_context* ctx, int32_t iteration, float dt, const hns_params* params,
+                          const hns_solve_options* opts, hns_solve_stats* stats);
+
+/* HNS_DownloadForOutput(void*, GridIndexedData&), HNanoSolver.cu:471-495: writes velocity into
+ * the "velocity" block (else the first Vec3f block) and every state field into the float block of
+ * the same name, where present. */
+hns_status hns_download_for_output(hns_context* ctx, hns_grid_data* data);
+
+/* HNS_GetActiveMask(void*, std::vector<unsigned char>&), HNanoSolver.cu:459-468. The
+ * reference's mask is identically 1 (Kernel.cu:74-75). Writes min(N, capacity) bytes; *n_out = N
+ * (0 for a null context, which the reference answers with an empty mask). */
+hns_status hns_get_active_mask(hns_context* ctx, uint8_t* mask, size_t capacity, size_t* n_out);
+
+/* HNS_DestroyContext(void** ctx), HNanoSolver.cu:51-56. Null-safe; sets *ctx to null. */
+hns_status hns_destroy_context(hns_context** ctx);
+
+/* Runs the context's work on a caller stream (a hipStream_t; null = the context's own stream). */
+hns_status hns_set_stream(hns_context* ctx, void* stream);
+/* Context sizes and whether a named state field exists. */
+hns_status hns_context_info(const hns_context* ctx, size_t* n_voxels, size_t* n_leaves, int32_t* n_state);
+int32_t hns_has_field(const hns_context* ctx, const char* name);
+
+/* Per-kernel device timing with HIP events (for measurement; off by default). */
+#define HNS_K_ADVECT_VELOCITY 0
+#define HNS_K_VORTICITY_FORCES 1
+#define HNS_K_DIVERGENCE_COMBUSTION 2
+#define HNS_K_RBGS 3
+#define HNS_K_RESIDUAL 4
+#define HNS_K_GRADIENT 5
+#define HNS_K_ADVECT_SCALARS 6
+#define HNS_K_CG 7
+#define HNS_K_OTHER 8
+#define HNS_K_COUNT 9
+hns_status hns_set_timing(hns_context* ctx, int32_t enable);
+/* Accumulated time (ms) and launch count for kernel class `which` since the last reset. */
+hns_status hns_get_timing(hns_context* ctx, int32_t which, double* total_ms, int64_t* launches);
+hns_status hns_reset_timing(hns_context* ctx);
+
+/* ---- standalone entry points (the HNanoAdvect / HNanoAdvectVelocity /
+ *      HNanoProjectNonDivergent SOP nodes); each builds its topology from data->coords,
+ *      updates the payload in place and frees its device memory before returning ---------- */
+
+/* AdvectIndexGrid(GridIndexedData&, float dt, float voxelSize, const cudaStream_t&),
+ * src/Cuda/Advection.cu:13-112, :169-171: every float block is advected by the single Vec3f
+ * block with advect_scalar (Kernel.cu:294-344; misses read 0). */
+hns_status hns_advect_index_grid(hns_grid_data* data, float dt, float voxel_size, void* stream);
+/* AdvectIndexGridVelocity(...), Advection.cu:114-166, :173-175: the Vec3f block advects itself
+ * (advect_vector, Kernel.cu:349-426). */
+hns_status hns_advect_index_grid_velocity(hns_grid_data* data, float dt, float voxel_size, void* stream);
+/* Divergence(GridIndexedData&, float voxelSize, const cudaStream_t&), PressureProjection.cu:81-129:
+ * writes the divergence of the Vec3f block into the "divergence" float block. */
+hns_status hns_divergence(hns_grid_data* data, float voxel_size, void* stream);
+/* ProjectNonDivergent(GridIndexedData&, size_t iterations, float voxelSize, const cudaStream_t&),
+ * PressureProjection.cu:9-78, :132-135: divergence, `iterations` RBGS iterations from p = 0 with
+ * omega = 2/(1+sin(3.14159*voxelSize)), gradient subtraction, in place on the Vec3f block. */
+hns_status hns_project_non_divergent(hns_grid_data* data, size_t iterations, float voxel_size, void* stream);
+
+/* ---- device-array kernel API ----------------------------------------------------------
+ * Each call launches one kernel of the step on caller-owned device arrays (pointers from
+ * hipMalloc or a framework allocator) and synchronises `stream` unless async != 0.
+ * Layouts: scalars leaf-dense (length N); velocity SoA (x plane, y plane, z plane; 3N);
+ * pressure and divergence for the RBGS sweep "colour-split" (see hnsk_to_colour_split).
+ * Missing neighbours read 0 except in hnsk_advect_scalars, which reads element 0 as
+ * the reference's advect_scalars does (Kernel.cu:187-188, :229-230, :256-257). */
+
+/* advect_vector, Kernel.cu:349-426. sdf may be null. */
+hns_status hnsk_advect_velocity(const hns_topology* topo, const float* vel, float* out, float dt, float inv_dx,
+                                const float* sdf, void* stream, int32_t async);
+/* advect_scalars, Kernel.cu:163-289: m fields share one back-trace with the velocity `vel`. */
+hns_status hnsk_advect_scalars(const hns_topology* topo, const float* vel, int32_t m, const float* const* in,
+                               float* const* out, float dt, float inv_dx, const float* sdf, void* stream,
+                               int32_t async);
+/* advect_scalar, Kernel.cu:294-344 (misses read 0). */
+hns_status hnsk_advect_scalar(const hns_topology* topo, const float* vel, const float* in, float* out, float dt,
+                              float inv_dx, const float* sdf, void* stream, int32_t async);
+/* vorticityConfinement (Kernel.cu:682-727, out of place) followed by the velocity source
+ * (vel += src*dt, Kernel.cu:739-755; src_vel SoA or null) and gravity (Kernel.cu:834-838). */
+hns_status hnsk_vorticity_forces(const hns_topology* topo, const float* vel, float* out, float dt, float inv_dx,
+                                 float vorticity_scale, float factor_scale, const float* src_vel, float gravity,
+                                 void* stream, int32_t async);
+/* divergence (Kernel.cu:432-453); colour_split != 0 writes the colour-split layout. */
+hns_status hnsk_divergence(const hns_topology* topo, const float* vel, float* div, float inv_dx,
+                           int32_t colour_split, void* stream, int32_t async);
+/* One red-black Gauss-Seidel colour sweep (Kernel.cu:459-489) on colour-split p and div. */
+hns_status hnsk_rbgs_sweep(const hns_topology* topo, const float* div_cs, float* p_cs, float dx, int32_t colour,
+                           float omega, void* stream, int32_t async);
+/* ||div - L p||^2 and ||div||^2 in FP64 (colour-split p and div); synchronous. */
+hns_status hnsk_residual(const hns_topology* topo, const float* p_cs, const float* div_cs, float dx,
+                         double* r2, double* d2, void* stream);
+/* subtractPressureGradient, Kernel.cu:495-541 (colour-split p; sdf may be null). */
+hns_status hnsk_subtract_gradient(const hns_topology* topo, const float* vel, const float* p_cs, float* out,
+                                  float inv_dx, const float* sdf, void* stream, int32_t async);
+/* enforceCollisionBoundaries, Kernel.cu:132-158, in place on SoA velocity. */
+hns_status hnsk_enforce_collision(const hns_topology* topo, float* vel, const float* sdf, float voxel_size,
+                                  void* stream, int32_t async);
+/* combustion_oxygen (Kernel.cu:641-677, in place, div read-modify-write in leaf-dense layout)
+ * followed by temperature_buoyancy (Kernel.cu:546-563) from vel to vel_out. */
+hns_status hnsk_combustion_buoyancy(const hns_topology* topo, float* fuel, float* waste, float* temperature,
+                                    float* flame, float* div, const float* vel, float* vel_out, float dt,
+                                    float temp_gain, float expansion, float ambient, float buoyancy,
+                                    void* stream, int32_t async);
+/* Layout converters: leaf-dense <-> colour-split (colour c = (x+y+z)&1 first for c = 0; inside a
+ * colour, leaf l, local (x,y,z) sits at c*N/2 + 256*l + 32*x + 4*y + (z>>1)). */
+hns_status hnsk_to_colour_split(const hns_topology* topo, const float* in, float* out, void* stream, int32_t async);
+hns_status hnsk_from_colour_split(const hns_topology* topo, const float* in, float* out, void* stream,
+                                  int32_t async);
+/* AoS (x,y,z per voxel) <-> SoA velocity. */
+hns_status hnsk_aos_to_soa(size_t n, const float* in, float* out, void* stream, int32_t async);
+hns_status hnsk_soa_to_aos(size_t n, const float* in, float* out, void* stream, int32_t async);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HNS_H_ */

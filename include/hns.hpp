@@ -1,0 +1,188 @@
+// hns.hpp — header-only C++ adapter that restores the reference's C++ launch API on top of the
+// C ABI in hns.h, so SOP source written against src/Cuda/HNanoSolver.cu keeps compiling:
+//
+//   reference (extern "C", C++ references, throws)           this adapter
+//   CreateIndexGrid(data, GridHandle&, voxelSize)            CreateIndexGrid(data, hns::TopologyHandle&, voxelSize)
+//   HNS_EnsureContext(void**, data, handle, vs, hasColl)     HNS_EnsureContext(void**, data, handle, vs, hasColl)
+//   HNS_UploadSources(void*, data)                           HNS_UploadSources(void*, data)
+//   HNS_Advance(void*, int iteration, float dt, params)      HNS_Advance(void*, int iteration, float dt, params)
+//   HNS_DownloadForOutput(void*, data)                       HNS_DownloadForOutput(void*, data)
+//   HNS_GetActiveMask(void*, std::vector<unsigned char>&)    HNS_GetActiveMask(void*, std::vector<unsigned char>&)
+//   HNS_DestroyContext(void**)                               HNS_DestroyContext(void**)
+//   AdvectIndexGrid / AdvectIndexGridVelocity /              same names; the stream argument is any
+//   Divergence / ProjectNonDivergent (stream&)               pointer-sized stream handle (or nullptr)
+//
+// The payload type is any class shaped like HNS::GridIndexedData (src/Utils/GridData.hpp:16-166):
+// size(), pCoords() (3 int32 per voxel), getBlocksOfType<T>() and pValues<T>(name). Its Vec3 type
+// is HNS_VEC3_TYPE (define it before including, e.g. openvdb::Vec3f; 3 packed floats). The
+// GridHandle<cuda::DeviceBuffer> becomes hns::TopologyHandle. CombustionParams may be the SOP's
+// own struct: it is copied into hns_params after a size check (72 bytes, Kernels.cuh:5-24).
+// Failures throw std::runtime_error, as the reference's CUDA_CHECK does (Utils.cuh:10-18).
+#ifndef HNS_HPP_
+#define HNS_HPP_
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hns.h"
+
+#ifndef HNS_VEC3_TYPE
+#error "define HNS_VEC3_TYPE (the payload's Vec3f type, e.g. openvdb::Vec3f) before including hns.hpp"
+#endif
+
+namespace hns {
+
+using Vec3 = HNS_VEC3_TYPE;
+static_assert(sizeof(Vec3) == 3 * sizeof(float), "HNS_VEC3_TYPE must be three packed floats");
+
+inline void check(hns_status s) {
+  if (s != HNS_OK) throw std::runtime_error(std::string(hns_last_error()));
+}
+
+// Owns a device topology (the role of nanovdb::GridHandle<nanovdb::cuda::DeviceBuffer>).
+class TopologyHandle {
+ public:
+  TopologyHandle() = default;
+  TopologyHandle(const TopologyHandle&) = delete;
+  TopologyHandle& operator=(const TopologyHandle&) = delete;
+  TopologyHandle(TopologyHandle&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+  TopologyHandle& operator=(TopologyHandle&& o) noexcept {
+    if (this != &o) {
+      reset();
+      h_ = o.h_;
+      o.h_ = nullptr;
+    }
+    return *this;
+  }
+  ~TopologyHandle() { reset(); }
+  void reset() {
+    if (h_) hns_topology_destroy(&h_);
+  }
+  hns_topology* get() const { return h_; }
+  hns_topology** out() { return &h_; }
+  explicit operator bool() const { return h_ != nullptr; }
+
+ private:
+  hns_topology* h_ = nullptr;
+};
+
+// Flat C view of a GridIndexedData-shaped payload; valid while `data` is alive and unchanged.
+template <class GridData>
+struct PayloadView {
+  std::vector<std::string> names;
+  std::vector<hns_value_block> blocks;
+  hns_grid_data view{};
+
+  explicit PayloadView(GridData& data) {
+    const std::vector<std::string> f = data.template getBlocksOfType<float>();
+    const std::vector<std::string> v = data.template getBlocksOfType<Vec3>();
+    names.reserve(f.size() + v.size());
+    // insertion order is kept per type; the C side only relies on the Vec3 order
+    // (velocity = "velocity" else the first Vec3 block, HNanoSolver.cu:99-110)
+    for (const auto& n : f) names.push_back(n);
+    for (const auto& n : v) names.push_back(n);
+    for (size_t i = 0; i < names.size(); ++i) {
+      hns_value_block b;
+      b.name = names[i].c_str();
+      if (i < f.size()) {
+        b.type = HNS_FLOAT;
+        b.values = static_cast<void*>(data.template pValues<float>(names[i]));
+      } else {
+        b.type = HNS_VEC3F;
+        b.values = static_cast<void*>(data.template pValues<Vec3>(names[i]));
+      }
+      blocks.push_back(b);
+    }
+    view.size = data.size();
+    view.coords = reinterpret_cast<const int32_t*>(data.pCoords());
+    view.num_blocks = blocks.size();
+    view.blocks = blocks.empty() ? nullptr : blocks.data();
+  }
+  hns_grid_data* get() { return &view; }
+};
+
+template <class P>
+inline hns_params to_params(const P& p) {
+  static_assert(sizeof(P) == sizeof(hns_params), "CombustionParams must have the 72-byte layout of Kernels.cuh:5-24");
+  hns_params out;
+  std::memcpy(&out, &p, sizeof(out));
+  return out;
+}
+
+}  // namespace hns
+
+// ---- the reference's entry points ------------------------------------------------------------
+
+template <class GridData>
+inline void CreateIndexGrid(GridData& data, hns::TopologyHandle& handle, const float voxelSize) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_create_index_grid(v.get(), voxelSize, handle.out()));
+}
+
+template <class GridData>
+inline void HNS_EnsureContext(void** ctx, GridData& data, const hns::TopologyHandle& handle, float voxelSize,
+                              bool hasCollision) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_ensure_context(reinterpret_cast<hns_context**>(ctx), v.get(), handle.get(), voxelSize,
+                                hasCollision ? 1 : 0));
+}
+
+// The reference returns silently on a null context (HNanoSolver.cu:160).
+template <class GridData>
+inline void HNS_UploadSources(void* ctx, GridData& data) {
+  if (!ctx) return;
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_upload_sources(static_cast<hns_context*>(ctx), v.get()));
+}
+
+// The reference returns silently on a null context (HNanoSolver.cu:225).
+template <class Params>
+inline void HNS_Advance(void* ctx, int iteration, float dt, const Params& params) {
+  if (!ctx) return;
+  const hns_params p = hns::to_params(params);
+  hns::check(hns_advance(static_cast<hns_context*>(ctx), iteration, dt, &p));
+}
+
+// The reference dereferences a null context here (HNanoSolver.cu:472-473); this throws instead.
+template <class GridData>
+inline void HNS_DownloadForOutput(void* ctx, GridData& data) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_download_for_output(static_cast<hns_context*>(ctx), v.get()));
+}
+
+inline void HNS_GetActiveMask(void* ctx, std::vector<unsigned char>& hostMask) {
+  size_t n = 0;
+  hns::check(hns_get_active_mask(static_cast<hns_context*>(ctx), nullptr, 0, &n));
+  hostMask.resize(n);
+  if (n) hns::check(hns_get_active_mask(static_cast<hns_context*>(ctx), hostMask.data(), n, &n));
+}
+
+inline void HNS_DestroyContext(void** ctx) { hns::check(hns_destroy_context(reinterpret_cast<hns_context**>(ctx))); }
+
+template <class GridData, class Stream>
+inline void AdvectIndexGrid(GridData& data, const float dt, const float voxelSize, const Stream& stream) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_advect_index_grid(v.get(), dt, voxelSize, reinterpret_cast<void*>(stream)));
+}
+
+template <class GridData, class Stream>
+inline void AdvectIndexGridVelocity(GridData& data, const float dt, const float voxelSize, const Stream& stream) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_advect_index_grid_velocity(v.get(), dt, voxelSize, reinterpret_cast<void*>(stream)));
+}
+
+template <class GridData, class Stream>
+inline void Divergence(GridData& data, const float voxelSize, const Stream& stream) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_divergence(v.get(), voxelSize, reinterpret_cast<void*>(stream)));
+}
+
+template <class GridData, class Stream>
+inline void ProjectNonDivergent(GridData& data, const size_t iterations, const float voxelSize, const Stream& stream) {
+  hns::PayloadView<GridData> v(data);
+  hns::check(hns_project_non_divergent(v.get(), iterations, voxelSize, reinterpret_cast<void*>(stream)));
+}
+
+#endif  // HNS_HPP_

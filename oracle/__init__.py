@@ -1,0 +1,246 @@
+"""TEST INFRASTRUCTURE — ctypes wrapper of the CPU oracle (oracle/liboracle_hns.so).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import this module;
+the product path never does. The oracle restates the reference's algorithm on the host (see the
+header of hns_oracle.cpp for the file:line it follows); `dtype=np.float64` runs the same
+sequence of operations in double precision (the FP64 host oracle).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Dict, Optional
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_DIR, "liboracle_hns.so")
+_lib = None
+
+
+class OracleParams(C.Structure):
+    _fields_ = [("expansionRate", C.c_float), ("temperatureRelease", C.c_float), ("buoyancyStrength", C.c_float),
+                ("ambientTemp", C.c_float), ("vorticityScale", C.c_float), ("factorScale", C.c_float),
+                ("disturbanceEnable", C.c_float), ("disturbanceStrength", C.c_float),
+                ("disturbanceSwirl", C.c_float), ("disturbanceThreshold", C.c_float),
+                ("disturbanceGain", C.c_float), ("disturbanceFrequency", C.c_float), ("substeps", C.c_int32),
+                ("gravity", C.c_float), ("maskDensityMin", C.c_float), ("maskDensityMax", C.c_float),
+                ("downloadVel", C.c_int32), ("simReset", C.c_int32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _DIR], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        _lib.oracle_topo_create.restype = vp
+        _lib.oracle_topo_create.argtypes = [vp, C.c_int64]
+        _lib.oracle_topo_destroy.argtypes = [vp]
+        _lib.oracle_offset.restype = C.c_int64
+        _lib.oracle_offset.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+        _lib.oracle_leaf_order.argtypes = [vp, C.c_int64, vp]
+        _lib.oracle_sample_nearest_f32.restype = C.c_float
+        _lib.oracle_sample_nearest_f32.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32]
+        _lib.oracle_sample_trilinear_f32.restype = C.c_float
+        _lib.oracle_sample_trilinear_f32.argtypes = [vp, vp, C.c_float, C.c_float, C.c_float]
+        _lib.oracle_sample_trilinear_vec_f32.argtypes = [vp, vp, C.c_float, C.c_float, C.c_float, vp]
+        _lib.oracle_num_threads.restype = C.c_int
+        _lib.oracle_set_num_threads.argtypes = [C.c_int]
+        for suf in ("f32", "f64"):
+            f = lambda n: getattr(_lib, "oracle_%s_%s" % (n, suf))
+            f("advect_vector").argtypes = [vp, vp, vp, C.c_float, C.c_float, vp]
+            f("advect_scalars").argtypes = [vp, vp, C.c_int, vp, vp, C.c_float, C.c_float, vp]
+            f("advect_scalar").argtypes = [vp, vp, vp, vp, C.c_float, C.c_float, vp]
+            f("vorticity").argtypes = [vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float]
+            f("divergence").argtypes = [vp, vp, vp, C.c_float]
+            f("rbgs").argtypes = [vp, vp, vp, C.c_float, C.c_int, C.c_float]
+            f("residual").argtypes = [vp, vp, vp, C.c_float, vp, vp]
+            f("subtract_gradient").argtypes = [vp, vp, vp, vp, C.c_float, vp]
+            f("enforce_collision").argtypes = [vp, vp, vp, C.c_float]
+            f("combustion").argtypes = [C.c_int64, vp, vp, vp, vp, vp, C.c_float, C.c_float]
+            f("buoyancy").argtypes = [C.c_int64, vp, vp, C.c_float, C.c_float, C.c_float]
+            f("advance").restype = C.c_int
+            f("advance").argtypes = [vp, C.c_float, C.c_int, C.c_float, C.POINTER(OracleParams), vp, C.c_int, vp, vp,
+                                     vp, C.c_int, vp, vp, vp, vp]
+    return _lib
+
+
+def set_threads(n: int):
+    load().oracle_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(load().oracle_num_threads())
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def leaf_order(origins: np.ndarray) -> np.ndarray:
+    o = np.ascontiguousarray(origins, dtype=np.int32).reshape(-1, 3)
+    perm = np.empty(len(o), np.int64)
+    load().oracle_leaf_order(o.ctypes.data, len(o), perm.ctypes.data)
+    return perm
+
+
+class Oracle:
+    """Topology + kernels of the reference step on the host. Arrays are leaf-dense; Vec3 fields
+    are (N, 3) AoS as the reference stores them."""
+
+    def __init__(self, coords: np.ndarray, dtype=np.float32):
+        self.coords = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
+        self.n = len(self.coords)
+        self.dtype = np.dtype(dtype)
+        self.suf = "f32" if self.dtype == np.float32 else "f64"
+        self._t = load().oracle_topo_create(self.coords.ctypes.data, self.n)
+        if not self._t:
+            raise ValueError("oracle: coords are not a leaf-dense payload")
+
+    def __del__(self):
+        if getattr(self, "_t", None):
+            load().oracle_topo_destroy(self._t)
+            self._t = None
+
+    def _f(self, name):
+        return getattr(load(), "oracle_%s_%s" % (name, self.suf))
+
+    def _a(self, a, shape=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if shape is not None:
+            a = a.reshape(shape)
+        return a
+
+    def offset(self, x, y, z) -> int:
+        return int(load().oracle_offset(self._t, int(x), int(y), int(z)))
+
+    def nearest(self, data, x, y, z) -> float:
+        d = np.ascontiguousarray(data, np.float32)
+        return float(load().oracle_sample_nearest_f32(self._t, d.ctypes.data, int(x), int(y), int(z)))
+
+    def trilinear(self, data, x, y, z) -> float:
+        d = np.ascontiguousarray(data, np.float32)
+        return float(load().oracle_sample_trilinear_f32(self._t, d.ctypes.data, x, y, z))
+
+    def trilinear_vec(self, data, x, y, z) -> np.ndarray:
+        d = np.ascontiguousarray(data, np.float32).reshape(-1, 3)
+        out = np.empty(3, np.float32)
+        load().oracle_sample_trilinear_vec_f32(self._t, d.ctypes.data, x, y, z, out.ctypes.data)
+        return out
+
+    def advect_vector(self, vel, dt, inv_dx, sdf=None):
+        v = self._a(vel, (self.n, 3))
+        out = np.empty_like(v)
+        s = self._a(sdf) if sdf is not None else None
+        self._f("advect_vector")(self._t, v.ctypes.data, out.ctypes.data, dt, inv_dx, _p(s))
+        return out
+
+    def advect_scalars(self, vel, fields, dt, inv_dx, sdf=None):
+        v = self._a(vel, (self.n, 3))
+        ins = [self._a(f) for f in fields]
+        outs = [np.empty_like(f) for f in ins]
+        pin = (C.c_void_p * max(1, len(ins)))(*[f.ctypes.data for f in ins])
+        pout = (C.c_void_p * max(1, len(outs)))(*[f.ctypes.data for f in outs])
+        s = self._a(sdf) if sdf is not None else None
+        self._f("advect_scalars")(self._t, v.ctypes.data, len(ins), C.addressof(pin), C.addressof(pout), dt, inv_dx, _p(s))
+        return outs
+
+    def advect_scalar(self, vel, field, dt, inv_dx, sdf=None):
+        v = self._a(vel, (self.n, 3))
+        f = self._a(field)
+        out = np.empty_like(f)
+        s = self._a(sdf) if sdf is not None else None
+        self._f("advect_scalar")(self._t, v.ctypes.data, f.ctypes.data, out.ctypes.data, dt, inv_dx, _p(s))
+        return out
+
+    def vorticity(self, vel, dt, inv_dx, scale, factor_scale):
+        v = self._a(vel, (self.n, 3))
+        out = np.empty_like(v)
+        self._f("vorticity")(self._t, v.ctypes.data, out.ctypes.data, dt, inv_dx, scale, factor_scale)
+        return out
+
+    def divergence(self, vel, inv_dx):
+        v = self._a(vel, (self.n, 3))
+        out = np.empty(self.n, self.dtype)
+        self._f("divergence")(self._t, v.ctypes.data, out.ctypes.data, inv_dx)
+        return out
+
+    def rbgs(self, div, p, dx, colour, omega):
+        """One colour sweep, in place on p (returned)."""
+        d = self._a(div)
+        q = self._a(p).copy()
+        self._f("rbgs")(self._t, d.ctypes.data, q.ctypes.data, dx, int(colour), omega)
+        return q
+
+    def residual(self, p, div, dx):
+        r2, d2 = C.c_double(), C.c_double()
+        pa, da = self._a(p), self._a(div)  # keep the arrays alive across the call
+        self._f("residual")(self._t, pa.ctypes.data, da.ctypes.data, dx, C.byref(r2), C.byref(d2))
+        return r2.value, d2.value
+
+    def subtract_gradient(self, vel, p, inv_dx, sdf=None):
+        v = self._a(vel, (self.n, 3))
+        out = np.empty_like(v)
+        s = self._a(sdf) if sdf is not None else None
+        pa = self._a(p)
+        self._f("subtract_gradient")(self._t, v.ctypes.data, pa.ctypes.data, out.ctypes.data, inv_dx, _p(s))
+        return out
+
+    def enforce_collision(self, vel, sdf, voxel_size):
+        v = self._a(vel, (self.n, 3)).copy()
+        sa = self._a(sdf)
+        self._f("enforce_collision")(self._t, v.ctypes.data, sa.ctypes.data, voxel_size)
+        return v
+
+    def combustion(self, fuel, waste, temp, div, flame, gain, expansion):
+        arrs = [self._a(a).copy() for a in (fuel, waste, temp, div, flame)]
+        self._f("combustion")(self.n, *[a.ctypes.data for a in arrs], gain, expansion)
+        return arrs  # fuel, waste, temp, div, flame
+
+    def buoyancy(self, vel, temp, dt, ambient, beta):
+        v = self._a(vel, (self.n, 3)).copy()
+        ta = self._a(temp)
+        self._f("buoyancy")(self.n, v.ctypes.data, ta.ctypes.data, dt, ambient, beta)
+        return v
+
+    def advance(self, voxel_size, iterations, dt, params: Dict[str, float], velocity, fields: Dict[str, np.ndarray],
+                sources: Optional[Dict[str, np.ndarray]] = None, src_vel=None, sdf=None):
+        """HNS_Advance on host copies. `fields` maps state names to arrays (the context's state at call
+        start); the combustion fields it lacks are created as the reference's needOrMake does.
+        Returns (velocity, fields) after the call."""
+        P = OracleParams()
+        for k, _ in OracleParams._fields_:
+            v = params[k]
+            setattr(P, k, int(v) if k in ("substeps", "downloadVel", "simReset") else float(v))
+        vel = self._a(velocity, (self.n, 3)).copy()
+        names = list(fields.keys())
+        arrs = [self._a(fields[n]).copy() for n in names]
+        present = [1] * len(names)
+        for extra in ("fuel", "waste", "temperature", "flame"):
+            if extra not in fields:
+                names.append(extra)
+                arrs.append(np.zeros(self.n, self.dtype))
+                present.append(0)
+        cn = (C.c_char_p * len(names))(*[n.encode() for n in names])
+        cf = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        cp = np.asarray(present, np.int32)
+        sources = sources or {}
+        snames = list(sources.keys())
+        sarrs = [self._a(sources[n]) for n in snames]
+        csn = (C.c_char_p * max(1, len(snames)))(*[n.encode() for n in snames])
+        csa = (C.c_void_p * max(1, len(sarrs)))(*[a.ctypes.data for a in sarrs])
+        sv = self._a(src_vel, (self.n, 3)) if src_vel is not None else None
+        sd = self._a(sdf) if sdf is not None else None
+        st = self._f("advance")(self._t, voxel_size, int(iterations), dt, C.byref(P), vel.ctypes.data, len(names),
+                                C.addressof(cn), C.addressof(cf), cp.ctypes.data, len(snames), C.addressof(csn),
+                                C.addressof(csa), _p(sv), _p(sd))
+        if st != 0:
+            raise RuntimeError("oracle advance failed: %d" % st)
+        return vel, dict(zip(names, arrs))

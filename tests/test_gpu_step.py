@@ -1,0 +1,243 @@
+"""GPU parity of the whole step through the C ABI (HNS_* lifecycle and the standalone SOP entry
+points) against the CPU oracle, plus the error paths and determinism of the boundary."""
+import numpy as np
+import pytest
+
+import hnanosolver_amd as hns
+from hnanosolver_amd import scenes
+from oracle import Oracle
+
+from _helpers import TOL_REL, canonical_coords, max_abs, plane_sdf, ragged_origins, ragged_scene, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print("CASE", name, flush=True)
+
+
+def _payload(coords, velocity, fields, sources=None, sdf=None, src_vel=None):
+    d = hns.GridIndexedData(coords)
+    for k, v in fields.items():
+        d.add_value_block(k, v.copy())
+    d.add_value_block("velocity", velocity.copy())
+    if sdf is not None:
+        d.add_value_block("collision_sdf", sdf)
+    for k, v in (sources or {}).items():
+        d.add_value_block(k, v)
+    if src_vel is not None:
+        d.add_value_block("src_vel", src_vel)
+    return d
+
+
+def _out_payload(coords, names):
+    d = hns.GridIndexedData(coords)
+    for k in names:
+        d.add_value_block(k, np.zeros(len(coords), np.float32))
+    d.add_value_block("velocity", np.zeros((len(coords), 3), np.float32))
+    return d
+
+
+def _run_gpu(coords, dx, velocity, fields, params, K, dt, calls, sources=None, sdf=None, src_vel=None):
+    data = _payload(coords, velocity, fields, sources, sdf, src_vel)
+    topo = hns.create_index_grid(data, dx)
+    ctx = hns.ensure_context(None, data, topo, dx, has_collision=sdf is not None)
+    hns.upload_sources(ctx, data)
+    p = hns.make_params(**params)
+    for _ in range(calls):
+        hns.advance(ctx, K, dt, p)
+    out = _out_payload(coords, list(dict.fromkeys(list(fields) + ["fuel", "waste", "temperature", "flame"])))
+    hns.download_for_output(ctx, out)
+    res = {k: out[k] for k in out.blocks if k != "velocity" and ctx.has_field(k)}
+    hns.destroy_context(ctx)
+    return out["velocity"], res
+
+
+def _run_oracle(coords, dx, velocity, fields, params, K, dt, calls, sources=None, sdf=None, src_vel=None,
+                dtype=np.float32):
+    o = Oracle(coords, dtype)
+    vel, f = velocity, dict(fields)
+    for _ in range(calls):
+        vel, f = o.advance(dx, K, dt, params, vel, f, sources, src_vel, sdf)
+    return vel, f
+
+
+def _compare(tag, gv, gf, ov, of, o64=None):
+    worst = 0.0
+    for k in ["velocity"] + sorted(of):
+        g = gv if k == "velocity" else gf[k]
+        r = ov if k == "velocity" else of[k]
+        e = rel_err(g, r)
+        line = "  %-8s %-12s rel %.3e  max-abs %.3e" % (tag, k, e, max_abs(g, r))
+        if o64 is not None:
+            r64 = o64[0] if k == "velocity" else o64[1][k]
+            line += "  | vs FP64: gpu %.3e  oracle-f32 %.3e" % (max_abs(g, r64), max_abs(r, r64))
+        print(line, flush=True)
+        worst = max(worst, e)
+    return worst
+
+
+@pytest.mark.parametrize("calls", [1, 3])
+def test_smoke64_step(hip, calls):
+    _case("smoke64 calls=%d" % calls)
+    s = scenes.smoke64()
+    args = (s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt, calls)
+    gv, gf = _run_gpu(*args)
+    ov, of = _run_oracle(*args)
+    o64 = _run_oracle(*args, dtype=np.float64)
+    assert set(gf) == set(of)
+    assert _compare("smoke64", gv, gf, ov, of, o64) <= TOL_REL
+
+
+def test_fire_step(hip):
+    _case("fire precond128 (fire parameters, K=100)")
+    s = scenes.precond128(4)
+    args = (s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt, 2)
+    gv, gf = _run_gpu(*args)
+    ov, of = _run_oracle(*args)
+    o64 = _run_oracle(*args, dtype=np.float64)
+    assert _compare("fire", gv, gf, ov, of, o64) <= TOL_REL
+
+
+def test_all_features_step(hip):
+    """Sources (src_vel, src_density, src_temperature, src_fuel), collision SDF, gravity,
+    vorticity with factorScale 1, the disturbance re-solve and 2 substeps, on a ragged topology."""
+    _case("all features ragged")
+    c, u, f, dx, dt = ragged_scene(3, cfl=1.2)
+    n = len(c)
+    rng = np.random.default_rng(21)
+    sources = {"src_density": rng.random(n).astype(np.float32), "src_temperature": rng.random(n).astype(np.float32) * 5,
+               "src_fuel": rng.random(n).astype(np.float32) * 0.1}
+    src_vel = (rng.standard_normal((n, 3)) * 0.5).astype(np.float32)
+    sdf = plane_sdf(c, offset=-0.4, dx=dx)
+    params = scenes.default_params(vorticityScale=0.3, factorScale=1.0, disturbanceEnable=1.0, substeps=2, gravity=2.0,
+                                   ambientTemp=0.5)
+    args = (c, dx, u, f, params, 12, dt, 2)
+    kw = dict(sources=sources, sdf=sdf, src_vel=src_vel)
+    gv, gf = _run_gpu(*args, **kw)
+    ov, of = _run_oracle(*args, **kw)
+    o64 = _run_oracle(*args, **kw, dtype=np.float64)
+    assert _compare("all", gv, gf, ov, of, o64) <= TOL_REL
+
+
+def test_standalone_entry_points(hip):
+    _case("standalone AdvectIndexGrid / Velocity / Divergence / ProjectNonDivergent")
+    c, u, f, dx, dt = ragged_scene(4, cfl=1.0)
+    o = Oracle(c)
+    d = hns.GridIndexedData(c)
+    d.add_value_block("density", f["density"].copy())
+    d.add_value_block("temperature", f["temperature"].copy())
+    d.add_value_block("vel", u.copy())
+    hns.advect_index_grid(d, dt, dx)
+    for k in ("density", "temperature"):
+        assert rel_err(d[k], o.advect_scalar(u, f[k], dt, 1.0 / dx)) <= TOL_REL
+    dv = hns.GridIndexedData(c)
+    dv.add_value_block("vel", u.copy())
+    hns.advect_index_grid_velocity(dv, dt, dx)
+    assert rel_err(dv["vel"], o.advect_vector(u, dt, 1.0 / dx)) <= TOL_REL
+    dd = hns.GridIndexedData(c)
+    dd.add_value_block("vel", u.copy())
+    dd.add_value_block("divergence", np.zeros(len(c), np.float32))
+    hns.divergence(dd, dx)
+    div = o.divergence(u, 1.0 / dx)
+    assert rel_err(dd["divergence"], div) <= TOL_REL
+    dp = hns.GridIndexedData(c)
+    dp.add_value_block("vel", u.copy())
+    K = 20
+    hns.project_non_divergent(dp, K, dx)
+    omega = np.float32(2.0 / (1.0 + np.sin(3.14159 * np.float64(np.float32(dx)))))
+    p = np.zeros(len(c), np.float32)
+    for _ in range(K):
+        p = o.rbgs(div, p, dx, 0, omega)
+        p = o.rbgs(div, p, dx, 1, omega)
+    ref = o.subtract_gradient(u, p, 1.0 / dx)
+    e = rel_err(dp["vel"], ref)
+    print("  project rel %.3e max-abs %.3e" % (e, max_abs(dp["vel"], ref)))
+    assert e <= TOL_REL
+
+
+def test_context_reuse_and_recreate(hip):
+    """HNS_EnsureContext keeps device state when N and voxel size are unchanged and re-seeds
+    from the host payload otherwise (HNanoSolver.cu:64-155)."""
+    _case("context reuse")
+    s = scenes.smoke64()
+    data = _payload(s.coords, s.velocity, s.fields)
+    topo = hns.create_index_grid(data, s.voxel_size)
+    ctx = hns.ensure_context(None, data, topo, s.voxel_size)
+    p = hns.make_params(**s.params)
+    hns.advance(ctx, 5, s.dt, p)
+    h0 = ctx.handle.value
+    ctx = hns.ensure_context(ctx, data, topo, s.voxel_size)  # reuse: state kept
+    assert ctx.handle.value == h0
+    assert ctx.has_field("fuel")
+    ctx = hns.ensure_context(ctx, data, topo, s.voxel_size * 2)  # voxel size changed: new context
+    assert not ctx.has_field("fuel") and ctx.has_field("density")
+    n, L, ns = ctx.info()
+    assert (n, L, ns) == (s.n, s.n_leaves, 1)
+    m = hns.get_active_mask(ctx)
+    assert m.shape == (s.n,) and (m == 1).all()
+    hns.destroy_context(ctx)
+    assert not ctx
+    assert hns.get_active_mask(None).size == 0
+
+
+def test_boundary_errors(hip):
+    _case("boundary errors")
+    c = canonical_coords(ragged_origins(5))
+    # leaves out of voxelsToGrid order
+    L = len(c) // 512
+    swapped = c.reshape(L, 512, 3)[[1, 0] + list(range(2, L))].reshape(-1, 3)
+    with pytest.raises(hns.HNSError) as e:
+        hns.create_index_grid(hns.GridIndexedData(swapped), 0.1)
+    assert e.value.status == -5
+    # not leaf-dense (voxel order inside a leaf broken)
+    bad = c.copy()
+    bad[[3, 4]] = bad[[4, 3]]
+    with pytest.raises(hns.HNSError) as e:
+        hns.create_index_grid(hns.GridIndexedData(bad), 0.1)
+    assert e.value.status == -4
+    with pytest.raises(hns.HNSError) as e:
+        hns.create_index_grid(hns.GridIndexedData(c[:700]), 0.1)
+    assert e.value.status == -4
+    with pytest.raises(hns.HNSError) as e:
+        hns.create_index_grid(hns.GridIndexedData(np.zeros((0, 3), np.int32)), 0.1)
+    assert e.value.status == -2
+    # no velocity block
+    d = hns.GridIndexedData(c)
+    d.add_value_block("density", np.zeros(len(c), np.float32))
+    topo = hns.create_index_grid(d, 0.1)
+    with pytest.raises(hns.HNSError) as e:
+        hns.ensure_context(None, d, topo, 0.1)
+    assert e.value.status == -6
+    # payload / topology size mismatch
+    d2 = hns.GridIndexedData(c[:512])
+    d2.add_value_block("velocity", np.zeros((512, 3), np.float32))
+    with pytest.raises(hns.HNSError) as e:
+        hns.ensure_context(None, d2, topo, 0.1)
+    assert e.value.status == -4
+
+
+def test_determinism(hip):
+    _case("determinism fire")
+    s = scenes.precond128(6)
+    args = (s.coords, s.voxel_size, s.velocity, s.fields, s.params, 30, s.dt, 2)
+    a = _run_gpu(*args)
+    b = _run_gpu(*args)
+    np.testing.assert_array_equal(a[0], b[0])
+    for k in a[1]:
+        np.testing.assert_array_equal(a[1][k], b[1][k])
+
+
+def test_rbgs_to_tolerance(hip):
+    _case("rbgs to tolerance smoke64")
+    s = scenes.smoke64()
+    data = _payload(s.coords, s.velocity, s.fields)
+    topo = hns.create_index_grid(data, s.voxel_size)
+    ctx = hns.ensure_context(None, data, topo, s.voxel_size)
+    p = hns.make_params(**s.params)
+    st = hns.advance(ctx, 0, s.dt, p, solver="rbgs_tol", tol=1e-3, check_every=25, max_iters=20000)
+    print("  iterations %d rel residual %.3e" % (st.iterations, st.rel_residual))
+    assert st.rel_residual <= 1e-3 and st.iterations > 0
+    st2 = hns.advance(ctx, 60, s.dt, p, stats=True)
+    assert st2.iterations == 60 and 0 < st2.rel_residual < 1
+    hns.destroy_context(ctx)

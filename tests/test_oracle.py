@@ -1,0 +1,385 @@
+"""CPU pins of the oracle's kernels (rows a2-a15): closed forms, invariants, an FP64 sparse
+direct solve, and the composition of the step. These fix what the oracle computes without
+re-typing its formulas: each check is a property the reference kernel's mathematics implies
+(src/Cuda/Kernel.cu, cited per test) that a plausible slip (a dropped term, a wrong sign or
+index, a transposed operand, a wrong miss rule) would break."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from hnanosolver_amd import scenes
+
+from _helpers import canonical_coords, plane_sdf, ragged_scene
+
+
+def block_coords(nl=4, origin=(0, 0, 0)):
+    """Dense block of nl^3 leaves."""
+    g = np.arange(nl) * 8
+    org = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3) + np.asarray(origin)
+    return canonical_coords(org)
+
+
+def interior(c, lo, hi, margin):
+    return ((c >= lo + margin) & (c <= hi - margin)).all(1)
+
+
+# ---------------------------------------------------------------- advection (a4, a14, a2)
+def test_advect_scalars_zero_velocity_is_identity():
+    c, u, f, dx, dt = ragged_scene(0)
+    o = oracle.Oracle(c)
+    out = o.advect_scalars(np.zeros_like(u), [f["density"], f["fuel"]], dt, 1 / dx)
+    np.testing.assert_array_equal(out[0], f["density"])
+    np.testing.assert_array_equal(out[1], f["fuel"])
+
+
+@pytest.mark.parametrize("shift", [1, 2, -1])
+def test_integer_cfl_is_exact_translation(shift):
+    c = block_coords(4)
+    o = oracle.Oracle(c)
+    dx, dt = 0.125, 0.25  # exact binary fractions: shift * dx / dt * dt / dx == shift
+    u = np.zeros((len(c), 3), np.float32)
+    u[:, 0] = shift * dx / dt
+    f = (np.sin(0.3 * c[:, 0]) + np.cos(0.2 * c[:, 1]) * c[:, 2] * 0.1).astype(np.float32)
+    m = interior(c, 0, 31, 3)
+    lut = {tuple(p): i for i, p in enumerate(c)}
+    src = np.array([lut[(p[0] - shift, p[1], p[2])] for p in c[m]])
+    for out in (o.advect_scalars(u, [f], dt, 1 / dx)[0], o.advect_scalar(u, f, dt, 1 / dx)):
+        np.testing.assert_array_equal(out[m], f[src])
+    # a uniform velocity field advects itself unchanged
+    np.testing.assert_array_equal(o.advect_vector(u, dt, 1 / dx)[m], u[m])
+
+
+def test_subcell_shift_exact_on_affine_field():
+    """Trilinear samples of an affine field are exact, the MacCormack correction vanishes and the
+    clamp is inactive for a single-axis sub-cell shift: out = f(x - s) in the interior."""
+    c = block_coords(3)
+    o = oracle.Oracle(c)
+    dx, dt = 0.125, 0.25
+    s = 0.3125
+    u = np.zeros((len(c), 3), np.float32)
+    u[:, 1] = s * dx / dt
+    a = np.array([0.5, -0.25, 0.125])
+    f = (c @ a + 1.0).astype(np.float32)
+    m = interior(c, 0, 23, 3)
+    exp = (c[m] - [0, s, 0]) @ a + 1.0
+    for out in (o.advect_scalars(u, [f], dt, 1 / dx)[0], o.advect_scalar(u, f, dt, 1 / dx)):
+        np.testing.assert_allclose(out[m], exp, rtol=0, atol=2e-6)
+
+
+def test_advect_scalars_misses_read_element_zero():
+    """Quirk Q1 (Kernel.cu:187-188, :229-230, :256-257): advect_scalars maps a miss to element 0,
+    so a constant field is preserved even where the trace leaves the domain; advect_scalar maps a
+    miss to 0 and loses mass there."""
+    c, u, f, dx, dt = ragged_scene(1, cfl=3.0)
+    o = oracle.Oracle(c)
+    k = np.full(len(c), 0.75, np.float32)
+    # weights sum to 1 up to rounding: the constant survives to an ulp everywhere
+    np.testing.assert_allclose(o.advect_scalars(u, [k], dt, 1 / dx)[0], k, rtol=0, atol=4e-7)
+    single = o.advect_scalar(u, k, dt, 1 / dx)
+    assert (single < 0.7).sum() > 100 and (single <= 0.75 + 4e-7).all()
+
+
+def test_advect_scalars_global_max_principle():
+    c, u, f, dx, dt = ragged_scene(2, cfl=2.0)
+    o = oracle.Oracle(c)
+    d = f["temperature"]
+    out = o.advect_scalars(u, [d], dt, 1 / dx)[0]
+    assert out.max() <= d.max() and out.min() >= d.min()
+
+
+def test_advect_vector_miss_reads_zero():
+    """advect_vector samples with misses as 0 (Stencils.hpp:83): a uniform field loses speed only
+    where the back-trace or the clamp neighbourhood leaves the domain."""
+    c = block_coords(2)
+    o = oracle.Oracle(c)
+    u = np.zeros((len(c), 3), np.float32)
+    u[:, 2] = 0.8
+    out = o.advect_vector(u, 0.1, 10.0)
+    at_low_face = c[:, 2] == 0
+    np.testing.assert_array_equal(out[~at_low_face & (c[:, 2] < 15)], u[~at_low_face & (c[:, 2] < 15)])
+    assert (out[at_low_face, 2] < 0.8).all()
+
+
+# ---------------------------------------------------------------- vorticity (a5)
+def test_vorticity_factor_scale_below_one_is_identity():
+    c, u, f, dx, dt = ragged_scene(3)
+    o = oracle.Oracle(c)
+    for F in (0.5, 0.0, -0.9):
+        np.testing.assert_array_equal(o.vorticity(u, dt, 1 / dx, 0.2, F), u)
+
+
+def test_vorticity_constant_curl_has_no_force():
+    """Solid-body rotation has constant |curl|, so the confinement normal and force vanish."""
+    c = block_coords(4)
+    o = oracle.Oracle(c)
+    u = np.stack([-(c[:, 1] - 16.0), c[:, 0] - 16.0, np.zeros(len(c))], -1).astype(np.float32)
+    out = o.vorticity(u, 0.05, 10.0, 0.3, 1.0)
+    m = interior(c, 0, 31, 3)
+    np.testing.assert_array_equal(out[m], u[m])
+    assert not np.array_equal(out[~m], u[~m])
+
+
+def test_vorticity_shear_closed_form():
+    """u = (0, 0, x^2) with f = 0.5 inv_dx: curl = (0, -4 f x, 0), |curl| = 4 f |x| grows away from
+    x = 0, so grad |curl| = 8 f^2 sign(x) e_x, N = sign(x) e_x, and the force
+    eps (N x curl)_z = eps Nx wy = -4 eps f |x| points along -z on both sides."""
+    c = block_coords(4, origin=(-16, -16, -16))
+    o = oracle.Oracle(c)
+    x = c[:, 0].astype(np.float64)
+    u = np.stack([np.zeros_like(x), np.zeros_like(x), x * x], -1).astype(np.float32)
+    inv_dx, eps, dt = 2.0, 0.5, 0.25
+    f = 0.5 * inv_dx
+    out = o.vorticity(u, dt, inv_dx, eps, 1.0)
+    m = interior(c, -16, 15, 3) & (np.abs(c[:, 0]) >= 2)
+    g = 8 * f * f
+    exp = x * x - dt * eps * 4 * f * np.abs(x) * g / (g + 1e-5)
+    np.testing.assert_allclose(out[m, 2], exp[m], rtol=1e-6, atol=1e-5)
+    np.testing.assert_array_equal(out[m, :2], u[m, :2])
+
+
+# ---------------------------------------------------------------- divergence / gradient (a7, a12)
+def test_divergence_linear_field_and_boundary_misses():
+    c = block_coords(2)
+    o = oracle.Oracle(c)
+    a, b, d = 2.0, -3.0, 5.0
+    u = np.stack([a * c[:, 0], b * c[:, 1], d * c[:, 2]], -1).astype(np.float32)
+    inv_dx = 4.0
+    div = o.divergence(u, inv_dx)
+    m = interior(c, 0, 15, 1)
+    np.testing.assert_array_equal(div[m], np.float32((a + b + d) * inv_dx))
+    # at x = 15 the +x neighbour is missing and reads 0: xp = cx/2, xm = (cx + u(x-1))/2
+    e = (c[:, 0] == 15) & interior(c[:, 1:], 0, 15, 1)
+    cx = a * 15.0
+    exp = ((cx * 0.5 - (cx + a * 14.0) * 0.5) + b + d) * inv_dx
+    np.testing.assert_allclose(div[e], exp, rtol=1e-6)
+
+
+def test_gradient_quadratic_pressure():
+    c = block_coords(2)
+    o = oracle.Oracle(c)
+    x, y, z = [c[:, i].astype(np.float64) for i in range(3)]
+    p = (x * x + 2 * y - 3 * z).astype(np.float32)
+    u = np.zeros((len(c), 3), np.float32)
+    inv_dx = 0.5
+    out = o.subtract_gradient(u, p, inv_dx)
+    m = interior(c, 0, 15, 1)
+    exp = -np.stack([2 * x, np.full_like(x, 2.0), np.full_like(x, -3.0)], -1) * inv_dx
+    np.testing.assert_array_equal(out[m], exp[m].astype(np.float32))
+
+
+# ---------------------------------------------------------------- pressure (a10, a11)
+def laplacian(c, dx, diag=6.0):
+    """Dirichlet-0 7-point operator A = -L on the leaf-dense voxels, assembled from coordinates."""
+    n = len(c)
+    lut = {tuple(p): i for i, p in enumerate(c)}
+    rows, cols, vals = [], [], []
+    for i, p in enumerate(c):
+        rows.append(i); cols.append(i); vals.append(diag)
+        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            j = lut.get((p[0] + d[0], p[1] + d[1], p[2] + d[2]))
+            if j is not None:
+                rows.append(i); cols.append(j); vals.append(-1.0)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n)) / (dx * dx)
+
+
+def test_rbgs_converges_to_direct_solve_and_energy_decreases():
+    s = scenes.smoke64()
+    c = canonical_coords(np.asarray([[24, 24, 24], [32, 24, 24], [24, 32, 24], [32, 32, 24], [24, 24, 32],
+                                     [32, 24, 32], [24, 32, 32]], np.int32))
+    o = oracle.Oracle(c)
+    o64 = oracle.Oracle(c, np.float64)
+    dx = s.voxel_size
+    rng = np.random.default_rng(4)
+    div = rng.standard_normal(len(c)).astype(np.float32)
+    dx64 = float(np.float32(dx))
+    A = laplacian(c, dx64)
+    p_star = spla.spsolve(A.tocsc(), -div.astype(np.float64))   # L p = div  <=>  A p = -div
+    # the sweep multiplies by the float constant 0.166666667f (Kernel.cu:475), so its fixed point
+    # solves the operator with diagonal 1 / float32(0.166666667) instead of 6
+    inv6 = float(np.float32(0.166666667))
+    A_ref = laplacian(c, dx64, diag=1.0 / inv6)
+    p_fix = spla.spsolve(A_ref.tocsc(), -div.astype(np.float64))
+    omega = 2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(dx)))
+    p = np.zeros(len(c))
+    energy = []
+    for it in range(400):
+        p = o64.rbgs(div, p, dx, 0, omega)
+        p = o64.rbgs(div, p, dx, 1, omega)
+        e = p - p_fix
+        energy.append(float(e @ (A_ref @ e)))
+    live = [e for e in energy if e > 1e-24 * energy[0]]
+    assert len(live) > 50 and all(b < a for a, b in zip(live, live[1:]))
+    np.testing.assert_allclose(p, p_fix, rtol=0, atol=1e-10 * np.abs(p_fix).max())
+    assert np.abs(p_fix - p_star).max() <= 1e-6 * np.abs(p_star).max()
+    r2, d2 = o64.residual(p_star, div, dx)
+    assert np.sqrt(r2 / d2) < 1e-12
+    r2, d2 = o.residual(np.zeros(len(c), np.float32), div, dx)
+    assert r2 == pytest.approx(d2, rel=1e-12)
+    # FP32 RBGS reaches the FP64 solution to FP32 accuracy
+    q = np.zeros(len(c), np.float32)
+    for it in range(400):
+        q = o.rbgs(div, q, dx, 0, np.float32(omega))
+        q = o.rbgs(div, q, dx, 1, np.float32(omega))
+    assert np.abs(q - p_star).max() <= 1e-5 * np.abs(p_star).max()
+
+
+def test_rbgs_sweep_touches_only_its_colour():
+    c, u, f, dx, dt = ragged_scene(4)
+    o = oracle.Oracle(c)
+    rng = np.random.default_rng(5)
+    div = rng.standard_normal(len(c)).astype(np.float32)
+    p = rng.standard_normal(len(c)).astype(np.float32)
+    col = c.sum(1) & 1
+    for k in (0, 1):
+        q = o.rbgs(div, p, dx, k, 1.5)
+        np.testing.assert_array_equal(q[col != k], p[col != k])
+        assert not np.array_equal(q[col == k], p[col == k])
+
+
+# ---------------------------------------------------------------- pointwise (a6, a8, a9, a13)
+def test_combustion_branches():
+    """combustion_oxygen (Kernel.cu:641-677), one voxel per branch, values by hand."""
+    c = block_coords(1)
+    o = oracle.Oracle(c)
+    n = len(c)
+    fuel = np.zeros(n, np.float32); waste = np.zeros(n, np.float32); temp = np.zeros(n, np.float32)
+    flame = np.zeros(n, np.float32); div = np.zeros(n, np.float32)
+    fuel[0], waste[0], temp[0], flame[0] = 0.5, 0.2, 1.0, 0.1      # burn = min(0.3, 0.5) = 0.3
+    fuel[1], waste[1] = 0.0005, 0.1                                 # snaps to 0, burn 0
+    fuel[2], waste[2], temp[2] = 0.7, 0.5, 3.0                      # oxygen < 0: unchanged
+    fuel[3], waste[3] = 0.02, 0.0                                   # burn = 0.02, flame = 0.2
+    f2, w2, t2, d2, fl2 = o.combustion(fuel, waste, temp, div, flame, 0.9, 0.1)
+    assert f2[0] == pytest.approx(0.2) and w2[0] == pytest.approx(0.8) and fl2[0] == 1.0
+    assert t2[0] == pytest.approx(1.0 + 0.3 * 0.9) and d2[0] == pytest.approx(0.03)
+    assert f2[1] == 0.0 and w2[1] == pytest.approx(0.1) and d2[1] == 0.0
+    assert (f2[2], w2[2], t2[2], d2[2]) == (np.float32(0.7), np.float32(0.5), np.float32(3.0), 0.0)
+    assert fl2[3] == pytest.approx(0.2) and f2[3] == 0.0 and w2[3] == pytest.approx(0.04)
+
+
+def test_buoyancy_closed_form():
+    c = block_coords(1)
+    o = oracle.Oracle(c)
+    n = len(c)
+    u = np.ones((n, 3), np.float32)
+    T = np.linspace(0, 50, n).astype(np.float32)
+    out = o.buoyancy(u, T, 0.1, 23.0, 2.0)
+    hot = T > 23.0
+    np.testing.assert_allclose(out[hot, 1], 1.0 + 0.1 * (T[hot] - 23.0) * 2.0, rtol=1e-6)
+    np.testing.assert_array_equal(out[~hot], u[~hot])
+    np.testing.assert_array_equal(out[:, [0, 2]], u[:, [0, 2]])
+    np.testing.assert_array_equal(o.buoyancy(u, T, 0.1, 23.0, -2.0), u)
+
+
+def test_collision_planar_sdf_closed_form():
+    c = block_coords(2)
+    o = oracle.Oracle(c)
+    dx = 0.1
+    nrm = np.array([0.6, 0.8, 0.0])
+    sdf = plane_sdf(c, nrm, offset=1.0, dx=dx)
+    rng = np.random.default_rng(6)
+    u = rng.standard_normal((len(c), 3)).astype(np.float32)
+    out = o.enforce_collision(u, sdf, dx)
+    np.testing.assert_array_equal(out[sdf < 0], 0.0)
+    band = (sdf >= 0) & (sdf < 0.1) & interior(c, 0, 15, 1)
+    assert band.sum() > 10
+    b = 1.0 - sdf[band] / 0.1
+    v = u[band].astype(np.float64)
+    tang = v - (v @ nrm)[:, None] * nrm
+    np.testing.assert_allclose(out[band], v * (1 - b)[:, None] + tang * b[:, None], rtol=1e-5, atol=1e-6)
+    far = sdf >= 0.1
+    np.testing.assert_array_equal(out[far], u[far])
+
+
+# ---------------------------------------------------------------- the step (a15)
+def compose_step(o, dx, K, dt, P, vel, fields, sources=None, src_vel=None, sdf=None):
+    """HNS_Advance restated from HNanoSolver.cu:223-456 as a sequence of oracle kernels."""
+    f32 = np.float32
+    sources = sources or {}
+    fields = {k: v.copy() for k, v in fields.items()}
+    exists = set(fields)
+    vel = vel.copy()
+    inv_dx = f32(1.0) / f32(dx)
+    subs = max(1, int(P["substeps"]))
+    dts = f32(dt) / f32(subs)
+    if sdf is not None:
+        vel = o.enforce_collision(vel, sdf, dx)
+    adv_names = list(fields)
+    omega = f32(2.0) / (f32(1.0) + np.sin(f32(3.14159) * f32(dx), dtype=np.float32))
+    for _ in range(subs):
+        a = o.advect_vector(vel, dts, inv_dx, sdf)
+        a = o.vorticity(a, dts, inv_dx, P["vorticityScale"], P["factorScale"])
+        if src_vel is not None:
+            a = a + src_vel * dts
+        for sname, dname in (("src_temperature", "temperature"), ("src_density", "density"), ("src_fuel", "fuel")):
+            if sname in sources and dname in exists:
+                fields[dname] = fields[dname] + dts * sources[sname]
+        if P["gravity"] != 0:
+            a[:, 1] -= f32(P["gravity"]) * dts
+        div = o.divergence(a, inv_dx)
+        for k in ("fuel", "waste", "temperature", "flame"):
+            if k not in exists:
+                fields[k] = np.zeros(len(vel), np.float32)
+                exists.add(k)
+        fu, wa, te, div, fl = o.combustion(fields["fuel"], fields["waste"], fields["temperature"], div,
+                                           fields["flame"], P["temperatureRelease"], P["expansionRate"])
+        fields.update(fuel=fu, waste=wa, temperature=te, flame=fl)
+        a = o.buoyancy(a, te, dts, P["ambientTemp"], P["buoyancyStrength"])
+        p = np.zeros(len(vel), np.float32)
+        for _ in range(K):
+            p = o.rbgs(div, p, dx, 0, omega)
+            p = o.rbgs(div, p, dx, 1, omega)
+        if P["disturbanceEnable"] > 0:
+            div = o.divergence(a, inv_dx)
+            p = np.zeros(len(vel), np.float32)
+            for _ in range(max(2, K // 2)):
+                p = o.rbgs(div, p, dx, 0, omega)
+                p = o.rbgs(div, p, dx, 1, omega)
+        vel = o.subtract_gradient(a, p, inv_dx, sdf)
+        if sdf is not None:
+            vel = o.enforce_collision(vel, sdf, dx)
+        outs = o.advect_scalars(vel, [fields[k] for k in adv_names], dts, inv_dx, sdf)
+        fields.update(dict(zip(adv_names, outs)))
+    return vel, fields
+
+
+def test_step_is_the_composition_smoke():
+    s = scenes.smoke64()
+    o = oracle.Oracle(s.coords)
+    v1, f1 = o.advance(s.voxel_size, 6, s.dt, s.params, s.velocity, s.fields)
+    v2, f2 = compose_step(o, s.voxel_size, 6, s.dt, s.params, s.velocity, s.fields)
+    np.testing.assert_array_equal(v1, v2)
+    assert set(f1) == set(f2)
+    for k in f1:
+        np.testing.assert_array_equal(f1[k], f2[k])
+
+
+def test_step_is_the_composition_all_features():
+    c, u, f, dx, dt = ragged_scene(5, cfl=1.2)
+    n = len(c)
+    rng = np.random.default_rng(8)
+    sources = {"src_density": rng.random(n).astype(np.float32), "src_temperature": rng.random(n).astype(np.float32),
+               "src_fuel": rng.random(n).astype(np.float32) * 0.1}
+    src_vel = rng.standard_normal((n, 3)).astype(np.float32)
+    sdf = plane_sdf(c, offset=-0.4, dx=dx)
+    P = scenes.default_params(vorticityScale=0.3, factorScale=1.0, disturbanceEnable=1.0, substeps=2, gravity=2.0,
+                              ambientTemp=0.5)
+    fields = {"density": f["density"], "temperature": f["temperature"]}  # fuel is created by needOrMake
+    o = oracle.Oracle(c)
+    v1, f1 = o.advance(dx, 5, dt, P, u, fields, sources, src_vel, sdf)
+    v2, f2 = compose_step(o, dx, 5, dt, P, u, fields, sources, src_vel, sdf)
+    np.testing.assert_array_equal(v1, v2)
+    for k in f1:
+        np.testing.assert_array_equal(f1[k], f2[k], err_msg=k)
+    # fuel did not exist at call start: created, sources skipped in substep 0, not advected
+    assert "fuel" in f1
+
+
+def test_fp64_oracle_agrees_with_fp32():
+    s = scenes.smoke64()
+    v32, f32 = oracle.Oracle(s.coords).advance(s.voxel_size, s.iterations, s.dt, s.params, s.velocity, s.fields)
+    v64, f64 = oracle.Oracle(s.coords, np.float64).advance(s.voxel_size, s.iterations, s.dt, s.params, s.velocity,
+                                                          s.fields)
+    assert np.abs(v32 - v64).max() <= 1e-5 * np.abs(v64).max()
+    assert np.abs(f32["density"] - f64["density"]).max() <= 1e-5
