@@ -17,12 +17,15 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhns.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-SOURCES = ["topology.hip", "kernels_step.hip", "kernels_pressure.hip", "solvers.hip", "api.hip"]
+SOURCES = ["topology.hip", "kernels_step.hip", "kernels_pressure.hip", "kernels_cg.hip", "solvers.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HNS_OFFLOAD_ARCH", "gfx950")
-# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in device code (build flags)
-DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 in device code.
+# -ffp-contract=off: the kernels execute exactly the multiplies and adds they write (fused only
+# where they call fmaf, as the reference does with __fmaf_rn / fmaf), the same rule the CPU oracle
+# is compiled with, so FP32 results are reproducible operation for operation.
+DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=off"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall", "-Wno-unused-result",
           "-I", os.path.join(ROOT, "include")]
 

@@ -815,6 +815,45 @@ hns_status hnsk_residual(const hns_topology* topo, const float* p_cs, const floa
   });
 }
 
+hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, float* p_cs, float dx, float omega,
+                               int32_t iterations, const hns_solve_options* opts, hns_solve_stats* stats,
+                               void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!div_cs || !p_cs || iterations < 0) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (opts && opts->struct_size < sizeof(hns_solve_options))
+      fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_options.struct_size too small");
+    if (stats && stats->struct_size < sizeof(hns_solve_stats))
+      fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_stats.struct_size too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Solvers sol;
+    sol.init(T.L, topo->s->N);
+    Timing tm;
+    SolveRequest req;
+    req.omega = omega;
+    req.dx = dx;
+    req.want_residual = true;
+    if (opts) {
+      req.solver = opts->solver;
+      req.tol = opts->tol;
+      req.max_iters = opts->max_iters;
+      req.check_every = opts->check_every;
+    }
+    const SolveResult res = sol.solve(T, div_cs, p_cs, iterations, req, tm, s);
+    HNS_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iterations = res.iterations;
+      stats->rel_residual = res.rel_residual;
+      stats->solve_ms = res.ms;
+    }
+    if (opts && opts->solver != HNS_SOLVER_RBGS_FIXED && !(res.rel_residual <= opts->tol)) {
+      g_err = "pressure solve reached max_iters before tol";
+      return HNS_ERR_NOT_CONVERGED;
+    }
+    return HNS_OK;
+  });
+}
+
 hns_status hnsk_subtract_gradient(const hns_topology* topo, const float* vel, const float* p_cs, float* out,
                                   float inv_dx, const float* sdf, void* stream, int32_t async) {
   return guard([&] {

@@ -126,6 +126,10 @@ void launch_aos_soa(size_t n, const float* in, float* out, bool to_soa, hipStrea
 // One colour sweep; first != 0 treats p as zero before the sweep (iteration 0 from p = 0).
 void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,
                  hipStream_t s);
+// One full red-black iteration (both colours), temporally blocked in LDS; p_in != p_out.
+// first != 0 treats p_in as zero (it is not read).
+void launch_rbgs_iteration(const DevTopo& T, const float* div_cs, const float* p_in, float* p_out, float dx,
+                           float omega, bool first, hipStream_t s);
 // Writes per-workgroup partial sums (2 doubles per leaf) then reduces into out[0..1].
 void launch_residual(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* partial,
                      double* out, hipStream_t s);

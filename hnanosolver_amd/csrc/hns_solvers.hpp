@@ -98,6 +98,8 @@ class Solvers {
  private:
   int32_t L_ = 0;
   size_t N_ = 0;
+  bool per_colour_ = false;
+  DBuf<float> pb_;  // ping-pong partner of p for the temporally blocked iteration
   DBuf<double> partial_, red_;
   double* h_red_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;

@@ -16,6 +16,13 @@ namespace {
 
 constexpr int kNbXm = 4, kNbXp = 22, kNbYm = 10, kNbYp = 16, kNbZm = 12, kNbZp = 14;  // (dx+1)*9+(dy+1)*3+(dz+1)
 
+// The SOR update of redBlackGaussSeidelUpdate (Kernel.cu:487-488), shared by both kernels so they
+// produce identical bits: pGS = (sum6 - div dx^2) / 6 with the reference's 0.166666667f.
+__device__ __forceinline__ float sor_update(float s, float dv, float pOld, float dx2, float omega) {
+  const float pGS = (s - dv * dx2) * 0.166666667f;
+  return pOld + omega * (pGS - pOld);
+}
+
 template <int COLOUR, bool FIRST>
 __global__ __launch_bounds__(256) void k_rbgs(DevTopo T, const float* __restrict__ div, float* __restrict__ p,
                                               float dx2, float omega) {
@@ -48,8 +55,89 @@ __global__ __launch_bounds__(256) void k_rbgs(DevTopo T, const float* __restrict
   }
   const float dv = div[COLOUR * half + base + t];
   const float pOld = FIRST ? 0.f : pc[base + t];
-  const float pGS = (s - dv * dx2) * 0.166666667f;
-  pc[base + t] = pOld + omega * (pGS - pOld);
+  pc[base + t] = sor_update(s, dv, pOld, dx2, omega);
+}
+
+// One full red-black iteration per launch (colour 0 then colour 1), temporally blocked in LDS.
+// The workgroup stages its leaf plus a 2-voxel halo of p (12^3) and the divergence of the inner
+// 10^3, updates colour 0 on the 10^3 (its halo ring included, so colour 1 of the 8^3 core sees
+// exactly the values the global colour-0 sweep would produce), then colour 1 on the 8^3 core, and
+// writes the core. Cells of absent leaves stay 0 and are never updated (the Dirichlet boundary of
+// Kernel.cu:477-482). Reads p_in, writes p_out (distinct: neighbours read the halo from p_in while
+// this workgroup writes). HBM per voxel and iteration: 4 B p + 4 B div read, 4 B p written.
+constexpr int kT = 12;  // tile edge: 8 + 2 * 2
+__device__ __forceinline__ int tile_idx(int x, int y, int z) { return (x * kT + y) * kT + z; }
+
+template <bool FIRST>
+__global__ __launch_bounds__(512) void k_rbgs_iter(DevTopo T, const float* __restrict__ div,
+                                                   const float* __restrict__ p_in, float* __restrict__ p_out,
+                                                   float dx2, float omega) {
+  __shared__ int32_t s_nb[27];
+  __shared__ float s_p[kT * kT * kT];
+  __shared__ float s_d[10 * 10 * 10];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const int32_t tid = threadIdx.x;
+  if (tid < 27) s_nb[tid] = T.nbr[kNbrStride * l + tid];
+  __syncthreads();
+  const int32_t half = T.L * kHalfLeaf;
+  // tile coordinate t in [0,12) is local coordinate t-2; its leaf offset is (t-2)>>3 in {-1,0,1}
+  for (int c = tid; c < kT * kT * kT; c += 512) {
+    const int tx = c / (kT * kT), ty = (c / kT) % kT, tz = c % kT;
+    float v = 0.f;
+    if (!FIRST) {
+      const int lx = tx - 2, ly = ty - 2, lz = tz - 2;
+      const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
+      if (nl >= 0) {
+        const int x = lx & 7, y = ly & 7, z = lz & 7;
+        v = p_in[((x + y + z) & 1) * half + nl * kHalfLeaf + cs_local(x, y, z)];
+      }
+    }
+    s_p[c] = v;
+  }
+  for (int c = tid; c < 1000; c += 512) {
+    const int tx = c / 100 + 1, ty = (c / 10) % 10 + 1, tz = c % 10 + 1;
+    const int lx = tx - 2, ly = ty - 2, lz = tz - 2;
+    const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
+    float v = 0.f;
+    if (nl >= 0) {
+      const int x = lx & 7, y = ly & 7, z = lz & 7;
+      v = div[((x + y + z) & 1) * half + nl * kHalfLeaf + cs_local(x, y, z)];
+    }
+    s_d[c] = v;
+  }
+  __syncthreads();
+  // colour 0 on the 10^3 region [1,11)^3: 100 rows of 5 cells
+  if (tid < 500) {
+    const int row = tid / 5, j = tid % 5;
+    const int tx = 1 + row / 10, ty = 1 + row % 10;
+    const int tz = 1 + ((tx + ty + 1) & 1) + 2 * j;  // (tx+ty+tz) even: colour 0
+    const int lx = tx - 2, ly = ty - 2, lz = tz - 2;
+    if (s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)] >= 0) {
+      const float s = s_p[tile_idx(tx + 1, ty, tz)] + s_p[tile_idx(tx - 1, ty, tz)] + s_p[tile_idx(tx, ty + 1, tz)] +
+                      s_p[tile_idx(tx, ty - 1, tz)] + s_p[tile_idx(tx, ty, tz + 1)] + s_p[tile_idx(tx, ty, tz - 1)];
+      const int c = tile_idx(tx, ty, tz);
+      s_p[c] = sor_update(s, s_d[((tx - 1) * 10 + (ty - 1)) * 10 + (tz - 1)], s_p[c], dx2, omega);
+    }
+  }
+  __syncthreads();
+  // colour 1 on the 8^3 core [2,10)^3: 64 rows of 4 cells (always present)
+  if (tid < 256) {
+    const int row = tid >> 2, j = tid & 3;
+    const int tx = 2 + (row >> 3), ty = 2 + (row & 7);
+    const int tz = 2 + ((tx + ty + 1) & 1) + 2 * j;  // (tx+ty+tz) odd: colour 1
+    const float s = s_p[tile_idx(tx + 1, ty, tz)] + s_p[tile_idx(tx - 1, ty, tz)] + s_p[tile_idx(tx, ty + 1, tz)] +
+                    s_p[tile_idx(tx, ty - 1, tz)] + s_p[tile_idx(tx, ty, tz + 1)] + s_p[tile_idx(tx, ty, tz - 1)];
+    const int c = tile_idx(tx, ty, tz);
+    s_p[c] = sor_update(s, s_d[((tx - 1) * 10 + (ty - 1)) * 10 + (tz - 1)], s_p[c], dx2, omega);
+  }
+  __syncthreads();
+  // write the core: lane = colour-split local index within each colour
+  {
+    const int colour = tid >> 8, t = tid & 255;
+    const int x = t >> 5, y = (t >> 2) & 7, z = ((t & 3) << 1) | ((x + y + colour) & 1);
+    p_out[colour * half + l * kHalfLeaf + t] = s_p[tile_idx(x + 2, y + 2, z + 2)];
+  }
 }
 
 // Residual r = div - (sum6 p_nbr - 6 p) / dx^2 in FP64; per-leaf partial sums of r^2 and div^2.
@@ -141,6 +229,15 @@ void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, i
     if (first) k_rbgs<1, true><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
     else k_rbgs<1, false><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
   }
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_rbgs_iteration(const DevTopo& T, const float* div_cs, const float* p_in, float* p_out, float dx,
+                           float omega, bool first, hipStream_t s) {
+  const unsigned g = grid_for_leaves(T.L);
+  const float dx2 = dx * dx;
+  if (first) k_rbgs_iter<true><<<g, 512, 0, s>>>(T, div_cs, p_in, p_out, dx2, omega);
+  else k_rbgs_iter<false><<<g, 512, 0, s>>>(T, div_cs, p_in, p_out, dx2, omega);
   HNS_HIP(hipGetLastError());
 }
 

@@ -3,12 +3,25 @@
 // hns_solve_options (no reference counterpart).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 
+#include "hns_cg.hpp"
 #include "hns_solvers.hpp"
 
 namespace hns {
 
-struct CgState {};
+struct CgState {
+  DBuf<float> r, d0, d1, q;
+  DBuf<double> rho, scal;
+  DBuf<int> flag;
+  DBuf<unsigned> counter;
+  int* h_flag = nullptr;
+  int cap = 0;
+  ~CgState() {
+    if (h_flag) (void)hipHostFree(h_flag);
+  }
+};
 
 Solvers::Solvers() = default;
 
@@ -22,6 +35,11 @@ Solvers::~Solvers() {
 void Solvers::init(int32_t L, size_t N) {
   L_ = L;
   N_ = N;
+  // HNS_RBGS_KERNEL=sweep selects one launch per colour (the reference's structure) instead of the
+  // temporally blocked full iteration; both give identical results.
+  const char* k = std::getenv("HNS_RBGS_KERNEL");
+  per_colour_ = k && std::strcmp(k, "sweep") == 0;
+  pb_.alloc(N);
   partial_.alloc(2 * (size_t)L);
   red_.alloc(2);
   if (!h_red_) HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&h_red_), 2 * sizeof(double), hipHostMallocDefault));
@@ -29,16 +47,34 @@ void Solvers::init(int32_t L, size_t N) {
   if (!e1_) HNS_HIP(hipEventCreate(&e1_));
 }
 
+// n iterations continuing from p (or from p = 0 when first_it == 0); the result ends in p.
 void Solvers::rbgs_iters(const DevTopo& T, const float* div, float* p, int first_it, int n, const SolveRequest& r,
                          Timing& tm, hipStream_t s) {
-  for (int it = first_it; it < first_it + n; ++it) {
-    tm.begin(HNS_K_RBGS, s);
-    launch_rbgs(T, div, p, r.dx, 0, r.omega, it == 0, s);
-    tm.end(s);
-    tm.begin(HNS_K_RBGS, s);
-    launch_rbgs(T, div, p, r.dx, 1, r.omega, it == 0, s);
-    tm.end(s);
+  if (per_colour_) {
+    for (int it = first_it; it < first_it + n; ++it) {
+      tm.begin(HNS_K_RBGS, s);
+      launch_rbgs(T, div, p, r.dx, 0, r.omega, it == 0, s);
+      tm.end(s);
+      tm.begin(HNS_K_RBGS, s);
+      launch_rbgs(T, div, p, r.dx, 1, r.omega, it == 0, s);
+      tm.end(s);
+    }
+    return;
   }
+  // ping-pong between p and pb_: a first iteration does not read its input, so it may write p
+  // directly; choose the first destination so that the last one is p
+  float* src = p;
+  float* dst = ((n & 1) == (first_it == 0 ? 1 : 0)) ? p : pb_.p;
+  if (first_it != 0 && dst == p) dst = pb_.p;  // odd n from a live p: finish with a copy
+  for (int k = 0; k < n; ++k) {
+    const int it = first_it + k;
+    tm.begin(HNS_K_RBGS, s);
+    launch_rbgs_iteration(T, div, src, dst, r.dx, r.omega, it == 0, s);
+    tm.end(s);
+    src = dst;
+    dst = (dst == p) ? pb_.p : p;
+  }
+  if (n > 0 && src != p) HNS_HIP(hipMemcpyAsync(p, src, N_ * sizeof(float), hipMemcpyDeviceToDevice, s));
 }
 
 double Solvers::rel_residual(const DevTopo& T, const float* div, const float* p, float dx, Timing& tm,
@@ -97,8 +133,60 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
   return res;
 }
 
-SolveResult Solvers::solve_cg(const DevTopo&, const float*, float*, const SolveRequest&, Timing&, hipStream_t) {
-  fail(HNS_ERR_INVALID_ARGUMENT, "conjugate-gradient solvers are not built yet");
+SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
+                              hipStream_t s) {
+  if (r.solver == HNS_SOLVER_PCG_LEARNED) fail(HNS_ERR_INVALID_ARGUMENT, "the learned preconditioner is not loaded");
+  const int cap = r.max_iters > 0 ? r.max_iters : 100000;
+  if (!cg_) cg_ = new CgState();
+  CgState& G = *cg_;
+  if (!G.r.p) {
+    G.r.alloc(N_);
+    G.d0.alloc(N_);
+    G.d1.alloc(N_);
+    G.q.alloc(N_);
+    G.scal.alloc(kScalarSlots);
+    G.flag.alloc(2);
+    G.counter.alloc(1);
+    HNS_HIP(hipMemsetAsync(G.counter.p, 0, sizeof(unsigned), s));
+    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 2 * sizeof(int), hipHostMallocDefault));
+  }
+  if (G.cap < cap) {
+    G.rho.alloc((size_t)cap + 1);
+    G.cap = cap;
+  }
+  CgScalars S;
+  S.partial = partial_.p;
+  S.counter = G.counter.p;
+  S.rho = G.rho.p;
+  S.scal = G.scal.p;
+  S.flag = G.flag.p;
+  // the recurrence residual runs slightly ahead of the true one in FP32: stop it at tol/2
+  const double t = 0.5 * (double)r.tol;
+  S.tol2 = t * t;
+  tm.begin(HNS_K_CG, s);
+  launch_cg_init(T.L, div, r.dx, p, G.r.p, S, s);
+  tm.end(s);
+  int done = 0, batch = 8;
+  G.h_flag[0] = 0;
+  G.h_flag[1] = 0;
+  while (done < cap) {
+    const int nb = std::min(batch, cap - done);
+    for (int k = 0; k < nb; ++k) {
+      const int it = done + k;
+      tm.begin(HNS_K_CG, s);
+      launch_cg_iteration(T, it, p, G.r.p, (it & 1) ? G.d1.p : G.d0.p, (it & 1) ? G.d0.p : G.d1.p, G.q.p, S, s);
+      tm.end(s);
+    }
+    done += nb;
+    HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    HNS_HIP(hipStreamSynchronize(s));
+    if (G.h_flag[0]) break;
+    batch = std::min(batch * 2, 128);
+  }
+  SolveResult res;
+  res.iterations = G.h_flag[1];
+  res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
+  return res;
 }
 
 }  // namespace hns

@@ -65,6 +65,17 @@ def residual(topo, p_cs, div_cs, dx):
     return r2.value, d2.value
 
 
+def pressure_solve(topo, div_cs, p_cs, dx, omega=1.0, iterations=0, solver="rbgs", tol=1e-5, max_iters=100000,
+                   check_every=0):
+    """Whole pressure solve from p = 0; returns hns_solve_stats (raises unless converged)."""
+    from . import SOLVERS, hns_solve_options, hns_solve_stats
+    o = hns_solve_options(C.sizeof(hns_solve_options), SOLVERS[solver], float(tol), int(max_iters), int(check_every))
+    s = hns_solve_stats(C.sizeof(hns_solve_stats), 0, -1.0, 0.0)
+    _check(lib().hnsk_pressure_solve(topo.handle, _ptr(div_cs), _ptr(p_cs), dx, omega, int(iterations), C.byref(o),
+                                     C.byref(s), _stream()))
+    return s
+
+
 def subtract_gradient(topo, vel, p_cs, out, inv_dx, sdf=None, asynchronous=False):
     _check(lib().hnsk_subtract_gradient(topo.handle, _ptr(vel), _ptr(p_cs), _ptr(out), inv_dx, _ptr(sdf), _stream(),
                                         int(asynchronous)))

@@ -237,6 +237,11 @@ hns_status hnsk_rbgs_sweep(const hns_topology* topo, const float* div_cs, float*
 /* ||div - L p||^2 and ||div||^2 in FP64 (colour-split p and div); synchronous. */
 hns_status hnsk_residual(const hns_topology* topo, const float* p_cs, const float* div_cs, float dx,
                          double* r2, double* d2, void* stream);
+/* A whole pressure solve from p = 0 with one of the HNS_SOLVER_* methods on colour-split div and
+ * p (RBGS uses `iterations` and `omega`; the to-tolerance solvers use opts). Synchronous. */
+hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, float* p_cs, float dx, float omega,
+                               int32_t iterations, const hns_solve_options* opts, hns_solve_stats* stats,
+                               void* stream);
 /* subtractPressureGradient, Kernel.cu:495-541 (colour-split p; sdf may be null). */
 hns_status hnsk_subtract_gradient(const hns_topology* topo, const float* vel, const float* p_cs, float* out,
                                   float inv_dx, const float* sdf, void* stream, int32_t async);

@@ -62,6 +62,8 @@ def load():
             f("divergence").argtypes = [vp, vp, vp, C.c_float]
             f("rbgs").argtypes = [vp, vp, vp, C.c_float, C.c_int, C.c_float]
             f("residual").argtypes = [vp, vp, vp, C.c_float, vp, vp]
+            f("cg").restype = C.c_int
+            f("cg").argtypes = [vp, vp, C.c_float, C.c_double, C.c_int, vp, vp]
             f("subtract_gradient").argtypes = [vp, vp, vp, vp, C.c_float, vp]
             f("enforce_collision").argtypes = [vp, vp, vp, C.c_float]
             f("combustion").argtypes = [C.c_int64, vp, vp, vp, vp, vp, C.c_float, C.c_float]
@@ -184,6 +186,14 @@ class Oracle:
         pa, da = self._a(p), self._a(div)  # keep the arrays alive across the call
         self._f("residual")(self._t, pa.ctypes.data, da.ctypes.data, dx, C.byref(r2), C.byref(d2))
         return r2.value, d2.value
+
+    def cg(self, div, dx, tol=1e-5, max_iters=100000):
+        """CG solve of L p = div from p = 0; returns (p, iterations, recurrence relative residual)."""
+        d = self._a(div)
+        p = np.empty(self.n, self.dtype)
+        rel = C.c_double()
+        it = self._f("cg")(self._t, d.ctypes.data, dx, float(tol), int(max_iters), p.ctypes.data, C.byref(rel))
+        return p, int(it), rel.value
 
     def subtract_gradient(self, vel, p, inv_dx, sdf=None):
         v = self._a(vel, (self.n, 3))

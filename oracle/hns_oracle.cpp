@@ -493,6 +493,59 @@ void residual(const Topo& t, const R* p, const R* div, R dx, double* r2, double*
   *d2 = sd;
 }
 
+// Conjugate gradients on A p = b, A = (6 p - sum6 p_nbr) (the Dirichlet-0 operator of the RBGS
+// sweep scaled by dx^2), b = -div dx^2, from p = 0 until ||r|| <= tol ||b|| (textbook CG,
+// Hestenes & Stiefel 1952), in the arithmetic type R with FP64 dot products. New: the reference
+// has no to-tolerance solver.
+template <typename R>
+int conjugateGradient(const Topo& t, const R* div, R dx, double tol, int maxIters, R* p, double* relOut) {
+  const int64_t N = t.N;
+  std::vector<R> r(N), d(N), q(N);
+  const R dx2 = dx * dx;
+  double rr = 0.0;
+#pragma omp parallel for reduction(+ : rr) schedule(static)
+  for (int64_t i = 0; i < N; ++i) {
+    p[i] = R(0);
+    r[i] = -div[i] * dx2;
+    d[i] = r[i];
+    rr += double(r[i]) * double(r[i]);
+  }
+  const double bb = rr;
+  int it = 0;
+  while (it < maxIters && rr > tol * tol * bb) {
+    double dq = 0.0;
+#pragma omp parallel reduction(+ : dq)
+    {
+      Acc a(&t);
+#pragma omp for schedule(static)
+      for (int64_t idx = 0; idx < N; ++idx) {
+        const int32_t x = t.coords[3 * idx], y = t.coords[3 * idx + 1], z = t.coords[3 * idx + 2];
+        const R s = nearest0(a, d.data(), x + 1, y, z) + nearest0(a, d.data(), x - 1, y, z) +
+                    nearest0(a, d.data(), x, y + 1, z) + nearest0(a, d.data(), x, y - 1, z) +
+                    nearest0(a, d.data(), x, y, z + 1) + nearest0(a, d.data(), x, y, z - 1);
+        q[idx] = R(6) * d[idx] - s;
+        dq += double(d[idx]) * double(q[idx]);
+      }
+    }
+    if (!(dq > 0.0)) break;
+    const R alpha = R(rr / dq);
+    double rr2 = 0.0;
+#pragma omp parallel for reduction(+ : rr2) schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+      p[i] += alpha * d[i];
+      r[i] -= alpha * q[i];
+      rr2 += double(r[i]) * double(r[i]);
+    }
+    const R beta = R(rr2 / rr);
+    rr = rr2;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) d[i] = r[i] + beta * d[i];
+    ++it;
+  }
+  *relOut = bb > 0.0 ? std::sqrt(rr / bb) : 0.0;
+  return it;
+}
+
 // subtractPressureGradient, Kernel.cu:495-541.
 template <typename R>
 void subtractGradient(const Topo& t, const V3<R>* vel, const R* p, V3<R>* out, R inv_dx, const R* sdf) {
@@ -826,6 +879,9 @@ void oracle_set_num_threads(int n) {
   }                                                                                                                  \
   void oracle_residual_##SUF(void* t, const R* p, const R* div, float dx, double* r2, double* d2) {                   \
     oracle::residual<R>(*static_cast<Topo*>(t), p, div, R(dx), r2, d2);                                              \
+  }                                                                                                                  \
+  int oracle_cg_##SUF(void* t, const R* div, float dx, double tol, int maxIters, R* p, double* rel) {              \
+    return oracle::conjugateGradient<R>(*static_cast<Topo*>(t), div, R(dx), tol, maxIters, p, rel);                 \
   }                                                                                                                  \
   void oracle_subtract_gradient_##SUF(void* t, const R* vel, const R* p, R* out, float inv_dx, const R* sdf) {        \
     oracle::subtractGradient<R>(*static_cast<Topo*>(t), reinterpret_cast<const V3<R>*>(vel), p,                       \

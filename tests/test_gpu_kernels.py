@@ -300,3 +300,84 @@ def test_layout_roundtrips(hip):
     K.soa_to_aos(soa, aos2)
     assert torch.equal(aos2, aos)
     np.testing.assert_array_equal(soa.cpu().numpy().reshape(3, -1).T, u)
+
+
+def test_cg_against_scipy_fixture_and_oracle(hip):
+    """GPU CG (colour-split FP32 vectors, FP64 dots) on the committed scipy direct solve, and on
+    the divergence of the first smoke64 step against the FP64 oracle CG."""
+    import os
+    torch = hip
+    _case("cg poisson_block + smoke64")
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "poisson_block.npz"))
+    c, div, p_ref, dx = fx["coords"], fx["div"], fx["p"], float(fx["dx"])
+    t = _topo(c, dx)
+    dcs = torch.empty(len(c), device="cuda")
+    pcs = torch.empty(len(c), device="cuda")
+    K.to_colour_split(t, _dev(torch, div), dcs)
+    st = K.pressure_solve(t, dcs, pcs, dx, solver="cg", tol=1e-5)
+    got = torch.empty_like(pcs)
+    K.from_colour_split(t, pcs, got)
+    p = got.cpu().numpy()
+    o = Oracle(c, np.float64)
+    r2, d2 = o.residual(p, div, dx)
+    print("  block: iterations %d, gpu rel %.3e, oracle-checked rel %.3e, |p - p_scipy| %.3e" % (
+        st.iterations, st.rel_residual, np.sqrt(r2 / d2), np.abs(p - p_ref).max()))
+    assert st.rel_residual <= 1e-5 and np.sqrt(r2 / d2) <= 1.01e-5
+    assert np.abs(p - p_ref).max() <= 1e-4 * np.abs(p_ref).max()
+    s = scenes.smoke64()
+    o32 = Oracle(s.coords)
+    d0 = o32.divergence(o32.advect_vector(s.velocity, s.dt, 1 / s.voxel_size), 1 / s.voxel_size)
+    t = _topo(s.coords, s.voxel_size)
+    dcs = torch.empty(s.n, device="cuda")
+    pcs = torch.empty(s.n, device="cuda")
+    K.to_colour_split(t, _dev(torch, d0), dcs)
+    st = K.pressure_solve(t, dcs, pcs, s.voxel_size, solver="cg", tol=1e-5)
+    _, it64, _ = Oracle(s.coords, np.float64).cg(d0, s.voxel_size, tol=1e-5)
+    got = torch.empty_like(pcs)
+    K.from_colour_split(t, pcs, got)
+    r2, d2 = Oracle(s.coords, np.float64).residual(got.cpu().numpy(), d0, s.voxel_size)
+    print("  smoke64: gpu iterations %d (FP64 oracle %d), rel %.3e (oracle-checked %.3e)" % (
+        st.iterations, it64, st.rel_residual, np.sqrt(r2 / d2)))
+    assert np.sqrt(r2 / d2) <= 1.01e-5
+    assert abs(st.iterations - it64) <= max(3, 0.15 * it64)
+
+
+def test_rbgs_to_tolerance_device_api(hip):
+    torch = hip
+    _case("rbgs_tol poisson_block")
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "poisson_block.npz"))
+    c, div, p_ref, dx = fx["coords"], fx["div"], fx["p"], float(fx["dx"])
+    t = _topo(c, dx)
+    dcs = torch.empty(len(c), device="cuda")
+    pcs = torch.empty(len(c), device="cuda")
+    K.to_colour_split(t, _dev(torch, div), dcs)
+    omega = float(np.float32(2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(dx)))))
+    st = K.pressure_solve(t, dcs, pcs, dx, omega=omega, solver="rbgs_tol", tol=1e-5, check_every=10)
+    got = torch.empty_like(pcs)
+    K.from_colour_split(t, pcs, got)
+    r2, d2 = Oracle(c, np.float64).residual(got.cpu().numpy(), div, dx)
+    print("  iterations %d rel %.3e (oracle-checked %.3e)" % (st.iterations, st.rel_residual, np.sqrt(r2 / d2)))
+    assert st.rel_residual <= 1e-5 and np.sqrt(r2 / d2) <= 1.01e-5
+
+
+@pytest.mark.parametrize("name,iters", [("ragged", 3), ("ragged", 4), ("smoke64", 7)])
+def test_blocked_iteration_matches_colour_sweeps_bitwise(hip, name, iters):
+    """The LDS temporally blocked full iteration (used by the solver) and the per-colour sweep
+    kernel compute the same bits from p = 0."""
+    torch = hip
+    _case("rbgs blocked vs sweeps %s K=%d" % (name, iters))
+    c, u, _, dx, dt = _scene(name)
+    t = _topo(c, dx)
+    rng = np.random.default_rng(13)
+    div = rng.standard_normal(len(c)).astype(np.float32)
+    omega = float(np.float32(2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(dx)))))
+    dcs = torch.empty(len(c), device="cuda")
+    K.to_colour_split(t, _dev(torch, div), dcs)
+    p_blocked = torch.full((len(c),), float("nan"), device="cuda")
+    K.pressure_solve(t, dcs, p_blocked, dx, omega=omega, iterations=iters, solver="rbgs")
+    p_sweep = torch.zeros(len(c), device="cuda")
+    for _ in range(iters):
+        K.rbgs_sweep(t, dcs, p_sweep, dx, 0, omega)
+        K.rbgs_sweep(t, dcs, p_sweep, dx, 1, omega)
+    assert torch.equal(p_blocked, p_sweep)

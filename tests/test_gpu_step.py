@@ -62,6 +62,32 @@ def _run_oracle(coords, dx, velocity, fields, params, K, dt, calls, sources=None
     return vel, f
 
 
+def _per_step_parity(tag, coords, dx, velocity, fields, params, K, dt, calls, sources=None, sdf=None, src_vel=None):
+    """Each GPU call (HNS_Advance) against the FP32 oracle advanced from the same state (the GPU's
+    state after the previous call); the FP64 oracle from that state is reported beside it."""
+    data = _payload(coords, velocity, fields, sources, sdf, src_vel)
+    topo = hns.create_index_grid(data, dx)
+    ctx = hns.ensure_context(None, data, topo, dx, has_collision=sdf is not None)
+    hns.upload_sources(ctx, data)
+    p = hns.make_params(**params)
+    o32, o64 = Oracle(coords), Oracle(coords, np.float64)
+    vel, f = velocity, dict(fields)
+    worst = 0.0
+    for c in range(calls):
+        hns.advance(ctx, K, dt, p)
+        out = _out_payload(coords, list(dict.fromkeys(list(fields) + ["fuel", "waste", "temperature", "flame"])))
+        hns.download_for_output(ctx, out)
+        gv = out["velocity"].copy()
+        gf = {k: out[k].copy() for k in out.blocks if k != "velocity" and ctx.has_field(k)}
+        ov, of = o32.advance(dx, K, dt, params, vel, f, sources, src_vel, sdf)
+        ref64 = o64.advance(dx, K, dt, params, vel, f, sources, src_vel, sdf)
+        assert set(gf) == set(of)
+        worst = max(worst, _compare("%s#%d" % (tag, c), gv, gf, ov, of, ref64))
+        vel, f = gv, gf
+    hns.destroy_context(ctx)
+    return worst
+
+
 def _compare(tag, gv, gf, ov, of, o64=None):
     worst = 0.0
     for k in ["velocity"] + sorted(of):
@@ -77,26 +103,32 @@ def _compare(tag, gv, gf, ov, of, o64=None):
     return worst
 
 
-@pytest.mark.parametrize("calls", [1, 3])
-def test_smoke64_step(hip, calls):
-    _case("smoke64 calls=%d" % calls)
+def test_smoke64_steps(hip):
+    """Three calls: the first advects only density (the fields needOrMake creates are not in the
+    advNames snapshot, HNanoSolver.cu:248-250), the later ones all five scalars."""
+    _case("smoke64 per-step x3")
     s = scenes.smoke64()
-    args = (s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt, calls)
+    assert _per_step_parity("smoke64", s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt,
+                            3) <= TOL_REL
+
+
+def test_smoke64_multi_step_drift(hip):
+    """Three calls chained on each side (no re-seeding): the GPU stays on the FP32 oracle."""
+    _case("smoke64 chained x3")
+    s = scenes.smoke64()
+    args = (s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt, 3)
     gv, gf = _run_gpu(*args)
     ov, of = _run_oracle(*args)
     o64 = _run_oracle(*args, dtype=np.float64)
     assert set(gf) == set(of)
-    assert _compare("smoke64", gv, gf, ov, of, o64) <= TOL_REL
+    assert _compare("smoke64x3", gv, gf, ov, of, o64) <= TOL_REL
 
 
-def test_fire_step(hip):
-    _case("fire precond128 (fire parameters, K=100)")
+def test_fire_steps(hip):
+    _case("fire precond128 (fire parameters, K=100) per-step x2")
     s = scenes.precond128(4)
-    args = (s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt, 2)
-    gv, gf = _run_gpu(*args)
-    ov, of = _run_oracle(*args)
-    o64 = _run_oracle(*args, dtype=np.float64)
-    assert _compare("fire", gv, gf, ov, of, o64) <= TOL_REL
+    assert _per_step_parity("fire", s.coords, s.voxel_size, s.velocity, s.fields, s.params, s.iterations, s.dt,
+                            2) <= TOL_REL
 
 
 def test_all_features_step(hip):
@@ -112,12 +144,10 @@ def test_all_features_step(hip):
     sdf = plane_sdf(c, offset=-0.4, dx=dx)
     params = scenes.default_params(vorticityScale=0.3, factorScale=1.0, disturbanceEnable=1.0, substeps=2, gravity=2.0,
                                    ambientTemp=0.5)
-    args = (c, dx, u, f, params, 12, dt, 2)
-    kw = dict(sources=sources, sdf=sdf, src_vel=src_vel)
-    gv, gf = _run_gpu(*args, **kw)
-    ov, of = _run_oracle(*args, **kw)
-    o64 = _run_oracle(*args, **kw, dtype=np.float64)
-    assert _compare("all", gv, gf, ov, of, o64) <= TOL_REL
+    # collision branches (a back-trace landing inside the SDF) make this scene discontinuous in its
+    # inputs: FP64 and FP32 runs of the same algorithm part ways, so parity is per step
+    assert _per_step_parity("all", c, dx, u, f, params, 12, dt, 2, sources=sources, sdf=sdf,
+                            src_vel=src_vel) <= TOL_REL
 
 
 def test_standalone_entry_points(hip):

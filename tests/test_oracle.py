@@ -225,6 +225,24 @@ def test_rbgs_converges_to_direct_solve_and_energy_decreases():
     assert np.abs(q - p_star).max() <= 1e-5 * np.abs(p_star).max()
 
 
+def test_cg_matches_scipy_fixture():
+    """The oracle's CG (FP64) against the committed scipy direct solve and scipy CG iteration count
+    (tests/golden/make_golden.py); the FP32 CG reaches the same solution to FP32 accuracy."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "poisson_block.npz"))
+    c, div, p_ref, dx = fx["coords"], fx["div"], fx["p"], float(fx["dx"])
+    o64 = oracle.Oracle(c, np.float64)
+    p, it, rel = o64.cg(div, dx, tol=1e-8)
+    assert rel <= 1e-8
+    assert abs(it - int(fx["scipy_cg_iters_1e8"])) <= 3
+    np.testing.assert_allclose(p, p_ref, rtol=0, atol=1e-7 * np.abs(p_ref).max())
+    r2, d2 = o64.residual(p, div, dx)
+    assert np.sqrt(r2 / d2) <= 2e-8
+    p32, it32, rel32 = oracle.Oracle(c).cg(div, dx, tol=1e-5)
+    assert rel32 <= 1e-5
+    assert np.abs(p32 - p_ref).max() <= 1e-4 * np.abs(p_ref).max()
+
+
 def test_rbgs_sweep_touches_only_its_colour():
     c, u, f, dx, dt = ragged_scene(4)
     o = oracle.Oracle(c)
