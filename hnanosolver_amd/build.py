@@ -17,7 +17,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libhns.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-SOURCES = ["topology.hip", "kernels_step.hip", "kernels_pressure.hip", "kernels_cg.hip", "solvers.hip", "api.hip"]
+SOURCES = ["topology.hip", "kernels_step.hip", "kernels_advect.hip", "kernels_pressure.hip", "kernels_cg.hip", "solvers.hip",
+           "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HNS_OFFLOAD_ARCH", "gfx950")

@@ -9,7 +9,6 @@ enum CgScalarSlot { kBB = 0, kSigma = 1, kScalarSlots = 4 };
 // Device scalars of one solve. rho[k] = r_k . r_k; flag[0] = stop, flag[1] = iterations done.
 struct CgScalars {
   double* partial;   // one FP64 per leaf
-  unsigned* counter; // last-workgroup counter (0 between launches)
   double* rho;       // max_iters + 1 entries
   double* scal;      // kScalarSlots entries
   int* flag;         // 2 entries

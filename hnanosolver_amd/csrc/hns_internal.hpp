@@ -122,6 +122,16 @@ void launch_enforce_collision(const DevTopo& T, float* u, const float* sdf, floa
 void launch_to_colour_split(const DevTopo& T, const float* in, float* out, bool inverse, hipStream_t s);
 void launch_aos_soa(size_t n, const float* in, float* out, bool to_soa, hipStream_t s);
 
+// ---- LDS-staged variants (kernels_advect.hip); HNS_ADVECT_KERNEL=gather disables them ----
+bool lds_kernels_enabled();
+void launch_advect_velocity_lds(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, const float* sdf,
+                                hipStream_t s);
+void launch_advect_scalars_lds(const DevTopo& T, const float* u, const FieldSet& f, float dt, float inv_dx,
+                               const float* sdf, hipStream_t s);
+// F must be +-1 (every stencil inside the 2-voxel halo)
+void launch_vorticity_forces_lds(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, float scale,
+                                 int32_t F, const float* src_vel, float gravity, hipStream_t s);
+
 // ---- pressure kernels (kernels_pressure.hip) -------------------------------------------
 // One colour sweep; first != 0 treats p as zero before the sweep (iteration 0 from p = 0).
 void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,

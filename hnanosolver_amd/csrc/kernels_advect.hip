@@ -1,0 +1,332 @@
+// LDS-staged semi-Lagrangian kernels for gfx950: advect_vector (Kernel.cu:349-426),
+// advect_scalars (Kernel.cu:163-289) and vorticityConfinement + forces (Kernel.cu:682-727,
+// :739-755, :834-838), with the same arithmetic as the gather kernels of kernels_step.hip.
+//
+// Each 512-lane workgroup stages its 8^3 leaf plus a 2-voxel halo (a 12^3 tile built from the
+// leaf's 27-entry neighbour row) in LDS. Every trilinear corner, clamp neighbour and curl stencil
+// inside the tile is then an LDS read instead of a global gather; a lane whose trace leaves the
+// tile (CFL above ~1 in the fire scenes) takes the global path for that voxel. Tile cells of
+// absent leaves hold the reference's miss value: 0 for advect_vector and the curl stencils
+// (Stencils.hpp:83), element 0 of the field for advect_scalars (Kernel.cu:187-188, :229-230).
+#include <cstdlib>
+#include <cstring>
+
+#include "hns_internal.hpp"
+
+namespace hns {
+namespace {
+
+constexpr int kH = 2;            // halo
+constexpr int kE = 8 + 2 * kH;   // tile edge
+constexpr int kE3 = kE * kE * kE;
+
+__device__ __forceinline__ int tix(int x, int y, int z) { return (x * kE + y) * kE + z; }
+
+// Fills NC planes of the tile from a leaf-dense field with plane stride `stride`; absent cells get
+// miss[c]. Ends with a barrier.
+template <int NC>
+__device__ inline void fill_tile(float* __restrict__ s, const float* __restrict__ f, int32_t stride,
+                                 const int32_t* s_nb, int l, const float* miss) {
+  for (int c = threadIdx.x; c < kE3; c += 512) {
+    const int tx = c / (kE * kE), ty = (c / kE) % kE, tz = c % kE;
+    const int lx = tx - kH, ly = ty - kH, lz = tz - kH;
+    const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
+    if (nl >= 0) {
+      const int32_t g = nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+#pragma unroll
+      for (int k = 0; k < NC; ++k) s[k * kE3 + c] = f[k * stride + g];
+    } else {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) s[k * kE3 + c] = miss[k];
+    }
+  }
+  (void)l;
+  __syncthreads();
+}
+
+__device__ __forceinline__ float3 f3(float a, float b, float c) { return make_float3(a, b, c); }
+__device__ __forceinline__ float3 tvec(const float* s, int c) { return f3(s[c], s[kE3 + c], s[2 * kE3 + c]); }
+
+// tile corner base of a sample position (tile coordinates), or -1 when the 2x2x2 stencil is not
+// inside the tile
+__device__ __forceinline__ bool tile_base(const LeafView& V, int32_t i, int32_t j, int32_t k, int& bi, int& bj,
+                                          int& bk) {
+  bi = i - (V.ox - kH);
+  bj = j - (V.oy - kH);
+  bk = k - (V.oz - kH);
+  return (unsigned)bi <= (unsigned)(kE - 2) && (unsigned)bj <= (unsigned)(kE - 2) && (unsigned)bk <= (unsigned)(kE - 2);
+}
+
+// TrilinearSampler on Vec3f from the tile (same operations as trilinear_vec).
+__device__ inline float3 tile_trilinear_vec(const float* s, int bi, int bj, int bk, float tx, float ty, float tz) {
+  float3 c[2][2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) c[a][b][d] = tvec(s, tix(bi + a, bj + b, bk + d));
+  auto lerp = [](float3 va, float3 vb, float w) {
+    return f3(fmaf(w, vb.x - va.x, va.x), fmaf(w, vb.y - va.y, va.y), fmaf(w, vb.z - va.z, va.z));
+  };
+  const float3 z0 = lerp(c[0][0][0], c[0][0][1], tz), z1 = lerp(c[0][1][0], c[0][1][1], tz);
+  const float3 z2 = lerp(c[1][0][0], c[1][0][1], tz), z3 = lerp(c[1][1][0], c[1][1][1], tz);
+  const float3 y0 = lerp(z0, z1, ty), y1 = lerp(z2, z3, ty);
+  return lerp(y0, y1, tx);
+}
+
+__device__ inline float3 sample_vec(const LeafView& V, const Vel& U, const float* s, float px, float py, float pz) {
+  const int32_t i = floor_i(px), j = floor_i(py), k = floor_i(pz);
+  int bi, bj, bk;
+  if (tile_base(V, i, j, k, bi, bj, bk))
+    return tile_trilinear_vec(s, bi, bj, bk, px - (float)i, py - (float)j, pz - (float)k);
+  return trilinear_vec(V, U, px, py, pz);
+}
+
+// same helpers as kernels_step.hip (getSDFNormal + applyNoSlipBoundary, Kernel.cu:88-127)
+__device__ inline float3 sdf_normal_g(const LeafView& V, const float* __restrict__ sdf, int32_t x, int32_t y,
+                                      int32_t z, float inv_dx) {
+  const float r = load_s(sdf, V.index(x + 1, y, z)), le = load_s(sdf, V.index(x - 1, y, z));
+  const float t = load_s(sdf, V.index(x, y + 1, z)), b = load_s(sdf, V.index(x, y - 1, z));
+  const float f = load_s(sdf, V.index(x, y, z + 1)), k = load_s(sdf, V.index(x, y, z - 1));
+  const float s = 0.5f * inv_dx;
+  const float gx = s * (r - le), gy = s * (t - b), gz = s * (f - k);
+  const float len = sqrtf(gx * gx + gy * gy + gz * gz);
+  if (!(len > 1e-6f)) return f3(0.f, 0.f, 0.f);
+  const float il = 1.0f / len;
+  return f3(il * gx, il * gy, il * gz);
+}
+__device__ inline float3 no_slip_blend_g(float3 v, float3 n, float blend) {
+  const float vdotn = v.x * n.x + v.y * n.y + v.z * n.z;
+  const float3 m = f3(v.x - vdotn * n.x, v.y - vdotn * n.y, v.z - vdotn * n.z);
+  const float a = 1.0f - blend;
+  return f3(a * v.x + blend * m.x, a * v.y + blend * m.y, a * v.z + blend * m.z);
+}
+
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(512) void k_advect_velocity_lds(DevTopo T, const float* __restrict__ u,
+                                                             float* __restrict__ out, float sdt, float inv_dx,
+                                                             const float* __restrict__ sdf) {
+  __shared__ int32_t s_nb[27];
+  __shared__ float s_u[3 * kE3];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const LeafView V = load_leaf(T, l, s_nb);
+  const int32_t N = T.L * kLeafVox;
+  const Vel U{u, u + N, u + 2 * N};
+  const float zero[3] = {0.f, 0.f, 0.f};
+  fill_tile<3>(s_u, u, N, s_nb, l, zero);
+  const int32_t n = threadIdx.x;
+  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
+  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
+  const int32_t idx = l * kLeafVox + n;
+  const int ct = tix(lx + kH, ly + kH, lz + kH);
+  const float3 vo = tvec(s_u, ct);
+  const float px = (float)x, py = (float)y, pz = (float)z;
+  float bx = px - sdt * vo.x, by = py - sdt * vo.y, bz = pz - sdt * vo.z;
+  if (sdf && trilinear_s(V, sdf, bx, by, bz) < 0.f) { bx = px; by = py; bz = pz; }
+  const float3 vf = sample_vec(V, U, s_u, bx, by, bz);
+  float fx = bx + sdt * vf.x, fy = by + sdt * vf.y, fz = bz + sdt * vf.z;
+  if (sdf && trilinear_s(V, sdf, fx, fy, fz) < 0.f) { fx = bx; fy = by; fz = bz; }
+  const float3 vb = sample_vec(V, U, s_u, fx, fy, fz);
+  float3 c = f3(vf.x + 0.5f * (vo.x - vb.x), vf.y + 0.5f * (vo.y - vb.y), vf.z + 0.5f * (vo.z - vb.z));
+  float3 mn = vo, mx = vo;
+  const int nct[6] = {ct - kE * kE, ct + kE * kE, ct - kE, ct + kE, ct - 1, ct + 1};
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const float3 v = tvec(s_u, nct[q]);
+    mn = f3(fminf(mn.x, v.x), fminf(mn.y, v.y), fminf(mn.z, v.z));
+    mx = f3(fmaxf(mx.x, v.x), fmaxf(mx.y, v.y), fmaxf(mx.z, v.z));
+  }
+  mn = f3(fminf(mn.x, vf.x), fminf(mn.y, vf.y), fminf(mn.z, vf.z));
+  mx = f3(fmaxf(mx.x, vf.x), fmaxf(mx.y, vf.y), fmaxf(mx.z, vf.z));
+  c = f3(fmaxf(mn.x, fminf(c.x, mx.x)), fmaxf(mn.y, fminf(c.y, mx.y)), fmaxf(mn.z, fminf(c.z, mx.z)));
+  if (sdf) {
+    const float sv = sdf[idx];
+    if (sv < 0.f) {
+      c = f3(0.f, 0.f, 0.f);
+    } else if (sv < 0.1f) {
+      c = no_slip_blend_g(c, sdf_normal_g(V, sdf, x, y, z, inv_dx), 1.0f - (sv / 1.5f));
+    }
+  }
+  out[idx] = c.x;
+  out[N + idx] = c.y;
+  out[2 * N + idx] = c.z;
+}
+
+// ---------------------------------------------------------------------------------------
+// advect_scalars: the velocity tile serves the back-trace; one scalar tile is refilled per field.
+struct Interp8T {
+  int32_t idx[8];  // tile index (fast path) or leaf-dense index (global path)
+  float w[8];
+};
+__device__ inline void interp_weights(float x, float y, float z, int32_t& i0, int32_t& j0, int32_t& k0, float* w) {
+  i0 = floor_i(x);
+  j0 = floor_i(y);
+  k0 = floor_i(z);
+  const float tx = x - (float)i0, ty = y - (float)j0, tz = z - (float)k0;
+  const float itx = 1.0f - tx, ity = 1.0f - ty, itz = 1.0f - tz;
+  const float w00 = itx * ity, w10 = tx * ity, w01 = itx * ty, w11 = tx * ty;
+  w[0] = w00 * itz; w[1] = w10 * itz; w[2] = w01 * itz; w[3] = w11 * itz;
+  w[4] = w00 * tz;  w[5] = w10 * tz;  w[6] = w01 * tz;  w[7] = w11 * tz;
+}
+// corner q of the explicit-weight stencil: (i0 + (q&1), j0 + ((q>>1)&1), k0 + (q>>2))
+__device__ inline bool setup_interp_tile(const LeafView& V, float x, float y, float z, Interp8T& d) {
+  int32_t i0, j0, k0;
+  interp_weights(x, y, z, i0, j0, k0, d.w);
+  int bi, bj, bk;
+  if (tile_base(V, i0, j0, k0, bi, bj, bk)) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) d.idx[q] = tix(bi + (q & 1), bj + ((q >> 1) & 1), bk + (q >> 2));
+    return true;
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) d.idx[q] = max(V.index(i0 + (q & 1), j0 + ((q >> 1) & 1), k0 + (q >> 2)), 0);
+  return false;
+}
+
+__global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const float* __restrict__ u, FieldSet F,
+                                                            float sdt, const float* __restrict__ sdf) {
+  __shared__ int32_t s_nb[27];
+  __shared__ float s_u[3 * kE3];
+  __shared__ float s_f[kE3];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const LeafView V = load_leaf(T, l, s_nb);
+  const int32_t N = T.L * kLeafVox;
+  const Vel U{u, u + N, u + 2 * N};
+  const float miss_u[3] = {u[0], u[N], u[2 * N]};  // advect_scalars reads element 0 on a miss
+  fill_tile<3>(s_u, u, N, s_nb, l, miss_u);
+  const int32_t n = threadIdx.x;
+  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
+  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
+  const int32_t idx = l * kLeafVox + n;
+  const int ct = tix(lx + kH, ly + kH, lz + kH);
+  const float px = (float)x, py = (float)y, pz = (float)z;
+  const float3 vc = tvec(s_u, ct);
+  float bx = px - sdt * vc.x, by = py - sdt * vc.y, bz = pz - sdt * vc.z;
+  if (sdf && trilinear_s(V, sdf, bx, by, bz) < 0.f) { bx = px; by = py; bz = pz; }
+  Interp8T bd;
+  const bool bt = setup_interp_tile(V, bx, by, bz, bd);
+  float vfx = 0.f, vfy = 0.f, vfz = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const float3 v = bt ? tvec(s_u, bd.idx[q]) : f3(U.x[bd.idx[q]], U.y[bd.idx[q]], U.z[bd.idx[q]]);
+    vfx = fmaf(bd.w[q], v.x, vfx);
+    vfy = fmaf(bd.w[q], v.y, vfy);
+    vfz = fmaf(bd.w[q], v.z, vfz);
+  }
+  float fx = bx + sdt * vfx, fy = by + sdt * vfy, fz = bz + sdt * vfz;
+  if (sdf && trilinear_s(V, sdf, fx, fy, fz) < 0.f) { fx = bx; fy = by; fz = bz; }
+  Interp8T fd;
+  const bool ft = setup_interp_tile(V, fx, fy, fz, fd);
+  const int nct[6] = {ct - kE * kE, ct + kE * kE, ct - kE, ct + kE, ct - 1, ct + 1};
+  for (int s = 0; s < F.m; ++s) {
+    const float* __restrict__ d = F.in[s];
+    const float miss_f[1] = {d[0]};
+    __syncthreads();  // previous field's tile fully consumed
+    fill_tile<1>(s_f, d, N, s_nb, l, miss_f);
+    const float phiOrig = s_f[ct];
+    float phiF = 0.f, phiB = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      phiF = fmaf(bt ? s_f[bd.idx[q]] : d[bd.idx[q]], bd.w[q], phiF);
+      phiB = fmaf(ft ? s_f[fd.idx[q]] : d[fd.idx[q]], fd.w[q], phiB);
+    }
+    const float corr = fmaf(0.5f, phiOrig - phiB, phiF);
+    float mn = phiOrig, mx = phiOrig;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      const float v = s_f[nct[q]];
+      mn = fminf(mn, v);
+      mx = fmaxf(mx, v);
+    }
+    mn = fminf(mn, phiF);
+    mx = fmaxf(mx, phiF);
+    F.out[s][idx] = fmaxf(mn, fminf(corr, mx));
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// vorticity confinement + sources + gravity with |factorScale| truncating to 1 (the fire
+// scenes): every stencil (curl at +-1, |curl| at +-1 +-1) lies inside the 2-voxel halo.
+__device__ inline float3 tile_curl(const float* s, int c, float factor) {
+  const float3 pX = tvec(s, c + kE * kE), mX = tvec(s, c - kE * kE);
+  const float3 pY = tvec(s, c + kE), mY = tvec(s, c - kE);
+  const float3 pZ = tvec(s, c + 1), mZ = tvec(s, c - 1);
+  return f3(((pY.z - mY.z) - (pZ.y - mZ.y)) * factor, ((pZ.x - mZ.x) - (pX.z - mX.z)) * factor,
+            ((pX.y - mX.y) - (pY.x - mY.x)) * factor);
+}
+__device__ inline float tile_curl_mag(const float* s, int c, float factor) {
+  const float3 w = tile_curl(s, c, factor);
+  // correctly rounded, as the reference's __fsqrt_rn (Utils.cuh:242); on gfx950 HIP's __fsqrt_rn
+  // is a bare v_sqrt_f32 while sqrtf carries the rounding correction
+  return sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
+}
+
+__global__ __launch_bounds__(512) void k_vorticity_forces_lds(DevTopo T, const float* __restrict__ u,
+                                                              float* __restrict__ out, float dt, float inv_dx,
+                                                              float scale, int32_t F, const float* __restrict__ src,
+                                                              float gravity) {
+  __shared__ int32_t s_nb[27];
+  __shared__ float s_u[3 * kE3];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const LeafView V = load_leaf(T, l, s_nb);
+  const int32_t N = T.L * kLeafVox;
+  const float zero[3] = {0.f, 0.f, 0.f};
+  fill_tile<3>(s_u, u, N, s_nb, l, zero);
+  const int32_t n = threadIdx.x;
+  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
+  const int32_t idx = l * kLeafVox + n;
+  const int ct = tix(lx + kH, ly + kH, lz + kH);
+  const float factor = 0.5f * inv_dx;
+  float3 v = tvec(s_u, ct);
+  const float3 w = tile_curl(s_u, ct, factor);
+  const int sx = F * kE * kE, sy = F * kE, sz = F;  // F is +-1 here
+  const float mpX = tile_curl_mag(s_u, ct + sx, factor), mmX = tile_curl_mag(s_u, ct - sx, factor);
+  const float mpY = tile_curl_mag(s_u, ct + sy, factor), mmY = tile_curl_mag(s_u, ct - sy, factor);
+  const float mpZ = tile_curl_mag(s_u, ct + sz, factor), mmZ = tile_curl_mag(s_u, ct - sz, factor);
+  const float gx = (mpX - mmX) * 0.5f * inv_dx;
+  const float gy = (mpY - mmY) * 0.5f * inv_dx;
+  const float gz = (mpZ - mmZ) * 0.5f * inv_dx;
+  const float gl = sqrtf(gx * gx + gy * gy + gz * gz) + 1e-5f;
+  const float Nx = gx / gl, Ny = gy / gl, Nz = gz / gl;
+  const float fx = scale * (Ny * w.z - Nz * w.y);
+  const float fy = scale * (Nz * w.x - Nx * w.z);
+  const float fz = scale * (Nx * w.y - Ny * w.x);
+  v = f3(v.x + dt * fx, v.y + dt * fy, v.z + dt * fz);
+  if (src) v = f3(v.x + dt * src[idx], v.y + dt * src[N + idx], v.z + dt * src[2 * N + idx]);
+  if (gravity != 0.0f) v.y -= gravity * dt;
+  out[idx] = v.x;
+  out[N + idx] = v.y;
+  out[2 * N + idx] = v.z;
+}
+
+}  // namespace
+
+bool lds_kernels_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("HNS_ADVECT_KERNEL");
+    return !(e && std::strcmp(e, "gather") == 0);
+  }();
+  return on;
+}
+
+void launch_advect_velocity_lds(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, const float* sdf,
+                                hipStream_t s) {
+  k_advect_velocity_lds<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, out, dt * inv_dx, inv_dx, sdf);
+  HNS_HIP(hipGetLastError());
+}
+void launch_advect_scalars_lds(const DevTopo& T, const float* u, const FieldSet& f, float dt, float inv_dx,
+                               const float* sdf, hipStream_t s) {
+  k_advect_scalars_lds<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, f, dt * inv_dx, sdf);
+  HNS_HIP(hipGetLastError());
+}
+void launch_vorticity_forces_lds(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, float scale,
+                                 int32_t F, const float* src_vel, float gravity, hipStream_t s) {
+  k_vorticity_forces_lds<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, out, dt, inv_dx, scale, F, src_vel, gravity);
+  HNS_HIP(hipGetLastError());
+}
+
+}  // namespace hns

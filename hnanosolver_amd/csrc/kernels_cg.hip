@@ -4,7 +4,7 @@
 // Operator: A = -L scaled by dx^2, (A p)_i = 6 p_i - sum of the 6 neighbours, missing neighbours
 // are 0 (the same Dirichlet boundary as RBGS, Kernel.cu:477-482); right-hand side b = -div dx^2.
 // Vectors are colour-split FP32 (the RBGS layout); dot products are FP64, reduced per leaf and
-// then by the last workgroup to finish, in leaf order (deterministic). alpha and beta are
+// then by a one-workgroup reduce kernel in a fixed order (deterministic). alpha and beta are
 // formed on the device from the reduced scalars, and a device flag stops the remaining
 // iterations of a batch once ||r||^2 <= tol^2 ||b||^2, so the host only synchronises per batch.
 //
@@ -33,37 +33,51 @@ __device__ inline double block_sum(double v, double* s_red) {
   return t;  // valid in lane 0
 }
 
-// Writes this leaf's partial; the last of the L leaf workgroups sums all partials in leaf order
-// into *out. Returns true in the workgroup that did the final sum. Partials cross XCDs (each has
-// its own L2), so they are stored and loaded as agent-scope atomics around agent-scope fences.
-__device__ inline bool last_block_reduce(double part, int32_t l, int32_t L, double* __restrict__ partial,
-                                         unsigned* __restrict__ counter, double* __restrict__ out,
-                                         double* s_red, int* s_last) {
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partial[l], part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    *s_last = (prev == (unsigned)(L - 1));
-  }
+// One workgroup sums the per-leaf partials in a fixed order (thread t takes t, t+1024, ...,
+// then a wave/LDS tree), so every reduction is deterministic. A single-address atomic counter
+// across all leaf workgroups would serialise 16k device-scope atomics per reduction.
+constexpr int kRed = 1024;
+__device__ inline double reduce_partials(const double* __restrict__ partial, int32_t L) {
+  __shared__ double s_red[kRed / 64];
+  double v = 0.0;
+  for (int32_t i = threadIdx.x; i < L; i += kRed) v += partial[i];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (!*s_last) return false;
-  __threadfence();
-  double acc = 0.0;
-  for (int32_t i = threadIdx.x; i < L; i += kTpb)
-    acc += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const double tot = block_sum(acc, s_red);
-  if (threadIdx.x == 0) {
-    *out = tot;
-    *counter = 0u;
+  double t = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kRed / 64; ++i) t += s_red[i];
+  return t;  // valid in thread 0
+}
+
+// mode 0: rho[0] = ||b||^2 = scal[kBB], flags reset (stop at once when b == 0)
+// mode 1: scal[kSigma] = d.q (stop on breakdown)
+// mode 2: rho[it+1] = r.r, iterations = it+1, stop when rho[it+1] <= tol2 ||b||^2
+__global__ __launch_bounds__(kRed) void k_cg_reduce(const double* __restrict__ partial, int32_t L, int mode, int32_t it,
+                                                    CgScalars S) {
+  if (mode != 0 && S.flag[0]) return;
+  const double t = reduce_partials(partial, L);
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    S.rho[0] = t;
+    S.scal[kBB] = t;
+    S.flag[0] = t == 0.0 ? 1 : 0;
+    S.flag[1] = 0;
+  } else if (mode == 1) {
+    S.scal[kSigma] = t;
+    if (!(t > 0.0)) S.flag[0] = 1;  // breakdown (A is SPD; only round-off gets here)
+  } else {
+    S.rho[it + 1] = t;
+    S.flag[1] = it + 1;
+    if (t <= S.tol2 * S.scal[kBB]) S.flag[0] = 1;
   }
-  return true;
 }
 
 // r = b = -div dx^2, x = 0; rho[0] = ||b||^2 = scal[kBB]
 __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __restrict__ div, float dx2,
                                                   float* __restrict__ x, float* __restrict__ r, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
-  __shared__ int s_last;
   const int32_t l = leaf_of_block();
   if (l >= L) return;
   const int32_t n = threadIdx.x;
@@ -74,11 +88,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __rest
   x[i] = 0.f;
   r[i] = b;
   const double part = block_sum((double)b * (double)b, s_red);
-  if (last_block_reduce(part, l, L, S.partial, S.counter, &S.rho[0], s_red, &s_last) && threadIdx.x == 0) {
-    S.scal[kBB] = S.rho[0];
-    S.flag[0] = S.rho[0] == 0.0 ? 1 : 0;  // b == 0 is solved by x = 0
-    S.flag[1] = 0;
-  }
+  if (threadIdx.x == 0) S.partial[l] = part;
 }
 
 // Iteration `it` (0-based): d_out = r + beta d_in with beta = rho[it]/rho[it-1] (d_out = r at it 0),
@@ -89,7 +99,6 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, const f
                                                   float* __restrict__ q, CgScalars S) {
   __shared__ int32_t s_nb[27];
   __shared__ double s_red[kTpb / 64];
-  __shared__ int s_last;
   if (S.flag[0]) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
@@ -118,9 +127,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, const f
   d_out[i] = dn;
   q[i] = qv;
   const double part = block_sum((double)dn * (double)qv, s_red);
-  if (last_block_reduce(part, l, T.L, S.partial, S.counter, &S.scal[kSigma], s_red, &s_last) && threadIdx.x == 0) {
-    if (!(S.scal[kSigma] > 0.0)) S.flag[0] = 1;  // breakdown (A is SPD; only round-off gets here)
-  }
+  if (threadIdx.x == 0) S.partial[l] = part;
 }
 
 // alpha = rho[it]/sigma; x += alpha d; r -= alpha q; rho[it+1] = r.r; convergence flag
@@ -128,7 +135,6 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
                                                     float* __restrict__ r, const float* __restrict__ d,
                                                     const float* __restrict__ q, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
-  __shared__ int s_last;
   if (S.flag[0]) return;
   const int32_t l = leaf_of_block();
   if (l >= L) return;
@@ -142,10 +148,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
   const float rv = fmaf(-alpha, q[i], r[i]);
   r[i] = rv;
   const double part = block_sum((double)rv * (double)rv, s_red);
-  if (last_block_reduce(part, l, L, S.partial, S.counter, &S.rho[it + 1], s_red, &s_last) && threadIdx.x == 0) {
-    S.flag[1] = it + 1;
-    if (S.rho[it + 1] <= S.tol2 * S.scal[kBB]) S.flag[0] = 1;
-  }
+  if (threadIdx.x == 0) S.partial[l] = part;
 }
 
 }  // namespace
@@ -153,12 +156,18 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
 void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, const CgScalars& S, hipStream_t s) {
   k_cg_init<<<grid_for_leaves(L), kTpb, 0, s>>>(L, div, dx * dx, x, r, S);
   HNS_HIP(hipGetLastError());
+  k_cg_reduce<<<1, kRed, 0, s>>>(S.partial, L, 0, 0, S);
+  HNS_HIP(hipGetLastError());
 }
 void launch_cg_iteration(const DevTopo& T, int32_t it, float* x, float* r, const float* d_in, float* d_out, float* q,
                          const CgScalars& S, hipStream_t s) {
   k_cg_spmv<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, it, d_in, r, d_out, q, S);
   HNS_HIP(hipGetLastError());
+  k_cg_reduce<<<1, kRed, 0, s>>>(S.partial, T.L, 1, it, S);
+  HNS_HIP(hipGetLastError());
   k_cg_update<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, it, x, r, d_out, q, S);
+  HNS_HIP(hipGetLastError());
+  k_cg_reduce<<<1, kRed, 0, s>>>(S.partial, T.L, 2, it, S);
   HNS_HIP(hipGetLastError());
 }
 

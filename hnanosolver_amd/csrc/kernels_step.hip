@@ -211,7 +211,9 @@ __device__ inline float3 curl_at(const LeafView& V, const Vel& U, int32_t x, int
 }
 __device__ inline float curl_mag(const LeafView& V, const Vel& U, int32_t x, int32_t y, int32_t z, float factor) {
   const float3 w = curl_at(V, U, x, y, z, factor);
-  return __fsqrt_rn(w.x * w.x + w.y * w.y + w.z * w.z);
+  // correctly rounded, as the reference's __fsqrt_rn (Utils.cuh:242); on gfx950 HIP's __fsqrt_rn
+  // is a bare v_sqrt_f32 while sqrtf carries the rounding correction
+  return sqrtf(w.x * w.x + w.y * w.y + w.z * w.z);
 }
 
 template <bool VORT>
@@ -449,11 +451,13 @@ __global__ void k_aos_soa(int64_t n, const float* __restrict__ in, float* __rest
 // ---- launchers ---------------------------------------------------------------------------
 void launch_advect_velocity(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, const float* sdf,
                             hipStream_t s) {
+  if (lds_kernels_enabled()) return launch_advect_velocity_lds(T, u, out, dt, inv_dx, sdf, s);
   k_advect_velocity<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, out, dt * inv_dx, inv_dx, sdf);
   HNS_HIP(hipGetLastError());
 }
 void launch_advect_scalars(const DevTopo& T, const float* u, const FieldSet& f, float dt, float inv_dx,
                            const float* sdf, hipStream_t s) {
+  if (lds_kernels_enabled()) return launch_advect_scalars_lds(T, u, f, dt, inv_dx, sdf, s);
   k_advect_scalars<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, f, dt * inv_dx, sdf);
   HNS_HIP(hipGetLastError());
 }
@@ -464,6 +468,8 @@ void launch_advect_scalar(const DevTopo& T, const float* u, const float* in, flo
 }
 void launch_vorticity_forces(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, float scale,
                              int32_t F, bool vort, const float* src_vel, float gravity, hipStream_t s) {
+  if (vort && (F == 1 || F == -1) && lds_kernels_enabled())
+    return launch_vorticity_forces_lds(T, u, out, dt, inv_dx, scale, F, src_vel, gravity, s);
   if (vort)
     k_vorticity_forces<true><<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, out, dt, inv_dx, scale, F, src_vel, gravity);
   else

@@ -15,7 +15,6 @@ struct CgState {
   DBuf<float> r, d0, d1, q;
   DBuf<double> rho, scal;
   DBuf<int> flag;
-  DBuf<unsigned> counter;
   int* h_flag = nullptr;
   int cap = 0;
   ~CgState() {
@@ -35,10 +34,11 @@ Solvers::~Solvers() {
 void Solvers::init(int32_t L, size_t N) {
   L_ = L;
   N_ = N;
-  // HNS_RBGS_KERNEL=sweep selects one launch per colour (the reference's structure) instead of the
-  // temporally blocked full iteration; both give identical results.
+  // One launch per colour (the reference's structure, measured faster on MI355X) unless
+  // HNS_RBGS_KERNEL=blocked selects the LDS temporally blocked full iteration; both give
+  // identical results.
   const char* k = std::getenv("HNS_RBGS_KERNEL");
-  per_colour_ = k && std::strcmp(k, "sweep") == 0;
+  per_colour_ = !(k && std::strcmp(k, "blocked") == 0);
   pb_.alloc(N);
   partial_.alloc(2 * (size_t)L);
   red_.alloc(2);
@@ -146,8 +146,6 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.q.alloc(N_);
     G.scal.alloc(kScalarSlots);
     G.flag.alloc(2);
-    G.counter.alloc(1);
-    HNS_HIP(hipMemsetAsync(G.counter.p, 0, sizeof(unsigned), s));
     HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 2 * sizeof(int), hipHostMallocDefault));
   }
   if (G.cap < cap) {
@@ -156,7 +154,6 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   }
   CgScalars S;
   S.partial = partial_.p;
-  S.counter = G.counter.p;
   S.rho = G.rho.p;
   S.scal = G.scal.p;
   S.flag = G.flag.p;

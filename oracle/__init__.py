@@ -68,6 +68,14 @@ def load():
             f("enforce_collision").argtypes = [vp, vp, vp, C.c_float]
             f("combustion").argtypes = [C.c_int64, vp, vp, vp, vp, vp, C.c_float, C.c_float]
             f("buoyancy").argtypes = [C.c_int64, vp, vp, C.c_float, C.c_float, C.c_float]
+            f("advect_scalars_adj").argtypes = [vp, vp, vp, vp, vp, vp, C.c_float, C.c_float]
+            f("advect_vector_adj").argtypes = [vp, vp, vp, vp, C.c_float, C.c_float]
+            f("divergence_adj").argtypes = [vp, vp, vp, C.c_float]
+            f("gradient_adj").argtypes = [vp, vp, vp, C.c_float]
+            f("rbgs_adj").argtypes = [vp, vp, vp, C.c_float, C.c_int, C.c_float]
+            f("diff_forward").argtypes = [vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float]
+            f("diff_loss_grad").restype = C.c_double
+            f("diff_loss_grad").argtypes = [vp, vp, vp, vp, C.c_int, C.c_float, C.c_float, C.c_int, C.c_float, vp, vp]
             f("advance").restype = C.c_int
             f("advance").argtypes = [vp, C.c_float, C.c_int, C.c_float, C.POINTER(OracleParams), vp, C.c_int, vp, vp,
                                      vp, C.c_int, vp, vp, vp, vp]
@@ -194,6 +202,56 @@ class Oracle:
         rel = C.c_double()
         it = self._f("cg")(self._t, d.ctypes.data, dx, float(tol), int(max_iters), p.ctypes.data, C.byref(rel))
         return p, int(it), rel.value
+
+    # ---- reverse mode of the differentiable step (FP64 host tape when dtype is float64) ----
+    def advect_scalars_adj(self, vel, field, gout, dt, inv_dx):
+        """Returns (d/d field, d/d vel) of sum(gout * advect_scalars(field; vel))."""
+        v, f, g = self._a(vel, (self.n, 3)), self._a(field), self._a(gout)
+        gin = np.zeros(self.n, self.dtype)
+        gvel = np.zeros((self.n, 3), self.dtype)
+        self._f("advect_scalars_adj")(self._t, v.ctypes.data, f.ctypes.data, g.ctypes.data, gin.ctypes.data,
+                                      gvel.ctypes.data, dt, inv_dx)
+        return gin, gvel
+
+    def advect_vector_adj(self, vel, gout, dt, inv_dx):
+        v, g = self._a(vel, (self.n, 3)), self._a(gout, (self.n, 3))
+        gvel = np.zeros((self.n, 3), self.dtype)
+        self._f("advect_vector_adj")(self._t, v.ctypes.data, g.ctypes.data, gvel.ctypes.data, dt, inv_dx)
+        return gvel
+
+    def divergence_adj(self, gdiv, inv_dx):
+        g = self._a(gdiv)
+        gvel = np.zeros((self.n, 3), self.dtype)
+        self._f("divergence_adj")(self._t, g.ctypes.data, gvel.ctypes.data, inv_dx)
+        return gvel
+
+    def gradient_adj(self, gu2, inv_dx):
+        """d/dp of sum(gu2 . u2) for u2 = u1 - grad p (minus the transpose of the gradient)."""
+        g = self._a(gu2, (self.n, 3))
+        gp = np.zeros(self.n, self.dtype)
+        self._f("gradient_adj")(self._t, g.ctypes.data, gp.ctypes.data, inv_dx)
+        return gp
+
+    def rbgs_adj(self, gp, dx, K, omega):
+        """d/d div of sum(gp * p_K(div)) for K RBGS iterations from p = 0."""
+        g = self._a(gp)
+        gdiv = np.zeros(self.n, self.dtype)
+        self._f("rbgs_adj")(self._t, g.ctypes.data, gdiv.ctypes.data, dx, int(K), omega)
+        return gdiv
+
+    def diff_forward(self, vel, rho, steps, dt, dx, K, omega):
+        v, r = self._a(vel, (self.n, 3)).copy(), self._a(rho).copy()
+        self._f("diff_forward")(self._t, v.ctypes.data, r.ctypes.data, int(steps), dt, dx, int(K), omega)
+        return v, r
+
+    def diff_loss_grad(self, vel, rho, target, steps, dt, dx, K, omega):
+        """L = sum (rho_S - target)^2 and (dL/dvel0, dL/drho0)."""
+        v, r, tg = self._a(vel, (self.n, 3)), self._a(rho), self._a(target)
+        gv = np.zeros((self.n, 3), self.dtype)
+        gr = np.zeros(self.n, self.dtype)
+        loss = self._f("diff_loss_grad")(self._t, v.ctypes.data, r.ctypes.data, tg.ctypes.data, int(steps), dt, dx,
+                                         int(K), omega, gv.ctypes.data, gr.ctypes.data)
+        return loss, gv, gr
 
     def subtract_gradient(self, vel, p, inv_dx, sdf=None):
         v = self._a(vel, (self.n, 3))

@@ -611,6 +611,349 @@ void buoyancy(int64_t N, V3<R>* vel, const R* temp, R dt, R ambient, R beta) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Reverse mode of the differentiable smoke step (new; the reference has no adjoint). Forward:
+//   u1 = advect_vector(u), div = divergence(u1), p = K RBGS iterations from 0 (Kernel.cu:459-489),
+//   u2 = u1 - grad p, rho' = advect_scalars(rho; u2).
+// Each adjoint below is the exact transpose of the forward operation as written (trilinear
+// weights differentiated inside the cell, the MacCormack clamp through the branch it took, the
+// truncated RBGS sweeps transposed in reverse order, misses scattered where they gathered).
+// Adjoints accumulate (+=) into their outputs. Serial loops keep the scatters deterministic.
+// ---------------------------------------------------------------------------------------
+
+// weights and their gradient for one trilinear cell: corner q = (q&1, (q>>1)&1, q>>2)
+template <typename R>
+struct TriCell {
+  int32_t i0, j0, k0;
+  R w[8];
+  R dw[8][3];
+};
+template <typename R>
+static TriCell<R> triCell(const V3<R>& p) {
+  TriCell<R> c;
+  c.i0 = floorI(p[0]);
+  c.j0 = floorI(p[1]);
+  c.k0 = floorI(p[2]);
+  const R t[3] = {p[0] - R(c.i0), p[1] - R(c.j0), p[2] - R(c.k0)};
+  for (int q = 0; q < 8; ++q) {
+    const int b[3] = {q & 1, (q >> 1) & 1, q >> 2};
+    R f[3], df[3];
+    for (int a = 0; a < 3; ++a) {
+      f[a] = b[a] ? t[a] : R(1) - t[a];
+      df[a] = b[a] ? R(1) : R(-1);
+    }
+    c.w[q] = f[0] * f[1] * f[2];
+    c.dw[q][0] = df[0] * f[1] * f[2];
+    c.dw[q][1] = f[0] * df[1] * f[2];
+    c.dw[q][2] = f[0] * f[1] * df[2];
+  }
+  return c;
+}
+
+// Adjoint of advect_scalars for one field (Kernel.cu:163-289): gin += d/d(in), gvel += d/d(vel).
+template <typename R>
+void advectScalarsAdjoint(const Topo& t, const V3<R>* vel, const R* in, const R* gout, R* gin, V3<R>* gvel, R dt,
+                          R inv_dx) {
+  const R sdt = dt * inv_dx;
+  static const int offs[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+  Acc a(&t);
+  auto idxOf = [&](int32_t x, int32_t y, int32_t z) {
+    const int64_t off = a.offset(x, y, z);
+    return off == 0 ? int64_t(0) : off - 1;  // miss -> element 0
+  };
+  for (int64_t idx = 0; idx < t.N; ++idx) {
+    const R g = gout[idx];
+    if (g == R(0)) continue;
+    const int32_t* c = &t.coords[3 * idx];
+    const int64_t orig = idxOf(c[0], c[1], c[2]);
+    const V3<R> pos = posOf<R>(t, idx);
+    const V3<R> vc = vel[orig];
+    const V3<R> bp = pos - vc * sdt;
+    const TriCell<R> B = triCell(bp);
+    int64_t bi[8];
+    V3<R> velF;
+    R phiF = R(0);
+    // recomputed exactly as advectScalars does, so the clamp takes the same branch
+    for (int q = 0; q < 8; ++q) {
+      bi[q] = idxOf(B.i0 + (q & 1), B.j0 + ((q >> 1) & 1), B.k0 + (q >> 2));
+      const V3<R>& v = vel[bi[q]];
+      velF = V3<R>(std::fma(B.w[q], v[0], velF[0]), std::fma(B.w[q], v[1], velF[1]), std::fma(B.w[q], v[2], velF[2]));
+      phiF = std::fma(in[bi[q]], B.w[q], phiF);
+    }
+    const V3<R> fp = bp + velF * sdt;
+    const TriCell<R> F = triCell(fp);
+    int64_t fi[8];
+    R phiB = R(0);
+    for (int q = 0; q < 8; ++q) {
+      fi[q] = idxOf(F.i0 + (q & 1), F.j0 + ((q >> 1) & 1), F.k0 + (q >> 2));
+      phiB = std::fma(in[fi[q]], F.w[q], phiB);
+    }
+    const R corr = std::fma(R(0.5f), in[orig] - phiB, phiF);
+    // clamp: which argument produced the output (first minimum / maximum in evaluation order)
+    int64_t nb[6];
+    for (int n = 0; n < 6; ++n) nb[n] = idxOf(c[0] + offs[n][0], c[1] + offs[n][1], c[2] + offs[n][2]);
+    R mn = in[orig], mx = in[orig];
+    int64_t amn = orig, amx = orig;  // -1 marks phiF
+    for (int n = 0; n < 6; ++n) {
+      if (in[nb[n]] < mn) { mn = in[nb[n]]; amn = nb[n]; }
+      if (in[nb[n]] > mx) { mx = in[nb[n]]; amx = nb[n]; }
+    }
+    if (phiF < mn) { mn = phiF; amn = -1; }
+    if (phiF > mx) { mx = phiF; amx = -1; }
+    R gcorr = R(0), gphiF = R(0);
+    const R c1 = std::fmin(corr, mx);
+    if (mn > c1) {
+      if (amn < 0) gphiF += g; else gin[amn] += g;
+    } else if (corr > mx) {
+      if (amx < 0) gphiF += g; else gin[amx] += g;
+    } else {
+      gcorr = g;
+    }
+    gphiF += gcorr;
+    gin[orig] += R(0.5) * gcorr;
+    const R gphiB = R(-0.5) * gcorr;
+    V3<R> gfp;
+    for (int q = 0; q < 8; ++q) {
+      gin[fi[q]] += F.w[q] * gphiB;
+      for (int k = 0; k < 3; ++k) gfp[k] += F.dw[q][k] * in[fi[q]] * gphiB;
+    }
+    V3<R> gbp = gfp;
+    const V3<R> gvelF = gfp * sdt;
+    for (int q = 0; q < 8; ++q) {
+      gvel[bi[q]] = gvel[bi[q]] + gvelF * B.w[q];
+      gin[bi[q]] += B.w[q] * gphiF;
+      const R vdot = vel[bi[q]].dot(gvelF);
+      for (int k = 0; k < 3; ++k) gbp[k] += B.dw[q][k] * (vdot + in[bi[q]] * gphiF);
+    }
+    gvel[orig] = gvel[orig] - gbp * sdt;
+  }
+}
+
+// Adjoint of advect_vector (Kernel.cu:349-426), no collision: gvel += d/d(vel) of out.
+template <typename R>
+void advectVectorAdjoint(const Topo& t, const V3<R>* vel, const V3<R>* gout, V3<R>* gvel, R dt, R inv_dx) {
+  const R sdt = dt * inv_dx;
+  Acc a(&t);
+  auto idxOf = [&](int32_t x, int32_t y, int32_t z) { return a.offset(x, y, z) - 1; };  // -1: miss (reads 0)
+  auto at = [&](int64_t i) { return i < 0 ? V3<R>() : vel[i]; };
+  for (int64_t idx = 0; idx < t.N; ++idx) {
+    const V3<R> g = gout[idx];
+    if (g[0] == R(0) && g[1] == R(0) && g[2] == R(0)) continue;
+    const int32_t* c = &t.coords[3 * idx];
+    const V3<R> pos = posOf<R>(t, idx);
+    const V3<R> vo = vel[idx];
+    const V3<R> bp = pos - vo * sdt;
+    const TriCell<R> B = triCell(bp);
+    int64_t bi[8];
+    for (int q = 0; q < 8; ++q) bi[q] = idxOf(B.i0 + (q & 1), B.j0 + ((q >> 1) & 1), B.k0 + (q >> 2));
+    const V3<R> vf = trilinearVec(a, vel, bp);  // as advectVector computes it
+    const V3<R> fp = bp + vf * sdt;
+    const TriCell<R> F = triCell(fp);
+    int64_t fi[8];
+    for (int q = 0; q < 8; ++q) fi[q] = idxOf(F.i0 + (q & 1), F.j0 + ((q >> 1) & 1), F.k0 + (q >> 2));
+    const V3<R> vb = trilinearVec(a, vel, fp);
+    const V3<R> corr = vf + R(0.5f) * (vo - vb);
+    int64_t nb[6];
+    const int offs[6][3] = {{-1, 0, 0}, {1, 0, 0}, {0, -1, 0}, {0, 1, 0}, {0, 0, -1}, {0, 0, 1}};
+    for (int n = 0; n < 6; ++n) nb[n] = idxOf(c[0] + offs[n][0], c[1] + offs[n][1], c[2] + offs[n][2]);
+    V3<R> gvo, gvf, gcorr;
+    for (int k = 0; k < 3; ++k) {
+      // argument ids: -2 = vo, -3 = vf, -4 = a missing neighbour (value 0), >= 0 neighbour index
+      R mn = vo[k], mx = vo[k];
+      int64_t amn = -2, amx = -2;
+      for (int n = 0; n < 6; ++n) {
+        const R v = nb[n] < 0 ? R(0) : vel[nb[n]][k];
+        const int64_t id = nb[n] < 0 ? -4 : nb[n];
+        if (v < mn) { mn = v; amn = id; }
+        if (v > mx) { mx = v; amx = id; }
+      }
+      if (vf[k] < mn) { mn = vf[k]; amn = -3; }
+      if (vf[k] > mx) { mx = vf[k]; amx = -3; }
+      const R c1 = std::fmin(corr[k], mx);
+      int64_t arg = -5;  // -5: the corrected value
+      if (mn > c1) arg = amn;
+      else if (corr[k] > mx) arg = amx;
+      if (arg == -5) gcorr[k] = g[k];
+      else if (arg == -2) gvo[k] += g[k];
+      else if (arg == -3) gvf[k] += g[k];
+      else if (arg >= 0) gvel[arg][k] += g[k];
+    }
+    gvf = gvf + gcorr;
+    gvo = gvo + gcorr * R(0.5);
+    const V3<R> gvb = gcorr * R(-0.5);
+    V3<R> gfp;
+    for (int q = 0; q < 8; ++q) {
+      if (fi[q] >= 0) gvel[fi[q]] = gvel[fi[q]] + gvb * F.w[q];
+      const R d = at(fi[q]).dot(gvb);
+      for (int k = 0; k < 3; ++k) gfp[k] += F.dw[q][k] * d;
+    }
+    V3<R> gbp = gfp;
+    gvf = gvf + gfp * sdt;
+    for (int q = 0; q < 8; ++q) {
+      if (bi[q] >= 0) gvel[bi[q]] = gvel[bi[q]] + gvf * B.w[q];
+      const R d = at(bi[q]).dot(gvf);
+      for (int k = 0; k < 3; ++k) gbp[k] += B.dw[q][k] * d;
+    }
+    gvo = gvo - gbp * sdt;
+    gvel[idx] = gvel[idx] + gvo;
+  }
+}
+
+// Adjoint of divergence (Kernel.cu:432-453): div = 0.5 inv_dx sum_axes (u(x+e) - u(x-e)), the centre
+// terms cancelling; gvel_a(y) += 0.5 inv_dx (gdiv(y - e_a) - gdiv(y + e_a)) over present voxels.
+template <typename R>
+void divergenceAdjoint(const Topo& t, const R* gdiv, V3<R>* gvel, R inv_dx) {
+  const R h = R(0.5) * inv_dx;
+#pragma omp parallel
+  {
+    Acc a(&t);
+#pragma omp for schedule(static)
+    for (int64_t idx = 0; idx < t.N; ++idx) {
+      const int32_t* c = &t.coords[3 * idx];
+      for (int k = 0; k < 3; ++k) {
+        int32_t m[3] = {c[0], c[1], c[2]}, p[3] = {c[0], c[1], c[2]};
+        m[k] -= 1;
+        p[k] += 1;
+        gvel[idx][k] += h * (nearest0(a, gdiv, m[0], m[1], m[2]) - nearest0(a, gdiv, p[0], p[1], p[2]));
+      }
+    }
+  }
+}
+
+// Adjoint of the gradient subtraction (Kernel.cu:495-541) w.r.t. p: u2 = u1 - 0.5 inv_dx (p(x+e) - p(x-e)),
+// so d/dp of sum(gu2 . u2) is gp(y) += 0.5 inv_dx sum_axes (gu2_a(y + e_a) - gu2_a(y - e_a)) (the minus
+// sign of the subtraction included). d/du1 is the identity.
+template <typename R>
+void gradientAdjoint(const Topo& t, const V3<R>* gu2, R* gp, R inv_dx) {
+  const R h = R(0.5) * inv_dx;
+#pragma omp parallel
+  {
+    Acc a(&t);
+#pragma omp for schedule(static)
+    for (int64_t idx = 0; idx < t.N; ++idx) {
+      const int32_t* c = &t.coords[3 * idx];
+      R s = R(0);
+      for (int k = 0; k < 3; ++k) {
+        int32_t m[3] = {c[0], c[1], c[2]}, p[3] = {c[0], c[1], c[2]};
+        m[k] -= 1;
+        p[k] += 1;
+        s += nearest0(a, gu2, p[0], p[1], p[2])[k] - nearest0(a, gu2, m[0], m[1], m[2])[k];
+      }
+      gp[idx] += h * s;
+    }
+  }
+}
+
+// Adjoint of K RBGS iterations from p = 0 (div -> p is linear): gdiv += R_K^T gp. Each colour sweep
+// p_c <- (1-w) p_c + w inv6 (sum nbr p_(1-c) - dx^2 div_c) is transposed, last sweep first:
+// the other colour gathers w inv6 lambda_c from its neighbours, then lambda_c feeds div_c with
+// -w inv6 dx^2 and keeps (1-w) of itself.
+template <typename R>
+void rbgsAdjoint(const Topo& t, const R* gp_in, R* gdiv, R dx, int K, R omega) {
+  const int64_t N = t.N;
+  const R inv6 = R(0.166666667f), dx2 = dx * dx;
+  std::vector<R> lam(gp_in, gp_in + N), add(N);
+  for (int it = K - 1; it >= 0; --it) {
+    for (int c = 1; c >= 0; --c) {
+#pragma omp parallel
+      {
+        Acc a(&t);
+#pragma omp for schedule(static)
+        for (int64_t idx = 0; idx < N; ++idx) {
+          const int32_t i = t.coords[3 * idx], j = t.coords[3 * idx + 1], k = t.coords[3 * idx + 2];
+          add[idx] = R(0);
+          if (((i + j + k) & 1) == c) continue;
+          const R s = nearest0(a, lam.data(), i + 1, j, k) + nearest0(a, lam.data(), i - 1, j, k) +
+                      nearest0(a, lam.data(), i, j + 1, k) + nearest0(a, lam.data(), i, j - 1, k) +
+                      nearest0(a, lam.data(), i, j, k + 1) + nearest0(a, lam.data(), i, j, k - 1);
+          add[idx] = omega * inv6 * s;
+        }
+      }
+#pragma omp parallel for schedule(static)
+      for (int64_t idx = 0; idx < N; ++idx) {
+        const int32_t i = t.coords[3 * idx], j = t.coords[3 * idx + 1], k = t.coords[3 * idx + 2];
+        if (((i + j + k) & 1) == c) {
+          gdiv[idx] += -omega * inv6 * dx2 * lam[idx];
+          lam[idx] *= (R(1) - omega);
+        } else {
+          lam[idx] += add[idx];
+        }
+      }
+    }
+  }
+  // the first iteration starts from p = 0: lambda left on p0 has nowhere to go
+}
+
+// Forward of the differentiable step on (vel, rho), recording the intermediates its adjoint needs.
+template <typename R>
+void diffStep(const Topo& t, std::vector<V3<R>>& vel, std::vector<R>& rho, R dt, R dx, int K, R omega,
+              std::vector<V3<R>>* u2Out) {
+  const int64_t N = t.N;
+  const R inv_dx = R(1) / dx;
+  std::vector<V3<R>> u1(N), u2(N);
+  std::vector<R> div(N), p(N, R(0)), out(N);
+  advectVector(t, vel.data(), u1.data(), dt, inv_dx, (const R*)nullptr);
+  divergence(t, u1.data(), div.data(), inv_dx);
+  for (int it = 0; it < K; ++it) {
+    rbgsColour(t, div.data(), p.data(), dx, 0, omega);
+    rbgsColour(t, div.data(), p.data(), dx, 1, omega);
+  }
+  subtractGradient(t, u1.data(), p.data(), u2.data(), inv_dx, (const R*)nullptr);
+  const R* ins[1] = {rho.data()};
+  R* outs[1] = {out.data()};
+  advectScalars(t, u2.data(), 1, ins, outs, dt, inv_dx, (const R*)nullptr);
+  if (u2Out) *u2Out = u2;
+  vel = u2;
+  rho = out;
+}
+
+// Loss L = sum (rho_S - target)^2 after S steps from (vel0, rho0) and its gradient w.r.t. vel0 and
+// rho0 (the FP64 host tape stores every step's inputs).
+template <typename R>
+R diffLossGrad(const Topo& t, const R* vel0, const R* rho0, const R* target, int steps, R dt, R dx, int K, R omega,
+               R* gvel0, R* grho0) {
+  const int64_t N = t.N;
+  const R inv_dx = R(1) / dx;
+  std::vector<std::vector<V3<R>>> tapeU(steps), tapeU2(steps);
+  std::vector<std::vector<R>> tapeRho(steps);
+  std::vector<V3<R>> vel(reinterpret_cast<const V3<R>*>(vel0), reinterpret_cast<const V3<R>*>(vel0) + N);
+  std::vector<R> rho(rho0, rho0 + N);
+  for (int s = 0; s < steps; ++s) {
+    tapeU[s] = vel;
+    tapeRho[s] = rho;
+    diffStep(t, vel, rho, dt, dx, K, omega, &tapeU2[s]);
+  }
+  R loss = R(0);
+  std::vector<R> grho(N);
+  for (int64_t i = 0; i < N; ++i) {
+    const R d = rho[i] - target[i];
+    loss += d * d;
+    grho[i] = R(2) * d;
+  }
+  std::vector<V3<R>> gvel(N);
+  for (int s = steps - 1; s >= 0; --s) {
+    std::vector<R> grhoIn(N, R(0));
+    std::vector<V3<R>> gu2 = gvel;  // adjoint of this step's output velocity
+    advectScalarsAdjoint(t, tapeU2[s].data(), tapeRho[s].data(), grho.data(), grhoIn.data(), gu2.data(), dt, inv_dx);
+    std::vector<R> gp(N, R(0)), gdiv(N, R(0));
+    gradientAdjoint(t, gu2.data(), gp.data(), inv_dx);  // already d/dp of u2 = u1 - grad p
+    rbgsAdjoint(t, gp.data(), gdiv.data(), dx, K, omega);
+    std::vector<V3<R>> gu1 = gu2;
+    divergenceAdjoint(t, gdiv.data(), gu1.data(), inv_dx);
+    std::vector<V3<R>> gu(N);
+    advectVectorAdjoint(t, tapeU[s].data(), gu1.data(), gu.data(), dt, inv_dx);
+    gvel = gu;
+    grho = grhoIn;
+  }
+  for (int64_t i = 0; i < N; ++i) {
+    gvel0[3 * i] = gvel[i][0];
+    gvel0[3 * i + 1] = gvel[i][1];
+    gvel0[3 * i + 2] = gvel[i][2];
+    grho0[i] = grho[i];
+  }
+  return loss;
+}
+
+// ---------------------------------------------------------------------------------------
 // HNS_Advance sequence, HNanoSolver.cu:223-456.
 // ---------------------------------------------------------------------------------------
 struct Params {  // CombustionParams, Kernels.cuh:5-24 (72 bytes)
@@ -903,7 +1246,43 @@ void oracle_set_num_threads(int n) {
                               present, nSrc, srcNames, srcs, srcVelV3, sdf);                                         \
   }
 
+#define ORACLE_ADJOINTS(SUF, R)                                                                                      \
+  void oracle_advect_scalars_adj_##SUF(void* t, const R* vel, const R* in, const R* gout, R* gin, R* gvel, float dt,  \
+                                       float inv_dx) {                                                               \
+    oracle::advectScalarsAdjoint<R>(*static_cast<Topo*>(t), reinterpret_cast<const V3<R>*>(vel), in, gout, gin,      \
+                                    reinterpret_cast<V3<R>*>(gvel), R(dt), R(inv_dx));                               \
+  }                                                                                                                  \
+  void oracle_advect_vector_adj_##SUF(void* t, const R* vel, const R* gout, R* gvel, float dt, float inv_dx) {        \
+    oracle::advectVectorAdjoint<R>(*static_cast<Topo*>(t), reinterpret_cast<const V3<R>*>(vel),                      \
+                                   reinterpret_cast<const V3<R>*>(gout), reinterpret_cast<V3<R>*>(gvel), R(dt),      \
+                                   R(inv_dx));                                                                       \
+  }                                                                                                                  \
+  void oracle_divergence_adj_##SUF(void* t, const R* gdiv, R* gvel, float inv_dx) {                                  \
+    oracle::divergenceAdjoint<R>(*static_cast<Topo*>(t), gdiv, reinterpret_cast<V3<R>*>(gvel), R(inv_dx));          \
+  }                                                                                                                  \
+  void oracle_gradient_adj_##SUF(void* t, const R* gu2, R* gp, float inv_dx) {                                       \
+    oracle::gradientAdjoint<R>(*static_cast<Topo*>(t), reinterpret_cast<const V3<R>*>(gu2), gp, R(inv_dx));         \
+  }                                                                                                                  \
+  void oracle_rbgs_adj_##SUF(void* t, const R* gp, R* gdiv, float dx, int K, float omega) {                          \
+    oracle::rbgsAdjoint<R>(*static_cast<Topo*>(t), gp, gdiv, R(dx), K, R(omega));                                   \
+  }                                                                                                                  \
+  void oracle_diff_forward_##SUF(void* t, R* vel, R* rho, int steps, float dt, float dx, int K, float omega) {       \
+    const Topo& T = *static_cast<Topo*>(t);                                                                          \
+    std::vector<V3<R>> v(reinterpret_cast<V3<R>*>(vel), reinterpret_cast<V3<R>*>(vel) + T.N);                        \
+    std::vector<R> r(rho, rho + T.N);                                                                                \
+    for (int s = 0; s < steps; ++s) oracle::diffStep<R>(T, v, r, R(dt), R(dx), K, R(omega), nullptr);               \
+    std::copy(v.begin(), v.end(), reinterpret_cast<V3<R>*>(vel));                                                    \
+    std::copy(r.begin(), r.end(), rho);                                                                              \
+  }                                                                                                                  \
+  double oracle_diff_loss_grad_##SUF(void* t, const R* vel0, const R* rho0, const R* target, int steps, float dt,    \
+                                     float dx, int K, float omega, R* gvel0, R* grho0) {                             \
+    return double(oracle::diffLossGrad<R>(*static_cast<Topo*>(t), vel0, rho0, target, steps, R(dt), R(dx), K,       \
+                                          R(omega), gvel0, grho0));                                                  \
+  }
+
 ORACLE_KERNELS(f32, float)
 ORACLE_KERNELS(f64, double)
+ORACLE_ADJOINTS(f32, float)
+ORACLE_ADJOINTS(f64, double)
 
 }  // extern "C"

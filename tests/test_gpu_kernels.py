@@ -51,7 +51,9 @@ def _dev(torch, a):
 
 def _report(tag, got, ref):
     e, m = rel_err(got, ref), max_abs(got, ref)
-    print("  %-28s rel %.3e  max-abs %.3e" % (tag, e, m), flush=True)
+    diff = int(np.count_nonzero(np.asarray(got, np.float32) != np.asarray(ref, np.float32)))
+    print("  %-28s rel %.3e  max-abs %.3e  elements differing from the oracle: %d of %d" % (
+        tag, e, m, diff, np.size(ref)), flush=True)
     return e
 
 
@@ -375,7 +377,12 @@ def test_blocked_iteration_matches_colour_sweeps_bitwise(hip, name, iters):
     dcs = torch.empty(len(c), device="cuda")
     K.to_colour_split(t, _dev(torch, div), dcs)
     p_blocked = torch.full((len(c),), float("nan"), device="cuda")
-    K.pressure_solve(t, dcs, p_blocked, dx, omega=omega, iterations=iters, solver="rbgs")
+    import os
+    os.environ["HNS_RBGS_KERNEL"] = "blocked"
+    try:
+        K.pressure_solve(t, dcs, p_blocked, dx, omega=omega, iterations=iters, solver="rbgs")
+    finally:
+        del os.environ["HNS_RBGS_KERNEL"]
     p_sweep = torch.zeros(len(c), device="cuda")
     for _ in range(iters):
         K.rbgs_sweep(t, dcs, p_sweep, dx, 0, omega)

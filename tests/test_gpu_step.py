@@ -94,7 +94,8 @@ def _compare(tag, gv, gf, ov, of, o64=None):
         g = gv if k == "velocity" else gf[k]
         r = ov if k == "velocity" else of[k]
         e = rel_err(g, r)
-        line = "  %-8s %-12s rel %.3e  max-abs %.3e" % (tag, k, e, max_abs(g, r))
+        nd = int(np.count_nonzero(np.asarray(g) != np.asarray(r)))
+        line = "  %-8s %-12s rel %.3e  max-abs %.3e  differing %d/%d" % (tag, k, e, max_abs(g, r), nd, np.size(r))
         if o64 is not None:
             r64 = o64[0] if k == "velocity" else o64[1][k]
             line += "  | vs FP64: gpu %.3e  oracle-f32 %.3e" % (max_abs(g, r64), max_abs(r, r64))
