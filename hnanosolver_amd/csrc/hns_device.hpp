@@ -124,6 +124,43 @@ __device__ __forceinline__ LeafView load_leaf(const DevTopo& T, int32_t l, int32
   return v;
 }
 
+// The six face neighbours of a leaf (uniform per workgroup, so they live in SGPRs): the fast
+// path of every 7-point stencil (divergence, gradient, residual, CG), no LDS or hash involved.
+struct FaceRow {
+  int32_t l, xm, xp, ym, yp, zm, zp;
+};
+__device__ __forceinline__ FaceRow face_row(const DevTopo& T, int32_t l) {
+  const int32_t* r = T.nbr + kNbrStride * l;
+  return FaceRow{l, r[4], r[22], r[10], r[16], r[12], r[14]};
+}
+// Leaf holding local coordinate (lx,ly,lz) moved by DIR along axis AX, with the wrapped local
+// coordinate in (nx,ny,nz); -1 when that leaf is absent.
+template <int AX, int DIR>
+__device__ __forceinline__ int32_t face_step(const FaceRow& F, int lx, int ly, int lz, int& nx, int& ny, int& nz) {
+  nx = lx;
+  ny = ly;
+  nz = lz;
+  int& c = AX == 0 ? nx : (AX == 1 ? ny : nz);
+  c += DIR;
+  if ((unsigned)c < 8u) return F.l;
+  c &= 7;
+  if (AX == 0) return DIR > 0 ? F.xp : F.xm;
+  if (AX == 1) return DIR > 0 ? F.yp : F.ym;
+  return DIR > 0 ? F.zp : F.zm;
+}
+template <int AX, int DIR>
+__device__ __forceinline__ int32_t face_dense(const FaceRow& F, int lx, int ly, int lz) {
+  int nx, ny, nz;
+  const int32_t leaf = face_step<AX, DIR>(F, lx, ly, lz, nx, ny, nz);
+  return leaf < 0 ? -1 : leaf * kLeafVox + ((nx << 6) | (ny << 3) | nz);
+}
+template <int AX, int DIR>
+__device__ __forceinline__ int32_t face_cs(const FaceRow& F, int lx, int ly, int lz, int32_t half) {
+  int nx, ny, nz;
+  const int32_t leaf = face_step<AX, DIR>(F, lx, ly, lz, nx, ny, nz);
+  return leaf < 0 ? -1 : ((nx + ny + nz) & 1) * half + leaf * kHalfLeaf + cs_local(nx, ny, nz);
+}
+
 struct Vel {
   const float* __restrict__ x;
   const float* __restrict__ y;

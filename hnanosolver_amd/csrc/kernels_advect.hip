@@ -8,6 +8,7 @@
 // tile (CFL above ~1 in the fire scenes) takes the global path for that voxel. Tile cells of
 // absent leaves hold the reference's miss value: 0 for advect_vector and the curl stencils
 // (Stencils.hpp:83), element 0 of the field for advect_scalars (Kernel.cu:187-188, :229-230).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -185,18 +186,47 @@ __device__ inline bool setup_interp_tile(const LeafView& V, float x, float y, fl
   return false;
 }
 
+// The velocity tile and all M field tiles are filled in one pass (dynamic LDS: (3 + M) 12^3
+// floats), so the loads of every tile are in flight together and there is one barrier.
 __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const float* __restrict__ u, FieldSet F,
                                                             float sdt, const float* __restrict__ sdf) {
   __shared__ int32_t s_nb[27];
-  __shared__ float s_u[3 * kE3];
-  __shared__ float s_f[kE3];
+  extern __shared__ float s_dyn[];
+  float* s_u = s_dyn;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const LeafView V = load_leaf(T, l, s_nb);
   const int32_t N = T.L * kLeafVox;
   const Vel U{u, u + N, u + 2 * N};
-  const float miss_u[3] = {u[0], u[N], u[2 * N]};  // advect_scalars reads element 0 on a miss
-  fill_tile<3>(s_u, u, N, s_nb, l, miss_u);
+  {
+    // advect_scalars reads element 0 on a miss (velocity and fields alike)
+    const float mu0 = u[0], mu1 = u[N], mu2 = u[2 * N];
+    float mf[FieldSet::kMax];
+#pragma unroll
+    for (int s = 0; s < FieldSet::kMax; ++s) mf[s] = s < F.m ? F.in[s][0] : 0.f;
+    for (int c = threadIdx.x; c < kE3; c += 512) {
+      const int tx = c / (kE * kE), ty = (c / kE) % kE, tz = c % kE;
+      const int lx = tx - kH, ly = ty - kH, lz = tz - kH;
+      const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
+      if (nl >= 0) {
+        const int32_t g = nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+        s_u[c] = u[g];
+        s_u[kE3 + c] = u[N + g];
+        s_u[2 * kE3 + c] = u[2 * N + g];
+#pragma unroll
+        for (int s = 0; s < FieldSet::kMax; ++s)
+          if (s < F.m) s_u[(3 + s) * kE3 + c] = F.in[s][g];
+      } else {
+        s_u[c] = mu0;
+        s_u[kE3 + c] = mu1;
+        s_u[2 * kE3 + c] = mu2;
+#pragma unroll
+        for (int s = 0; s < FieldSet::kMax; ++s)
+          if (s < F.m) s_u[(3 + s) * kE3 + c] = mf[s];
+      }
+    }
+    __syncthreads();
+  }
   const int32_t n = threadIdx.x;
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
   const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
@@ -223,9 +253,7 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   const int nct[6] = {ct - kE * kE, ct + kE * kE, ct - kE, ct + kE, ct - 1, ct + 1};
   for (int s = 0; s < F.m; ++s) {
     const float* __restrict__ d = F.in[s];
-    const float miss_f[1] = {d[0]};
-    __syncthreads();  // previous field's tile fully consumed
-    fill_tile<1>(s_f, d, N, s_nb, l, miss_f);
+    const float* s_f = s_dyn + (3 + s) * kE3;
     const float phiOrig = s_f[ct];
     float phiF = 0.f, phiB = 0.f;
 #pragma unroll
@@ -320,8 +348,20 @@ void launch_advect_velocity_lds(const DevTopo& T, const float* u, float* out, fl
 }
 void launch_advect_scalars_lds(const DevTopo& T, const float* u, const FieldSet& f, float dt, float inv_dx,
                                const float* sdf, hipStream_t s) {
-  k_advect_scalars_lds<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, f, dt * inv_dx, sdf);
-  HNS_HIP(hipGetLastError());
+  // at most 5 field tiles per launch: (3 + 5) 12^3 floats = 55 KiB of LDS, under the 64 KiB a
+  // workgroup can always allocate; more fields take further launches (the trace is recomputed)
+  constexpr int kPerLaunch = 5;
+  for (int b = 0; b < f.m; b += kPerLaunch) {
+    FieldSet g{};
+    g.m = std::min(kPerLaunch, f.m - b);
+    for (int q = 0; q < g.m; ++q) {
+      g.in[q] = f.in[b + q];
+      g.out[q] = f.out[b + q];
+    }
+    const size_t lds = (size_t)(3 + g.m) * kE3 * sizeof(float);
+    k_advect_scalars_lds<<<grid_for_leaves(T.L), 512, lds, s>>>(T, u, g, dt * inv_dx, sdf);
+    HNS_HIP(hipGetLastError());
+  }
 }
 void launch_vorticity_forces_lds(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, float scale,
                                  int32_t F, const float* src_vel, float gravity, hipStream_t s) {

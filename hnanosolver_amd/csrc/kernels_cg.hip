@@ -97,19 +97,18 @@ __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __rest
 __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, const float* __restrict__ d_in,
                                                   const float* __restrict__ r, float* __restrict__ d_out,
                                                   float* __restrict__ q, CgScalars S) {
-  __shared__ int32_t s_nb[27];
   __shared__ double s_red[kTpb / 64];
   if (S.flag[0]) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
-  const LeafView V = load_leaf(T, l, s_nb);
+  const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int32_t n = threadIdx.x;
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
   const int32_t i = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
-  const int32_t nb[6] = {V.cs_index(x + 1, y, z, half), V.cs_index(x - 1, y, z, half), V.cs_index(x, y + 1, z, half),
-                         V.cs_index(x, y - 1, z, half), V.cs_index(x, y, z + 1, half), V.cs_index(x, y, z - 1, half)};
+  const int32_t nb[6] = {face_cs<0, 1>(Fr, lx, ly, lz, half), face_cs<0, -1>(Fr, lx, ly, lz, half),
+                         face_cs<1, 1>(Fr, lx, ly, lz, half), face_cs<1, -1>(Fr, lx, ly, lz, half),
+                         face_cs<2, 1>(Fr, lx, ly, lz, half), face_cs<2, -1>(Fr, lx, ly, lz, half)};
   float s = 0.f, dn;
   if (it == 0) {
 #pragma unroll

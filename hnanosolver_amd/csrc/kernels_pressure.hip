@@ -9,6 +9,8 @@
 // p and writes p as fully coalesced 1 KiB runs, and gathers the other colour's 6 neighbours
 // from the same leaf or a face-neighbour leaf. Per sweep and voxel of the domain that is
 // 2 B div + 2 B p + 2 B p written + 2 B of the other colour = 8 B; 16 B per iteration.
+#include <cstdlib>
+
 #include "hns_internal.hpp"
 
 namespace hns {
@@ -23,12 +25,13 @@ __device__ __forceinline__ float sor_update(float s, float dv, float pOld, float
   return pOld + omega * (pGS - pOld);
 }
 
-template <int COLOUR, bool FIRST>
-__global__ __launch_bounds__(256) void k_rbgs(DevTopo T, const float* __restrict__ div, float* __restrict__ p,
-                                              float dx2, float omega) {
-  const int32_t l = leaf_of_block();
+// LPW leaves per workgroup (256 lanes each); the workgroup group index is XCD-mapped.
+template <int COLOUR, bool FIRST, int LPW>
+__global__ __launch_bounds__(256 * LPW) void k_rbgs(DevTopo T, const float* __restrict__ div, float* __restrict__ p,
+                                                    float dx2, float omega) {
+  const int32_t l = leaf_of_block() * LPW + (LPW > 1 ? (int32_t)(threadIdx.x >> 8) : 0);
   if (l >= T.L) return;
-  const int32_t t = threadIdx.x;
+  const int32_t t = threadIdx.x & 255;
   const int32_t x = t >> 5, y = (t >> 2) & 7, zh = t & 3;
   const int32_t z = (zh << 1) | ((x + y + COLOUR) & 1);
   const int32_t half = T.L * kHalfLeaf;
@@ -144,19 +147,20 @@ __global__ __launch_bounds__(512) void k_rbgs_iter(DevTopo T, const float* __res
 __global__ __launch_bounds__(512) void k_residual(DevTopo T, const float* __restrict__ p,
                                                   const float* __restrict__ div, double inv_dx2,
                                                   double* __restrict__ partial) {
-  __shared__ int32_t s_nb[27];
   __shared__ double s_red[2][8];
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
-  const LeafView V = load_leaf(T, l, s_nb);
+  const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int32_t n = threadIdx.x;
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
   const int32_t own = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
-  const double sum = (double)load_s(p, V.cs_index(x + 1, y, z, half)) + (double)load_s(p, V.cs_index(x - 1, y, z, half)) +
-                     (double)load_s(p, V.cs_index(x, y + 1, z, half)) + (double)load_s(p, V.cs_index(x, y - 1, z, half)) +
-                     (double)load_s(p, V.cs_index(x, y, z + 1, half)) + (double)load_s(p, V.cs_index(x, y, z - 1, half));
+  const double sum = (double)load_s(p, face_cs<0, 1>(Fr, lx, ly, lz, half)) +
+                     (double)load_s(p, face_cs<0, -1>(Fr, lx, ly, lz, half)) +
+                     (double)load_s(p, face_cs<1, 1>(Fr, lx, ly, lz, half)) +
+                     (double)load_s(p, face_cs<1, -1>(Fr, lx, ly, lz, half)) +
+                     (double)load_s(p, face_cs<2, 1>(Fr, lx, ly, lz, half)) +
+                     (double)load_s(p, face_cs<2, -1>(Fr, lx, ly, lz, half));
   const double d = (double)div[own];
   const double r = d - (sum - 6.0 * (double)p[own]) * inv_dx2;
   double a = r * r, b = d * d;
@@ -218,16 +222,37 @@ __global__ __launch_bounds__(1024) void k_reduce_pairs(const double* __restrict_
 
 }  // namespace
 
+namespace {
+template <int LPW>
+void launch_rbgs_lpw(const DevTopo& T, const float* div_cs, float* p_cs, float dx2, int colour, float omega, bool first,
+                     hipStream_t s) {
+  const unsigned g = grid_for_leaves((T.L + LPW - 1) / LPW);
+  const unsigned b = 256 * LPW;
+  if (colour == 0) {
+    if (first) k_rbgs<0, true, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+    else k_rbgs<0, false, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+  } else {
+    if (first) k_rbgs<1, true, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+    else k_rbgs<1, false, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+  }
+}
+int rbgs_lpw() {
+  static const int v = [] {
+    const char* e = std::getenv("HNS_RBGS_LPW");
+    const int k = e ? std::atoi(e) : 1;
+    return (k == 2 || k == 4) ? k : 1;
+  }();
+  return v;
+}
+}  // namespace
+
 void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,
                  hipStream_t s) {
-  const unsigned g = grid_for_leaves(T.L);
   const float dx2 = dx * dx;
-  if (colour == 0) {
-    if (first) k_rbgs<0, true><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
-    else k_rbgs<0, false><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
-  } else {
-    if (first) k_rbgs<1, true><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
-    else k_rbgs<1, false><<<g, 256, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+  switch (rbgs_lpw()) {
+    case 2: launch_rbgs_lpw<2>(T, div_cs, p_cs, dx2, colour, omega, first, s); break;
+    case 4: launch_rbgs_lpw<4>(T, div_cs, p_cs, dx2, colour, omega, first, s); break;
+    default: launch_rbgs_lpw<1>(T, div_cs, p_cs, dx2, colour, omega, first, s); break;
   }
   HNS_HIP(hipGetLastError());
 }

@@ -264,23 +264,21 @@ __global__ __launch_bounds__(512) void k_vorticity_forces(DevTopo T, const float
 template <bool COMB, bool CS>
 __global__ __launch_bounds__(512) void k_divergence(DevTopo T, const float* __restrict__ u, float* __restrict__ div,
                                                     float inv_dx, CombustionArgs C, float* __restrict__ u_out) {
-  __shared__ int32_t s_nb[27];
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
-  const LeafView V = load_leaf(T, l, s_nb);
+  const FaceRow Fr = face_row(T, l);
   const int32_t N = T.L * kLeafVox;
   const Vel U{u, u + N, u + 2 * N};
   const int32_t n = threadIdx.x;
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
   const int32_t idx = l * kLeafVox + n;
   const float cx = U.x[idx], cy = U.y[idx], cz = U.z[idx];
-  const float xp = (cx + load_s(U.x, V.index(x + 1, y, z))) * 0.5f;
-  const float xm = (cx + load_s(U.x, V.index(x - 1, y, z))) * 0.5f;
-  const float yp = (cy + load_s(U.y, V.index(x, y + 1, z))) * 0.5f;
-  const float ym = (cy + load_s(U.y, V.index(x, y - 1, z))) * 0.5f;
-  const float zp = (cz + load_s(U.z, V.index(x, y, z + 1))) * 0.5f;
-  const float zm = (cz + load_s(U.z, V.index(x, y, z - 1))) * 0.5f;
+  const float xp = (cx + load_s(U.x, face_dense<0, 1>(Fr, lx, ly, lz))) * 0.5f;
+  const float xm = (cx + load_s(U.x, face_dense<0, -1>(Fr, lx, ly, lz))) * 0.5f;
+  const float yp = (cy + load_s(U.y, face_dense<1, 1>(Fr, lx, ly, lz))) * 0.5f;
+  const float ym = (cy + load_s(U.y, face_dense<1, -1>(Fr, lx, ly, lz))) * 0.5f;
+  const float zp = (cz + load_s(U.z, face_dense<2, 1>(Fr, lx, ly, lz))) * 0.5f;
+  const float zm = (cz + load_s(U.z, face_dense<2, -1>(Fr, lx, ly, lz))) * 0.5f;
   float d = (xp - xm + yp - ym + zp - zm) * inv_dx;
   if (COMB) {
     float fuel = C.fuel[idx];
@@ -368,11 +366,13 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
   const LeafView V = load_leaf(T, l, s_nb);
   const int32_t N = T.L * kLeafVox, half = N >> 1;
   const int32_t n = threadIdx.x;
-  const int32_t x = V.ox + (n >> 6), y = V.oy + ((n >> 3) & 7), z = V.oz + (n & 7);
+  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
+  const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
   const int32_t idx = l * kLeafVox + n;
-  const float pxp = load_s(p, V.cs_index(x + 1, y, z, half)), pxm = load_s(p, V.cs_index(x - 1, y, z, half));
-  const float pyp = load_s(p, V.cs_index(x, y + 1, z, half)), pym = load_s(p, V.cs_index(x, y - 1, z, half));
-  const float pzp = load_s(p, V.cs_index(x, y, z + 1, half)), pzm = load_s(p, V.cs_index(x, y, z - 1, half));
+  const FaceRow Fr = face_row(T, l);
+  const float pxp = load_s(p, face_cs<0, 1>(Fr, lx, ly, lz, half)), pxm = load_s(p, face_cs<0, -1>(Fr, lx, ly, lz, half));
+  const float pyp = load_s(p, face_cs<1, 1>(Fr, lx, ly, lz, half)), pym = load_s(p, face_cs<1, -1>(Fr, lx, ly, lz, half));
+  const float pzp = load_s(p, face_cs<2, 1>(Fr, lx, ly, lz, half)), pzm = load_s(p, face_cs<2, -1>(Fr, lx, ly, lz, half));
   const float s = 0.5f * inv_dx;
   float3 v = f3(u[idx] - s * (pxp - pxm), u[N + idx] - s * (pyp - pym), u[2 * N + idx] - s * (pzp - pzm));
   if (sdf) {
