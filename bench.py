@@ -3,9 +3,13 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload fire256_8m]
 
 A step is one HNS_Advance call (one substep) of the fire scene: advect velocity, vorticity
-confinement, divergence + combustion + buoyancy, 100 red-black Gauss-Seidel iterations,
-gradient subtraction, MacCormack advection of the five scalars. The inputs are resident in HBM
-before timing starts (the context holds the state between calls). Prints one JSON line.
+confinement, divergence + combustion + buoyancy, the pressure solve, gradient subtraction,
+MacCormack advection of the five scalars. The headline `value` is the step with the pressure
+solved to 1e-5 relative residual by the fastest solver built (conjugate gradients until the
+learned preconditioner is faster); the plain red-black Gauss-Seidel solve to the same residual and
+the reference's own fixed 100-iteration step are reported beside it under "legs". The inputs are
+resident in HBM before timing starts (the context holds the state between calls). Prints one JSON
+line.
 """
 from __future__ import annotations
 
@@ -22,15 +26,23 @@ sys.path.insert(0, ROOT)
 
 METRIC = "voxel-steps/s at 8M voxels to 1e-5 residual, plain vs learned; % HBM peak"
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes of one RBGS colour sweep per voxel of the domain: read the other colour's p
-# (2 B), own div (2 B), own p (2 B), write own p (2 B)
-RBGS_SWEEP_BYTES = 8.0
+TOL = 1e-5
+# algorithmic bytes per voxel of the domain, per launch
+KERNEL_BYTES = {
+    # RBGS colour sweep: the other colour's p (2 B), own div (2 B), own p (2 B), write own p (2 B)
+    "rbgs": 8.0,
+    # CG product: read r (or z) and d_in, write d_out and q
+    "cg_product": 16.0,
+    # CG update: read x, d, r, q, write x, r
+    "cg_update": 24.0,
+}
+KERNEL_NAMES = {"rbgs": "k_rbgs", "cg_product": "k_cg_spmv", "cg_update": "k_cg_update"}
 
 
-def step_bytes(K: int, m_scalars: int) -> float:
+def step_bytes(solve_bytes: float, m_scalars: int) -> float:
     """Compulsory bytes per voxel-step of the fire step (FP32; stencil reuse perfect):
-    adv u 24 + vorticity 24 + div/combustion/buoyancy 60 + 16 K + gradient 28 + scalars (12 + 8 M)."""
-    return 24 + 24 + 60 + 16 * K + 28 + 12 + 8 * m_scalars
+    adv u 24 + vorticity 24 + div/combustion/buoyancy 60 + solve + gradient 28 + scalars (12 + 8 M)."""
+    return 24 + 24 + 60 + solve_bytes + 28 + 12 + 8 * m_scalars
 
 
 def make_scene(name):
@@ -47,32 +59,113 @@ def payload(sc):
     return d
 
 
-def cpu_baseline(sc, budget_s: float = 20.0):
-    """The CPU oracle (a from-scratch restatement of the reference step; the reference has no CPU
-    solver) on the host cores, on a bounded sample: whole steps of the same workload until about
-    budget_s seconds have run."""
-    import oracle
-    o = oracle.Oracle(sc.coords)
-    vel, f = sc.velocity, dict(sc.fields)
-    t0 = time.perf_counter()
-    steps = 0
-    while True:
-        vel, f = o.advance(sc.voxel_size, sc.iterations, sc.dt, sc.params, vel, f)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or steps >= 3:
-            break
-    return {"value": sc.n * steps / el, "unit": "voxel-steps/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": "%d full step(s) of %s (N=%d, K=%d) on the CPU oracle in %.1f s" % (steps, sc.name, sc.n,
-                                                                                     sc.iterations, el)}
-
-
-def read_traffic(workload):
-    path = os.path.join(ROOT, "profiles", "rbgs_%s_traffic.json" % workload)
+def read_traffic(kernel, workload):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (KERNEL_NAMES.get(kernel, kernel), workload))
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        for line in fh:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(sc, solver, iters_per_step, budget_s):
+    """The CPU oracle (a from-scratch restatement of the reference step; the reference has no CPU
+    solver) on the host cores, on a bounded sample of the same workload: one step without the
+    pressure solve, plus a bounded number of solver iterations on the same domain, scaled to the
+    iteration count the GPU step needed per step."""
+    import oracle
+    o = oracle.Oracle(sc.coords)
+    t0 = time.perf_counter()
+    o.advance(sc.voxel_size, 0, sc.dt, sc.params, sc.velocity, dict(sc.fields))
+    t_rest = time.perf_counter() - t0
+    div = np.random.default_rng(0).standard_normal(sc.n).astype(np.float32)
+    if solver == "cg":
+        run = lambda n: o.cg(div, sc.voxel_size, tol=0.0, max_iters=n)  # noqa: E731
+    else:
+        omega = float(np.float32(2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(sc.voxel_size)))))
+        p = np.zeros(sc.n, np.float32)
+
+        def run(n):
+            q = p
+            for _ in range(n):
+                q = o.rbgs(div, o.rbgs(div, q, sc.voxel_size, 0, omega), sc.voxel_size, 1, omega)
+    t0 = time.perf_counter()
+    run(2)
+    t_probe = (time.perf_counter() - t0) / 2
+    n = int(max(2, min(iters_per_step, (budget_s - t_rest) / max(t_probe, 1e-6))))
+    t0 = time.perf_counter()
+    run(n)
+    t_iter = (time.perf_counter() - t0) / n
+    t_step = t_rest + iters_per_step * t_iter
+    return {"value": sc.n / t_step, "unit": "voxel-steps/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": "%s (N=%d): one oracle step without the pressure solve (%.2f s) + %d %s iterations "
+                      "(%.3f s each), scaled to the GPU's %.0f iterations per step" % (
+                          sc.name, sc.n, t_rest, n, solver, t_iter, iters_per_step)}
+
+
+def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrument, K):
+    """Fresh context from the same initial payload; `warmup` untimed steps, then `steps` timed ones."""
+    ctx = hns.ensure_context(None, data, topo, sc.voxel_size)
+    stream = torch.cuda.current_stream()
+    ctx.set_stream(stream.cuda_stream)
+
+    def step():
+        if solver == "rbgs":
+            hns.advance(ctx, K, sc.dt, params)
+            return None
+        return hns.advance(ctx, K, sc.dt, params, solver=solver, tol=TOL, max_iters=200000,
+                           check_every=100 if solver == "rbgs_tol" else 0)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    iters, rels = [], []
+    e0.record(stream)
+    for _ in range(steps):
+        st = step()
+        if st is not None:
+            iters.append(st.iterations)
+            rels.append(st.rel_residual)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    leg = {"solver": solver, "ms_per_step": ms, "voxel_steps_per_s": sc.n / (ms * 1e-3), "steps": steps,
+           "warmup": warmup}
+    if solver == "rbgs":
+        leg["iterations_per_step"] = K
+    else:
+        leg["iterations_per_step"] = float(np.mean(iters))
+        leg["max_rel_residual"] = float(np.max(rels))
+    print("[bench] %-11s %9.3f ms/step  %s" % (solver, ms, {k: v for k, v in leg.items() if "iter" in k or "rel" in k}),
+          file=sys.stderr, flush=True)
+    if instrument:
+        # HIP events around every launch of each kernel class, same step count
+        ctx.set_timing(True)
+        ctx.reset_timing()
+        for _ in range(steps):
+            step()
+        torch.cuda.synchronize()
+        tm = ctx.timing()
+        ctx.set_timing(False)
+        leg["kernels"] = {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps,
+                              "avg_us": 1e3 * v[0] / max(v[1], 1)} for k, v in tm.items() if v[1]}
+    leg["m_scalars"] = sum(1 for k in ("density", "temperature", "fuel", "waste", "flame") if ctx.has_field(k))
+    hns.destroy_context(ctx)
+    return leg
+
+
+def roofline(leg, kernel, n, workload):
+    k = leg["kernels"][kernel]
+    algo = KERNEL_BYTES[kernel] * n
+    achieved = algo / (k["avg_us"] * 1e-6) / 1e9
+    return {"bound": "hbm", "kernel": KERNEL_NAMES[kernel], "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(kernel, workload),
+            "algorithmic_bytes_per_launch": algo, "avg_launch_us": k["avg_us"]}
 
 
 def main(argv=None):
@@ -81,12 +174,13 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fire256_8m")
+    ap.add_argument("--legs", default="cg,rbgs,rbgs_tol", help="solver legs; the first to-tolerance one is value")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the step runs on a single MI355X (sharding across GPUs is a "
-              "design in DESIGN.md, not built); run with --gpus 1", file=sys.stderr)
+              "design, not built); run with --gpus 1", file=sys.stderr)
         return 2
 
     import torch
@@ -97,63 +191,45 @@ def main(argv=None):
     sc = make_scene(a.workload)
     data = payload(sc)
     topo = hns.create_index_grid(data, sc.voxel_size)
-    ctx = hns.ensure_context(None, data, topo, sc.voxel_size)
-    stream = torch.cuda.current_stream()
-    ctx.set_stream(stream.cuda_stream)
     params = hns.make_params(**sc.params)
     K = sc.iterations
-    print("[bench] N=%d leaves=%d K=%d; warmup %d" % (sc.n, sc.n_leaves, K, a.warmup), file=sys.stderr, flush=True)
-    for _ in range(a.warmup):
-        hns.advance(ctx, K, sc.dt, params)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(a.steps):
-        hns.advance(ctx, K, sc.dt, params)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / a.steps
-    value = sc.n / (ms * 1e-3)
-    print("[bench] timed %d steps: %.3f ms/step (wall %.3f ms/step)" % (a.steps, ms, wall * 1e3 / a.steps),
-          file=sys.stderr, flush=True)
-
-    # instrumented pass: HIP events around every launch of each kernel class, same step count
-    ctx.set_timing(True)
-    ctx.reset_timing()
-    for _ in range(a.steps):
-        hns.advance(ctx, K, sc.dt, params)
-    torch.cuda.synchronize()
-    tm = ctx.timing()
-    ctx.set_timing(False)
-    rb_ms, rb_n = tm["rbgs"]
-    avg_s = rb_ms * 1e-3 / max(rb_n, 1)
-    algo = RBGS_SWEEP_BYTES * sc.n
-    achieved = algo / avg_s / 1e9
-    kernels = {k: {"ms_per_step": v[0] / a.steps, "launches_per_step": v[1] / a.steps,
-                   "avg_us": 1e3 * v[0] / max(v[1], 1)} for k, v in tm.items() if v[1]}
-    m_scalars = sum(1 for k in ("density", "temperature", "fuel", "waste", "flame") if ctx.has_field(k))
-    step_b = step_bytes(K, m_scalars)
+    print("[bench] N=%d leaves=%d K=%d" % (sc.n, sc.n_leaves, K), file=sys.stderr, flush=True)
+    legs = {}
+    for solver in a.legs.split(","):
+        slow = solver == "rbgs_tol"
+        legs[solver] = run_leg(hns, torch, sc, data, topo, params, solver,
+                               steps=min(a.steps, 2) if slow else a.steps, warmup=min(a.warmup, 1) if slow else a.warmup,
+                               instrument=not slow, K=K)
+    tol_legs = [s for s in legs if s in ("cg", "pcg_learned")]
+    head = min(tol_legs, key=lambda s: legs[s]["ms_per_step"]) if tol_legs else next(iter(legs))
+    L = legs[head]
+    value = L["voxel_steps_per_s"]
+    if head == "rbgs":
+        kern, solve_b = "rbgs", 2 * KERNEL_BYTES["rbgs"] * K
+    else:
+        kern = max(("cg_product", "cg_update"), key=lambda k: L["kernels"][k]["ms_per_step"])
+        solve_b = (KERNEL_BYTES["cg_product"] + KERNEL_BYTES["cg_update"]) * L["iterations_per_step"]
+    step_b = step_bytes(solve_b, L["m_scalars"])
     out = {
-        "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": sc.name, "voxels": sc.n, "leaves": sc.n_leaves, "rbgs_iterations": K, "dt": sc.dt,
-                   "voxel_size": sc.voxel_size, "advected_scalars": m_scalars, "solver": "rbgs_fixed",
-                   "notes": sc.notes},
-        "roofline": {"bound": "hbm", "kernel": "rbgs_sweep", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(sc.name),
-                     "algorithmic_bytes_per_launch": algo, "avg_launch_us": avg_s * 1e6},
+        "metric": METRIC, "value": value, "unit": "voxel-steps/s", "n_gpus": 1, "steps": L["steps"],
+        "warmup": L["warmup"], "ms_per_step": L["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": sc.name, "voxels": sc.n, "leaves": sc.n_leaves, "dt": sc.dt,
+                   "voxel_size": sc.voxel_size, "advected_scalars": L["m_scalars"], "solver": head,
+                   "tol": TOL if head != "rbgs" else None, "notes": sc.notes},
+        "roofline": roofline(L, kern, sc.n, sc.name),
         "step_roofline": {"bytes_per_voxel_step": step_b, "achieved_GBs": step_b * value / 1e9,
                           "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
-        "kernels": kernels,
+        "legs": legs,
     }
-    hns.destroy_context(ctx)
+    if "rbgs" in legs and "kernels" in legs["rbgs"]:
+        out["roofline_rbgs_sweep"] = roofline(legs["rbgs"], "rbgs", sc.n, sc.name)
+    if "rbgs_tol" in legs and head != "rbgs":
+        out["speedup_vs_plain_rbgs_to_tol"] = legs["rbgs_tol"]["ms_per_step"] / L["ms_per_step"]
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sc, a.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(sc, "cg" if head != "rbgs" else "rbgs", L["iterations_per_step"],
+                                           a.cpu_budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -34,7 +34,7 @@ STATUS = {0: "HNS_OK", -1: "HNS_ERR_INVALID_ARGUMENT", -2: "HNS_ERR_EMPTY", -3: 
 HNS_FLOAT, HNS_VEC3F = 1, 3
 SOLVERS = {"rbgs": 0, "rbgs_tol": 1, "cg": 2, "pcg_learned": 3}
 KERNEL_CLASSES = ["advect_velocity", "vorticity_forces", "divergence_combustion", "rbgs", "residual", "gradient",
-                  "advect_scalars", "cg", "other"]
+                  "advect_scalars", "cg_product", "cg_update", "precond", "other"]
 
 
 class HNSError(RuntimeError):

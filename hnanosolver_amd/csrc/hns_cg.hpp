@@ -4,21 +4,43 @@
 
 namespace hns {
 
-enum CgScalarSlot { kBB = 0, kSigma = 1, kScalarSlots = 4 };
+// Workgroups of every CG launch: each processes a strided run of leaves and leaves one FP64
+// partial per dot product, so a consumer kernel can re-reduce the partials itself.
+constexpr int kCgMaxGroups = 2048;
+inline unsigned cg_groups(int32_t L) {
+  const unsigned g = grid_for_leaves(L);
+  return g < (unsigned)kCgMaxGroups ? g : (unsigned)kCgMaxGroups;
+}
 
-// Device scalars of one solve. rho[k] = r_k . r_k; flag[0] = stop, flag[1] = iterations done.
+// Device scalars of one solve. rho[k] = r_k . r_k (CG) or r_k . z_k (PCG), alpha[k] the step
+// length of iteration k; flag[0] = stop, flag[1] = iterations done when stopped.
 struct CgScalars {
-  double* partial;   // one FP64 per leaf
-  double* rho;       // max_iters + 1 entries
-  double* scal;      // kScalarSlots entries
-  int* flag;         // 2 entries
-  double tol2;       // stop when rho[k] <= tol2 * ||b||^2
+  double* pa;     // kCgSlots partials per workgroup, written by init/replace/update/rz
+  double* pb;     // one partial per workgroup (d . q), written by the product kernel
+  double* rho;    // max_iters + 1 entries
+  double* alpha;  // max_iters + 1 entries
+  int* flag;      // 2 entries
+  double tol2;    // stop when r.r <= tol2 * ||b||^2
+  double* bb;     // ||b||^2
 };
+// pa slots: r.r, r.z, q_{k-1}.z
+enum CgSlot { kSlotRR = 0, kSlotRZ = 1, kSlotQZ = 2, kCgSlots = 4 };
+// the recurrence residual is replaced by b - A x every this many iterations
+constexpr int kResidualReplace = 64;
 
-// r = b = -div dx^2, x = 0; rho[0] = ||b||^2.
+// r = b = -div dx^2, x = 0; partials of ||b||^2; stop flags cleared.
 void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, const CgScalars& S, hipStream_t s);
-// One CG iteration `it`: d_out = r + beta d_in, q = A d_out, x += alpha d_out, r -= alpha q.
-void launch_cg_iteration(const DevTopo& T, int32_t it, float* x, float* r, const float* d_in, float* d_out, float* q,
-                         const CgScalars& S, hipStream_t s);
+// r = b - A x (residual replacement); partials of r.r; stop flags cleared.
+void launch_cg_replace(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
+                       hipStream_t s);
+// One CG iteration `it` in two launches. Product: d_out = r + beta d_in (beta = 0 when `fresh`),
+// q = A d_out. Update: x += alpha d_out, r -= alpha q. With z != nullptr it is a flexible
+// preconditioned iteration (z = M r in place and launch_cg_rz run on it): d_out = z + beta d_in.
+void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* z, const float* d_in,
+                       float* d_out, float* q, const CgScalars& S, hipStream_t s);
+void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d, const float* q, const CgScalars& S,
+                      hipStream_t s);
+// Partials of r.z and q.z (q of the previous iteration) after the preconditioner wrote z.
+void launch_cg_rz(int32_t L, const float* r, const float* z, const float* q, const CgScalars& S, hipStream_t s);
 
 }  // namespace hns

@@ -161,6 +161,32 @@ __device__ __forceinline__ int32_t face_cs(const FaceRow& F, int lx, int ly, int
   return leaf < 0 ? -1 : ((nx + ny + nz) & 1) * half + leaf * kHalfLeaf + cs_local(nx, ny, nz);
 }
 
+// Colour-split voxel pairs: a 256-lane workgroup covers a leaf with lane t = (x<<5)|(y<<2)|zz
+// owning v0 = (x,y,2zz) in colour c0 = (x+y)&1 and v1 = (x,y,2zz+1) in colour c0^1, both at
+// in-plane offset 256*l + t. Their 6-neighbour sums need no per-lane index arithmetic beyond
+// the leaf faces: +-x is +-32 (+-224 across a face), +-y is +-4 (+-28), z stays inside the pair
+// or moves one slot (3 across a face). get(c, i) returns the value at colour plane c, index i.
+// Sums are in the order +x, -x, +y, -y, +z, -z.
+template <class T, class G>
+__device__ __forceinline__ void pair_face_sums(const FaceRow& F, int32_t t, G get, T v0, T v1, T& s0, T& s1) {
+  const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
+  const int c0 = (x + y) & 1, c1 = c0 ^ 1;
+  const int32_t b = F.l * kHalfLeaf + t;
+  auto at = [&](int c, int32_t leaf, int32_t off) -> T { return leaf < 0 ? T(0) : get(c, leaf * kHalfLeaf + off); };
+  const T xp0 = x < 7 ? get(c1, b + 32) : at(c1, F.xp, t - 224);
+  const T xm0 = x > 0 ? get(c1, b - 32) : at(c1, F.xm, t + 224);
+  const T yp0 = y < 7 ? get(c1, b + 4) : at(c1, F.yp, t - 28);
+  const T ym0 = y > 0 ? get(c1, b - 4) : at(c1, F.ym, t + 28);
+  const T zm0 = zz > 0 ? get(c1, b - 1) : at(c1, F.zm, t + 3);
+  const T xp1 = x < 7 ? get(c0, b + 32) : at(c0, F.xp, t - 224);
+  const T xm1 = x > 0 ? get(c0, b - 32) : at(c0, F.xm, t + 224);
+  const T yp1 = y < 7 ? get(c0, b + 4) : at(c0, F.yp, t - 28);
+  const T ym1 = y > 0 ? get(c0, b - 4) : at(c0, F.ym, t + 28);
+  const T zp1 = zz < 3 ? get(c0, b + 1) : at(c0, F.zp, t - 3);
+  s0 = xp0 + xm0 + yp0 + ym0 + v1 + zm0;
+  s1 = xp1 + xm1 + yp1 + ym1 + zp1 + v0;
+}
+
 struct Vel {
   const float* __restrict__ x;
   const float* __restrict__ y;

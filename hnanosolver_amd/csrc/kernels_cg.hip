@@ -1,172 +1,272 @@
-// Conjugate gradients for the pressure Poisson problem L p = div (HNS_SOLVER_CG; no reference
-// counterpart: the reference runs a fixed RBGS count, src/Cuda/HNanoSolver.cu:366-373).
+// Conjugate gradients for the pressure Poisson problem L p = div (HNS_SOLVER_CG and the flexible
+// preconditioned variant behind HNS_SOLVER_PCG_LEARNED; no reference counterpart: the reference
+// runs a fixed RBGS count, src/Cuda/HNanoSolver.cu:366-373).
 //
 // Operator: A = -L scaled by dx^2, (A p)_i = 6 p_i - sum of the 6 neighbours, missing neighbours
 // are 0 (the same Dirichlet boundary as RBGS, Kernel.cu:477-482); right-hand side b = -div dx^2.
-// Vectors are colour-split FP32 (the RBGS layout); dot products are FP64, reduced per leaf and
-// then by a one-workgroup reduce kernel in a fixed order (deterministic). alpha and beta are
-// formed on the device from the reduced scalars, and a device flag stops the remaining
-// iterations of a batch once ||r||^2 <= tol^2 ||b||^2, so the host only synchronises per batch.
+// Vectors are colour-split FP32 (the RBGS layout); dot products are FP64.
 //
-// Per iteration and voxel: spmv reads r, d (8 B) and writes d, q (8 B); update reads x, d, r, q
-// (16 B) and writes x, r (8 B): 40 B.
+// Two launches per iteration and no separate reduction kernel: every 256-lane workgroup walks a
+// strided run of leaves (lane t owns in-plane offset 256*l + t of both colour planes) and leaves
+// one FP64 partial per dot product (at most 2048 partials); the next kernel's workgroups each
+// re-reduce all partials in the same fixed order, so every workgroup holds bitwise the same
+// alpha, beta and stop decision, and the result is deterministic. The stop test
+// (||r||^2 <= tol^2 ||b||^2) runs on the device; the host synchronises per batch. The recurrence
+// residual is replaced by b - A x every kResidualReplace iterations and before a stop is
+// accepted, so FP32 drift of the recurrence cannot end the solve early.
+//
+// Flexible PCG (Notay's truncated FCG(1)): beta_k = -alpha_{k-1} (q_{k-1} . z_k) / (r_{k-1} . z_{k-1}),
+// which is the Polak-Ribiere form z_k . (r_k - r_{k-1}) / (r_{k-1} . z_{k-1}) without keeping r_{k-1}.
+//
+// Per iteration and voxel: product reads r (or z), d (8 B) and writes d, q (8 B); update reads
+// x, d, r, q (16 B) and writes x, r (8 B): 40 B.
 #include "hns_cg.hpp"
 
 namespace hns {
 namespace {
 
-constexpr int kTpb = 512;
+constexpr int kTpb = 256;
 
-// Block-wide FP64 sum over 512 lanes (8 wave64): shuffles, then LDS.
-__device__ inline double block_sum(double v, double* s_red) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) s_red[w] = v;
-  __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < kTpb / 64; ++i) t += s_red[i];
-  }
-  __syncthreads();
-  return t;  // valid in lane 0
+// Strided leaf run of this workgroup: the leaves are cut into 8 contiguous ranges, one per XCD
+// (workgroup b runs on XCD b % 8), and the XCD's workgroups interleave inside their range.
+struct LeafRun {
+  int32_t first, end, step;
+};
+__device__ inline LeafRun leaf_run(int32_t L) {
+  const int32_t b = (int32_t)blockIdx.x, W = (int32_t)(gridDim.x >> 3);
+  const int32_t per = (L + 7) / 8;
+  const int32_t lo = (b & 7) * per;
+  return LeafRun{lo + (b >> 3), min(L, lo + per), W};
 }
 
-// One workgroup sums the per-leaf partials in a fixed order (thread t takes t, t+1024, ...,
-// then a wave/LDS tree), so every reduction is deterministic. A single-address atomic counter
-// across all leaf workgroups would serialise 16k device-scope atomics per reduction.
-constexpr int kRed = 1024;
-__device__ inline double reduce_partials(const double* __restrict__ partial, int32_t L) {
-  __shared__ double s_red[kRed / 64];
-  double v = 0.0;
-  for (int32_t i = threadIdx.x; i < L; i += kRed) v += partial[i];
+// Sum of a 256-lane value over the workgroup; the total is returned to every lane.
+__device__ inline double block_sum_all(double v, double* s_red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
   if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
   __syncthreads();
   double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int i = 0; i < kRed / 64; ++i) t += s_red[i];
-  return t;  // valid in thread 0
+#pragma unroll
+  for (int i = 0; i < kTpb / 64; ++i) t += s_red[i];
+  __syncthreads();
+  return t;
 }
 
-// mode 0: rho[0] = ||b||^2 = scal[kBB], flags reset (stop at once when b == 0)
-// mode 1: scal[kSigma] = d.q (stop on breakdown)
-// mode 2: rho[it+1] = r.r, iterations = it+1, stop when rho[it+1] <= tol2 ||b||^2
-__global__ __launch_bounds__(kRed) void k_cg_reduce(const double* __restrict__ partial, int32_t L, int mode, int32_t it,
-                                                    CgScalars S) {
-  if (mode != 0 && S.flag[0]) return;
-  const double t = reduce_partials(partial, L);
-  if (threadIdx.x != 0) return;
-  if (mode == 0) {
-    S.rho[0] = t;
-    S.scal[kBB] = t;
-    S.flag[0] = t == 0.0 ? 1 : 0;
+// Sum of slot k of the G partials (stride `stride` doubles), fixed order, returned to all lanes.
+__device__ inline double reduce_partials(const double* __restrict__ p, int stride, int k, int G, double* s_red) {
+  double v = 0.0;
+  for (int i = threadIdx.x; i < G; i += kTpb) v += p[i * stride + k];
+  return block_sum_all(v, s_red);
+}
+
+__device__ inline void clear_flags(const CgScalars& S) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.flag[0] = 0;
     S.flag[1] = 0;
-  } else if (mode == 1) {
-    S.scal[kSigma] = t;
-    if (!(t > 0.0)) S.flag[0] = 1;  // breakdown (A is SPD; only round-off gets here)
-  } else {
-    S.rho[it + 1] = t;
-    S.flag[1] = it + 1;
-    if (t <= S.tol2 * S.scal[kBB]) S.flag[0] = 1;
   }
 }
 
-// r = b = -div dx^2, x = 0; rho[0] = ||b||^2 = scal[kBB]
+// r = b = -div dx^2, x = 0; partials of ||b||^2 into pa slot kSlotRR; stop flags cleared
 __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __restrict__ div, float dx2,
                                                   float* __restrict__ x, float* __restrict__ r, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
-  const int32_t l = leaf_of_block();
-  if (l >= L) return;
-  const int32_t n = threadIdx.x;
+  const LeafRun R = leaf_run(L);
   const int32_t half = L * kHalfLeaf;
-  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t i = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
-  const float b = -div[i] * dx2;
-  x[i] = 0.f;
-  r[i] = b;
-  const double part = block_sum((double)b * (double)b, s_red);
-  if (threadIdx.x == 0) S.partial[l] = part;
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+    const int32_t i0 = l * kHalfLeaf + threadIdx.x, i1 = half + i0;
+    const float b0 = -div[i0] * dx2, b1 = -div[i1] * dx2;
+    x[i0] = 0.f;
+    x[i1] = 0.f;
+    r[i0] = b0;
+    r[i1] = b1;
+    acc += (double)b0 * (double)b0 + (double)b1 * (double)b1;
+  }
+  const double t = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = t;
+  clear_flags(S);
 }
 
-// Iteration `it` (0-based): d_out = r + beta d_in with beta = rho[it]/rho[it-1] (d_out = r at it 0),
-// evaluated at this voxel and its 6 neighbours; q = A d_out; scal[kSigma] = d_out . q. d_in and
-// d_out are distinct buffers, so no workgroup reads a value another one has already replaced.
-__global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, const float* __restrict__ d_in,
-                                                  const float* __restrict__ r, float* __restrict__ d_out,
+// Residual replacement: r = b - A x; partials of r.r into pa slot kSlotRR; stop flags cleared.
+__global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __restrict__ div, float dx2,
+                                                     const float* __restrict__ x, float* __restrict__ r,
+                                                     CgScalars S) {
+  __shared__ double s_red[kTpb / 64];
+  const LeafRun R = leaf_run(T.L);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  auto get = [&](int c, int32_t i) { return x[c * half + i]; };
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+    const FaceRow Fr = face_row(T, l);
+    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
+    const float x0 = x[i0], x1 = x[i1];
+    float s0, s1;
+    pair_face_sums(Fr, t, get, x0, x1, s0, s1);
+    const float r0 = -div[i0] * dx2 - (6.0f * x0 - s0);
+    const float r1 = -div[i1] * dx2 - (6.0f * x1 - s1);
+    r[i0] = r0;
+    r[i1] = r1;
+    acc += (double)r0 * (double)r0 + (double)r1 * (double)r1;
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = tt;
+  clear_flags(S);
+}
+
+// Iteration `it` (0-based), product half: reduce r.r (and r.z, q_{k-1}.z) from pa; stop when
+// converged; d_out = s + beta d_in (s = r, or z when PRE) at the voxel pair and its neighbours;
+// q = A d_out; partials of d_out . q into pb. `fresh` restarts the direction (beta = 0) and, at
+// it == 0, records ||b||^2. d_in and d_out are distinct buffers, so no workgroup reads a value
+// another one has already replaced.
+template <bool PRE>
+__global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fresh, const float* __restrict__ src,
+                                                  const float* __restrict__ d_in, float* __restrict__ d_out,
                                                   float* __restrict__ q, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
   if (S.flag[0]) return;
-  const int32_t l = leaf_of_block();
-  if (l >= T.L) return;
-  const FaceRow Fr = face_row(T, l);
-  const int32_t half = T.L * kHalfLeaf;
-  const int32_t n = threadIdx.x;
-  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t i = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
-  const int32_t nb[6] = {face_cs<0, 1>(Fr, lx, ly, lz, half), face_cs<0, -1>(Fr, lx, ly, lz, half),
-                         face_cs<1, 1>(Fr, lx, ly, lz, half), face_cs<1, -1>(Fr, lx, ly, lz, half),
-                         face_cs<2, 1>(Fr, lx, ly, lz, half), face_cs<2, -1>(Fr, lx, ly, lz, half)};
-  float s = 0.f, dn;
-  if (it == 0) {
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-      if (nb[k] >= 0) s += r[nb[k]];
-    dn = r[i];
-  } else {
-    const float beta = (float)(S.rho[it] / S.rho[it - 1]);
-#pragma unroll
-    for (int k = 0; k < 6; ++k)
-      if (nb[k] >= 0) s += fmaf(beta, d_in[nb[k]], r[nb[k]]);
-    dn = fmaf(beta, d_in[i], r[i]);
+  const int G = (int)gridDim.x;
+  const double rr = reduce_partials(S.pa, kCgSlots, kSlotRR, G, s_red);
+  const double bb = it == 0 ? rr : S.bb[0];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (rr <= S.tol2 * bb) {  // identical decision in every workgroup
+    if (lead) {
+      if (it == 0) S.bb[0] = bb;
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+    return;
   }
-  const float qv = 6.0f * dn - s;
-  d_out[i] = dn;
-  q[i] = qv;
-  const double part = block_sum((double)dn * (double)qv, s_red);
-  if (threadIdx.x == 0) S.partial[l] = part;
+  double gam = rr, beta = 0.0;
+  if (PRE) {
+    gam = reduce_partials(S.pa, kCgSlots, kSlotRZ, G, s_red);
+    if (!fresh) beta = -S.alpha[it - 1] * reduce_partials(S.pa, kCgSlots, kSlotQZ, G, s_red) / S.rho[it - 1];
+  } else if (!fresh) {
+    beta = rr / S.rho[it - 1];
+  }
+  if (lead) {
+    if (it == 0) S.bb[0] = bb;
+    S.rho[it] = gam;
+  }
+  const float bf = (float)beta;
+  const LeafRun R = leaf_run(T.L);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+    const FaceRow Fr = face_row(T, l);
+    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
+    float d0, d1, s0, s1;
+    if (fresh) {
+      auto get = [&](int c, int32_t i) { return src[c * half + i]; };
+      d0 = src[i0];
+      d1 = src[i1];
+      pair_face_sums(Fr, t, get, d0, d1, s0, s1);
+    } else {
+      auto get = [&](int c, int32_t i) { return fmaf(bf, d_in[c * half + i], src[c * half + i]); };
+      d0 = fmaf(bf, d_in[i0], src[i0]);
+      d1 = fmaf(bf, d_in[i1], src[i1]);
+      pair_face_sums(Fr, t, get, d0, d1, s0, s1);
+    }
+    const float q0 = 6.0f * d0 - s0, q1 = 6.0f * d1 - s1;
+    d_out[i0] = d0;
+    d_out[i1] = d1;
+    q[i0] = q0;
+    q[i1] = q1;
+    acc += (double)d0 * (double)q0 + (double)d1 * (double)q1;
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pb[blockIdx.x] = tt;
 }
 
-// alpha = rho[it]/sigma; x += alpha d; r -= alpha q; rho[it+1] = r.r; convergence flag
+// Update half: sigma = d.q from pb; alpha = rho[it]/sigma; x += alpha d; r -= alpha q;
+// partials of r.r into pa.
 __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float* __restrict__ x,
                                                     float* __restrict__ r, const float* __restrict__ d,
                                                     const float* __restrict__ q, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
   if (S.flag[0]) return;
-  const int32_t l = leaf_of_block();
-  if (l >= L) return;
-  const int32_t n = threadIdx.x;
+  const double sigma = reduce_partials(S.pb, 1, 0, (int)gridDim.x, s_red);
+  if (!(sigma > 0.0)) {  // breakdown (A is SPD; only round-off gets here); same in every workgroup
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+    return;
+  }
+  const double alpha_d = S.rho[it] / sigma;
+  if (blockIdx.x == 0 && threadIdx.x == 0) S.alpha[it] = alpha_d;
+  const float alpha = (float)alpha_d;
+  const LeafRun R = leaf_run(L);
   const int32_t half = L * kHalfLeaf;
-  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t i = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
-  const float alpha = (float)(S.rho[it] / S.scal[kSigma]);
-  const float dv = d[i];
-  x[i] = fmaf(alpha, dv, x[i]);
-  const float rv = fmaf(-alpha, q[i], r[i]);
-  r[i] = rv;
-  const double part = block_sum((double)rv * (double)rv, s_red);
-  if (threadIdx.x == 0) S.partial[l] = part;
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = tt;
+}
+
+// Partials of r.z and q.z (q of the previous iteration) into pa slots kSlotRZ, kSlotQZ.
+__global__ __launch_bounds__(kTpb) void k_cg_rz(int32_t L, const float* __restrict__ r, const float* __restrict__ z,
+                                                const float* __restrict__ q, CgScalars S) {
+  __shared__ double s_red[kTpb / 64];
+  if (S.flag[0]) return;
+  const LeafRun R = leaf_run(L);
+  const int32_t half = L * kHalfLeaf;
+  double a = 0.0, b = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
+      const double zv = (double)z[i];
+      a += (double)r[i] * zv;
+      b += (double)q[i] * zv;
+    }
+  }
+  const double ta = block_sum_all(a, s_red);
+  const double tb = block_sum_all(b, s_red);
+  if (threadIdx.x == 0) {
+    S.pa[blockIdx.x * kCgSlots + kSlotRZ] = ta;
+    S.pa[blockIdx.x * kCgSlots + kSlotQZ] = tb;
+  }
 }
 
 }  // namespace
 
 void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, const CgScalars& S, hipStream_t s) {
-  k_cg_init<<<grid_for_leaves(L), kTpb, 0, s>>>(L, div, dx * dx, x, r, S);
-  HNS_HIP(hipGetLastError());
-  k_cg_reduce<<<1, kRed, 0, s>>>(S.partial, L, 0, 0, S);
+  k_cg_init<<<cg_groups(L), kTpb, 0, s>>>(L, div, dx * dx, x, r, S);
   HNS_HIP(hipGetLastError());
 }
-void launch_cg_iteration(const DevTopo& T, int32_t it, float* x, float* r, const float* d_in, float* d_out, float* q,
-                         const CgScalars& S, hipStream_t s) {
-  k_cg_spmv<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, it, d_in, r, d_out, q, S);
+void launch_cg_replace(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
+                       hipStream_t s) {
+  k_cg_replace<<<cg_groups(T.L), kTpb, 0, s>>>(T, div, dx * dx, x, r, S);
   HNS_HIP(hipGetLastError());
-  k_cg_reduce<<<1, kRed, 0, s>>>(S.partial, T.L, 1, it, S);
+}
+void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* z, const float* d_in,
+                       float* d_out, float* q, const CgScalars& S, hipStream_t s) {
+  if (z)
+    k_cg_spmv<true><<<cg_groups(T.L), kTpb, 0, s>>>(T, it, fresh ? 1 : 0, z, d_in, d_out, q, S);
+  else
+    k_cg_spmv<false><<<cg_groups(T.L), kTpb, 0, s>>>(T, it, fresh ? 1 : 0, r, d_in, d_out, q, S);
   HNS_HIP(hipGetLastError());
-  k_cg_update<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, it, x, r, d_out, q, S);
+}
+void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d, const float* q, const CgScalars& S,
+                      hipStream_t s) {
+  k_cg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, x, r, d, q, S);
   HNS_HIP(hipGetLastError());
-  k_cg_reduce<<<1, kRed, 0, s>>>(S.partial, T.L, 2, it, S);
+}
+void launch_cg_rz(int32_t L, const float* r, const float* z, const float* q, const CgScalars& S, hipStream_t s) {
+  k_cg_rz<<<cg_groups(L), kTpb, 0, s>>>(L, r, z, q, S);
   HNS_HIP(hipGetLastError());
 }
 

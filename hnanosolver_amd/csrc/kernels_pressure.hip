@@ -144,27 +144,26 @@ __global__ __launch_bounds__(512) void k_rbgs_iter(DevTopo T, const float* __res
 }
 
 // Residual r = div - (sum6 p_nbr - 6 p) / dx^2 in FP64; per-leaf partial sums of r^2 and div^2.
-__global__ __launch_bounds__(512) void k_residual(DevTopo T, const float* __restrict__ p,
+// 256 lanes per leaf, each owning a colour-split voxel pair.
+__global__ __launch_bounds__(256) void k_residual(DevTopo T, const float* __restrict__ p,
                                                   const float* __restrict__ div, double inv_dx2,
                                                   double* __restrict__ partial) {
-  __shared__ double s_red[2][8];
+  __shared__ double s_red[2][4];
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int32_t n = threadIdx.x;
-  const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
-  const int32_t own = ((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz);
-  const double sum = (double)load_s(p, face_cs<0, 1>(Fr, lx, ly, lz, half)) +
-                     (double)load_s(p, face_cs<0, -1>(Fr, lx, ly, lz, half)) +
-                     (double)load_s(p, face_cs<1, 1>(Fr, lx, ly, lz, half)) +
-                     (double)load_s(p, face_cs<1, -1>(Fr, lx, ly, lz, half)) +
-                     (double)load_s(p, face_cs<2, 1>(Fr, lx, ly, lz, half)) +
-                     (double)load_s(p, face_cs<2, -1>(Fr, lx, ly, lz, half));
-  const double d = (double)div[own];
-  const double r = d - (sum - 6.0 * (double)p[own]) * inv_dx2;
-  double a = r * r, b = d * d;
-  // wave64 reduction with DPP/permute shuffles, then across the 8 waves through LDS
+  const int c0 = ((n >> 5) + (n >> 2)) & 1;
+  const int32_t i0 = c0 * half + l * kHalfLeaf + n, i1 = (c0 ^ 1) * half + l * kHalfLeaf + n;
+  auto get = [&](int c, int32_t i) { return (double)p[c * half + i]; };
+  const double p0 = (double)p[i0], p1 = (double)p[i1];
+  double s0, s1;
+  pair_face_sums(Fr, n, get, p0, p1, s0, s1);
+  const double d0 = (double)div[i0], d1 = (double)div[i1];
+  const double r0 = d0 - (s0 - 6.0 * p0) * inv_dx2, r1 = d1 - (s1 - 6.0 * p1) * inv_dx2;
+  double a = r0 * r0 + r1 * r1, b = d0 * d0 + d1 * d1;
+  // wave64 reduction with DPP/permute shuffles, then across the 4 waves through LDS
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_down(a, off, 64);
@@ -179,7 +178,7 @@ __global__ __launch_bounds__(512) void k_residual(DevTopo T, const float* __rest
   if (n == 0) {
     double ta = 0.0, tb = 0.0;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < 4; ++i) {
       ta += s_red[0][i];
       tb += s_red[1][i];
     }
@@ -269,7 +268,7 @@ void launch_rbgs_iteration(const DevTopo& T, const float* div_cs, const float* p
 void launch_residual(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* partial,
                      double* out, hipStream_t s) {
   const double inv_dx2 = 1.0 / ((double)dx * (double)dx);
-  k_residual<<<grid_for_leaves(T.L), 512, 0, s>>>(T, p_cs, div_cs, inv_dx2, partial);
+  k_residual<<<grid_for_leaves(T.L), 256, 0, s>>>(T, p_cs, div_cs, inv_dx2, partial);
   HNS_HIP(hipGetLastError());
   k_reduce_pairs<<<1, 1024, 0, s>>>(partial, T.L, out);
   HNS_HIP(hipGetLastError());

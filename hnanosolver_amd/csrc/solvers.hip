@@ -13,7 +13,7 @@ namespace hns {
 
 struct CgState {
   DBuf<float> r, d0, d1, q;
-  DBuf<double> rho, scal;
+  DBuf<double> rho, alpha, pa, pb, bb;
   DBuf<int> flag;
   int* h_flag = nullptr;
   int cap = 0;
@@ -144,44 +144,70 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.d0.alloc(N_);
     G.d1.alloc(N_);
     G.q.alloc(N_);
-    G.scal.alloc(kScalarSlots);
+    G.pa.alloc((size_t)kCgMaxGroups * kCgSlots);
+    G.pb.alloc(kCgMaxGroups);
+    G.bb.alloc(1);
     G.flag.alloc(2);
     HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 2 * sizeof(int), hipHostMallocDefault));
   }
   if (G.cap < cap) {
     G.rho.alloc((size_t)cap + 1);
+    G.alpha.alloc((size_t)cap + 1);
     G.cap = cap;
   }
   CgScalars S;
-  S.partial = partial_.p;
+  S.pa = G.pa.p;
+  S.pb = G.pb.p;
   S.rho = G.rho.p;
-  S.scal = G.scal.p;
+  S.alpha = G.alpha.p;
   S.flag = G.flag.p;
-  // the recurrence residual runs slightly ahead of the true one in FP32: stop it at tol/2
-  const double t = 0.5 * (double)r.tol;
-  S.tol2 = t * t;
-  tm.begin(HNS_K_CG, s);
+  S.bb = G.bb.p;
+  S.tol2 = (double)r.tol * (double)r.tol;
+  tm.begin(HNS_K_CG_UPDATE, s);
   launch_cg_init(T.L, div, r.dx, p, G.r.p, S, s);
   tm.end(s);
-  int done = 0, batch = 8;
-  G.h_flag[0] = 0;
-  G.h_flag[1] = 0;
-  while (done < cap) {
-    const int nb = std::min(batch, cap - done);
-    for (int k = 0; k < nb; ++k) {
-      const int it = done + k;
-      tm.begin(HNS_K_CG, s);
-      launch_cg_iteration(T, it, p, G.r.p, (it & 1) ? G.d1.p : G.d0.p, (it & 1) ? G.d0.p : G.d1.p, G.q.p, S, s);
+  // The device stops at the first iteration whose recurrence residual passes the test. The stop
+  // is accepted only when that residual was just replaced by b - A x; otherwise the residual is
+  // replaced and the same iteration is tested again (and continued if it fails).
+  int it = 0, batch = 8, replaced_at = 0;
+  bool stopped = false;
+  while (it < cap) {
+    const int nb = std::min(batch, cap - it);
+    for (int k = 0; k < nb; ++k, ++it) {
+      if (it > 0 && it % kResidualReplace == 0) {
+        tm.begin(HNS_K_CG_UPDATE, s);
+        launch_cg_replace(T, div, r.dx, p, G.r.p, S, s);
+        tm.end(s);
+        replaced_at = it;
+      }
+      float* d_in = (it & 1) ? G.d1.p : G.d0.p;
+      float* d_out = (it & 1) ? G.d0.p : G.d1.p;
+      tm.begin(HNS_K_CG_PRODUCT, s);
+      launch_cg_product(T, it, it == 0, G.r.p, nullptr, d_in, d_out, G.q.p, S, s);
+      tm.end(s);
+      tm.begin(HNS_K_CG_UPDATE, s);
+      launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
       tm.end(s);
     }
-    done += nb;
     HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HNS_HIP(hipStreamSynchronize(s));
-    if (G.h_flag[0]) break;
+    if (G.h_flag[0]) {
+      it = G.h_flag[1];
+      if (it == replaced_at) {
+        stopped = true;
+        break;
+      }
+      tm.begin(HNS_K_CG_UPDATE, s);
+      launch_cg_replace(T, div, r.dx, p, G.r.p, S, s);
+      tm.end(s);
+      replaced_at = it;
+      batch = 8;
+      continue;
+    }
     batch = std::min(batch * 2, 128);
   }
   SolveResult res;
-  res.iterations = G.h_flag[1];
+  res.iterations = stopped ? G.h_flag[1] : it;
   res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
   return res;
 }

@@ -178,9 +178,11 @@ int32_t hns_has_field(const hns_context* ctx, const char* name);
 #define HNS_K_RESIDUAL 4
 #define HNS_K_GRADIENT 5
 #define HNS_K_ADVECT_SCALARS 6
-#define HNS_K_CG 7
-#define HNS_K_OTHER 8
-#define HNS_K_COUNT 9
+#define HNS_K_CG_PRODUCT 7 /* CG/PCG: direction update + operator product (+ d.q partials) */
+#define HNS_K_CG_UPDATE 8  /* CG/PCG: x, r update (+ r.r partials) */
+#define HNS_K_PRECOND 9    /* PCG: the learned preconditioner (+ r.z partials) */
+#define HNS_K_OTHER 10
+#define HNS_K_COUNT 11
 hns_status hns_set_timing(hns_context* ctx, int32_t enable);
 /* Accumulated time (ms) and launch count for kernel class `which` since the last reset. */
 hns_status hns_get_timing(hns_context* ctx, int32_t which, double* total_ms, int64_t* launches);
