@@ -194,6 +194,12 @@ class Context:
     def set_stream(self, stream_ptr: int):
         _check(lib().hns_set_stream(self._h, C.c_void_p(stream_ptr)))
 
+    def set_preconditioner(self, pc):
+        """HNS_SOLVER_PCG_LEARNED solves of this context use `pc` (a precond.Preconditioner, kept
+        alive by the context; None detaches)."""
+        _check(lib().hns_set_preconditioner(self.handle, pc.handle if pc is not None else None))
+        self._pc = pc
+
     def set_timing(self, on: bool):
         _check(lib().hns_set_timing(self._h, 1 if on else 0))
 

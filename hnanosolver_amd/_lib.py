@@ -38,7 +38,7 @@ class hns_solve_options(C.Structure):
 
 class hns_solve_stats(C.Structure):
     _fields_ = [("struct_size", C.c_uint32), ("iterations", C.c_int32), ("rel_residual", C.c_double),
-                ("solve_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("fallback_iterations", C.c_int32), ("reserved", C.c_int32)]
 
 
 assert C.sizeof(hns_params) == 72
@@ -90,6 +90,27 @@ _SIGS = [
     ("hnsk_from_colour_split", C.c_int32, [_VP, _VP, _VP, _VP, C.c_int32]),
     ("hnsk_aos_to_soa", C.c_int32, [C.c_size_t, _VP, _VP, _VP, C.c_int32]),
     ("hnsk_soa_to_aos", C.c_int32, [C.c_size_t, _VP, _VP, _VP, C.c_int32]),
+    ("hns_precond_num_params", C.c_int64, []),
+    ("hns_precond_create", C.c_int32, [_VP, C.c_int64, C.POINTER(_VP)]),
+    ("hns_precond_create_device", C.c_int32, [_VP, C.c_int64, C.POINTER(_VP)]),
+    ("hns_precond_refresh", C.c_int32, [_VP, _VP]),
+    ("hns_precond_destroy", C.c_int32, [C.POINTER(_VP)]),
+    ("hns_set_preconditioner", C.c_int32, [_VP, _VP]),
+    ("hnsk_precond_apply", C.c_int32, [_VP, _VP, _VP, _VP, C.c_int32, _VP]),
+    ("hnsk_precond_backward", C.c_int32, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    ("hnsk_precond_adam", C.c_int32, [C.c_int64, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float, C.c_float,
+                                      C.c_int32, _VP]),
+    ("hnsk_poisson_apply", C.c_int32, [_VP, _VP, _VP, C.c_float, C.c_float, _VP, _VP, _VP]),
+    ("hnsk_pressure_solve_learned", C.c_int32, [_VP, _VP, _VP, _VP, C.c_float, C.POINTER(hns_solve_options),
+                                                C.POINTER(hns_solve_stats), _VP]),
+    ("hnsk_diff_forward", C.c_int32, [_VP, _VP, _VP, C.c_int32, C.c_float, C.c_float, C.c_int32, C.c_float,
+                                      C.c_int32, _VP, _VP, C.POINTER(_VP), _VP]),
+    ("hnsk_diff_backward", C.c_int32, [_VP, _VP, _VP, _VP, _VP, _VP]),
+    ("hns_diff_tape_info", C.c_int32, [_VP, c_int32_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("hns_diff_tape_destroy", C.c_int32, [C.POINTER(_VP)]),
+    ("hnsk_advect_scalars_adj", C.c_int32, [_VP, _VP, _VP, _VP, _VP, _VP, C.c_float, C.c_float, _VP]),
+    ("hnsk_advect_vector_adj", C.c_int32, [_VP, _VP, _VP, _VP, C.c_float, C.c_float, _VP]),
+    ("hnsk_rbgs_adj", C.c_int32, [_VP, _VP, _VP, C.c_float, C.c_int32, C.c_float, _VP]),
 ]
 
 EXPORTS = [s[0] for s in _SIGS]

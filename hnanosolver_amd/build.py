@@ -1,7 +1,9 @@
-"""Builds libhns.so (HIP, gfx950) in-tree, and the CPU oracle used by the tests.
+"""Builds libhns.so (HIP, gfx950) in-tree, the hns_torch autograd extension over it, and the CPU
+oracle used by the tests.
 
-    python -m hnanosolver_amd.build            # library + oracle
+    python -m hnanosolver_amd.build            # library + torch extension + oracle
     python -m hnanosolver_amd.build --lib      # library only
+    python -m hnanosolver_amd.build --torch    # torch extension only (needs libhns.so)
 """
 from __future__ import annotations
 
@@ -18,7 +20,7 @@ LIB = os.path.join(PKG, "libhns.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 SOURCES = ["topology.hip", "kernels_step.hip", "kernels_advect.hip", "kernels_pressure.hip", "kernels_cg.hip", "solvers.hip",
-           "api.hip"]
+           "kernels_precond.hip", "precond.hip", "kernels_adjoint.hip", "adjoint.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HNS_OFFLOAD_ARCH", "gfx950")
@@ -57,6 +59,30 @@ def build_lib(verbose: bool = False) -> str:
     return LIB
 
 
+TORCH_EXT = os.path.join(PKG, "hns_torch.so")
+
+
+def build_torch_ext() -> str:
+    """hns_torch.so: the torch::autograd::Function of the differentiable step (host C++ against
+    LibTorch, linked to the in-tree libhns.so through an $ORIGIN rpath)."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+    inc = []
+    for d in ce.include_paths() + [sysconfig.get_paths()["include"]]:
+        inc += ["-isystem", d]
+    libdir = ce.library_paths()[0]
+    abi = "1" if torch.compiled_with_cxx11_abi() else "0"
+    tmp = TORCH_EXT + ".tmp"
+    _run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-shared", "-fPIC", "-w",
+          "-D_GLIBCXX_USE_CXX11_ABI=" + abi, "-DTORCH_EXTENSION_NAME=hns_torch", "-DTORCH_API_INCLUDE_EXTENSION_H",
+          *inc, os.path.join(CSRC, "torch_ext.cpp"), "-o", tmp, "-L" + libdir, "-ltorch", "-ltorch_cpu", "-lc10",
+          "-ltorch_python", "-L" + PKG, "-l:libhns.so", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath," + libdir])
+    os.replace(tmp, TORCH_EXT)
+    return TORCH_EXT
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", ORACLE_DIR])
     return os.path.join(ORACLE_DIR, "liboracle_hns.so")
@@ -66,12 +92,15 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--lib", action="store_true")
     ap.add_argument("--oracle", action="store_true")
+    ap.add_argument("--torch", action="store_true")
     ap.add_argument("-v", action="store_true")
     a = ap.parse_args(argv)
-    both = not (a.lib or a.oracle)
-    if a.lib or both:
+    every = not (a.lib or a.oracle or a.torch)
+    if a.lib or every:
         print("built", build_lib(a.v))
-    if a.oracle or both:
+    if a.torch or every:
+        print("built", build_torch_ext())
+    if a.oracle or every:
         print("built", build_oracle())
 
 

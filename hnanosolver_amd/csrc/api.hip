@@ -11,13 +11,23 @@
 #include <string>
 #include <vector>
 
+#include "hns_adjoint.hpp"
 #include "hns_internal.hpp"
+#include "hns_precond.hpp"
 #include "hns_solvers.hpp"
 
 using namespace hns;
 
 struct hns_topology {
   std::shared_ptr<TopoStore> s;
+};
+struct hns_precond {
+  int device = 0;
+  LearnedPrecond net;
+  PcWork work;
+};
+struct hns_diff_tape {
+  std::unique_ptr<DiffTape> t;
 };
 
 namespace {
@@ -107,6 +117,7 @@ struct hns_context {
   std::vector<DBuf<float>> advTmp;
   Solvers solvers;
   Timing timing;
+  hns_precond* pc = nullptr;  // not owned
 
   ~hns_context() {
     if (own) (void)hipStreamDestroy(own);
@@ -170,6 +181,10 @@ void advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, c
     req.check_every = opts->check_every;
   }
   req.want_residual = (stats != nullptr);
+  if (C->pc) {
+    req.pc = &C->pc->net;
+    req.pcw = &C->pc->work;
+  }
 
   for (int sub = 0; sub < SUBS; ++sub) {
     // 1) advect velocity u -> A
@@ -245,6 +260,7 @@ void advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, c
       stats->iterations = res.iterations;
       stats->rel_residual = res.rel_residual;
       stats->solve_ms = res.ms;
+      stats->fallback_iterations = res.fallback;
     }
     // 5) projection into the state velocity, then collision
     tm.begin(HNS_K_GRADIENT, s);
@@ -845,6 +861,7 @@ hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, fl
       stats->iterations = res.iterations;
       stats->rel_residual = res.rel_residual;
       stats->solve_ms = res.ms;
+      stats->fallback_iterations = res.fallback;
     }
     if (opts && opts->solver != HNS_SOLVER_RBGS_FIXED && !(res.rel_residual <= opts->tol)) {
       g_err = "pressure solve reached max_iters before tol";
@@ -933,6 +950,239 @@ hns_status hnsk_soa_to_aos(size_t n, const float* in, float* out, void* stream, 
     if ((!in || !out || in == out) && n) fail(HNS_ERR_INVALID_ARGUMENT, "in and out must be distinct arrays");
     launch_aos_soa(n, in, out, false, static_cast<hipStream_t>(stream));
     finish(stream, async);
+    return HNS_OK;
+  });
+}
+
+/* ---- learned preconditioner ------------------------------------------------------------- */
+
+int64_t hns_precond_num_params(void) { return pc_num_params(); }
+
+hns_status hns_precond_create(const float* params, int64_t n, hns_precond** out) {
+  return guard([&] {
+    if (!out) fail(HNS_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (!params || n != pc_num_params()) fail(HNS_ERR_INVALID_ARGUMENT, "expected hns_precond_num_params() weights");
+    require_device();
+    auto p = std::make_unique<hns_precond>();
+    HNS_HIP(hipGetDevice(&p->device));
+    p->net.load_host(params);
+    p->net.refresh(nullptr);
+    HNS_HIP(hipDeviceSynchronize());
+    *out = p.release();
+    return HNS_OK;
+  });
+}
+
+hns_status hns_precond_create_device(float* params_dev, int64_t n, hns_precond** out) {
+  return guard([&] {
+    if (!out) fail(HNS_ERR_INVALID_ARGUMENT, "null output handle");
+    *out = nullptr;
+    if (!params_dev || n != pc_num_params()) fail(HNS_ERR_INVALID_ARGUMENT, "expected hns_precond_num_params() weights");
+    require_device();
+    auto p = std::make_unique<hns_precond>();
+    HNS_HIP(hipGetDevice(&p->device));
+    p->net.set_device_params(params_dev);
+    p->net.refresh(nullptr);
+    HNS_HIP(hipDeviceSynchronize());
+    *out = p.release();
+    return HNS_OK;
+  });
+}
+
+hns_status hns_precond_refresh(hns_precond* p, void* stream) {
+  return guard([&] {
+    if (!p) fail(HNS_ERR_INVALID_ARGUMENT, "null preconditioner");
+    p->net.refresh(static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hns_precond_destroy(hns_precond** p) {
+  return guard([&] {
+    if (!p) fail(HNS_ERR_INVALID_ARGUMENT, "null handle pointer");
+    if (*p) {
+      (void)hipDeviceSynchronize();
+      delete *p;
+      *p = nullptr;
+    }
+    return HNS_OK;
+  });
+}
+
+hns_status hns_set_preconditioner(hns_context* ctx, hns_precond* p) {
+  return guard([&] {
+    if (!ctx) fail(HNS_ERR_NULL_CONTEXT, "null context");
+    ctx->pc = p;
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_precond_apply(const hns_topology* topo, hns_precond* p, const float* r_cs, float* z_cs, int32_t keep,
+                              void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!p || !r_cs || !z_cs) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    p->net.forward(T, p->work, r_cs, z_cs, keep != 0, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_precond_backward(const hns_topology* topo, hns_precond* p, const float* r_cs, const float* dz_cs,
+                                 float* grad, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!p || !r_cs || !dz_cs || !grad) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    p->net.backward(T, p->work, r_cs, dz_cs, grad, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_precond_adam(int64_t n, float* params, const float* grad, float* m, float* v, float lr, float beta1,
+                             float beta2, float eps, int32_t step, void* stream) {
+  return guard([&] {
+    if (n < 0 || !params || !grad || !m || !v || step < 1) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    launch_pc_adam(n, params, grad, m, v, lr, beta1, beta2, eps, step, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_poisson_apply(const hns_topology* topo, const float* x_cs, const float* y_cs, float a, float b,
+                              float* out_cs, double* work, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!x_cs || !y_cs || !out_cs || !work || out_cs == x_cs) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    launch_pc_poisson(T, x_cs, y_cs, a, b, out_cs, work + 4, work, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_pressure_solve_learned(const hns_topology* topo, hns_precond* p, const float* div_cs, float* p_cs,
+                                       float dx, const hns_solve_options* opts, hns_solve_stats* stats,
+                                       void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!p || !div_cs || !p_cs || !opts) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    if (opts->struct_size < sizeof(hns_solve_options))
+      fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_options.struct_size too small");
+    if (stats && stats->struct_size < sizeof(hns_solve_stats))
+      fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_stats.struct_size too small");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    Solvers sol;
+    sol.init(T.L, topo->s->N);
+    Timing tm;
+    SolveRequest req;
+    req.dx = dx;
+    req.omega = 2.0f / (1.0f + sinf(3.14159f * dx));  // the fallback RBGS, HNanoSolver.cu:365
+    req.want_residual = true;
+    req.solver = HNS_SOLVER_PCG_LEARNED;
+    req.tol = opts->tol;
+    req.max_iters = opts->max_iters;
+    req.pc = &p->net;
+    req.pcw = &p->work;
+    const SolveResult res = sol.solve(T, div_cs, p_cs, 0, req, tm, s);
+    HNS_HIP(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iterations = res.iterations;
+      stats->rel_residual = res.rel_residual;
+      stats->solve_ms = res.ms;
+      stats->fallback_iterations = res.fallback;
+    }
+    if (!(res.rel_residual <= opts->tol)) {
+      g_err = "pressure solve reached max_iters before tol";
+      return HNS_ERR_NOT_CONVERGED;
+    }
+    return HNS_OK;
+  });
+}
+
+/* ---- reverse mode ------------------------------------------------------------------------ */
+
+hns_status hnsk_diff_forward(const hns_topology* topo, const float* vel0, const float* rho0, int32_t steps, float dt,
+                             float dx, int32_t iterations, float omega, int32_t checkpoint_every, float* vel_out,
+                             float* rho_out, hns_diff_tape** tape, void* stream) {
+  return guard([&] {
+    if (!tape) fail(HNS_ERR_INVALID_ARGUMENT, "null tape pointer");
+    *tape = nullptr;
+    (void)topo_of(topo);
+    if (!vel0 || !rho0 || steps < 1 || iterations < 0 || !(dx > 0.f))
+      fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    auto t = std::make_unique<hns_diff_tape>();
+    t->t = std::make_unique<DiffTape>(topo->s, steps, dt, dx, iterations, omega, checkpoint_every);
+    t->t->forward(vel0, rho0, vel_out, rho_out, static_cast<hipStream_t>(stream));
+    *tape = t.release();
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_diff_backward(hns_diff_tape* tape, const float* gvel_out, const float* grho_out, float* gvel0,
+                              float* grho0, void* stream) {
+  return guard([&] {
+    if (!tape || !tape->t || !gvel0 || !grho0) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    tape->t->backward(gvel_out, grho_out, gvel0, grho0, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hns_diff_tape_info(const hns_diff_tape* tape, int32_t* checkpoints, int64_t* bytes,
+                              int64_t* recompute_mismatches) {
+  return guard([&] {
+    if (!tape || !tape->t) fail(HNS_ERR_INVALID_ARGUMENT, "null tape");
+    if (checkpoints) *checkpoints = tape->t->checkpoints();
+    if (bytes) *bytes = tape->t->bytes();
+    if (recompute_mismatches) *recompute_mismatches = tape->t->mismatches();
+    return HNS_OK;
+  });
+}
+
+hns_status hns_diff_tape_destroy(hns_diff_tape** tape) {
+  return guard([&] {
+    if (!tape) fail(HNS_ERR_INVALID_ARGUMENT, "null tape pointer");
+    if (*tape) {
+      (void)hipDeviceSynchronize();
+      delete *tape;
+      *tape = nullptr;
+    }
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_advect_scalars_adj(const hns_topology* topo, const float* vel, const float* in, const float* gout,
+                                   float* gin, float* gvel, float dt, float inv_dx, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!vel || !in || !gout || !gin || !gvel) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    const size_t N = topo->s->N;
+    DBuf<float> far(4 * N);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HNS_HIP(hipMemsetAsync(far.p, 0, 4 * N * sizeof(float), s));
+    launch_advect_scalars_adj(T, vel, in, gout, gin, gvel, dt, inv_dx, far.p, s);
+    HNS_HIP(hipStreamSynchronize(s));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_advect_vector_adj(const hns_topology* topo, const float* vel, const float* gout, float* gvel, float dt,
+                                  float inv_dx, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!vel || !gout || !gvel) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    const size_t N = topo->s->N;
+    DBuf<float> far(3 * N);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HNS_HIP(hipMemsetAsync(far.p, 0, 3 * N * sizeof(float), s));
+    launch_advect_vector_adj(T, vel, gout, gvel, dt, inv_dx, far.p, s);
+    HNS_HIP(hipStreamSynchronize(s));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_rbgs_adj(const hns_topology* topo, float* g_cs, float* gdiv_cs, float dx, int32_t iterations,
+                         float omega, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!g_cs || !gdiv_cs || iterations < 0) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    launch_rbgs_adj(T, g_cs, gdiv_cs, dx, iterations, omega, static_cast<hipStream_t>(stream));
     return HNS_OK;
   });
 }

@@ -30,9 +30,10 @@ constexpr int kResidualReplace = 64;
 
 // r = b = -div dx^2, x = 0; partials of ||b||^2; stop flags cleared.
 void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, const CgScalars& S, hipStream_t s);
-// r = b - A x (residual replacement); partials of r.r; stop flags cleared.
+// r = b - A x (residual replacement); partials of r.r. Periodic (force = false): skipped once
+// the solve has stopped. Forced (re-testing a stop): always runs and clears the stop flags.
 void launch_cg_replace(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
-                       hipStream_t s);
+                       bool force, hipStream_t s);
 // One CG iteration `it` in two launches. Product: d_out = r + beta d_in (beta = 0 when `fresh`),
 // q = A d_out. Update: x += alpha d_out, r -= alpha q. With z != nullptr it is a flexible
 // preconditioned iteration (z = M r in place and launch_cg_rz run on it): d_out = z + beta d_in.

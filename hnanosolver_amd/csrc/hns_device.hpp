@@ -68,6 +68,19 @@ __device__ __forceinline__ int32_t leaf_of_block() {
 }
 inline unsigned grid_for_leaves(int32_t L) { return 8u * (unsigned)((L + 7) / 8); }
 
+// Strided leaf run of a workgroup of a grid smaller than the leaf count (gridDim.x a multiple of
+// 8): the leaves are cut into 8 contiguous ranges, one per XCD, and the XCD's workgroups
+// interleave inside their range.
+struct LeafRun {
+  int32_t first, end, step;
+};
+__device__ inline LeafRun leaf_run(int32_t L) {
+  const int32_t b = (int32_t)blockIdx.x, W = (int32_t)(gridDim.x >> 3);
+  const int32_t per = (L + 7) / 8;
+  const int32_t lo = (b & 7) * per;
+  return LeafRun{lo + (b >> 3), min(L, lo + per), W};
+}
+
 // float -> int round-down (Stencils.hpp:28-30, __float2int_rd); far/NaN positions are clamped
 // to a coordinate outside every domain so they read as misses.
 __device__ __forceinline__ int32_t floor_i(float v) {

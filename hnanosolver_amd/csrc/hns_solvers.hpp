@@ -65,7 +65,12 @@ struct Timing {
   hipEvent_t a_ = nullptr;
 };
 
+class LearnedPrecond;
+struct PcWork;
+
 struct SolveRequest {
+  LearnedPrecond* pc = nullptr;  // HNS_SOLVER_PCG_LEARNED
+  PcWork* pcw = nullptr;
   int solver = HNS_SOLVER_RBGS_FIXED;
   float tol = 1e-5f;
   int max_iters = 100000;
@@ -76,6 +81,7 @@ struct SolveRequest {
 };
 struct SolveResult {
   int iterations = 0;
+  int fallback = 0;  // RBGS iterations after a learned PCG that missed tol
   double rel_residual = -1.0;
   double ms = 0.0;
 };

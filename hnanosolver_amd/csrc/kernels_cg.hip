@@ -27,18 +27,6 @@ namespace {
 
 constexpr int kTpb = 256;
 
-// Strided leaf run of this workgroup: the leaves are cut into 8 contiguous ranges, one per XCD
-// (workgroup b runs on XCD b % 8), and the XCD's workgroups interleave inside their range.
-struct LeafRun {
-  int32_t first, end, step;
-};
-__device__ inline LeafRun leaf_run(int32_t L) {
-  const int32_t b = (int32_t)blockIdx.x, W = (int32_t)(gridDim.x >> 3);
-  const int32_t per = (L + 7) / 8;
-  const int32_t lo = (b & 7) * per;
-  return LeafRun{lo + (b >> 3), min(L, lo + per), W};
-}
-
 // Sum of a 256-lane value over the workgroup; the total is returned to every lane.
 __device__ inline double block_sum_all(double v, double* s_red) {
 #pragma unroll
@@ -87,32 +75,35 @@ __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __rest
   clear_flags(S);
 }
 
-// Residual replacement: r = b - A x; partials of r.r into pa slot kSlotRR; stop flags cleared.
+// Residual replacement: r = b - A x (evaluated in FP64, stored FP32); partials of r.r into pa
+// slot kSlotRR. A periodic replacement (force == 0) is skipped once the solve has stopped; a
+// forced one (re-testing a stop) runs and clears the stop flags.
 __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __restrict__ div, float dx2,
                                                      const float* __restrict__ x, float* __restrict__ r,
-                                                     CgScalars S) {
+                                                     CgScalars S, int force) {
   __shared__ double s_red[kTpb / 64];
+  if (!force && S.flag[0]) return;
   const LeafRun R = leaf_run(T.L);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const int c0 = ((t >> 5) + (t >> 2)) & 1;
-  auto get = [&](int c, int32_t i) { return x[c * half + i]; };
+  auto get = [&](int c, int32_t i) { return (double)x[c * half + i]; };
   double acc = 0.0;
   for (int32_t l = R.first; l < R.end; l += R.step) {
     const FaceRow Fr = face_row(T, l);
     const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
-    const float x0 = x[i0], x1 = x[i1];
-    float s0, s1;
+    const double x0 = (double)x[i0], x1 = (double)x[i1];
+    double s0, s1;
     pair_face_sums(Fr, t, get, x0, x1, s0, s1);
-    const float r0 = -div[i0] * dx2 - (6.0f * x0 - s0);
-    const float r1 = -div[i1] * dx2 - (6.0f * x1 - s1);
+    const float r0 = (float)(-(double)div[i0] * (double)dx2 - (6.0 * x0 - s0));
+    const float r1 = (float)(-(double)div[i1] * (double)dx2 - (6.0 * x1 - s1));
     r[i0] = r0;
     r[i1] = r1;
     acc += (double)r0 * (double)r0 + (double)r1 * (double)r1;
   }
   const double tt = block_sum_all(acc, s_red);
   if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = tt;
-  clear_flags(S);
+  if (force) clear_flags(S);
 }
 
 // Iteration `it` (0-based), product half: reduce r.r (and r.z, q_{k-1}.z) from pa; stop when
@@ -248,8 +239,8 @@ void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, c
   HNS_HIP(hipGetLastError());
 }
 void launch_cg_replace(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
-                       hipStream_t s) {
-  k_cg_replace<<<cg_groups(T.L), kTpb, 0, s>>>(T, div, dx * dx, x, r, S);
+                       bool force, hipStream_t s) {
+  k_cg_replace<<<cg_groups(T.L), kTpb, 0, s>>>(T, div, dx * dx, x, r, S, force ? 1 : 0);
   HNS_HIP(hipGetLastError());
 }
 void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* z, const float* d_in,

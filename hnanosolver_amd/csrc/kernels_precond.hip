@@ -1,0 +1,723 @@
+// Kernels of the learned pressure preconditioner (no reference counterpart; the reference runs a
+// fixed red-black Gauss-Seidel count, src/Cuda/HNanoSolver.cu:364-373).
+//
+// Convolutions are implicit GEMMs on gfx950 MFMA (v_mfma_f32_16x16x32_bf16, BF16 operands, FP32
+// accumulation), written transposed so the output is channel-contiguous per lane:
+//   D[cout][cell] = sum_k W^T[cout][k] X[k][cell],  k = (tap, cin) flattened tap-major,
+// one MFMA per 16 output channels x 16 cells x 32 k. The B operand of lane l is 8 consecutive
+// input channels (16 B) of the neighbour of cell (l & 15) at the lane group's tap, gathered
+// straight from the level's channel-contiguous bf16 tensor through the leaf neighbour table; the
+// A operand is the weight tile, pre-swizzled so each lane loads its 16 B contiguously. Lane l
+// ends with channels 4*(l>>4) .. +3 of cell (l & 15), stored as one 8 B write.
+// Backward-data is the same kernel on the mirrored, transposed weights; weight gradients are
+// MFMAs over cells (k = cell) on tiles staged and transposed through LDS, one workgroup per
+// (tap, cell range), partials reduced in a fixed order.
+#include <algorithm>
+#include <cmath>
+
+#include "hns_cg.hpp"
+#include "hns_precond.hpp"
+
+namespace hns {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+const PcLayer kPcNet[kPcLayers] = {{"c0", 27, 1, 16, 0},  {"c1", 27, 16, 32, 1}, {"c2", 27, 32, 64, 2},
+                                   {"c3", 27, 64, 64, 2}, {"u2", 1, 64, 32, 2},  {"c5", 27, 32, 32, 1},
+                                   {"u1", 1, 32, 16, 1},  {"c6", 27, 16, 16, 0}, {"out", 1, 16, 1, 0}};
+
+namespace {
+inline int64_t swz_size(int taps, int cin, int cout) {
+  if (cout < 16 || cin == 0) return 0;
+  const int64_t nk = (taps * cin + 31) / 32;
+  return nk * (cout / 16) * 64 * 8;
+}
+}  // namespace
+
+PcOffsets::PcOffsets() {
+  param[0] = fwd[0] = bwd[0] = 0;
+  for (int i = 0; i < kPcLayers; ++i) {
+    const PcLayer& Ly = kPcNet[i];
+    param[i + 1] = param[i] + (int64_t)Ly.taps * Ly.cin * Ly.cout;
+    fwd[i + 1] = fwd[i] + swz_size(Ly.taps, Ly.cin, Ly.cout);
+    // backward data is not needed for the input layer (r has no gradient) nor the fused output
+    bwd[i + 1] = bwd[i] + ((i == kC0 || i == kOut) ? 0 : swz_size(Ly.taps, Ly.cout, Ly.cin));
+  }
+}
+const PcOffsets& pc_offsets() {
+  static const PcOffsets o;
+  return o;
+}
+
+namespace {
+
+constexpr int kTpb = 256;
+
+__device__ __forceinline__ floatx4 mfma(const bf16x8& a, const bf16x8& b, const floatx4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 zero8() {
+  bf16x8 z;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) z[v] = (__bf16)0.0f;
+  return z;
+}
+
+// Neighbour of level cell `c` (coordinates leaf, cx, cy, cz at edge 2^lgS) at offset (dx,dy,dz)
+// in {-1,0,1}^3; -1 when the neighbour's leaf is absent.
+__device__ __forceinline__ int32_t nbr_cell(const DevTopo& T, int lgS, int32_t leaf, int cx, int cy, int cz, int dx,
+                                            int dy, int dz) {
+  const int Sm = (1 << lgS) - 1;
+  int nx = cx + dx, ny = cy + dy, nz = cz + dz;
+  const int ox = nx >> lgS, oy = ny >> lgS, oz = nz >> lgS;  // -1, 0 or 1
+  int32_t nl = leaf;
+  if (ox | oy | oz) nl = T.nbr[kNbrStride * leaf + (ox + 1) * 9 + (oy + 1) * 3 + (oz + 1)];
+  if (nl < 0) return -1;
+  nx &= Sm;
+  ny &= Sm;
+  nz &= Sm;
+  return (nl << (3 * lgS)) | (nx << (2 * lgS)) | (ny << lgS) | nz;
+}
+
+struct CellPos {
+  int32_t c, leaf;
+  int x, y, z;
+  bool ok;
+};
+__device__ __forceinline__ CellPos cell_pos(int32_t c, int32_t ncell, int lgS) {
+  CellPos p;
+  p.c = c;
+  p.ok = c < ncell;
+  p.leaf = c >> (3 * lgS);
+  const int32_t loc = c & ((1 << (3 * lgS)) - 1);
+  p.x = loc >> (2 * lgS);
+  p.y = (loc >> lgS) & ((1 << lgS) - 1);
+  p.z = loc & ((1 << lgS) - 1);
+  return p;
+}
+
+// ---- weight swizzle: FP32 master W[tap][cin][cout] -> per-lane MFMA A operands -------------
+// TRANS: the operand of the transposed conv with mirrored taps (backward data).
+__global__ void k_pc_swizzle(const float* __restrict__ W, __bf16* __restrict__ dst, int taps, int cin, int cout,
+                             int trans, int64_t count) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  const int Ci = trans ? cout : cin, Co = trans ? cin : cout;
+  const int MT = Co / 16;
+  const int v = (int)(e & 7), l = (int)((e >> 3) & 63);
+  const int64_t tile = e >> 9;
+  const int mt = (int)(tile % MT), kc = (int)(tile / MT);
+  const int kb = 32 * kc + 8 * (l >> 4);
+  int tap, ci;
+  if (Ci == 1) {
+    tap = kb + v;
+    ci = 0;
+  } else {
+    tap = kb / Ci;
+    ci = kb % Ci + v;
+  }
+  const int co = 16 * mt + (l & 15);
+  float w = 0.f;
+  if (tap < taps) {
+    if (trans) {
+      const int t0 = taps == 27 ? 26 - tap : 0;
+      w = W[((int64_t)t0 * cin + co) * cout + ci];
+    } else {
+      w = W[((int64_t)tap * cin + ci) * cout + co];
+    }
+  }
+  dst[e] = (__bf16)w;
+}
+
+// ---- input layer: h0 = relu(conv3(r; c0)), r FP32 colour-split, K = 27 taps padded to 32 ------
+template <int NT>
+__global__ __launch_bounds__(kTpb) void k_pc_input(DevTopo T, const float* __restrict__ r,
+                                                   const bf16x8* __restrict__ w, __bf16* __restrict__ h0) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int32_t ncell = T.L * kLeafVox;
+  const int32_t half = T.L * kHalfLeaf;
+  const int32_t cell0 = ((int32_t)blockIdx.x * 4 + wave) * (16 * NT);
+  if (cell0 >= ncell) return;
+  const bf16x8 a = w[lane];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const CellPos P = cell_pos(cell0 + 16 * t + j, ncell, 3);
+    bf16x8 b = zero8();
+    if (P.ok) {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int tap = 8 * g + v;
+        if (tap < 27) {
+          const int32_t nc = nbr_cell(T, 3, P.leaf, P.x, P.y, P.z, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1);
+          if (nc >= 0) {
+            const int32_t nl = nc >> 9, x = (nc >> 6) & 7, y = (nc >> 3) & 7, z = nc & 7;
+            b[v] = (__bf16)r[((x + y + z) & 1) * half + nl * kHalfLeaf + cs_local(x, y, z)];
+          }
+        }
+      }
+    }
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(a, b, acc);
+    if (P.ok) {
+      bf16x4 o;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[v], 0.f);
+      *reinterpret_cast<bf16x4*>(h0 + (int64_t)P.c * 16 + 4 * g) = o;
+    }
+  }
+}
+
+// ---- generic sparse conv (3x3x3 or 1x1) on a level -------------------------------------------
+// ACT 0: none; 1: relu; 2: keep where mask > 0 (relu backward); 3: relu, then the fused 16 -> 1
+// output projection into z (FP32 colour-split) with the relu'd values optionally kept in y.
+template <int TAPS, int CIN, int COUT, int ACT, int NT>
+__global__ __launch_bounds__(kTpb) void k_pc_conv(DevTopo T, int lgS, const __bf16* __restrict__ x,
+                                                  const bf16x8* __restrict__ w, __bf16* __restrict__ y,
+                                                  const __bf16* __restrict__ mask, const float* __restrict__ wout,
+                                                  float* __restrict__ z) {
+  constexpr int NK = (TAPS * CIN + 31) / 32, MT = COUT / 16;
+  static_assert(CIN % 8 == 0 && COUT % 16 == 0, "channel counts");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int32_t ncell = T.L << (3 * lgS);
+  const int32_t cell0 = ((int32_t)blockIdx.x * 4 + wave) * (16 * NT);
+  if (cell0 >= ncell) return;  // wave-uniform; the kernel has no barrier
+  CellPos P[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) P[t] = cell_pos(cell0 + 16 * t + j, ncell, lgS);
+  floatx4 acc[NT][MT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[t][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int kc = 0; kc < NK; ++kc) {
+    const int kb = 32 * kc + 8 * g;
+    const int tap = kb / CIN, ci0 = kb % CIN;
+    bf16x8 b[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      b[t] = zero8();
+      if (P[t].ok && tap < TAPS) {
+        const int32_t nc = TAPS == 1 ? P[t].c
+                                     : nbr_cell(T, lgS, P[t].leaf, P[t].x, P[t].y, P[t].z, tap / 9 - 1,
+                                                (tap / 3) % 3 - 1, tap % 3 - 1);
+        if (nc >= 0) b[t] = *reinterpret_cast<const bf16x8*>(x + (int64_t)nc * CIN + ci0);
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const bf16x8 a = w[(kc * MT + m) * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t][m] = mfma(a, b[t], acc[t][m]);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    if (ACT == 3) {
+      // relu, then z = sum_co w_out[co] h[co]: 4 channels per lane, summed over the 4 lane groups
+      float part = 0.f;
+      bf16x4 o;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        o[v] = (__bf16)fmaxf(acc[t][0][v], 0.f);
+        part += (float)o[v] * (float)(__bf16)wout[4 * g + v];
+      }
+      if (y && P[t].ok) *reinterpret_cast<bf16x4*>(y + (int64_t)P[t].c * 16 + 4 * g) = o;
+      part += __shfl_xor(part, 16, 64);
+      part += __shfl_xor(part, 32, 64);
+      if (g == 0 && P[t].ok) {
+        const int32_t half = T.L * kHalfLeaf;
+        z[((P[t].x + P[t].y + P[t].z) & 1) * half + P[t].leaf * kHalfLeaf + cs_local(P[t].x, P[t].y, P[t].z)] = part;
+      }
+      continue;
+    }
+    if (!P[t].ok) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int64_t o0 = (int64_t)P[t].c * COUT + 16 * m + 4 * g;
+      bf16x4 o;
+      if (ACT == 2) {
+        const bf16x4 mk = *reinterpret_cast<const bf16x4*>(mask + o0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (__bf16)((float)mk[v] > 0.f ? acc[t][m][v] : 0.f);
+      } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (__bf16)(ACT == 1 ? fmaxf(acc[t][m][v], 0.f) : acc[t][m][v]);
+      }
+      *reinterpret_cast<bf16x4*>(y + o0) = o;
+    }
+  }
+}
+
+// ---- pooling (children sum times scale) and upsample-add -------------------------------------
+__global__ void k_pc_pool(int32_t L, int lgSc, int c8, const __bf16* __restrict__ x, __bf16* __restrict__ y,
+                          float scale) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ncell = (int64_t)L << (3 * lgSc);
+  if (e >= ncell * c8) return;
+  const int32_t cc = (int32_t)(e / c8);
+  const int part = (int)(e % c8);
+  const CellPos P = cell_pos(cc, (int32_t)ncell, lgSc);
+  const int lgSf = lgSc + 1, C = 8 * c8;
+  float s[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int32_t fc = (P.leaf << (3 * lgSf)) | ((2 * P.x + (k >> 2)) << (2 * lgSf)) |
+                       ((2 * P.y + ((k >> 1) & 1)) << lgSf) | (2 * P.z + (k & 1));
+    const bf16x8 v = *reinterpret_cast<const bf16x8*>(x + (int64_t)fc * C + 8 * part);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s[q] += (float)v[q];
+  }
+  bf16x8 o;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) o[q] = (__bf16)(s[q] * scale);
+  *reinterpret_cast<bf16x8*>(y + (int64_t)cc * C + 8 * part) = o;
+}
+
+template <bool MASK>
+__global__ void k_pc_upadd(int32_t L, int lgSf, int c8, const __bf16* __restrict__ a, const __bf16* __restrict__ b,
+                           __bf16* __restrict__ y, float scale, const __bf16* __restrict__ mask) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t ncell = (int64_t)L << (3 * lgSf);
+  if (e >= ncell * c8) return;
+  const int32_t fc = (int32_t)(e / c8);
+  const int part = (int)(e % c8);
+  const CellPos P = cell_pos(fc, (int32_t)ncell, lgSf);
+  const int lgSc = lgSf - 1, C = 8 * c8;
+  const int32_t pc = (P.leaf << (3 * lgSc)) | ((P.x >> 1) << (2 * lgSc)) | ((P.y >> 1) << lgSc) | (P.z >> 1);
+  const int64_t o0 = (int64_t)fc * C + 8 * part;
+  const bf16x8 va = *reinterpret_cast<const bf16x8*>(a + o0);
+  const bf16x8 vb = *reinterpret_cast<const bf16x8*>(b + (int64_t)pc * C + 8 * part);
+  bf16x8 mk;
+  if (MASK) mk = *reinterpret_cast<const bf16x8*>(mask + o0);
+  bf16x8 o;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    float v = (float)va[q] + scale * (float)vb[q];
+    if (MASK && !((float)mk[q] > 0.f)) v = 0.f;
+    o[q] = (__bf16)v;
+  }
+  *reinterpret_cast<bf16x8*>(y + o0) = o;
+}
+
+// ---- output backward: dh6 = w_out * dz where h6 > 0 ------------------------------------------
+__global__ void k_pc_output_bwd(int32_t L, const float* __restrict__ dz, const __bf16* __restrict__ h6,
+                                const float* __restrict__ wout, __bf16* __restrict__ dh) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t ncell = L * kLeafVox;
+  if (e >= (int64_t)ncell * 2) return;
+  const int32_t c = (int32_t)(e >> 1);
+  const int part = (int)(e & 1);
+  const CellPos P = cell_pos(c, ncell, 3);
+  const float d = dz[((P.x + P.y + P.z) & 1) * (L * kHalfLeaf) + P.leaf * kHalfLeaf + cs_local(P.x, P.y, P.z)];
+  const int64_t o0 = (int64_t)c * 16 + 8 * part;
+  const bf16x8 h = *reinterpret_cast<const bf16x8*>(h6 + o0);
+  bf16x8 o;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) o[q] = (__bf16)((float)h[q] > 0.f ? (float)(__bf16)wout[8 * part + q] * d : 0.f);
+  *reinterpret_cast<bf16x8*>(dh + o0) = o;
+}
+
+// ---- weight gradients ------------------------------------------------------------------------
+// Workgroup (tap, range): partial[range][tap][ci][co] = sum over the range's cells of
+// x[nbr(c, tap)][ci] dy[c][co]. 64-cell blocks of x (gathered) and dy are staged in LDS; MFMA
+// operands take 8 consecutive cells of one channel (k = cell).
+constexpr int kWgBlock = 64;
+template <int TAPS, int CIN, int COUT>
+__global__ __launch_bounds__(kTpb) void k_pc_wgrad(DevTopo T, int lgS, int R, const __bf16* __restrict__ x,
+                                                   const __bf16* __restrict__ dy, float* __restrict__ partial) {
+  constexpr int XS = CIN + 8, YS = COUT + 8;  // padded LDS rows
+  constexpr int NI = CIN / 16, NJ = COUT / 16, NTILE = NI * NJ, TPW = (NTILE + 3) / 4;
+  __shared__ __bf16 sx[kWgBlock * XS];
+  __shared__ __bf16 sy[kWgBlock * YS];
+  const int tap = (int)(blockIdx.x % TAPS), rg = (int)(blockIdx.x / TAPS);
+  const int dxx = TAPS == 1 ? 0 : tap / 9 - 1, dyy = TAPS == 1 ? 0 : (tap / 3) % 3 - 1,
+            dzz = TAPS == 1 ? 0 : tap % 3 - 1;
+  const int32_t ncell = T.L << (3 * lgS);
+  const int32_t per = ((ncell + R - 1) / R + kWgBlock - 1) / kWgBlock * kWgBlock;
+  const int32_t cb0 = rg * per, ce = min(ncell, cb0 + per);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  floatx4 acc[TPW];
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int32_t cb = cb0; cb < ce; cb += kWgBlock) {
+    for (int e = threadIdx.x; e < kWgBlock * (CIN / 8); e += kTpb) {
+      const int row = e / (CIN / 8), part = e % (CIN / 8);
+      const int32_t c = cb + row;
+      bf16x8 v = zero8();
+      if (c < ce) {
+        const CellPos P = cell_pos(c, ncell, lgS);
+        const int32_t nc = TAPS == 1 ? c : nbr_cell(T, lgS, P.leaf, P.x, P.y, P.z, dxx, dyy, dzz);
+        if (nc >= 0) v = *reinterpret_cast<const bf16x8*>(x + (int64_t)nc * CIN + 8 * part);
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sx[row * XS + 8 * part + q] = v[q];
+    }
+    for (int e = threadIdx.x; e < kWgBlock * (COUT / 8); e += kTpb) {
+      const int row = e / (COUT / 8), part = e % (COUT / 8);
+      const int32_t c = cb + row;
+      bf16x8 v = zero8();
+      if (c < ce) v = *reinterpret_cast<const bf16x8*>(dy + (int64_t)c * COUT + 8 * part);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) sy[row * YS + 8 * part + q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kWgBlock / 32; ++kk) {
+#pragma unroll
+      for (int q = 0; q < TPW; ++q) {
+        const int tile = wave + 4 * q;
+        if (tile < NTILE) {
+          const int it = tile / NJ, jt = tile % NJ;
+          bf16x8 a, b;
+#pragma unroll
+          for (int v = 0; v < 8; ++v) {
+            const int row = 32 * kk + 8 * g + v;
+            a[v] = sx[row * XS + 16 * it + j];
+            b[v] = sy[row * YS + 16 * jt + j];
+          }
+          acc[q] = mfma(a, b, acc[q]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  float* out = partial + ((int64_t)rg * TAPS + tap) * CIN * COUT;
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const int tile = wave + 4 * q;
+    if (tile < NTILE) {
+      const int it = tile / NJ, jt = tile % NJ;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[(16 * it + 4 * g + v) * COUT + 16 * jt + j] = acc[q][v];
+    }
+  }
+}
+
+// Input layer (cin = 1, FP32 source rounded to bf16 as in the forward): VALU, 16 accumulators.
+__global__ __launch_bounds__(kTpb) void k_pc_wgrad_input(DevTopo T, int R, const float* __restrict__ r,
+                                                         const __bf16* __restrict__ dh0,
+                                                         float* __restrict__ partial) {
+  __shared__ float s_red[4][16];
+  const int tap = (int)(blockIdx.x % 27), rg = (int)(blockIdx.x / 27);
+  const int32_t ncell = T.L * kLeafVox, half = T.L * kHalfLeaf;
+  const int32_t per = (ncell + R - 1) / R;
+  const int32_t cb0 = rg * per, ce = min(ncell, cb0 + per);
+  float acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+  for (int32_t c = cb0 + (int32_t)threadIdx.x; c < ce; c += kTpb) {
+    const CellPos P = cell_pos(c, ncell, 3);
+    const int32_t nc = nbr_cell(T, 3, P.leaf, P.x, P.y, P.z, tap / 9 - 1, (tap / 3) % 3 - 1, tap % 3 - 1);
+    if (nc < 0) continue;
+    const int32_t nl = nc >> 9, x = (nc >> 6) & 7, y = (nc >> 3) & 7, z = nc & 7;
+    const float rv = (float)(__bf16)r[((x + y + z) & 1) * half + nl * kHalfLeaf + cs_local(x, y, z)];
+    const bf16x8 d0 = *reinterpret_cast<const bf16x8*>(dh0 + (int64_t)c * 16);
+    const bf16x8 d1 = *reinterpret_cast<const bf16x8*>(dh0 + (int64_t)c * 16 + 8);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      acc[q] += rv * (float)d0[q];
+      acc[8 + q] += rv * (float)d1[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    float v = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int q = threadIdx.x;
+    partial[((int64_t)rg * 27 + tap) * 16 + q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  }
+}
+
+// Output projection: partial[range][co] = sum_c h6[c][co] dz[c].
+__global__ __launch_bounds__(kTpb) void k_pc_wgrad_output(DevTopo T, int R, const float* __restrict__ dz,
+                                                          const __bf16* __restrict__ h6,
+                                                          float* __restrict__ partial) {
+  __shared__ float s_red[4][16];
+  const int rg = (int)blockIdx.x;
+  const int32_t ncell = T.L * kLeafVox, half = T.L * kHalfLeaf;
+  const int32_t per = (ncell + R - 1) / R;
+  const int32_t cb0 = rg * per, ce = min(ncell, cb0 + per);
+  float acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = 0.f;
+  for (int32_t c = cb0 + (int32_t)threadIdx.x; c < ce; c += kTpb) {
+    const CellPos P = cell_pos(c, ncell, 3);
+    const float d = dz[((P.x + P.y + P.z) & 1) * half + P.leaf * kHalfLeaf + cs_local(P.x, P.y, P.z)];
+    const bf16x8 h0 = *reinterpret_cast<const bf16x8*>(h6 + (int64_t)c * 16);
+    const bf16x8 h1 = *reinterpret_cast<const bf16x8*>(h6 + (int64_t)c * 16 + 8);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      acc[q] += d * (float)h0[q];
+      acc[8 + q] += d * (float)h1[q];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    float v = acc[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    const int q = threadIdx.x;
+    partial[(int64_t)rg * 16 + q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  }
+}
+
+// grad[e] += sum_r partial[r][e], fixed order
+__global__ void k_pc_reduce(int64_t n, int R, const float* __restrict__ partial, float* __restrict__ grad) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = 0.f;
+  for (int r = 0; r < R; ++r) s += partial[(int64_t)r * n + e];
+  grad[e] += s;
+}
+
+__global__ void k_pc_adam(int64_t n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                          float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float gv = g[e];
+  const float mv = b1 * m[e] + (1.f - b1) * gv;
+  const float vv = b2 * v[e] + (1.f - b2) * gv * gv;
+  m[e] = mv;
+  v[e] = vv;
+  p[e] -= lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// out = a*y + b*(A x) on colour-split vectors; per-workgroup partials of out.out, y.y, out.x
+__device__ inline double block_sum_256(double v, double* s_red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double t = s_red[0] + s_red[1] + s_red[2] + s_red[3];
+  __syncthreads();
+  return t;
+}
+__global__ __launch_bounds__(kTpb) void k_pc_poisson(DevTopo T, const float* __restrict__ x,
+                                                     const float* __restrict__ y, float a, float b,
+                                                     float* __restrict__ out, double* __restrict__ part) {
+  __shared__ double s_red[4];
+  const LeafRun R = leaf_run(T.L);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  auto get = [&](int c, int32_t i) { return x[c * half + i]; };
+  double oo = 0.0, yy = 0.0, ox = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+    const FaceRow Fr = face_row(T, l);
+    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
+    const float x0 = x[i0], x1 = x[i1];
+    float s0, s1;
+    pair_face_sums(Fr, t, get, x0, x1, s0, s1);
+    const float y0 = y[i0], y1 = y[i1];
+    const float o0 = a * y0 + b * (6.0f * x0 - s0), o1 = a * y1 + b * (6.0f * x1 - s1);
+    out[i0] = o0;
+    out[i1] = o1;
+    oo += (double)o0 * o0 + (double)o1 * o1;
+    yy += (double)y0 * y0 + (double)y1 * y1;
+    ox += (double)o0 * x0 + (double)o1 * x1;
+  }
+  const double to = block_sum_256(oo, s_red);
+  const double ty = block_sum_256(yy, s_red);
+  const double tx = block_sum_256(ox, s_red);
+  if (threadIdx.x == 0) {
+    part[3 * blockIdx.x] = to;
+    part[3 * blockIdx.x + 1] = ty;
+    part[3 * blockIdx.x + 2] = tx;
+  }
+}
+__global__ __launch_bounds__(kTpb) void k_pc_sum3(const double* __restrict__ part, int G, double* __restrict__ sums) {
+  __shared__ double s_red[4];
+  double a = 0.0, b = 0.0, c = 0.0;
+  for (int i = threadIdx.x; i < G; i += kTpb) {
+    a += part[3 * i];
+    b += part[3 * i + 1];
+    c += part[3 * i + 2];
+  }
+  const double ta = block_sum_256(a, s_red);
+  const double tb = block_sum_256(b, s_red);
+  const double tc = block_sum_256(c, s_red);
+  if (threadIdx.x == 0) {
+    sums[0] = ta;
+    sums[1] = tb;
+    sums[2] = tc;
+  }
+}
+
+inline unsigned blocks_for(int64_t n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
+constexpr int kNT = 2;  // 16-cell tiles per wave
+
+// number of cell ranges of a weight-gradient launch
+inline int wgrad_ranges(int64_t ncell) {
+  const int64_t r = (ncell + 2047) / 2048;
+  return (int)std::max<int64_t>(1, std::min<int64_t>(64, r));
+}
+
+}  // namespace
+
+void launch_pc_swizzle(const float* params, __bf16* wfwd, __bf16* wbwd, hipStream_t s) {
+  const PcOffsets& O = pc_offsets();
+  for (int i = 0; i < kPcLayers; ++i) {
+    const PcLayer& Ly = kPcNet[i];
+    const int64_t nf = O.fwd[i + 1] - O.fwd[i], nb = O.bwd[i + 1] - O.bwd[i];
+    if (nf > 0) {
+      k_pc_swizzle<<<blocks_for(nf, 256), 256, 0, s>>>(params + O.param[i], wfwd + O.fwd[i], Ly.taps, Ly.cin, Ly.cout,
+                                                      0, nf);
+      HNS_HIP(hipGetLastError());
+    }
+    if (nb > 0) {
+      k_pc_swizzle<<<blocks_for(nb, 256), 256, 0, s>>>(params + O.param[i], wbwd + O.bwd[i], Ly.taps, Ly.cin, Ly.cout,
+                                                      1, nb);
+      HNS_HIP(hipGetLastError());
+    }
+  }
+}
+
+void launch_pc_input(const DevTopo& T, const float* r_cs, const __bf16* w, __bf16* h0, hipStream_t s) {
+  const int64_t ncell = (int64_t)T.L * kLeafVox;
+  k_pc_input<kNT><<<blocks_for(ncell, 64 * kNT), kTpb, 0, s>>>(T, r_cs, reinterpret_cast<const bf16x8*>(w), h0);
+  HNS_HIP(hipGetLastError());
+}
+
+#define HNS_PC_CONV(TAPS, CIN, COUT)                                                                     \
+  if (taps == TAPS && cin == CIN && cout == COUT) {                                                     \
+    if (act == 0)                                                                                       \
+      k_pc_conv<TAPS, CIN, COUT, 0, kNT><<<g, kTpb, 0, s>>>(T, lgS, x, wv, y, mask, nullptr, nullptr);  \
+    else if (act == 1)                                                                                  \
+      k_pc_conv<TAPS, CIN, COUT, 1, kNT><<<g, kTpb, 0, s>>>(T, lgS, x, wv, y, mask, nullptr, nullptr);  \
+    else                                                                                                \
+      k_pc_conv<TAPS, CIN, COUT, 2, kNT><<<g, kTpb, 0, s>>>(T, lgS, x, wv, y, mask, nullptr, nullptr);  \
+    HNS_HIP(hipGetLastError());                                                                         \
+    return;                                                                                             \
+  }
+
+void launch_pc_conv(const DevTopo& T, int level, int taps, int cin, int cout, const __bf16* x, const __bf16* w,
+                    __bf16* y, int act, const __bf16* mask, hipStream_t s) {
+  const int lgS = 3 - level;
+  const int64_t ncell = (int64_t)T.L << (3 * lgS);
+  const unsigned g = blocks_for(ncell, 64 * kNT);
+  const bf16x8* wv = reinterpret_cast<const bf16x8*>(w);
+  // forward layers and their backward-data transposes
+  HNS_PC_CONV(27, 16, 32)
+  HNS_PC_CONV(27, 32, 64)
+  HNS_PC_CONV(27, 64, 64)
+  HNS_PC_CONV(1, 64, 32)
+  HNS_PC_CONV(27, 32, 32)
+  HNS_PC_CONV(1, 32, 16)
+  HNS_PC_CONV(27, 16, 16)
+  HNS_PC_CONV(27, 32, 16)
+  HNS_PC_CONV(27, 64, 32)
+  HNS_PC_CONV(1, 32, 64)
+  HNS_PC_CONV(1, 16, 32)
+  fail(HNS_ERR_INVALID_ARGUMENT, "no preconditioner conv kernel for this shape");
+}
+#undef HNS_PC_CONV
+
+void launch_pc_output(const DevTopo& T, const __bf16* x, const __bf16* w, const float* params_out, float* z_cs,
+                      __bf16* h, hipStream_t s) {
+  const int64_t ncell = (int64_t)T.L * kLeafVox;
+  k_pc_conv<27, 16, 16, 3, kNT><<<blocks_for(ncell, 64 * kNT), kTpb, 0, s>>>(
+      T, 3, x, reinterpret_cast<const bf16x8*>(w), h, nullptr, params_out, z_cs);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_pool(int32_t L, int level, int c, const __bf16* x, __bf16* y, float scale, hipStream_t s) {
+  const int lgSc = 3 - (level + 1);
+  const int64_t n = ((int64_t)L << (3 * lgSc)) * (c / 8);
+  k_pc_pool<<<blocks_for(n, 256), 256, 0, s>>>(L, lgSc, c / 8, x, y, scale);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_upadd(int32_t L, int level, int c, const __bf16* a, const __bf16* b, __bf16* y, float scale,
+                     const __bf16* mask, hipStream_t s) {
+  const int lgSf = 3 - level;
+  const int64_t n = ((int64_t)L << (3 * lgSf)) * (c / 8);
+  if (mask)
+    k_pc_upadd<true><<<blocks_for(n, 256), 256, 0, s>>>(L, lgSf, c / 8, a, b, y, scale, mask);
+  else
+    k_pc_upadd<false><<<blocks_for(n, 256), 256, 0, s>>>(L, lgSf, c / 8, a, b, y, scale, nullptr);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_output_bwd(const DevTopo& T, const float* dz_cs, const __bf16* h6, const float* params_out,
+                          __bf16* dh, hipStream_t s) {
+  const int64_t n = (int64_t)T.L * kLeafVox * 2;
+  k_pc_output_bwd<<<blocks_for(n, 256), 256, 0, s>>>(T.L, dz_cs, h6, params_out, dh);
+  HNS_HIP(hipGetLastError());
+}
+
+#define HNS_PC_WGRAD(TAPS, CIN, COUT)                                                             \
+  if (Ly.taps == TAPS && Ly.cin == CIN && Ly.cout == COUT) {                                      \
+    k_pc_wgrad<TAPS, CIN, COUT><<<(unsigned)(R * TAPS), kTpb, 0, s>>>(T, lgS, R, x, dy, partial); \
+    HNS_HIP(hipGetLastError());                                                                   \
+    done = true;                                                                                  \
+  }
+
+void launch_pc_wgrad(const DevTopo& T, int layer, const __bf16* x, const __bf16* dy, float* partial, float* grad,
+                     hipStream_t s) {
+  const PcLayer& Ly = kPcNet[layer];
+  const int lgS = 3 - Ly.level;
+  const int R = wgrad_ranges((int64_t)T.L << (3 * lgS));
+  bool done = false;
+  HNS_PC_WGRAD(27, 16, 32)
+  HNS_PC_WGRAD(27, 32, 64)
+  HNS_PC_WGRAD(27, 64, 64)
+  HNS_PC_WGRAD(1, 64, 32)
+  HNS_PC_WGRAD(27, 32, 32)
+  HNS_PC_WGRAD(1, 32, 16)
+  HNS_PC_WGRAD(27, 16, 16)
+  if (!done) fail(HNS_ERR_INVALID_ARGUMENT, "no weight-gradient kernel for this layer");
+  const int64_t n = (int64_t)Ly.taps * Ly.cin * Ly.cout;
+  k_pc_reduce<<<blocks_for(n, 256), 256, 0, s>>>(n, R, partial, grad + pc_offsets().param[layer]);
+  HNS_HIP(hipGetLastError());
+}
+#undef HNS_PC_WGRAD
+
+void launch_pc_wgrad_input(const DevTopo& T, const float* r_cs, const __bf16* dh0, float* partial, float* grad,
+                           hipStream_t s) {
+  const int R = wgrad_ranges((int64_t)T.L * kLeafVox);
+  k_pc_wgrad_input<<<(unsigned)(R * 27), kTpb, 0, s>>>(T, R, r_cs, dh0, partial);
+  HNS_HIP(hipGetLastError());
+  k_pc_reduce<<<blocks_for(27 * 16, 256), 256, 0, s>>>(27 * 16, R, partial, grad + pc_offsets().param[kC0]);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_wgrad_output(const DevTopo& T, const float* dz_cs, const __bf16* h6, float* partial, float* grad,
+                            hipStream_t s) {
+  const int R = wgrad_ranges((int64_t)T.L * kLeafVox);
+  k_pc_wgrad_output<<<(unsigned)R, kTpb, 0, s>>>(T, R, dz_cs, h6, partial);
+  HNS_HIP(hipGetLastError());
+  k_pc_reduce<<<1, 256, 0, s>>>(16, R, partial, grad + pc_offsets().param[kOut]);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                    int t, hipStream_t s) {
+  const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+  k_pc_adam<<<blocks_for(n, 256), 256, 0, s>>>(n, p, g, m, v, lr, b1, b2, eps, c1, c2);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_poisson(const DevTopo& T, const float* x, const float* y, float a, float b, float* out, double* part,
+                       double* sums, hipStream_t s) {
+  const unsigned G = cg_groups(T.L);
+  k_pc_poisson<<<G, kTpb, 0, s>>>(T, x, y, a, b, out, part);
+  HNS_HIP(hipGetLastError());
+  k_pc_sum3<<<1, kTpb, 0, s>>>(part, (int)G, sums);
+  HNS_HIP(hipGetLastError());
+}
+
+}  // namespace hns

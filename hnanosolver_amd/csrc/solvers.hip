@@ -7,12 +7,13 @@
 #include <cstring>
 
 #include "hns_cg.hpp"
+#include "hns_precond.hpp"
 #include "hns_solvers.hpp"
 
 namespace hns {
 
 struct CgState {
-  DBuf<float> r, d0, d1, q;
+  DBuf<float> r, d0, d1, q, z;
   DBuf<double> rho, alpha, pa, pb, bb;
   DBuf<int> flag;
   int* h_flag = nullptr;
@@ -135,7 +136,8 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
 
 SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
                               hipStream_t s) {
-  if (r.solver == HNS_SOLVER_PCG_LEARNED) fail(HNS_ERR_INVALID_ARGUMENT, "the learned preconditioner is not loaded");
+  const bool pre = r.solver == HNS_SOLVER_PCG_LEARNED;
+  if (pre && (!r.pc || !r.pcw)) fail(HNS_ERR_INVALID_ARGUMENT, "no learned preconditioner is set");
   const int cap = r.max_iters > 0 ? r.max_iters : 100000;
   if (!cg_) cg_ = new CgState();
   CgState& G = *cg_;
@@ -144,6 +146,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.d0.alloc(N_);
     G.d1.alloc(N_);
     G.q.alloc(N_);
+    HNS_HIP(hipMemsetAsync(G.q.p, 0, N_ * sizeof(float), s));
     G.pa.alloc((size_t)kCgMaxGroups * kCgSlots);
     G.pb.alloc(kCgMaxGroups);
     G.bb.alloc(1);
@@ -169,21 +172,30 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   // The device stops at the first iteration whose recurrence residual passes the test. The stop
   // is accepted only when that residual was just replaced by b - A x; otherwise the residual is
   // replaced and the same iteration is tested again (and continued if it fails).
-  int it = 0, batch = 8, replaced_at = 0;
+  if (pre && !G.z.p) G.z.alloc(N_);
+  // a preconditioned iteration costs far more than a sync: check the stop flag every 4
+  const int batch0 = pre ? 4 : 8, batch_max = pre ? 4 : 128;
+  int it = 0, batch = batch0, replaced_at = 0;
   bool stopped = false;
   while (it < cap) {
     const int nb = std::min(batch, cap - it);
     for (int k = 0; k < nb; ++k, ++it) {
       if (it > 0 && it % kResidualReplace == 0) {
         tm.begin(HNS_K_CG_UPDATE, s);
-        launch_cg_replace(T, div, r.dx, p, G.r.p, S, s);
+        launch_cg_replace(T, div, r.dx, p, G.r.p, S, false, s);
         tm.end(s);
         replaced_at = it;
       }
       float* d_in = (it & 1) ? G.d1.p : G.d0.p;
       float* d_out = (it & 1) ? G.d0.p : G.d1.p;
+      if (pre) {
+        tm.begin(HNS_K_PRECOND, s);
+        r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
+        launch_cg_rz(T.L, G.r.p, G.z.p, G.q.p, S, s);
+        tm.end(s);
+      }
       tm.begin(HNS_K_CG_PRODUCT, s);
-      launch_cg_product(T, it, it == 0, G.r.p, nullptr, d_in, d_out, G.q.p, S, s);
+      launch_cg_product(T, it, it == 0, G.r.p, pre ? G.z.p : nullptr, d_in, d_out, G.q.p, S, s);
       tm.end(s);
       tm.begin(HNS_K_CG_UPDATE, s);
       launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
@@ -198,17 +210,26 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         break;
       }
       tm.begin(HNS_K_CG_UPDATE, s);
-      launch_cg_replace(T, div, r.dx, p, G.r.p, S, s);
+      launch_cg_replace(T, div, r.dx, p, G.r.p, S, true, s);
       tm.end(s);
       replaced_at = it;
-      batch = 8;
+      batch = batch0;
       continue;
     }
-    batch = std::min(batch * 2, 128);
+    batch = std::min(batch * 2, batch_max);
   }
   SolveResult res;
   res.iterations = stopped ? G.h_flag[1] : it;
   res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
+  // plain RBGS is the fallback of the learned preconditioner: continue from the PCG iterate
+  if (pre && !(res.rel_residual <= (double)r.tol)) {
+    const int chunk = 50, cap_fb = 200000;
+    while (!(res.rel_residual <= (double)r.tol) && res.fallback < cap_fb) {
+      rbgs_iters(T, div, p, 1, chunk, r, tm, s);
+      res.fallback += chunk;
+      res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
+    }
+  }
   return res;
 }
 

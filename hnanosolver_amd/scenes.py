@@ -275,6 +275,20 @@ def control128(seed: int = 3, target_seed: int = 33) -> Scene:
     return sc
 
 
+def control64(seed: int = 3, target_seed: int = 33) -> Scene:
+    """The density-control problem at test size: 64^3 box, ball of radius 28, 8 blobs (sigma 4)."""
+    org = leaves_touched(sphere(32, 32, 32, 28), (0, 0, 0), (64, 64, 64))
+    c = _coords(org)
+    dens = blobs(c, seed, 8, 4.0, (32, 32, 32), 18.0)
+    target = blobs(c, target_seed, 8, 4.0, (32, 32, 32), 18.0)
+    vel = curl_noise(c, seed, 3, 16.0, 1.0)
+    params = default_params(disturbanceEnable=0.0, gravity=0.0, vorticityScale=0.0)
+    sc = Scene("control64", c, 0.1, 0.05, 80, params, vel, {"density": dens},
+               notes="16 steps, RBGS 80, checkpoint every 4")
+    sc.sources["target_density"] = target
+    return sc
+
+
 def precond128(seed: int) -> Scene:
     """Learned-preconditioner scenes: the fire scene at 128^3 (cone of base radius 40 over 120)."""
     shape = y_cone(64, 64, 4, 120, 40.0)
@@ -283,5 +297,19 @@ def precond128(seed: int) -> Scene:
     return _fire("precond128", org, seed, 3, 0.1, 0.05, 100, fuel)
 
 
+def precond128_2m(seed: int) -> Scene:
+    """Learned-preconditioner training and evaluation scenes at about 2M voxels in 128^3: a +y
+    cylinder whose radius (48..62) and height (104..120) vary with the seed, fire fields and
+    3-octave curl noise as in the fire scene."""
+    u = hash_uniform(seed, 900, np.array([0, 1]), np.array([0, 0]), np.array([0, 0]))
+    r0 = 48.0 + 14.0 * float(u[0])
+    h = 104 + int(16 * float(u[1]))
+    shape = y_cylinder(64, 64, 4, h, r0)
+    org = leaves_touched(shape, (0, 0, 0), (128, 128, 128))
+    fuel = lambda x, y, z: shape(x, y, z) & (y < 4 + 8)  # noqa: E731
+    return _fire("precond128_2m", org, seed, 3, 0.1, 0.05, 100, fuel,
+                 notes="cylinder R=%.1f H=%d in 128^3" % (r0, h))
+
+
 SCENES = {"smoke64": smoke64, "fire256_cone": fire256_cone, "fire256_8m": fire256_8m, "fire512": fire512,
-          "control128": control128}
+          "control128": control128, "control64": control64}

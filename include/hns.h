@@ -143,10 +143,12 @@ typedef struct hns_solve_options {
   int32_t check_every;   /* residual check period for HNS_SOLVER_RBGS_TOL (0 = default) */
 } hns_solve_options;
 typedef struct hns_solve_stats {
-  uint32_t struct_size;  /* sizeof(hns_solve_stats) */
-  int32_t iterations;    /* iterations of the last pressure solve */
-  double rel_residual;   /* ||div - L p||_2 / ||div||_2 after the last solve (FP64 sums) */
-  double solve_ms;       /* device time of the last solve */
+  uint32_t struct_size;          /* sizeof(hns_solve_stats) */
+  int32_t iterations;            /* iterations of the last pressure solve */
+  double rel_residual;           /* ||div - L p||_2 / ||div||_2 after the last solve (FP64 sums) */
+  double solve_ms;               /* device time of the last solve */
+  int32_t fallback_iterations;   /* RBGS iterations run after a learned PCG missed tol (0 if none) */
+  int32_t reserved;
 } hns_solve_stats;
 hns_status hns_advance_ex(hns_context* ctx, int32_t iteration, float dt, const hns_params* params,
                           const hns_solve_options* opts, hns_solve_stats* stats);
@@ -264,6 +266,72 @@ hns_status hnsk_from_colour_split(const hns_topology* topo, const float* in, flo
 /* AoS (x,y,z per voxel) <-> SoA velocity. */
 hns_status hnsk_aos_to_soa(size_t n, const float* in, float* out, void* stream, int32_t async);
 hns_status hnsk_soa_to_aos(size_t n, const float* in, float* out, void* stream, int32_t async);
+
+/* ---- learned pressure preconditioner (no reference counterpart: the reference projects with a
+ *      fixed red-black Gauss-Seidel count, HNanoSolver.cu:364-373) ------------------------------
+ * A sparse 3-level 3x3x3 convolutional network over the active leaves (16/32/64 channels, BF16
+ * weights and activations, FP32 accumulation) maps a residual r to a correction z; it is the
+ * preconditioner of the flexible PCG behind HNS_SOLVER_PCG_LEARNED. Weights are one FP32 vector,
+ * layer after layer (c0, c1, c2, c3, u2, c5, u1, c6, out), each W[tap][cin][cout]. A handle keeps
+ * the bf16 MFMA operands and activation scratch; calls sharing a handle must not overlap. */
+typedef struct hns_precond hns_precond;
+int64_t hns_precond_num_params(void);
+/* From host weights (copied to the device). */
+hns_status hns_precond_create(const float* params, int64_t n, hns_precond** out);
+/* Over caller-owned device weights (training); call hns_precond_refresh after changing them. */
+hns_status hns_precond_create_device(float* params_dev, int64_t n, hns_precond** out);
+hns_status hns_precond_refresh(hns_precond* p, void* stream);
+hns_status hns_precond_destroy(hns_precond** p);
+/* The context's HNS_SOLVER_PCG_LEARNED solves use `p` (not owned; null detaches). */
+hns_status hns_set_preconditioner(hns_context* ctx, hns_precond* p);
+/* z = M(r) on colour-split vectors; keep != 0 keeps the activations for hnsk_precond_backward. */
+hns_status hnsk_precond_apply(const hns_topology* topo, hns_precond* p, const float* r_cs, float* z_cs, int32_t keep,
+                              void* stream);
+/* grad (device, hns_precond_num_params() floats) += d/dparams sum(dz . z) of the last kept apply. */
+hns_status hnsk_precond_backward(const hns_topology* topo, hns_precond* p, const float* r_cs, const float* dz_cs,
+                                 float* grad, void* stream);
+/* One Adam step (bias-corrected, step >= 1) on n device parameters. */
+hns_status hnsk_precond_adam(int64_t n, float* params, const float* grad, float* m, float* v, float lr, float beta1,
+                             float beta2, float eps, int32_t step, void* stream);
+/* out = a*y + b*(A x) on colour-split vectors, A = 6 I - (6 face neighbours, 0 outside the domain)
+ * (the pressure operator scaled by -dx^2); work: device buffer of HNS_POISSON_WORK_DOUBLES
+ * doubles, which receives ||out||^2, ||y||^2 and out.x in work[0..2] (FP64). */
+#define HNS_POISSON_WORK_DOUBLES 6148
+hns_status hnsk_poisson_apply(const hns_topology* topo, const float* x_cs, const float* y_cs, float a, float b,
+                              float* out_cs, double* work, void* stream);
+/* ---- reverse mode of the differentiable step (no reference counterpart; the reference is
+ *      forward only). Step: u1 = advect_vector(u) (Kernel.cu:349-426), div = divergence(u1),
+ *      p = K RBGS iterations from 0 (HNanoSolver.cu:364-373), u2 = u1 - grad p, rho' =
+ *      advect_scalars(rho; u2) (Kernel.cu:163-289), u' = u2. Velocity SoA (3N), density N, device
+ *      memory. The adjoints are the exact transposes of those kernels. -------------------------- */
+typedef struct hns_diff_tape hns_diff_tape;
+/* Runs `steps` steps from (vel0, rho0), keeping the state every `checkpoint_every` steps in a new
+ * tape; vel_out/rho_out (nullable) receive the final state. */
+hns_status hnsk_diff_forward(const hns_topology* topo, const float* vel0, const float* rho0, int32_t steps, float dt,
+                             float dx, int32_t iterations, float omega, int32_t checkpoint_every, float* vel_out,
+                             float* rho_out, hns_diff_tape** tape, void* stream);
+/* Gradients of sum(gvel_out . u_S + grho_out . rho_S) w.r.t. vel0 and rho0 (gvel_out/grho_out
+ * nullable = 0); recomputes each segment from its checkpoint. Synchronous. */
+hns_status hnsk_diff_backward(hns_diff_tape* tape, const float* gvel_out, const float* grho_out, float* gvel0,
+                              float* grho0, void* stream);
+/* Checkpoints held, device bytes held by the tape, and elements where a recomputed segment end
+ * differed from the stored state in the last backward pass (0 expected). */
+hns_status hns_diff_tape_info(const hns_diff_tape* tape, int32_t* checkpoints, int64_t* bytes,
+                              int64_t* recompute_mismatches);
+hns_status hns_diff_tape_destroy(hns_diff_tape** tape);
+/* Single-kernel transposes (accumulate into gin/gvel/gdiv; each allocates its scratch):
+ * advect_scalars for one field, advect_vector, and K RBGS iterations from 0 (g_cs overwritten). */
+hns_status hnsk_advect_scalars_adj(const hns_topology* topo, const float* vel, const float* in, const float* gout,
+                                   float* gin, float* gvel, float dt, float inv_dx, void* stream);
+hns_status hnsk_advect_vector_adj(const hns_topology* topo, const float* vel, const float* gout, float* gvel, float dt,
+                                  float inv_dx, void* stream);
+hns_status hnsk_rbgs_adj(const hns_topology* topo, float* g_cs, float* gdiv_cs, float dx, int32_t iterations,
+                         float omega, void* stream);
+
+/* hnsk_pressure_solve with the learned preconditioner (flexible PCG to opts->tol). Synchronous. */
+hns_status hnsk_pressure_solve_learned(const hns_topology* topo, hns_precond* p, const float* div_cs, float* p_cs,
+                                       float dx, const hns_solve_options* opts, hns_solve_stats* stats,
+                                       void* stream);
 
 #ifdef __cplusplus
 }

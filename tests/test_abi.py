@@ -19,7 +19,7 @@ HEADER = os.path.join(ROOT, "include", "hns.h")
 
 def declared():
     src = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:hns_status|const char\*|int32_t)\s+(hnsk?_\w+)\s*\(", src, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:hns_status|const char\*|int32_t|int64_t)\s+(hnsk?_\w+)\s*\(", src, re.M)))
 
 
 def test_every_declared_symbol_is_exported_and_bound():

@@ -1,0 +1,202 @@
+"""GPU checks of the learned preconditioner: MFMA forward and backward kernels against the dense
+oracle (oracle/precond_net.py) on the same weights and inputs, the Poisson operator, PCG with an
+identity network against plain CG, and what a short GPU training run gains.
+
+Tolerances: the forward is compared with the oracle run with the same bf16 rounding points (the
+GPU accumulates in FP32, the oracle in FP64): ||z_gpu - z_ref|| <= 1e-2 ||z_ref||. Backward
+gradients keep bf16 activation gradients on the GPU and FP64 ones in the oracle: per layer
+||g_gpu - g_ref|| <= 5e-2 ||g_ref||."""
+import numpy as np
+import pytest
+
+import hnanosolver_amd as hns
+from hnanosolver_amd import kernels as K
+from hnanosolver_amd import precond as P
+from hnanosolver_amd import scenes
+from oracle import Oracle
+from oracle import precond_net as PN
+
+from _helpers import canonical_coords
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print("CASE", name, flush=True)
+
+
+def compact_ragged(seed, extent=6, fill=0.6):
+    """Leaves in an extent^3 block (8-voxel units) with holes, so some neighbours are absent."""
+    rng = np.random.default_rng(seed)
+    org = [(8 * x - 16, 8 * y, 8 * z + 8) for x in range(extent) for y in range(extent) for z in range(extent)
+           if rng.random() < fill]
+    return canonical_coords(np.array(org))
+
+
+def _topo(coords, dx=0.1):
+    return hns.create_index_grid(hns.GridIndexedData(coords), dx)
+
+
+def _cs(torch, topo, host):
+    d = torch.from_numpy(np.ascontiguousarray(host, np.float32)).cuda()
+    out = torch.empty_like(d)
+    K.to_colour_split(topo, d, out)
+    return out
+
+
+def _dense(torch, topo, dev_cs):
+    out = torch.empty_like(dev_cs)
+    K.from_colour_split(topo, dev_cs, out)
+    return out.cpu().numpy()
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_forward_matches_dense_oracle(hip):
+    torch = hip
+    for seed in (1, 2):
+        c = compact_ragged(seed)
+        _case("precond forward, compact ragged seed %d, %d leaves" % (seed, len(c) // 512))
+        topo = _topo(c)
+        w = P.init_params(seed + 4)
+        r = np.random.default_rng(seed).standard_normal(len(c)).astype(np.float32)
+        pc = P.Preconditioner(w)
+        z_cs = torch.empty(len(c), device="cuda")
+        pc.apply(topo, _cs(torch, topo, r), z_cs)
+        z = _dense(torch, topo, z_cs)
+        z_ref, _ = PN.NetOracle(c[::512], bf16_round=True).forward(w, r)
+        z64, _ = PN.NetOracle(c[::512]).forward(w, r)
+        e, e64 = _rel(z, z_ref), _rel(z, z64)
+        print("  rel err vs bf16 oracle %.2e, vs FP64 oracle %.2e, max-abs %.2e" % (e, e64, np.abs(z - z_ref).max()))
+        assert e <= 1e-2
+        # scale equivariance of the bias-free network holds on the device too
+        z2_cs = torch.empty_like(z_cs)
+        pc.apply(topo, _cs(torch, topo, 4.0 * r), z2_cs)
+        assert _rel(_dense(torch, topo, z2_cs), 4.0 * z) <= 1e-6
+        pc.close()
+
+
+def test_backward_matches_dense_oracle(hip):
+    torch = hip
+    c = compact_ragged(3)
+    _case("precond backward, compact ragged seed 3, %d leaves" % (len(c) // 512))
+    topo = _topo(c)
+    w = P.init_params(7)
+    rng = np.random.default_rng(8)
+    r = rng.standard_normal(len(c)).astype(np.float32)
+    dz = rng.standard_normal(len(c)).astype(np.float32)
+    params = torch.from_numpy(w).cuda()
+    pc = P.Preconditioner(params)
+    r_cs = _cs(torch, topo, r)
+    z_cs = torch.empty_like(r_cs)
+    pc.apply(topo, r_cs, z_cs, keep=True)
+    grad = torch.zeros_like(params)
+    pc.backward(topo, r_cs, _cs(torch, topo, dz), grad)
+    g = grad.cpu().numpy()
+    net = PN.NetOracle(c[::512], bf16_round=True)
+    _, cache = net.forward(w, r)
+    g_ref = net.backward(cache, dz.astype(np.float64))
+    worst = 0.0
+    for name, (o, (t, ci, co)) in PN.OFFSETS.items():
+        e = _rel(g[o:o + t * ci * co], g_ref[o:o + t * ci * co])
+        print("  %-3s |g| %.3e rel err %.2e" % (name, np.linalg.norm(g_ref[o:o + t * ci * co]), e))
+        worst = max(worst, e)
+    assert worst <= 5e-2
+    pc.close()
+
+
+def test_poisson_operator_matches_oracle(hip):
+    torch = hip
+    c = compact_ragged(4)
+    _case("poisson apply, compact ragged seed 4")
+    topo = _topo(c)
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(len(c)).astype(np.float32)
+    y = rng.standard_normal(len(c)).astype(np.float32)
+    out = torch.empty(len(c), device="cuda")
+    work = torch.zeros(P.POISSON_WORK, dtype=torch.float64, device="cuda")
+    P.poisson_apply(topo, _cs(torch, topo, x), _cs(torch, topo, y), 0.5, -2.0, out, work)
+    got = _dense(torch, topo, out)
+    ref = 0.5 * y.astype(np.float64) - 2.0 * PN.NetOracle(c[::512]).apply_A(x.astype(np.float64))
+    assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()
+    s = work[:2].cpu().numpy()
+    assert abs(s[0] - np.dot(ref, ref)) <= 1e-5 * np.dot(ref, ref)
+    assert abs(s[1] - np.dot(y.astype(np.float64), y)) <= 1e-6 * np.dot(y.astype(np.float64), y)
+    ox = np.dot(ref, x.astype(np.float64))
+    assert abs(s[2] - ox) <= 1e-5 * np.linalg.norm(ref) * np.linalg.norm(x)
+
+
+def identity_params():
+    """Weights whose network is z = r up to bf16 rounding of r: h0 = (relu(r), relu(-r)), every
+    other path zero, h6 = h0 through the centre tap, z = h6[0] - h6[1]."""
+    w = np.zeros(P.N_PARAMS, np.float32)
+    off = PN.OFFSETS
+    o, _ = off["c0"]
+    w[o + 13 * 16 + 0] = 1.0   # tap 13 (centre), cin 0, cout 0
+    w[o + 13 * 16 + 1] = -1.0
+    o, _ = off["c6"]
+    w[o + (13 * 16 + 0) * 16 + 0] = 1.0
+    w[o + (13 * 16 + 1) * 16 + 1] = 1.0
+    o, _ = off["out"]
+    w[o + 0], w[o + 1] = 1.0, -1.0
+    return w
+
+
+def _fire_div(torch, sc, topo):
+    return P.scene_divergence(sc, topo)
+
+
+def test_identity_network_pcg_reduces_to_cg(hip):
+    torch = hip
+    sc = scenes.smoke64()
+    _case("pcg identity network vs cg, smoke64")
+    topo = _topo(sc.coords, sc.voxel_size)
+    div = _fire_div(torch, sc, topo)
+    p1 = torch.empty_like(div)
+    p2 = torch.empty_like(div)
+    st_cg = K.pressure_solve(topo, div, p1, sc.voxel_size, solver="cg", tol=1e-5)
+    pc = P.Preconditioner(identity_params())
+    st = pc.solve(topo, div, p2, sc.voxel_size, tol=1e-5)
+    d0 = _dense(torch, topo, div)
+    r2, d2 = Oracle(sc.coords, np.float64).residual(_dense(torch, topo, p2), d0, sc.voxel_size)
+    print("  cg %d iterations, pcg(identity) %d iterations, rel %.2e (oracle-checked %.2e)" % (
+        st_cg.iterations, st.iterations, st.rel_residual, np.sqrt(r2 / d2)))
+    assert np.sqrt(r2 / d2) <= 1.01e-5
+    assert abs(st.iterations - st_cg.iterations) <= max(3, 0.1 * st_cg.iterations)
+    pc.close()
+
+
+@pytest.mark.timeout(300, method="thread")
+def test_training_gains_over_untrained_and_cg(hip):
+    """A short GPU training run on RBGS residual pairs of three scenes; on a held-out scene the
+    trained network must need fewer PCG iterations than the untrained one and than plain CG."""
+    torch = hip
+    _case("precond training: 3 train scenes, 1 held-out, 400 steps")
+    samples = P.make_samples(scenes.precond128_2m, [0, 1, 2], ks=(0, 4, 16, 64))
+    p_trained, hist = P.train(samples, 400, lr=2e-3, seed=0, log_every=100)
+    print("  loss history", [(s, round(l, 4)) for s, l in hist])
+    assert hist[-1][1] < hist[0][1]
+    sc = scenes.precond128_2m(200)
+    topo = _topo(sc.coords, sc.voxel_size)
+    div = P.scene_divergence(sc, topo)
+    p = torch.empty_like(div)
+    it = {}
+    it["cg"] = K.pressure_solve(topo, div, p, sc.voxel_size, solver="cg", tol=1e-5).iterations
+    cap = 3 * it["cg"]
+    for name, w in (("untrained", P.init_params(0)), ("trained", p_trained)):
+        pc = P.Preconditioner(w)
+        try:
+            st = pc.solve(topo, div, p, sc.voxel_size, tol=1e-5, max_iters=cap)
+            it[name] = st.iterations
+        except hns.HNSError as e:  # an untrained network may not converge at all
+            assert name == "untrained", e
+            it[name] = cap + 1
+        if name == "trained":
+            d0 = _dense(torch, topo, div)
+            r2, d2 = Oracle(sc.coords, np.float64).residual(_dense(torch, topo, p), d0, sc.voxel_size)
+            assert np.sqrt(r2 / d2) <= 1.01e-5
+        pc.close()
+    print("  held-out N=%d: PCG iterations to 1e-5 %s (%d = did not converge)" % (sc.n, it, cap + 1))
+    assert it["trained"] < it["untrained"] and it["trained"] < it["cg"]
