@@ -5,9 +5,10 @@
 A step is one HNS_Advance call (one substep) of the fire scene: advect velocity, vorticity
 confinement, divergence + combustion + buoyancy, the pressure solve, gradient subtraction,
 MacCormack advection of the five scalars. The headline `value` is the step with the pressure
-solved to 1e-5 relative residual by the fastest solver built (conjugate gradients until the
-learned preconditioner is faster); the plain red-black Gauss-Seidel solve to the same residual and
-the reference's own fixed 100-iteration step are reported beside it under "legs". The inputs are
+solved to 1e-5 relative residual by the fastest to-tolerance solver (conjugate gradients or the
+learned-preconditioned PCG, whichever is faster); the plain red-black Gauss-Seidel solve to the
+same residual, the learned PCG and the reference's own fixed 100-iteration step are reported
+beside it under "legs". The inputs are
 resident in HBM before timing starts (the context holds the state between calls). Prints one JSON
 line.
 """
@@ -111,6 +112,9 @@ def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrumen
     ctx = hns.ensure_context(None, data, topo, sc.voxel_size)
     stream = torch.cuda.current_stream()
     ctx.set_stream(stream.cuda_stream)
+    if solver == "pcg_learned":
+        from hnanosolver_amd import precond
+        ctx.set_preconditioner(precond.Preconditioner(precond.load_weights()))
 
     def step():
         if solver == "rbgs":
@@ -124,13 +128,14 @@ def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrumen
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    iters, rels = [], []
+    iters, rels, fallback = [], [], 0
     e0.record(stream)
     for _ in range(steps):
         st = step()
         if st is not None:
             iters.append(st.iterations)
             rels.append(st.rel_residual)
+            fallback += st.fallback_iterations
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
@@ -141,6 +146,8 @@ def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrumen
     else:
         leg["iterations_per_step"] = float(np.mean(iters))
         leg["max_rel_residual"] = float(np.max(rels))
+        if solver == "pcg_learned":
+            leg["fallback_rbgs_iterations"] = int(fallback)
     print("[bench] %-11s %9.3f ms/step  %s" % (solver, ms, {k: v for k, v in leg.items() if "iter" in k or "rel" in k}),
           file=sys.stderr, flush=True)
     if instrument:
@@ -174,7 +181,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fire256_8m")
-    ap.add_argument("--legs", default="cg,rbgs,rbgs_tol", help="solver legs; the first to-tolerance one is value")
+    ap.add_argument("--legs", default="cg,pcg_learned,rbgs,rbgs_tol",
+                    help="solver legs: cg, pcg_learned, rbgs (fixed K), rbgs_tol")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     a = ap.parse_args(argv)
@@ -196,10 +204,10 @@ def main(argv=None):
     print("[bench] N=%d leaves=%d K=%d" % (sc.n, sc.n_leaves, K), file=sys.stderr, flush=True)
     legs = {}
     for solver in a.legs.split(","):
-        slow = solver == "rbgs_tol"
+        slow = solver in ("rbgs_tol", "pcg_learned")  # hundreds of ms per step: a few steps suffice
         legs[solver] = run_leg(hns, torch, sc, data, topo, params, solver,
-                               steps=min(a.steps, 2) if slow else a.steps, warmup=min(a.warmup, 1) if slow else a.warmup,
-                               instrument=not slow, K=K)
+                               steps=min(a.steps, 3) if slow else a.steps, warmup=min(a.warmup, 1) if slow else a.warmup,
+                               instrument=solver != "rbgs_tol", K=K)
     tol_legs = [s for s in legs if s in ("cg", "pcg_learned")]
     head = min(tol_legs, key=lambda s: legs[s]["ms_per_step"]) if tol_legs else next(iter(legs))
     L = legs[head]
@@ -224,8 +232,9 @@ def main(argv=None):
     }
     if "rbgs" in legs and "kernels" in legs["rbgs"]:
         out["roofline_rbgs_sweep"] = roofline(legs["rbgs"], "rbgs", sc.n, sc.name)
-    if "rbgs_tol" in legs and head != "rbgs":
-        out["speedup_vs_plain_rbgs_to_tol"] = legs["rbgs_tol"]["ms_per_step"] / L["ms_per_step"]
+    if "rbgs_tol" in legs:
+        for s in tol_legs:
+            out["speedup_%s_vs_plain_rbgs_to_tol" % s] = legs["rbgs_tol"]["ms_per_step"] / legs[s]["ms_per_step"]
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(sc, "cg" if head != "rbgs" else "rbgs", L["iterations_per_step"],

@@ -142,8 +142,8 @@ void DiffTape::backward(const float* gvel_out, const float* grho_out, float* gve
     }
   }
   unsigned long long h = 0;
-  HNS_HIP(hipMemcpyAsync(&h, count_.p, sizeof(h), hipMemcpyDeviceToHost, s));
   HNS_HIP(hipStreamSynchronize(s));
+  HNS_HIP(hipMemcpy(&h, count_.p, sizeof(h), hipMemcpyDeviceToHost));
   mismatches_ = (int64_t)h;
 }
 

@@ -20,6 +20,16 @@ using namespace hns;
 
 struct hns_topology {
   std::shared_ptr<TopoStore> s;
+  // solver scratch of the device-API pressure solves on this topology, made on first use and
+  // reused (calls sharing a topology must not overlap)
+  mutable std::unique_ptr<Solvers> sol;
+  Solvers& solvers() const {
+    if (!sol) {
+      sol = std::make_unique<Solvers>();
+      sol->init(s->L, s->N);
+    }
+    return *sol;
+  }
 };
 struct hns_precond {
   int device = 0;
@@ -314,16 +324,8 @@ hns_status hns_create_index_grid(const hns_grid_data* data, float voxel_size, hn
     if (!data) fail(HNS_ERR_INVALID_ARGUMENT, "CreateIndexGrid: payload is null");
     if (data->size == 0) fail(HNS_ERR_EMPTY, "CreateIndexGrid: N == 0");
     require_device();
-    hipStream_t s = nullptr;
-    HNS_HIP(hipStreamCreate(&s));
-    std::shared_ptr<TopoStore> t;
-    try {
-      t = build_topology(data->coords, data->size, voxel_size, s);
-    } catch (...) {
-      (void)hipStreamDestroy(s);
-      throw;
-    }
-    (void)hipStreamDestroy(s);
+    // built on the null stream: the caller's later work on any blocking stream is ordered after it
+    std::shared_ptr<TopoStore> t = build_topology(data->coords, data->size, voxel_size, nullptr);
     if (*out) delete *out;
     *out = new hns_topology{t};
     return HNS_OK;
@@ -823,8 +825,8 @@ hns_status hnsk_residual(const hns_topology* topo, const float* p_cs, const floa
     DBuf<double> partial(2 * (size_t)T.L), out(2);
     launch_residual(T, p_cs, div_cs, dx, partial.p, out.p, s);
     double h[2];
-    HNS_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, s));
     HNS_HIP(hipStreamSynchronize(s));
+    HNS_HIP(hipMemcpy(h, out.p, sizeof(h), hipMemcpyDeviceToHost));
     *r2 = h[0];
     *d2 = h[1];
     return HNS_OK;
@@ -842,8 +844,7 @@ hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, fl
     if (stats && stats->struct_size < sizeof(hns_solve_stats))
       fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_stats.struct_size too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    Solvers sol;
-    sol.init(T.L, topo->s->N);
+    Solvers& sol = topo->solvers();
     Timing tm;
     SolveRequest req;
     req.omega = omega;
@@ -1068,8 +1069,7 @@ hns_status hnsk_pressure_solve_learned(const hns_topology* topo, hns_precond* p,
     if (stats && stats->struct_size < sizeof(hns_solve_stats))
       fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_stats.struct_size too small");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    Solvers sol;
-    sol.init(T.L, topo->s->N);
+    Solvers& sol = topo->solvers();
     Timing tm;
     SolveRequest req;
     req.dx = dx;

@@ -118,8 +118,8 @@ std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float
   k_check_order<<<(unsigned)((L + 255) / 256), 256, 0, s>>>(T->origins.p, L, derr.p);
   HNS_HIP(hipGetLastError());
   int herr = 0;
-  HNS_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HNS_HIP(hipStreamSynchronize(s));
+  HNS_HIP(hipMemcpy(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost));
   if (herr & kErrNotLeafDense)
     fail(HNS_ERR_TOPOLOGY, "topology: coords are not leaf-dense (each leaf must be 512 consecutive voxels in offset order)");
   if (herr & kErrRange) fail(HNS_ERR_TOPOLOGY, "topology: a coordinate lies outside [-2^23, 2^23)");

@@ -3,9 +3,9 @@
 Step: u1 = advect_vector(u), div = divergence(u1), p = K red-black Gauss-Seidel iterations from 0,
 u2 = u1 - grad p, rho' = advect_scalars(rho; u2), u' = u2 (src/Cuda/Kernel.cu:349-426, :432-453,
 :459-489, :495-541, :163-289). Gradients are exact transposes of those kernels, computed on the
-device with checkpoints every k steps. `DiffSteps` exposes the same as a torch autograd function
-(the C++ torch::autograd::Function of hns_torch when it is built, else a Python one over the same
-C ABI).
+device with checkpoints every k steps. `diff_steps` exposes the same as a torch autograd function:
+the C++ torch::autograd::Function of the in-tree hns_torch extension (LibTorch-ROCm host code over
+the same C ABI).
 """
 from __future__ import annotations
 
@@ -13,7 +13,6 @@ import ctypes as C
 import importlib
 import os
 import sys
-from typing import Optional
 
 from . import _check, lib
 
@@ -99,47 +98,11 @@ def torch_extension():
 
 def diff_steps(topo, vel0, rho0, steps, dt, dx, iterations, omega, checkpoint_every=4):
     """(u_S, rho_S) after `steps` differentiable steps, differentiable w.r.t. vel0 (SoA (3, N)) and
-    rho0 (N,) through torch.autograd."""
-    ext = torch_extension()
-    if ext is not None:
-        lib()  # libhns is loaded first so the extension's dependency resolves to the in-tree copy
-        import torch
-        return ext.diff_steps(vel0, rho0, int(topo.handle.value), int(steps), float(dt), float(dx), int(iterations),
-                              float(omega), int(checkpoint_every), int(torch.cuda.current_stream().cuda_stream))
-    return _PyDiffSteps.apply(vel0, rho0, topo, steps, dt, dx, iterations, omega, checkpoint_every)
-
-
-def _py_function():
+    rho0 (N,) through torch.autograd (raises ImportError when hns_torch is not built)."""
     import torch
-
-    class DiffStepsFn(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, vel0, rho0, topo, steps, dt, dx, iterations, omega, every):
-            v = torch.empty_like(vel0)
-            r = torch.empty_like(rho0)
-            ctx.tape = forward(topo, vel0.contiguous(), rho0.contiguous(), steps, dt, dx, iterations, omega, every,
-                               v, r)
-            return v, r
-
-        @staticmethod
-        def backward(ctx, gv, gr):
-            gv0 = torch.empty(gv.shape, dtype=gv.dtype, device=gv.device)
-            gr0 = torch.empty(gr.shape, dtype=gr.dtype, device=gr.device)
-            ctx.tape.backward(gv.contiguous(), gr.contiguous(), gv0, gr0)
-            ctx.tape.close()
-            return gv0, gr0, None, None, None, None, None, None, None
-
-    return DiffStepsFn
-
-
-class _Lazy:
-    def __init__(self):
-        self._fn = None
-
-    def apply(self, *a):
-        if self._fn is None:
-            self._fn = _py_function()
-        return self._fn.apply(*a)
-
-
-_PyDiffSteps = _Lazy()
+    ext = torch_extension()
+    if ext is None:
+        raise ImportError("hns_torch is not built; run `python -m hnanosolver_amd.build --torch`")
+    lib()  # libhns is loaded first so the extension's dependency resolves to the in-tree copy
+    return ext.diff_steps(vel0, rho0, int(topo.handle.value), int(steps), float(dt), float(dx), int(iterations),
+                          float(omega), int(checkpoint_every), int(torch.cuda.current_stream().cuda_stream))
