@@ -185,6 +185,9 @@ def main(argv=None):
                     help="solver legs: cg, pcg_learned, rbgs (fixed K), rbgs_tol")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="one 1 GiB device-to-device tensor copy first (reads and writes 2^30 bytes), the "
+                         "reference dispatch for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the step runs on a single MI355X (sharding across GPUs is a "
@@ -195,6 +198,12 @@ def main(argv=None):
 
     import hnanosolver_amd as hns
 
+    if a.copy_calibration:
+        src = torch.ones(1 << 28, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print("[bench] building scene %s" % a.workload, file=sys.stderr, flush=True)
     sc = make_scene(a.workload)
     data = payload(sc)
