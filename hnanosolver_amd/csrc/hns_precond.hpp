@@ -80,6 +80,11 @@ void launch_pc_conv(const DevTopo& T, int level, int taps, int cin, int cout, co
 // z = relu(conv3(x; c6)) . w_out (colour-split FP32); h (nullable) keeps relu(conv3(x; c6)) in bf16
 void launch_pc_output(const DevTopo& T, const __bf16* x, const __bf16* w, const float* params_out, float* z_cs,
                       __bf16* h, hipStream_t s);
+// Fused level-0 inference (one workgroup per leaf, leaf + halo tile in LDS): h0 = relu(conv3(r; c0))
+// and p1 = pool(h0); z = relu(conv3(h0 + up(g1); c6)) . w_out. Same results as the unfused kernels.
+void launch_pc_input_pool(const DevTopo& T, const float* r_cs, const __bf16* w, __bf16* h0, __bf16* p1, hipStream_t s);
+void launch_pc_output_fused(const DevTopo& T, const __bf16* h0, const __bf16* g1, const __bf16* w,
+                            const float* params_out, float* z_cs, hipStream_t s);
 // y_coarse = scale * sum of the 8 children of x (level `level` -> level + 1)
 void launch_pc_pool(int32_t L, int level, int c, const __bf16* x, __bf16* y, float scale, hipStream_t s);
 // y = a + scale * up(b) at level `level` (b at level + 1); with mask: y = (mask > 0) ? y : 0

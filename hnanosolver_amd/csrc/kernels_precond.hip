@@ -252,6 +252,137 @@ __global__ __launch_bounds__(kTpb) void k_pc_conv(DevTopo T, int lgS, const __bf
   }
 }
 
+// ---- fused level-0 inference kernels: one workgroup per leaf, a 10^3 tile (leaf + 1-voxel halo)
+// staged in LDS once instead of 27 gathers per voxel from the cache hierarchy -------------------
+constexpr int kT0 = 10, kT03 = kT0 * kT0 * kT0;
+__device__ __forceinline__ int tile0(int x, int y, int z) { return (x * kT0 + y) * kT0 + z; }
+// tile cell c (coordinates t-1 in {-1..8}^3 relative to the leaf) -> its leaf (or -1) and local offset
+__device__ __forceinline__ int32_t tile_leaf(const int32_t* s_nb, int c, int& lx, int& ly, int& lz) {
+  const int tx = c / (kT0 * kT0), ty = (c / kT0) % kT0, tz = c % kT0;
+  const int x = tx - 1, y = ty - 1, z = tz - 1;
+  const int ox = x < 0 ? -1 : (x > 7 ? 1 : 0), oy = y < 0 ? -1 : (y > 7 ? 1 : 0), oz = z < 0 ? -1 : (z > 7 ? 1 : 0);
+  lx = x & 7;
+  ly = y & 7;
+  lz = z & 7;
+  return s_nb[(ox + 1) * 9 + (oy + 1) * 3 + (oz + 1)];
+}
+
+// h0 = relu(conv3(r; c0)) written for the leaf, and p1 = pool(h0) from the same workgroup.
+// Same arithmetic as k_pc_input followed by k_pc_pool (same MFMA K order, same child order).
+__global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* __restrict__ r,
+                                                        const bf16x8* __restrict__ w, __bf16* __restrict__ h0,
+                                                        __bf16* __restrict__ p1) {
+  __shared__ int32_t s_nb[27];
+  __shared__ float s_r[kT03];
+  __shared__ __bf16 s_h[kLeafVox * 16];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  if (threadIdx.x < 27) s_nb[threadIdx.x] = T.nbr[kNbrStride * l + threadIdx.x];
+  __syncthreads();
+  const int32_t half = T.L * kHalfLeaf;
+  for (int c = threadIdx.x; c < kT03; c += kTpb) {
+    int lx, ly, lz;
+    const int32_t nl = tile_leaf(s_nb, c, lx, ly, lz);
+    s_r[c] = nl < 0 ? 0.f : r[((lx + ly + lz) & 1) * half + nl * kHalfLeaf + cs_local(lx, ly, lz)];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const bf16x8 a = w[lane];
+  for (int tile = wave; tile < kLeafVox / 16; tile += 4) {
+    const int cell = 16 * tile + j;
+    const int x = cell >> 6, y = (cell >> 3) & 7, z = cell & 7;
+    bf16x8 b;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int tap = 8 * g + v;
+      b[v] = tap < 27 ? (__bf16)s_r[tile0(x + 1 + tap / 9 - 1, y + 1 + (tap / 3) % 3 - 1, z + 1 + tap % 3 - 1)]
+                      : (__bf16)0.0f;
+    }
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = mfma(a, b, acc);
+    bf16x4 o;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[v], 0.f);
+    *reinterpret_cast<bf16x4*>(h0 + ((int64_t)l * kLeafVox + cell) * 16 + 4 * g) = o;
+    *reinterpret_cast<bf16x4*>(s_h + cell * 16 + 4 * g) = o;
+  }
+  __syncthreads();
+  // p1: 64 coarse cells x 16 channels, children in k_pc_pool's order
+  for (int e = threadIdx.x; e < 64 * 16; e += kTpb) {
+    const int cc = e >> 4, ch = e & 15;
+    const int cx = cc >> 4, cy = (cc >> 2) & 3, cz = cc & 3;
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int fc = ((2 * cx + (k >> 2)) << 6) | ((2 * cy + ((k >> 1) & 1)) << 3) | (2 * cz + (k & 1));
+      sum += (float)s_h[fc * 16 + ch];
+    }
+    p1[((int64_t)l * 64 + cc) * 16 + ch] = (__bf16)(sum * 0.125f);
+  }
+}
+
+// z = relu(conv3(s0; c6)) . w_out with s0 = h0 + up(g1) formed in LDS over the leaf and its halo.
+// Same arithmetic as k_pc_upadd at level 0 followed by the ACT 3 output conv.
+__global__ __launch_bounds__(kTpb) void k_pc_output_fused(DevTopo T, const __bf16* __restrict__ h0,
+                                                          const __bf16* __restrict__ g1,
+                                                          const bf16x8* __restrict__ w,
+                                                          const float* __restrict__ wout, float* __restrict__ z) {
+  __shared__ int32_t s_nb[27];
+  __shared__ __attribute__((aligned(16))) __bf16 s_s[kT03 * 16];
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  if (threadIdx.x < 27) s_nb[threadIdx.x] = T.nbr[kNbrStride * l + threadIdx.x];
+  __syncthreads();
+  for (int e = threadIdx.x; e < kT03 * 2; e += kTpb) {
+    const int c = e >> 1, part = e & 1;
+    int lx, ly, lz;
+    const int32_t nl = tile_leaf(s_nb, c, lx, ly, lz);
+    bf16x8 o = zero8();
+    if (nl >= 0) {
+      const bf16x8 va = *reinterpret_cast<const bf16x8*>(h0 + ((int64_t)nl * kLeafVox + ((lx << 6) | (ly << 3) | lz)) * 16 +
+                                                         8 * part);
+      const int32_t pc = (nl << 6) | ((lx >> 1) << 4) | ((ly >> 1) << 2) | (lz >> 1);
+      const bf16x8 vb = *reinterpret_cast<const bf16x8*>(g1 + (int64_t)pc * 16 + 8 * part);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) o[q] = (__bf16)((float)va[q] + 1.0f * (float)vb[q]);
+    }
+    *reinterpret_cast<bf16x8*>(s_s + c * 16 + 8 * part) = o;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  constexpr int NK = (27 * 16 + 31) / 32;
+  bf16x8 a[NK];
+#pragma unroll
+  for (int kc = 0; kc < NK; ++kc) a[kc] = w[kc * 64 + lane];
+  float wo[4];
+#pragma unroll
+  for (int v = 0; v < 4; ++v) wo[v] = (float)(__bf16)wout[4 * g + v];
+  const int32_t half = T.L * kHalfLeaf;
+  for (int tile = wave; tile < kLeafVox / 16; tile += 4) {
+    const int cell = 16 * tile + j;
+    const int x = cell >> 6, y = (cell >> 3) & 7, zz = cell & 7;
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < NK; ++kc) {
+      const int kb = 32 * kc + 8 * g;
+      const int tap = kb >> 4, ci0 = kb & 15;
+      bf16x8 b = zero8();
+      if (tap < 27)
+        b = *reinterpret_cast<const bf16x8*>(
+            s_s + tile0(x + tap / 9, y + (tap / 3) % 3, zz + tap % 3) * 16 + ci0);
+      acc = mfma(a[kc], b, acc);
+    }
+    float part = 0.f;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) part += (float)(__bf16)fmaxf(acc[v], 0.f) * wo[v];
+    part += __shfl_xor(part, 16, 64);
+    part += __shfl_xor(part, 32, 64);
+    if (g == 0) z[((x + y + zz) & 1) * half + l * kHalfLeaf + cs_local(x, y, zz)] = part;
+  }
+}
+
 // ---- pooling (children sum times scale) and upsample-add -------------------------------------
 __global__ void k_pc_pool(int32_t L, int lgSc, int c8, const __bf16* __restrict__ x, __bf16* __restrict__ y,
                           float scale) {
@@ -631,6 +762,18 @@ void launch_pc_output(const DevTopo& T, const __bf16* x, const __bf16* w, const 
   const int64_t ncell = (int64_t)T.L * kLeafVox;
   k_pc_conv<27, 16, 16, 3, kNT><<<blocks_for(ncell, 64 * kNT), kTpb, 0, s>>>(
       T, 3, x, reinterpret_cast<const bf16x8*>(w), h, nullptr, params_out, z_cs);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_input_pool(const DevTopo& T, const float* r_cs, const __bf16* w, __bf16* h0, __bf16* p1, hipStream_t s) {
+  k_pc_input_pool<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, reinterpret_cast<const bf16x8*>(w), h0, p1);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_output_fused(const DevTopo& T, const __bf16* h0, const __bf16* g1, const __bf16* w,
+                            const float* params_out, float* z_cs, hipStream_t s) {
+  k_pc_output_fused<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, h0, g1, reinterpret_cast<const bf16x8*>(w), params_out,
+                                                          z_cs);
   HNS_HIP(hipGetLastError());
 }
 

@@ -1,11 +1,21 @@
 // Host-side sequencing of the learned preconditioner's forward and backward passes (the network
 // is described in hns_precond.hpp; no reference counterpart).
+#include <cstdlib>
+
 #include "hns_precond.hpp"
 
 namespace hns {
 
 namespace {
 constexpr int64_t kWgradPartialFloats = 64LL * 27 * 64 * 64;  // ranges x largest layer
+// HNS_PRECOND_UNFUSED=1 runs inference through the unfused level-0 kernels (for comparison)
+bool fused_level0() {
+  static const bool v = [] {
+    const char* e = std::getenv("HNS_PRECOND_UNFUSED");
+    return !(e && e[0] == '1');
+  }();
+  return v;
+}
 }
 
 void PcWork::ensure(int32_t leaves, bool training) {
@@ -65,8 +75,14 @@ void LearnedPrecond::forward(const DevTopo& T, PcWork& W, const float* r_cs, flo
   const PcOffsets& O = pc_offsets();
   const __bf16* wf = wfwd_.p;
   const int32_t L = T.L;
-  launch_pc_input(T, r_cs, wf + O.fwd[kC0], W.h0.p, s);
-  launch_pc_pool(L, 0, 16, W.h0.p, W.p1.p, 0.125f, s);
+  // inference fuses the level-0 work into two leaf-tiled kernels; training keeps every activation
+  const bool fused = !keep && fused_level0();
+  if (fused) {
+    launch_pc_input_pool(T, r_cs, wf + O.fwd[kC0], W.h0.p, W.p1.p, s);
+  } else {
+    launch_pc_input(T, r_cs, wf + O.fwd[kC0], W.h0.p, s);
+    launch_pc_pool(L, 0, 16, W.h0.p, W.p1.p, 0.125f, s);
+  }
   launch_pc_conv(T, 1, 27, 16, 32, W.p1.p, wf + O.fwd[kC1], W.h1.p, 1, nullptr, s);
   launch_pc_pool(L, 1, 32, W.h1.p, W.p2.p, 0.125f, s);
   launch_pc_conv(T, 2, 27, 32, 64, W.p2.p, wf + O.fwd[kC2], W.h2.p, 1, nullptr, s);
@@ -75,8 +91,12 @@ void LearnedPrecond::forward(const DevTopo& T, PcWork& W, const float* r_cs, flo
   launch_pc_upadd(L, 1, 32, W.h1.p, W.g2.p, W.s1.p, 1.0f, nullptr, s);
   launch_pc_conv(T, 1, 27, 32, 32, W.s1.p, wf + O.fwd[kC5], W.h5.p, 1, nullptr, s);
   launch_pc_conv(T, 1, 1, 32, 16, W.h5.p, wf + O.fwd[kU1], W.g1.p, 0, nullptr, s);
-  launch_pc_upadd(L, 0, 16, W.h0.p, W.g1.p, W.s0.p, 1.0f, nullptr, s);
-  launch_pc_output(T, W.s0.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, keep ? W.h6.p : nullptr, s);
+  if (fused) {
+    launch_pc_output_fused(T, W.h0.p, W.g1.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, s);
+  } else {
+    launch_pc_upadd(L, 0, 16, W.h0.p, W.g1.p, W.s0.p, 1.0f, nullptr, s);
+    launch_pc_output(T, W.s0.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, keep ? W.h6.p : nullptr, s);
+  }
 }
 
 void LearnedPrecond::backward(const DevTopo& T, PcWork& W, const float* r_cs, const float* dz_cs, float* grad,

@@ -78,6 +78,25 @@ def test_forward_matches_dense_oracle(hip):
         pc.close()
 
 
+def test_fused_level0_inference_matches_unfused(hip):
+    """The leaf-tiled level-0 inference kernels give bitwise the unfused kernels' output (the
+    kept-activation path runs unfused)."""
+    torch = hip
+    for seed in (5, 6):
+        c = compact_ragged(seed, extent=7, fill=0.5)
+        _case("fused vs unfused level 0, compact ragged seed %d, %d leaves" % (seed, len(c) // 512))
+        topo = _topo(c)
+        pc = P.Preconditioner(P.load_weights())
+        r_cs = _cs(torch, topo, np.random.default_rng(seed).standard_normal(len(c)).astype(np.float32))
+        z1, z2 = torch.empty_like(r_cs), torch.empty_like(r_cs)
+        pc.apply(topo, r_cs, z1, keep=False)
+        pc.apply(topo, r_cs, z2, keep=True)
+        n = int((z1 != z2).sum())
+        print("  differing elements %d of %d" % (n, len(c)))
+        assert n == 0
+        pc.close()
+
+
 def test_backward_matches_dense_oracle(hip):
     torch = hip
     c = compact_ragged(3)
