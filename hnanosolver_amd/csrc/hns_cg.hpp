@@ -15,7 +15,7 @@ inline unsigned cg_groups(int32_t L) {
 // Device scalars of one solve. rho[k] = r_k . r_k (CG) or r_k . z_k (PCG), alpha[k] the step
 // length of iteration k; flag[0] = stop, flag[1] = iterations done when stopped.
 struct CgScalars {
-  double* pa;     // kCgSlots partials per workgroup, written by init/replace/update/rz
+  double* pa;     // kCgSlots rows of kCgMaxGroups partials (row = slot), written by init/replace/update/rz
   double* pb;     // one partial per workgroup (d . q), written by the product kernel
   double* rho;    // max_iters + 1 entries
   double* alpha;  // max_iters + 1 entries

@@ -200,6 +200,54 @@ __device__ __forceinline__ void pair_face_sums(const FaceRow& F, int32_t t, G ge
   s1 = xp1 + xm1 + yp1 + ym1 + zp1 + v0;
 }
 
+// LDS image of one leaf of a colour-split vector for 256-lane pair stencils: the two colour
+// planes (256 values each, at in-plane offset t), then the 6 face halos -x, +x, -y, +y, -z, +z
+// (64 values each, indexed (a<<3)|b by the two in-face coordinates in x, y, z order). Staged
+// once per leaf, it serves the 12 neighbour reads of every voxel pair from LDS instead of the
+// cache hierarchy.
+constexpr int kPairTile = 2 * kHalfLeaf + 6 * 64;
+constexpr int kPairHalo = 2 * kHalfLeaf;
+// Fills the 6 halos from the face neighbours of leaf F.l (absent leaves read as 0); get(c, i)
+// as in pair_face_sums. Every lane of the 256-lane workgroup calls it; the face index is
+// uniform per wave.
+template <class T, class G>
+__device__ __forceinline__ void pair_tile_halo(const FaceRow& F, G get, T* S) {
+  for (int e = threadIdx.x; e < 6 * 64; e += 256) {
+    const int f = e >> 6, a = (e >> 3) & 7, b = e & 7;
+    int vx, vy, vz;
+    int32_t nl;
+    switch (f) {
+      case 0: nl = F.xm; vx = 7; vy = a; vz = b; break;
+      case 1: nl = F.xp; vx = 0; vy = a; vz = b; break;
+      case 2: nl = F.ym; vx = a; vy = 7; vz = b; break;
+      case 3: nl = F.yp; vx = a; vy = 0; vz = b; break;
+      case 4: nl = F.zm; vx = a; vy = b; vz = 7; break;
+      default: nl = F.zp; vx = a; vy = b; vz = 0; break;
+    }
+    S[kPairHalo + e] = nl < 0 ? T(0) : get((vx + vy + vz) & 1, nl * kHalfLeaf + cs_local(vx, vy, vz));
+  }
+}
+// Same sums as pair_face_sums (same order, same values) from a staged tile.
+template <class T>
+__device__ __forceinline__ void pair_tile_sums(const T* S, int32_t t, T v0, T v1, T& s0, T& s1) {
+  const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
+  const int c0 = (x + y) & 1, c1 = c0 ^ 1;
+  const int z0 = 2 * zz, z1 = 2 * zz + 1;
+  const int p0 = c0 * kHalfLeaf + t, p1 = c1 * kHalfLeaf + t;
+  const T xp0 = S[x < 7 ? p1 + 32 : kPairHalo + 64 + y * 8 + z0];
+  const T xm0 = S[x > 0 ? p1 - 32 : kPairHalo + 0 + y * 8 + z0];
+  const T yp0 = S[y < 7 ? p1 + 4 : kPairHalo + 192 + x * 8 + z0];
+  const T ym0 = S[y > 0 ? p1 - 4 : kPairHalo + 128 + x * 8 + z0];
+  const T zm0 = S[zz > 0 ? p1 - 1 : kPairHalo + 256 + x * 8 + y];
+  const T xp1 = S[x < 7 ? p0 + 32 : kPairHalo + 64 + y * 8 + z1];
+  const T xm1 = S[x > 0 ? p0 - 32 : kPairHalo + 0 + y * 8 + z1];
+  const T yp1 = S[y < 7 ? p0 + 4 : kPairHalo + 192 + x * 8 + z1];
+  const T ym1 = S[y > 0 ? p0 - 4 : kPairHalo + 128 + x * 8 + z1];
+  const T zp1 = S[zz < 3 ? p0 + 1 : kPairHalo + 320 + x * 8 + y];
+  s0 = xp0 + xm0 + yp0 + ym0 + v1 + zm0;
+  s1 = xp1 + xm1 + yp1 + ym1 + zp1 + v0;
+}
+
 struct Vel {
   const float* __restrict__ x;
   const float* __restrict__ y;

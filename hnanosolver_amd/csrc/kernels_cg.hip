@@ -40,10 +40,10 @@ __device__ inline double block_sum_all(double v, double* s_red) {
   return t;
 }
 
-// Sum of slot k of the G partials (stride `stride` doubles), fixed order, returned to all lanes.
-__device__ inline double reduce_partials(const double* __restrict__ p, int stride, int k, int G, double* s_red) {
+// Sum of the G partials p[0..G), fixed order, returned to all lanes.
+__device__ inline double reduce_partials(const double* __restrict__ p, int G, double* s_red) {
   double v = 0.0;
-  for (int i = threadIdx.x; i < G; i += kTpb) v += p[i * stride + k];
+  for (int i = threadIdx.x; i < G; i += kTpb) v += p[i];
   return block_sum_all(v, s_red);
 }
 
@@ -71,7 +71,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __rest
     acc += (double)b0 * (double)b0 + (double)b1 * (double)b1;
   }
   const double t = block_sum_all(acc, s_red);
-  if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = t;
+  if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = t;
   clear_flags(S);
 }
 
@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __r
                                                      const float* __restrict__ x, float* __restrict__ r,
                                                      CgScalars S, int force) {
   __shared__ double s_red[kTpb / 64];
+  __shared__ double s_tile[2][kPairTile];
   if (!force && S.flag[0]) return;
   const LeafRun R = leaf_run(T.L);
   const int32_t half = T.L * kHalfLeaf;
@@ -89,12 +90,18 @@ __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __r
   const int c0 = ((t >> 5) + (t >> 2)) & 1;
   auto get = [&](int c, int32_t i) { return (double)x[c * half + i]; };
   double acc = 0.0;
-  for (int32_t l = R.first; l < R.end; l += R.step) {
+  int buf = 0;
+  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
+    double* St = s_tile[buf];
     const FaceRow Fr = face_row(T, l);
     const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
     const double x0 = (double)x[i0], x1 = (double)x[i1];
+    pair_tile_halo(Fr, get, St);
+    St[c0 * kHalfLeaf + t] = x0;
+    St[(c0 ^ 1) * kHalfLeaf + t] = x1;
+    __syncthreads();
     double s0, s1;
-    pair_face_sums(Fr, t, get, x0, x1, s0, s1);
+    pair_tile_sums(St, t, x0, x1, s0, s1);
     const float r0 = (float)(-(double)div[i0] * (double)dx2 - (6.0 * x0 - s0));
     const float r1 = (float)(-(double)div[i1] * (double)dx2 - (6.0 * x1 - s1));
     r[i0] = r0;
@@ -102,7 +109,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __r
     acc += (double)r0 * (double)r0 + (double)r1 * (double)r1;
   }
   const double tt = block_sum_all(acc, s_red);
-  if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = tt;
+  if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
   if (force) clear_flags(S);
 }
 
@@ -116,9 +123,10 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fre
                                                   const float* __restrict__ d_in, float* __restrict__ d_out,
                                                   float* __restrict__ q, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
+  __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
   const int G = (int)gridDim.x;
-  const double rr = reduce_partials(S.pa, kCgSlots, kSlotRR, G, s_red);
+  const double rr = reduce_partials(S.pa + kSlotRR * kCgMaxGroups, G, s_red);
   const double bb = it == 0 ? rr : S.bb[0];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (rr <= S.tol2 * bb) {  // identical decision in every workgroup
@@ -131,8 +139,8 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fre
   }
   double gam = rr, beta = 0.0;
   if (PRE) {
-    gam = reduce_partials(S.pa, kCgSlots, kSlotRZ, G, s_red);
-    if (!fresh) beta = -S.alpha[it - 1] * reduce_partials(S.pa, kCgSlots, kSlotQZ, G, s_red) / S.rho[it - 1];
+    gam = reduce_partials(S.pa + kSlotRZ * kCgMaxGroups, G, s_red);
+    if (!fresh) beta = -S.alpha[it - 1] * reduce_partials(S.pa + kSlotQZ * kCgMaxGroups, G, s_red) / S.rho[it - 1];
   } else if (!fresh) {
     beta = rr / S.rho[it - 1];
   }
@@ -146,21 +154,29 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fre
   const int t = threadIdx.x;
   const int c0 = ((t >> 5) + (t >> 2)) & 1;
   double acc = 0.0;
-  for (int32_t l = R.first; l < R.end; l += R.step) {
+  int buf = 0;
+  // d at the leaf and its face halos is staged in LDS (double-buffered: one barrier per leaf)
+  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
+    float* St = s_tile[buf];
     const FaceRow Fr = face_row(T, l);
     const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
-    float d0, d1, s0, s1;
+    float d0, d1;
     if (fresh) {
       auto get = [&](int c, int32_t i) { return src[c * half + i]; };
       d0 = src[i0];
       d1 = src[i1];
-      pair_face_sums(Fr, t, get, d0, d1, s0, s1);
+      pair_tile_halo(Fr, get, St);
     } else {
       auto get = [&](int c, int32_t i) { return fmaf(bf, d_in[c * half + i], src[c * half + i]); };
       d0 = fmaf(bf, d_in[i0], src[i0]);
       d1 = fmaf(bf, d_in[i1], src[i1]);
-      pair_face_sums(Fr, t, get, d0, d1, s0, s1);
+      pair_tile_halo(Fr, get, St);
     }
+    St[c0 * kHalfLeaf + t] = d0;
+    St[(c0 ^ 1) * kHalfLeaf + t] = d1;
+    __syncthreads();
+    float s0, s1;
+    pair_tile_sums(St, t, d0, d1, s0, s1);
     const float q0 = 6.0f * d0 - s0, q1 = 6.0f * d1 - s1;
     d_out[i0] = d0;
     d_out[i1] = d1;
@@ -179,7 +195,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
                                                     const float* __restrict__ q, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
   if (S.flag[0]) return;
-  const double sigma = reduce_partials(S.pb, 1, 0, (int)gridDim.x, s_red);
+  const double sigma = reduce_partials(S.pb, (int)gridDim.x, s_red);
   if (!(sigma > 0.0)) {  // breakdown (A is SPD; only round-off gets here); same in every workgroup
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       S.flag[1] = it;
@@ -204,7 +220,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
     }
   }
   const double tt = block_sum_all(acc, s_red);
-  if (threadIdx.x == 0) S.pa[blockIdx.x * kCgSlots + kSlotRR] = tt;
+  if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
 }
 
 // Partials of r.z and q.z (q of the previous iteration) into pa slots kSlotRZ, kSlotQZ.
@@ -227,8 +243,8 @@ __global__ __launch_bounds__(kTpb) void k_cg_rz(int32_t L, const float* __restri
   const double ta = block_sum_all(a, s_red);
   const double tb = block_sum_all(b, s_red);
   if (threadIdx.x == 0) {
-    S.pa[blockIdx.x * kCgSlots + kSlotRZ] = ta;
-    S.pa[blockIdx.x * kCgSlots + kSlotQZ] = tb;
+    S.pa[kSlotRZ * kCgMaxGroups + blockIdx.x] = ta;
+    S.pa[kSlotQZ * kCgMaxGroups + blockIdx.x] = tb;
   }
 }
 

@@ -25,35 +25,57 @@ __device__ __forceinline__ float sor_update(float s, float dv, float pOld, float
   return pOld + omega * (pGS - pOld);
 }
 
-// LPW leaves per workgroup (256 lanes each); the workgroup group index is XCD-mapped.
+// LPW leaves per workgroup (256 lanes each); the workgroup group index is XCD-mapped. The other
+// colour's plane of the leaf and the 6 half-faces of it that the sweep reads across the leaf
+// faces (32 voxels each: x faces indexed (y, z>>1), y faces (x, z>>1), z faces (x, y>>1)) are
+// staged in LDS first, so the 6 neighbour reads of every voxel come from LDS.
+constexpr int kRbTile = kHalfLeaf + 6 * 32;
 template <int COLOUR, bool FIRST, int LPW>
 __global__ __launch_bounds__(256 * LPW) void k_rbgs(DevTopo T, const float* __restrict__ div, float* __restrict__ p,
                                                     float dx2, float omega) {
-  const int32_t l = leaf_of_block() * LPW + (LPW > 1 ? (int32_t)(threadIdx.x >> 8) : 0);
-  if (l >= T.L) return;
+  __shared__ float s_o[LPW][kRbTile];
+  const int32_t lw = LPW > 1 ? (int32_t)(threadIdx.x >> 8) : 0;
+  const int32_t l = leaf_of_block() * LPW + lw;
+  const bool live = l < T.L;  // a partially filled workgroup still reaches the barrier
   const int32_t t = threadIdx.x & 255;
   const int32_t x = t >> 5, y = (t >> 2) & 7, zh = t & 3;
   const int32_t z = (zh << 1) | ((x + y + COLOUR) & 1);
   const int32_t half = T.L * kHalfLeaf;
-  const float* __restrict__ po = p + (1 - COLOUR) * half + 0;
+  const float* __restrict__ po = p + (1 - COLOUR) * half;
   float* __restrict__ pc = p + COLOUR * half;
   const int32_t base = l * kHalfLeaf;
+  float* S = s_o[lw];
+  constexpr bool kNbrs = !(FIRST && COLOUR == 0);  // the first colour-0 sweep sees p = 0
+  if (kNbrs) {
+    if (live) {
+      S[t] = po[base + t];
+      if (t < 192) {
+        const int f = t >> 5, a = (t >> 2) & 7, bh = t & 3;
+        constexpr int o = 1 - COLOUR;  // parity offset of the other colour
+        int vx, vy, vz, k;
+        switch (f) {
+          case 0: k = kNbXm; vx = 7; vy = a; vz = 2 * bh + ((a + 7 + o) & 1); break;
+          case 1: k = kNbXp; vx = 0; vy = a; vz = 2 * bh + ((a + o) & 1); break;
+          case 2: k = kNbYm; vx = a; vy = 7; vz = 2 * bh + ((a + 7 + o) & 1); break;
+          case 3: k = kNbYp; vx = a; vy = 0; vz = 2 * bh + ((a + o) & 1); break;
+          case 4: k = kNbZm; vx = a; vy = 2 * bh + ((a + 7 + o) & 1); vz = 7; break;
+          default: k = kNbZp; vx = a; vy = 2 * bh + ((a + o) & 1); vz = 0; break;
+        }
+        const int32_t nl = T.nbr[kNbrStride * l + k];
+        S[kHalfLeaf + t] = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(vx, vy, vz)];
+      }
+    }
+    __syncthreads();
+  }
+  if (!live) return;
   float s = 0.f;
-  if (!(FIRST && COLOUR == 0)) {
-    const int32_t* __restrict__ nr = T.nbr + kNbrStride * l;
-    float pxp, pxm, pyp, pym, pzp, pzm;
-    if (x < 7) pxp = po[base + cs_local(x + 1, y, z)];
-    else { const int32_t nl = nr[kNbXp]; pxp = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(0, y, z)]; }
-    if (x > 0) pxm = po[base + cs_local(x - 1, y, z)];
-    else { const int32_t nl = nr[kNbXm]; pxm = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(7, y, z)]; }
-    if (y < 7) pyp = po[base + cs_local(x, y + 1, z)];
-    else { const int32_t nl = nr[kNbYp]; pyp = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, 0, z)]; }
-    if (y > 0) pym = po[base + cs_local(x, y - 1, z)];
-    else { const int32_t nl = nr[kNbYm]; pym = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, 7, z)]; }
-    if (z < 7) pzp = po[base + cs_local(x, y, z + 1)];
-    else { const int32_t nl = nr[kNbZp]; pzp = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, y, 0)]; }
-    if (z > 0) pzm = po[base + cs_local(x, y, z - 1)];
-    else { const int32_t nl = nr[kNbZm]; pzm = nl < 0 ? 0.f : po[nl * kHalfLeaf + cs_local(x, y, 7)]; }
+  if (kNbrs) {
+    const float pxp = S[x < 7 ? cs_local(x + 1, y, z) : kHalfLeaf + 32 + y * 4 + (z >> 1)];
+    const float pxm = S[x > 0 ? cs_local(x - 1, y, z) : kHalfLeaf + 0 + y * 4 + (z >> 1)];
+    const float pyp = S[y < 7 ? cs_local(x, y + 1, z) : kHalfLeaf + 96 + x * 4 + (z >> 1)];
+    const float pym = S[y > 0 ? cs_local(x, y - 1, z) : kHalfLeaf + 64 + x * 4 + (z >> 1)];
+    const float pzp = S[z < 7 ? cs_local(x, y, z + 1) : kHalfLeaf + 160 + x * 4 + (y >> 1)];
+    const float pzm = S[z > 0 ? cs_local(x, y, z - 1) : kHalfLeaf + 128 + x * 4 + (y >> 1)];
     s = pxp + pxm + pyp + pym + pzp + pzm;
   }
   const float dv = div[COLOUR * half + base + t];
@@ -149,6 +171,7 @@ __global__ __launch_bounds__(256) void k_residual(DevTopo T, const float* __rest
                                                   const float* __restrict__ div, double inv_dx2,
                                                   double* __restrict__ partial) {
   __shared__ double s_red[2][4];
+  __shared__ double s_tile[kPairTile];
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const FaceRow Fr = face_row(T, l);
@@ -158,8 +181,12 @@ __global__ __launch_bounds__(256) void k_residual(DevTopo T, const float* __rest
   const int32_t i0 = c0 * half + l * kHalfLeaf + n, i1 = (c0 ^ 1) * half + l * kHalfLeaf + n;
   auto get = [&](int c, int32_t i) { return (double)p[c * half + i]; };
   const double p0 = (double)p[i0], p1 = (double)p[i1];
+  pair_tile_halo(Fr, get, s_tile);
+  s_tile[c0 * kHalfLeaf + n] = p0;
+  s_tile[(c0 ^ 1) * kHalfLeaf + n] = p1;
+  __syncthreads();
   double s0, s1;
-  pair_face_sums(Fr, n, get, p0, p1, s0, s1);
+  pair_tile_sums(s_tile, n, p0, p1, s0, s1);
   const double d0 = (double)div[i0], d1 = (double)div[i1];
   const double r0 = d0 - (s0 - 6.0 * p0) * inv_dx2, r1 = d1 - (s1 - 6.0 * p1) * inv_dx2;
   double a = r0 * r0 + r1 * r1, b = d0 * d0 + d1 * d1;
