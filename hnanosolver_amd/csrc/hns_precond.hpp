@@ -85,6 +85,14 @@ void launch_pc_output(const DevTopo& T, const __bf16* x, const __bf16* w, const 
 void launch_pc_input_pool(const DevTopo& T, const float* r_cs, const __bf16* w, __bf16* h0, __bf16* p1, hipStream_t s);
 void launch_pc_output_fused(const DevTopo& T, const __bf16* h0, const __bf16* g1, const __bf16* w,
                             const float* params_out, float* z_cs, hipStream_t s);
+// Fused level-1 inference (one wave per leaf, 6^3 tile in LDS). mode 0: ya = h1 = relu(conv3(xa; c1)),
+// yb = p2 = pool(h1). mode 1: yb = g1 = u1(relu(conv3(xa + up(xb); c5))) (xa = h1, xb = g2, w2 = u1).
+void launch_pc_level1(const DevTopo& T, int mode, const __bf16* xa, const __bf16* xb, const __bf16* w,
+                      const __bf16* w2, __bf16* ya, __bf16* yb, hipStream_t s);
+// Fused level-2 inference (8 leaves per workgroup, 4^3 tiles in LDS). mode 0: y = h2 = relu(conv3(x; c2)).
+// mode 1: y = g2 = u2(relu(conv3(x; c3))) (x = h2, w2 = u2).
+void launch_pc_level2(const DevTopo& T, int mode, const __bf16* x, const __bf16* w, const __bf16* w2, __bf16* y,
+                      hipStream_t s);
 // y_coarse = scale * sum of the 8 children of x (level `level` -> level + 1)
 void launch_pc_pool(int32_t L, int level, int c, const __bf16* x, __bf16* y, float scale, hipStream_t s);
 // y = a + scale * up(b) at level `level` (b at level + 1); with mask: y = (mask > 0) ? y : 0

@@ -40,6 +40,34 @@ __device__ inline double block_sum_all(double v, double* s_red) {
   return t;
 }
 
+// Sums of N rows of partials (row k at p + k * kCgMaxGroups, G entries each), one pass and one
+// barrier pair for all rows, each in the same fixed order as reduce_partials; all lanes get them.
+template <int N>
+__device__ inline void reduce_partial_rows(const double* __restrict__ p, int G, double* s_red, double (&out)[N]) {
+  double v[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = 0.0;
+  for (int i = threadIdx.x; i < G; i += kTpb)
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += p[k * kCgMaxGroups + i];
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) s_red[k * (kTpb / 64) + (threadIdx.x >> 6)] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kTpb / 64; ++i) t += s_red[k * (kTpb / 64) + i];
+    out[k] = t;
+  }
+  __syncthreads();
+}
+
 // Sum of the G partials p[0..G), fixed order, returned to all lanes.
 __device__ inline double reduce_partials(const double* __restrict__ p, int G, double* s_red) {
   double v = 0.0;
@@ -122,11 +150,14 @@ template <bool PRE>
 __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fresh, const float* __restrict__ src,
                                                   const float* __restrict__ d_in, float* __restrict__ d_out,
                                                   float* __restrict__ q, CgScalars S) {
-  __shared__ double s_red[kTpb / 64];
+  __shared__ double s_red[3 * (kTpb / 64)];
   __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
   const int G = (int)gridDim.x;
-  const double rr = reduce_partials(S.pa + kSlotRR * kCgMaxGroups, G, s_red);
+  static_assert(kSlotRR == 0 && kSlotRZ == 1 && kSlotQZ == 2, "slot rows read together");
+  double rows[PRE ? 3 : 1];
+  reduce_partial_rows(S.pa, G, s_red, rows);
+  const double rr = rows[kSlotRR];
   const double bb = it == 0 ? rr : S.bb[0];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (rr <= S.tol2 * bb) {  // identical decision in every workgroup
@@ -139,8 +170,8 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fre
   }
   double gam = rr, beta = 0.0;
   if (PRE) {
-    gam = reduce_partials(S.pa + kSlotRZ * kCgMaxGroups, G, s_red);
-    if (!fresh) beta = -S.alpha[it - 1] * reduce_partials(S.pa + kSlotQZ * kCgMaxGroups, G, s_red) / S.rho[it - 1];
+    gam = rows[PRE ? kSlotRZ : 0];
+    if (!fresh) beta = -S.alpha[it - 1] * rows[PRE ? kSlotQZ : 0] / S.rho[it - 1];
   } else if (!fresh) {
     beta = rr / S.rho[it - 1];
   }

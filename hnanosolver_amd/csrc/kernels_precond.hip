@@ -324,62 +324,366 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
 
 // z = relu(conv3(s0; c6)) . w_out with s0 = h0 + up(g1) formed in LDS over the leaf and its halo.
 // Same arithmetic as k_pc_upadd at level 0 followed by the ACT 3 output conv.
-__global__ __launch_bounds__(kTpb) void k_pc_output_fused(DevTopo T, const __bf16* __restrict__ h0,
-                                                          const __bf16* __restrict__ g1,
-                                                          const bf16x8* __restrict__ w,
-                                                          const float* __restrict__ wout, float* __restrict__ z) {
-  __shared__ int32_t s_nb[27];
-  __shared__ __attribute__((aligned(16))) __bf16 s_s[kT03 * 16];
-  const int32_t l = leaf_of_block();
-  if (l >= T.L) return;
-  if (threadIdx.x < 27) s_nb[threadIdx.x] = T.nbr[kNbrStride * l + threadIdx.x];
-  __syncthreads();
-  for (int e = threadIdx.x; e < kT03 * 2; e += kTpb) {
-    const int c = e >> 1, part = e & 1;
-    int lx, ly, lz;
-    const int32_t nl = tile_leaf(s_nb, c, lx, ly, lz);
-    bf16x8 o = zero8();
-    if (nl >= 0) {
-      const bf16x8 va = *reinterpret_cast<const bf16x8*>(h0 + ((int64_t)nl * kLeafVox + ((lx << 6) | (ly << 3) | lz)) * 16 +
-                                                         8 * part);
-      const int32_t pc = (nl << 6) | ((lx >> 1) << 4) | ((ly >> 1) << 2) | (lz >> 1);
-      const bf16x8 vb = *reinterpret_cast<const bf16x8*>(g1 + (int64_t)pc * 16 + 8 * part);
-#pragma unroll
-      for (int q = 0; q < 8; ++q) o[q] = (__bf16)((float)va[q] + 1.0f * (float)vb[q]);
-    }
-    *reinterpret_cast<bf16x8*>(s_s + c * 16 + 8 * part) = o;
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+//
+// Persistent workgroups, four per compute unit, walk a strided run of leaves with one LDS buffer
+// each. A leaf's h0 halo tile and g1 parent tile arrive by direct global-to-LDS loads (no
+// registers held), s0 = h0 + up(g1) is formed in place, then the MFMAs run; while one workgroup
+// waits for its loads the others on the compute unit keep the matrix cores busy. The neighbour row
+// of the next leaf is staged in LDS one leaf ahead, so no tile load waits on a lookup. Absent
+// neighbour leaves load a zero chunk.
+__device__ __attribute__((aligned(16))) __bf16 g_pc_zero[8];  // zero-initialised
+typedef __attribute__((address_space(3))) void lds_void;
+constexpr int kOutTileB = kT03 * 32;                     // s0 tile: 1000 cells x 16 bf16
+constexpr int kOutParB = 216 * 32;                       // g1 parent tile: 6^3 level-1 cells x 16 bf16
+constexpr int kOutBufB = kOutTileB + kOutParB;
+constexpr int kOutChunks = kT03 * 2, kOutParChunks = 216 * 2;
+constexpr int kOutIt = (kOutChunks + kTpb - 1) / kTpb;   // 8 tile chunks per lane
+constexpr int kOutParIt = (kOutParChunks + kTpb - 1) / kTpb;  // 2 parent chunks per lane
+// element offset (bf16) of tap t's cell row relative to the output cell's row in the 10^3 tile
+constexpr int out_tap_off(int t) { return t < 27 ? ((t / 9) * 100 + ((t / 3) % 3) * 10 + t % 3) * 16 : 0; }
+
+__global__ __launch_bounds__(kTpb, 4) void k_pc_output_fused(DevTopo T, const __bf16* __restrict__ h0,
+                                                             const __bf16* __restrict__ g1,
+                                                             const bf16x8* __restrict__ w,
+                                                             const float* __restrict__ wout, float* __restrict__ z) {
+  __shared__ __attribute__((aligned(16))) unsigned char s_buf[kOutBufB];
+  __shared__ int32_t s_nb[2][27];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, j = lane & 15;
+  const LeafRun R = leaf_run(T.L);
+  if (R.first >= R.end) return;  // uniform per workgroup
   constexpr int NK = (27 * 16 + 31) / 32;
   bf16x8 a[NK];
 #pragma unroll
   for (int kc = 0; kc < NK; ++kc) a[kc] = w[kc * 64 + lane];
+  const bool hi = g >= 2;                             // lanes of the odd tap of each k-step
+  const bool pad = hi && 2 * (NK - 1) + 1 >= 27;      // the last step's odd tap is padding
+  // lane j's cell in tile t is (t, 2 wave + (j >> 3), j & 7); its tile row is rowj + 1600 t
+  const int rowj = tile0(0, 2 * wave + (j >> 3), j & 7) * 16 + 8 * (g & 1);
+  // the odd tap of a k-step lies 16, 128 or 1248 elements after the even one
+  const int rb16 = rowj + (hi ? 16 : 0), rb128 = rowj + (hi ? 128 : 0), rb1248 = rowj + (hi ? 1248 : 0);
   float wo[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) wo[v] = (float)(__bf16)wout[4 * g + v];
   const int32_t half = T.L * kHalfLeaf;
-  for (int tile = wave; tile < kLeafVox / 16; tile += 4) {
-    const int cell = 16 * tile + j;
-    const int x = cell >> 6, y = (cell >> 3) & 7, zz = cell & 7;
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  __bf16* S = reinterpret_cast<__bf16*>(s_buf);
+  const __bf16* P = reinterpret_cast<const __bf16*>(s_buf + kOutTileB);
+  // Chunk descriptors, the same for every leaf: neighbour slot (bits 0-4), local voxel or level-1
+  // cell (bits 5-13), parent cell in the parent tile (bits 14-21, tile chunks only), data half
+  // (bit 22).
+  int desc[kOutIt + kOutParIt];
 #pragma unroll
+  for (int it = 0; it < kOutIt; ++it) {
+    const int e = min(tid + kTpb * it, kOutChunks - 1), c = e >> 1;
+    const int x = c / 100 - 1, y = (c / 10) % 10 - 1, zz = c % 10 - 1;
+    const int slot = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + ((zz >> 3) + 1);
+    const int loc = ((x & 7) << 6) | ((y & 7) << 3) | (zz & 7);
+    const int par = (((x >> 1) + 1) * 6 + ((y >> 1) + 1)) * 6 + ((zz >> 1) + 1);
+    desc[it] = slot | (loc << 5) | (par << 14) | ((e & 1) << 22);
+  }
+#pragma unroll
+  for (int it = 0; it < kOutParIt; ++it) {
+    const int e = min(tid + kTpb * it, kOutParChunks - 1), pc = e >> 1;
+    const int x = pc / 36 - 1, y = (pc / 6) % 6 - 1, zz = pc % 6 - 1;
+    const int slot = ((x >> 2) + 1) * 9 + ((y >> 2) + 1) * 3 + ((zz >> 2) + 1);
+    const int loc = ((x & 3) << 4) | ((y & 3) << 2) | (zz & 3);
+    desc[kOutIt + it] = slot | (loc << 5) | ((e & 1) << 22);
+  }
+  if (tid < 27) s_nb[0][tid] = T.nbr[kNbrStride * R.first + tid];
+  int nb = 0;
+  for (int32_t l = R.first; l < R.end; l += R.step, nb ^= 1) {
+    const int32_t ln = l + R.step;
+    __syncthreads();  // this leaf's neighbour row is visible; the previous leaf's MFMAs are done
+    // Every source address is formed (reading the neighbour row from LDS) before the first tile
+    // load is issued: an LDS read issued behind an LDS-bound load waits for that load.
+    const __bf16* src[kOutIt + kOutParIt];
+#pragma unroll
+    for (int it = 0; it < kOutIt + kOutParIt; ++it) {
+      int d = desc[it];
+      asm volatile("" : "+v"(d));  // keep derived values per leaf, not in registers across the loop
+      const int32_t nl = s_nb[nb][d & 31];
+      const int loc = (d >> 5) & 511, hf = (d >> 22) & 1;
+      src[it] = nl < 0 ? g_pc_zero
+                       : (it < kOutIt ? h0 + ((int64_t)nl * kLeafVox + loc) * 16 : g1 + ((int64_t)nl * 64 + loc) * 16) + 8 * hf;
+    }
+    const int tv = tid;
+#pragma unroll
+    for (int it = 0; it < kOutIt; ++it)
+      if (tv + kTpb * it < kOutChunks)
+        __builtin_amdgcn_global_load_lds(src[it], (lds_void*)(s_buf + (kTpb * it + 64 * wave) * 16), 16, 0, 0);
+#pragma unroll
+    for (int it = 0; it < kOutParIt; ++it)
+      if (tv + kTpb * it < kOutParChunks)
+        __builtin_amdgcn_global_load_lds(src[kOutIt + it], (lds_void*)(s_buf + kOutTileB + (kTpb * it + 64 * wave) * 16),
+                                         16, 0, 0);
+    if (tid < 27 && ln < R.end) s_nb[nb ^ 1][tid] = T.nbr[kNbrStride * ln + tid];
+    __builtin_amdgcn_s_waitcnt(0);  // this lane's tile loads have landed
+    __syncthreads();                // and everyone's
+    // s0 = h0 + up(g1) in place
+#pragma unroll 2
+    for (int it = 0; it < kOutIt; ++it) {
+      const int e = tv + kTpb * it;
+      if (e < kOutChunks) {
+        const int p = (desc[it] >> 14) & 255;
+        const bf16x8 va = *reinterpret_cast<const bf16x8*>(S + e * 8);
+        const bf16x8 vb = *reinterpret_cast<const bf16x8*>(P + p * 16 + 8 * (e & 1));
+        bf16x8 o;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) o[q] = (__bf16)((float)va[q] + 1.0f * (float)vb[q]);
+        *reinterpret_cast<bf16x8*>(S + e * 8) = o;
+      }
+    }
+    __syncthreads();
+    // Tiles go in pairs with two independent accumulators; the 8 x NK (tile, k-step) MFMAs run as
+    // one unrolled sequence with a ring of kRing B operands read ahead of their MFMA. Step st:
+    // pair st / (2 NK), k-step (st / 2) % NK, tile 2 pair + (st & 1).
+    constexpr int kRing = 4, kSteps = 8 * NK;
+    auto bsrc = [&](int st) {
+      const int t = 2 * (st / (2 * NK)) + (st & 1), kc = (st >> 1) % NK;
+      const int d = out_tap_off(2 * kc + 1) - out_tap_off(2 * kc);  // 16, 128, 1248 (or padding)
+      const int rb = d == 16 ? rb16 : (d == 128 ? rb128 : (d == 1248 ? rb1248 : rowj));
+      return reinterpret_cast<const bf16x8*>(S + rb + out_tap_off(2 * kc) + 1600 * t);
+    };
+    bf16x8 ring[kRing];
+#pragma unroll
+    for (int st = 0; st < kRing; ++st) ring[st] = *bsrc(st);
+    floatx4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+#pragma unroll
+    for (int st = 0; st < kSteps; ++st) {
+      const int u = st & 1, kc = (st >> 1) % NK, t = 2 * (st / (2 * NK)) + u;
+      const bf16x8 bv = ring[st % kRing];
+      if (st + kRing < kSteps) ring[st % kRing] = *bsrc(st + kRing);
+      acc[u] = mfma(a[kc], (kc == NK - 1 && pad) ? zero8() : bv, acc[u]);
+      if (kc == NK - 1) {
+        const int y = 2 * wave + (j >> 3), zz = j & 7;
+        float part = 0.f;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) part += (float)(__bf16)fmaxf(acc[u][v], 0.f) * wo[v];
+        part += __shfl_xor(part, 16, 64);
+        part += __shfl_xor(part, 32, 64);
+        if (g == 0) z[((t + y + zz) & 1) * half + l * kHalfLeaf + cs_local(t, y, zz)] = part;
+        acc[u] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
+    }
+  }
+}
+
+// ---- fused level-1 and level-2 inference kernels ----------------------------------------------
+// Each stages a halo tile of its input in LDS (absent leaves as 0, rows padded by 16 bytes so
+// consecutive rows start in different banks) and runs the same MFMA sequence per cell as
+// k_pc_conv (same K order), so results are bitwise those of the unfused kernels.
+
+// Leaf holding the tile cell (tx,ty,tz) of a tile of edge S+2 around a leaf of edge S = 1 << lgS
+// (tile coordinate t is level coordinate t-1), or -1, with the local coordinates in that leaf.
+__device__ __forceinline__ int32_t halo_leaf(const int32_t* __restrict__ nrow, int lgS, int tx, int ty, int tz,
+                                             int& lx, int& ly, int& lz) {
+  const int x = tx - 1, y = ty - 1, z = tz - 1, m = (1 << lgS) - 1;
+  lx = x & m;
+  ly = y & m;
+  lz = z & m;
+  return nrow[((x >> lgS) + 1) * 9 + ((y >> lgS) + 1) * 3 + ((z >> lgS) + 1)];
+}
+
+// Level 1 (4^3 cells per leaf): one wave per leaf, four leaves per workgroup. The wave stages its
+// leaf's 6^3 tile and computes all 4 cell tiles x both output-channel tiles, so every B operand
+// is read once from LDS and every A operand once per leaf.
+//   MODE 0 (c1):      tile = p1 (16 ch); h1 = relu(conv) to ya; p2 = pool(h1) to yb (level 2)
+//   MODE 1 (c5, u1):  tile = h1 + up(g2) (xa = h1, xb = g2); h5 = relu(conv) kept in LDS;
+//                     g1 = u1(h5) to yb (16 ch)
+constexpr int kT1 = 6, kT13 = kT1 * kT1 * kT1;
+template <int MODE>
+__global__ __launch_bounds__(kTpb) void k_pc_level1(DevTopo T, const __bf16* __restrict__ xa,
+                                                    const __bf16* __restrict__ xb, const bf16x8* __restrict__ w,
+                                                    const bf16x8* __restrict__ w2, __bf16* __restrict__ ya,
+                                                    __bf16* __restrict__ yb) {
+  constexpr int CIN = MODE == 0 ? 16 : 32, MT = 2, NK = (27 * CIN + 31) / 32, R = CIN + 8;
+  __shared__ __attribute__((aligned(16))) __bf16 s_t[4][kT13 * R];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int32_t grp = leaf_of_block();
+  if (grp * 4 >= T.L) return;  // uniform per workgroup
+  const int32_t l = grp * 4 + wave;
+  const bool live = l < T.L;
+  __bf16* S = s_t[wave];
+  if (live) {
+    const int32_t* nrow = T.nbr + kNbrStride * l;
+    for (int e = lane; e < kT13 * (CIN / 8); e += 64) {
+      const int c = e / (CIN / 8), part = e % (CIN / 8);
+      int lx, ly, lz;
+      const int32_t nl = halo_leaf(nrow, 2, c / 36, (c / 6) % 6, c % 6, lx, ly, lz);
+      bf16x8 o = zero8();
+      if (nl >= 0) {
+        const int32_t cell = (nl << 6) | (lx << 4) | (ly << 2) | lz;
+        const bf16x8 va = *reinterpret_cast<const bf16x8*>(xa + (int64_t)cell * CIN + 8 * part);
+        if (MODE == 0) {
+          o = va;
+        } else {
+          const int32_t pc = (nl << 3) | ((lx >> 1) << 2) | ((ly >> 1) << 1) | (lz >> 1);
+          const bf16x8 vb = *reinterpret_cast<const bf16x8*>(xb + (int64_t)pc * CIN + 8 * part);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) o[q] = (__bf16)((float)va[q] + 1.0f * (float)vb[q]);
+        }
+      }
+      *reinterpret_cast<bf16x8*>(S + c * R + 8 * part) = o;
+    }
+  }
+  __syncthreads();
+  floatx4 acc[4][MT];
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int m = 0; m < MT; ++m) acc[t][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int cy = j >> 2, cz = j & 3;  // cell 16 t + j has cx = t
+  if (live) {
     for (int kc = 0; kc < NK; ++kc) {
       const int kb = 32 * kc + 8 * g;
-      const int tap = kb >> 4, ci0 = kb & 15;
+      const int tap = kb / CIN, ci0 = kb % CIN;
+      const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
+      bf16x8 a[MT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) a[m] = w[(kc * MT + m) * 64 + lane];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        bf16x8 b = zero8();
+        if (tap < 27) b = *reinterpret_cast<const bf16x8*>(S + (((t + dx) * kT1 + cy + dy) * kT1 + cz + dz) * R + ci0);
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[t][m] = mfma(a[m], b, acc[t][m]);
+      }
+    }
+  }
+  __syncthreads();  // the tile region now holds the 64 x 32 relu'd outputs (cell * 32 + channel)
+  if (live) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        bf16x4 o;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][m][v], 0.f);
+        const int cell = 16 * t + j;
+        if (MODE == 0) *reinterpret_cast<bf16x4*>(ya + ((int64_t)l * 64 + cell) * 32 + 16 * m + 4 * g) = o;
+        *reinterpret_cast<bf16x4*>(S + cell * 32 + 16 * m + 4 * g) = o;
+      }
+  }
+  __syncthreads();
+  if (!live) return;
+  if (MODE == 0) {
+    // p2: 8 coarse cells x 8 groups of 4 channels, children in k_pc_pool's order
+    const int cc = lane >> 3, q4 = (lane & 7) * 4;
+    const int cx = cc >> 2, ccy = (cc >> 1) & 1, ccz = cc & 1;
+    float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int fc = ((2 * cx + (k >> 2)) << 4) | ((2 * ccy + ((k >> 1) & 1)) << 2) | (2 * ccz + (k & 1));
+      const bf16x4 v = *reinterpret_cast<const bf16x4*>(S + fc * 32 + q4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sum[q] += (float)v[q];
+    }
+    bf16x4 o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = (__bf16)(sum[q] * 0.125f);
+    *reinterpret_cast<bf16x4*>(yb + ((int64_t)l * 8 + cc) * 32 + q4) = o;
+  } else {
+    // g1 = u1(h5): K = 32 (one MFMA step), 16 output channels
+    const bf16x8 a = w2[lane];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + (16 * t + j) * 32 + 8 * g);
+      const floatx4 r = mfma(a, b, floatx4{0.f, 0.f, 0.f, 0.f});
+      bf16x4 o;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = (__bf16)r[v];
+      *reinterpret_cast<bf16x4*>(yb + ((int64_t)l * 64 + 16 * t + j) * 16 + 4 * g) = o;
+    }
+  }
+}
+
+// Level 2 (2^3 cells per leaf): a workgroup takes 8 consecutive leaves and stages their 4^3 tiles;
+// wave w computes output-channel tile w for all 64 cells (4 cell tiles of 2 leaves each), so the
+// weights of a layer are read once per 8 leaves.
+//   MODE 0 (c2):      tile = p2 (32 ch); h2 = relu(conv) to y (64 ch)
+//   MODE 1 (c3, u2):  tile = h2 (64 ch); h3 = relu(conv) kept in LDS; g2 = u2(h3) to y (32 ch)
+constexpr int kT2 = 4, kT23 = kT2 * kT2 * kT2;
+template <int MODE>
+__global__ __launch_bounds__(kTpb) void k_pc_level2(DevTopo T, const __bf16* __restrict__ x,
+                                                    const bf16x8* __restrict__ w, const bf16x8* __restrict__ w2,
+                                                    __bf16* __restrict__ y) {
+  constexpr int CIN = MODE == 0 ? 32 : 64, MT = 4, NK = (27 * CIN + 31) / 32, R = CIN + 8;
+  __shared__ __attribute__((aligned(16))) __bf16 S[8 * kT23 * R];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, j = lane & 15;
+  const int32_t l0 = leaf_of_block() * 8;
+  if (l0 >= T.L) return;  // uniform per workgroup
+  const int nlive = min(8, T.L - l0);
+  for (int e = threadIdx.x; e < 8 * kT23 * (CIN / 8); e += kTpb) {
+    const int slot = e / (kT23 * (CIN / 8)), rem = e % (kT23 * (CIN / 8));
+    const int c = rem / (CIN / 8), part = rem % (CIN / 8);
+    bf16x8 o = zero8();
+    if (slot < nlive) {
+      int lx, ly, lz;
+      const int32_t nl = halo_leaf(T.nbr + kNbrStride * (l0 + slot), 1, c >> 4, (c >> 2) & 3, c & 3, lx, ly, lz);
+      if (nl >= 0)
+        o = *reinterpret_cast<const bf16x8*>(x + (int64_t)((nl << 3) | (lx << 2) | (ly << 1) | lz) * CIN + 8 * part);
+    }
+    *reinterpret_cast<bf16x8*>(S + (slot * kT23 + c) * R + 8 * part) = o;
+  }
+  __syncthreads();
+  // lane j of cell tile t: leaf slot 2 t + (j >> 3), local cell j & 7
+  const int sj = j >> 3, cj = j & 7, cx = cj >> 2, cy = (cj >> 1) & 1, cz = cj & 1;
+  const int m = wave;
+  floatx4 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int kc = 0; kc < NK; ++kc) {
+    const int kb = 32 * kc + 8 * g;
+    const int tap = kb / CIN, ci0 = kb % CIN;
+    const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
+    const bf16x8 a = w[(kc * MT + m) * 64 + lane];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
       bf16x8 b = zero8();
       if (tap < 27)
         b = *reinterpret_cast<const bf16x8*>(
-            s_s + tile0(x + tap / 9, y + (tap / 3) % 3, zz + tap % 3) * 16 + ci0);
-      acc = mfma(a[kc], b, acc);
+            S + ((2 * t + sj) * kT23 + ((cx + dx) * kT2 + cy + dy) * kT2 + cz + dz) * R + ci0);
+      acc[t] = mfma(a, b, acc[t]);
     }
-    float part = 0.f;
+  }
+  if (MODE == 0) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) part += (float)(__bf16)fmaxf(acc[v], 0.f) * wo[v];
-    part += __shfl_xor(part, 16, 64);
-    part += __shfl_xor(part, 32, 64);
-    if (g == 0) z[((x + y + zz) & 1) * half + l * kHalfLeaf + cs_local(x, y, zz)] = part;
+    for (int t = 0; t < 4; ++t) {
+      if (2 * t + sj >= nlive) continue;
+      bf16x4 o;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][v], 0.f);
+      *reinterpret_cast<bf16x4*>(y + ((int64_t)(l0 + 2 * t + sj) * 8 + cj) * 64 + 16 * m + 4 * g) = o;
+    }
+    return;
+  }
+  __syncthreads();  // the tile region now holds h3 for the 64 cells (cell * 64 + channel)
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    bf16x4 o;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][v], 0.f);
+    *reinterpret_cast<bf16x4*>(S + ((2 * t + sj) * 8 + cj) * 64 + 16 * m + 4 * g) = o;
+  }
+  __syncthreads();
+  // g2 = u2(h3): K = 64 (two MFMA steps), 32 output channels; wave w: output tile w & 1, cell tiles 2 (w >> 1) + {0, 1}
+  const int m2 = wave & 1;
+#pragma unroll
+  for (int tt = 0; tt < 2; ++tt) {
+    const int t = 2 * (wave >> 1) + tt;
+    floatx4 r = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kc = 0; kc < 2; ++kc) {
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + ((2 * t + sj) * 8 + cj) * 64 + 32 * kc + 8 * g);
+      r = mfma(w2[(kc * 2 + m2) * 64 + lane], b, r);
+    }
+    if (2 * t + sj >= nlive) continue;
+    bf16x4 o;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o[v] = (__bf16)r[v];
+    *reinterpret_cast<bf16x4*>(y + ((int64_t)(l0 + 2 * t + sj) * 8 + cj) * 32 + 16 * m2 + 4 * g) = o;
   }
 }
 
@@ -689,6 +993,18 @@ __global__ __launch_bounds__(kTpb) void k_pc_sum3(const double* __restrict__ par
 }
 
 inline unsigned blocks_for(int64_t n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
+// Workgroups of a persistent leaf-run kernel: `per_cu` resident workgroups on each of the
+// device's compute units (a multiple of 8, the XCD count), at most one per leaf.
+unsigned persistent_groups(int32_t L, int per_cu) {
+  static const int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 256;
+    return n;
+  }();
+  const unsigned want = 8u * (unsigned)((cus * per_cu + 7) / 8);
+  return std::min(want, grid_for_leaves(L));
+}
 constexpr int kNT = 2;  // 16-cell tiles per wave
 
 // number of cell ranges of a weight-gradient launch
@@ -772,8 +1088,32 @@ void launch_pc_input_pool(const DevTopo& T, const float* r_cs, const __bf16* w, 
 
 void launch_pc_output_fused(const DevTopo& T, const __bf16* h0, const __bf16* g1, const __bf16* w,
                             const float* params_out, float* z_cs, hipStream_t s) {
-  k_pc_output_fused<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, h0, g1, reinterpret_cast<const bf16x8*>(w), params_out,
+  k_pc_output_fused<<<persistent_groups(T.L, 4), kTpb, 0, s>>>(T, h0, g1, reinterpret_cast<const bf16x8*>(w), params_out,
                                                           z_cs);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_level1(const DevTopo& T, int mode, const __bf16* xa, const __bf16* xb, const __bf16* w,
+                      const __bf16* w2, __bf16* ya, __bf16* yb, hipStream_t s) {
+  const unsigned g = grid_for_leaves((T.L + 3) / 4);
+  const bf16x8* wv = reinterpret_cast<const bf16x8*>(w);
+  const bf16x8* w2v = reinterpret_cast<const bf16x8*>(w2);
+  if (mode == 0)
+    k_pc_level1<0><<<g, kTpb, 0, s>>>(T, xa, xb, wv, w2v, ya, yb);
+  else
+    k_pc_level1<1><<<g, kTpb, 0, s>>>(T, xa, xb, wv, w2v, ya, yb);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_pc_level2(const DevTopo& T, int mode, const __bf16* x, const __bf16* w, const __bf16* w2, __bf16* y,
+                      hipStream_t s) {
+  const unsigned g = grid_for_leaves((T.L + 7) / 8);
+  const bf16x8* wv = reinterpret_cast<const bf16x8*>(w);
+  const bf16x8* w2v = reinterpret_cast<const bf16x8*>(w2);
+  if (mode == 0)
+    k_pc_level2<0><<<g, kTpb, 0, s>>>(T, x, wv, w2v, y);
+  else
+    k_pc_level2<1><<<g, kTpb, 0, s>>>(T, x, wv, w2v, y);
   HNS_HIP(hipGetLastError());
 }
 

@@ -8,8 +8,8 @@ namespace hns {
 
 namespace {
 constexpr int64_t kWgradPartialFloats = 64LL * 27 * 64 * 64;  // ranges x largest layer
-// HNS_PRECOND_UNFUSED=1 runs inference through the unfused level-0 kernels (for comparison)
-bool fused_level0() {
+// HNS_PRECOND_UNFUSED=1 runs inference through the layer-by-layer kernels (for comparison)
+bool fused_inference() {
   static const bool v = [] {
     const char* e = std::getenv("HNS_PRECOND_UNFUSED");
     return !(e && e[0] == '1');
@@ -75,14 +75,19 @@ void LearnedPrecond::forward(const DevTopo& T, PcWork& W, const float* r_cs, flo
   const PcOffsets& O = pc_offsets();
   const __bf16* wf = wfwd_.p;
   const int32_t L = T.L;
-  // inference fuses the level-0 work into two leaf-tiled kernels; training keeps every activation
-  const bool fused = !keep && fused_level0();
-  if (fused) {
+  // inference runs six leaf-tiled kernels (pool, upsample-add and the 1x1 convs fused into the
+  // 3x3x3 ones); training keeps every activation and runs the layers one by one
+  if (!keep && fused_inference()) {
     launch_pc_input_pool(T, r_cs, wf + O.fwd[kC0], W.h0.p, W.p1.p, s);
-  } else {
-    launch_pc_input(T, r_cs, wf + O.fwd[kC0], W.h0.p, s);
-    launch_pc_pool(L, 0, 16, W.h0.p, W.p1.p, 0.125f, s);
+    launch_pc_level1(T, 0, W.p1.p, nullptr, wf + O.fwd[kC1], nullptr, W.h1.p, W.p2.p, s);
+    launch_pc_level2(T, 0, W.p2.p, wf + O.fwd[kC2], nullptr, W.h2.p, s);
+    launch_pc_level2(T, 1, W.h2.p, wf + O.fwd[kC3], wf + O.fwd[kU2], W.g2.p, s);
+    launch_pc_level1(T, 1, W.h1.p, W.g2.p, wf + O.fwd[kC5], wf + O.fwd[kU1], nullptr, W.g1.p, s);
+    launch_pc_output_fused(T, W.h0.p, W.g1.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, s);
+    return;
   }
+  launch_pc_input(T, r_cs, wf + O.fwd[kC0], W.h0.p, s);
+  launch_pc_pool(L, 0, 16, W.h0.p, W.p1.p, 0.125f, s);
   launch_pc_conv(T, 1, 27, 16, 32, W.p1.p, wf + O.fwd[kC1], W.h1.p, 1, nullptr, s);
   launch_pc_pool(L, 1, 32, W.h1.p, W.p2.p, 0.125f, s);
   launch_pc_conv(T, 2, 27, 32, 64, W.p2.p, wf + O.fwd[kC2], W.h2.p, 1, nullptr, s);
@@ -91,12 +96,8 @@ void LearnedPrecond::forward(const DevTopo& T, PcWork& W, const float* r_cs, flo
   launch_pc_upadd(L, 1, 32, W.h1.p, W.g2.p, W.s1.p, 1.0f, nullptr, s);
   launch_pc_conv(T, 1, 27, 32, 32, W.s1.p, wf + O.fwd[kC5], W.h5.p, 1, nullptr, s);
   launch_pc_conv(T, 1, 1, 32, 16, W.h5.p, wf + O.fwd[kU1], W.g1.p, 0, nullptr, s);
-  if (fused) {
-    launch_pc_output_fused(T, W.h0.p, W.g1.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, s);
-  } else {
-    launch_pc_upadd(L, 0, 16, W.h0.p, W.g1.p, W.s0.p, 1.0f, nullptr, s);
-    launch_pc_output(T, W.s0.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, keep ? W.h6.p : nullptr, s);
-  }
+  launch_pc_upadd(L, 0, 16, W.h0.p, W.g1.p, W.s0.p, 1.0f, nullptr, s);
+  launch_pc_output(T, W.s0.p, wf + O.fwd[kC6], params_ + O.param[kOut], z_cs, keep ? W.h6.p : nullptr, s);
 }
 
 void LearnedPrecond::backward(const DevTopo& T, PcWork& W, const float* r_cs, const float* dz_cs, float* grad,

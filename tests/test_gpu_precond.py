@@ -78,13 +78,14 @@ def test_forward_matches_dense_oracle(hip):
         pc.close()
 
 
-def test_fused_level0_inference_matches_unfused(hip):
-    """The leaf-tiled level-0 inference kernels give bitwise the unfused kernels' output (the
-    kept-activation path runs unfused)."""
+def test_fused_inference_matches_unfused(hip):
+    """The six leaf-tiled inference kernels (levels 0, 1 and 2 with pool, upsample-add and the
+    1x1 convs fused) give bitwise the layer-by-layer kernels' output (the kept-activation path runs
+    layer by layer). Leaf counts that are not multiples of 4 or 8 leave partial workgroups."""
     torch = hip
     for seed in (5, 6):
         c = compact_ragged(seed, extent=7, fill=0.5)
-        _case("fused vs unfused level 0, compact ragged seed %d, %d leaves" % (seed, len(c) // 512))
+        _case("fused vs layer-by-layer inference, compact ragged seed %d, %d leaves" % (seed, len(c) // 512))
         topo = _topo(c)
         pc = P.Preconditioner(P.load_weights())
         r_cs = _cs(torch, topo, np.random.default_rng(seed).standard_normal(len(c)).astype(np.float32))
