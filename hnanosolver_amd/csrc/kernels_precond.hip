@@ -58,6 +58,21 @@ constexpr int kTpb = 256;
 __device__ __forceinline__ floatx4 mfma(const bf16x8& a, const bf16x8& b, const floatx4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
+// Empty device asm that makes a value opaque to the optimiser, so what is derived from it is
+// recomputed where it is used instead of being hoisted into registers across a loop (v: a
+// per-lane value, s: a wave-uniform one). The host pass of a template kernel sees nothing.
+template <class T>
+__device__ __forceinline__ void opaque_v(T& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v));
+#endif
+}
+template <class T>
+__device__ __forceinline__ void opaque_s(T& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+s"(v));
+#endif
+}
 __device__ __forceinline__ bf16x8 zero8() {
   bf16x8 z;
 #pragma unroll
@@ -273,8 +288,8 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
                                                         const bf16x8* __restrict__ w, __bf16* __restrict__ h0,
                                                         __bf16* __restrict__ p1) {
   __shared__ int32_t s_nb[27];
-  __shared__ float s_r[kT03];
-  __shared__ __bf16 s_h[kLeafVox * 16];
+  __shared__ __bf16 s_r[kT03];  // r rounded to bf16, as the MFMA operand takes it
+  __shared__ __attribute__((aligned(16))) __bf16 s_h[kLeafVox * 16];
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   if (threadIdx.x < 27) s_nb[threadIdx.x] = T.nbr[kNbrStride * l + threadIdx.x];
@@ -283,22 +298,26 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
   for (int c = threadIdx.x; c < kT03; c += kTpb) {
     int lx, ly, lz;
     const int32_t nl = tile_leaf(s_nb, c, lx, ly, lz);
-    s_r[c] = nl < 0 ? 0.f : r[((lx + ly + lz) & 1) * half + nl * kHalfLeaf + cs_local(lx, ly, lz)];
+    s_r[c] = (__bf16)(nl < 0 ? 0.f : r[((lx + ly + lz) & 1) * half + nl * kHalfLeaf + cs_local(lx, ly, lz)]);
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, j = lane & 15;
   const bf16x8 a = w[lane];
+  // taps 8 g + v of this lane: tile-cell offsets relative to the output cell's (0,0,0) neighbour
+  int off[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int tap = min(8 * g + v, 26);
+    off[v] = tile0(tap / 9, (tap / 3) % 3, tap % 3);
+  }
+  const int nvalid = min(8, 27 - 8 * g);  // taps beyond 26 are padding (zero)
   for (int tile = wave; tile < kLeafVox / 16; tile += 4) {
     const int cell = 16 * tile + j;
-    const int x = cell >> 6, y = (cell >> 3) & 7, z = cell & 7;
+    const int base = tile0(cell >> 6, (cell >> 3) & 7, cell & 7);
     bf16x8 b;
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int tap = 8 * g + v;
-      b[v] = tap < 27 ? (__bf16)s_r[tile0(x + 1 + tap / 9 - 1, y + 1 + (tap / 3) % 3 - 1, z + 1 + tap % 3 - 1)]
-                      : (__bf16)0.0f;
-    }
+    for (int v = 0; v < 8; ++v) b[v] = v < nvalid ? s_r[base + off[v]] : (__bf16)0.0f;
     floatx4 acc = {0.f, 0.f, 0.f, 0.f};
     acc = mfma(a, b, acc);
     bf16x4 o;
@@ -308,17 +327,22 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
     *reinterpret_cast<bf16x4*>(s_h + cell * 16 + 4 * g) = o;
   }
   __syncthreads();
-  // p1: 64 coarse cells x 16 channels, children in k_pc_pool's order
-  for (int e = threadIdx.x; e < 64 * 16; e += kTpb) {
-    const int cc = e >> 4, ch = e & 15;
+  // p1: 64 coarse cells x 4 groups of 4 channels, one per lane, children in k_pc_pool's order
+  {
+    const int cc = threadIdx.x >> 2, q4 = (threadIdx.x & 3) * 4;
     const int cx = cc >> 4, cy = (cc >> 2) & 3, cz = cc & 3;
-    float sum = 0.f;
+    float sum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int fc = ((2 * cx + (k >> 2)) << 6) | ((2 * cy + ((k >> 1) & 1)) << 3) | (2 * cz + (k & 1));
-      sum += (float)s_h[fc * 16 + ch];
+      const bf16x4 v = *reinterpret_cast<const bf16x4*>(s_h + fc * 16 + q4);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) sum[q] += (float)v[q];
     }
-    p1[((int64_t)l * 64 + cc) * 16 + ch] = (__bf16)(sum * 0.125f);
+    bf16x4 o;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) o[q] = (__bf16)(sum[q] * 0.125f);
+    *reinterpret_cast<bf16x4*>(p1 + ((int64_t)l * 64 + cc) * 16 + q4) = o;
   }
 }
 
@@ -333,6 +357,10 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
 // neighbour leaves load a zero chunk.
 __device__ __attribute__((aligned(16))) __bf16 g_pc_zero[8];  // zero-initialised
 typedef __attribute__((address_space(3))) void lds_void;
+// 16 bytes per lane from global memory straight into LDS at dst + 16 * lane (dst wave-uniform)
+__device__ __forceinline__ void lds_load16(const void* src, void* dst) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void*)dst, 16, 0, 0);
+}
 constexpr int kOutTileB = kT03 * 32;                     // s0 tile: 1000 cells x 16 bf16
 constexpr int kOutParB = 216 * 32;                       // g1 parent tile: 6^3 level-1 cells x 16 bf16
 constexpr int kOutBufB = kOutTileB + kOutParB;
@@ -400,7 +428,7 @@ __global__ __launch_bounds__(kTpb, 4) void k_pc_output_fused(DevTopo T, const __
 #pragma unroll
     for (int it = 0; it < kOutIt + kOutParIt; ++it) {
       int d = desc[it];
-      asm volatile("" : "+v"(d));  // keep derived values per leaf, not in registers across the loop
+      opaque_v(d);  // keep derived values per leaf, not in registers across the loop
       const int32_t nl = s_nb[nb][d & 31];
       const int loc = (d >> 5) & 511, hf = (d >> 22) & 1;
       src[it] = nl < 0 ? g_pc_zero
@@ -410,12 +438,11 @@ __global__ __launch_bounds__(kTpb, 4) void k_pc_output_fused(DevTopo T, const __
 #pragma unroll
     for (int it = 0; it < kOutIt; ++it)
       if (tv + kTpb * it < kOutChunks)
-        __builtin_amdgcn_global_load_lds(src[it], (lds_void*)(s_buf + (kTpb * it + 64 * wave) * 16), 16, 0, 0);
+        lds_load16(src[it], s_buf + (kTpb * it + 64 * wave) * 16);
 #pragma unroll
     for (int it = 0; it < kOutParIt; ++it)
       if (tv + kTpb * it < kOutParChunks)
-        __builtin_amdgcn_global_load_lds(src[kOutIt + it], (lds_void*)(s_buf + kOutTileB + (kTpb * it + 64 * wave) * 16),
-                                         16, 0, 0);
+        lds_load16(src[kOutIt + it], s_buf + kOutTileB + (kTpb * it + 64 * wave) * 16);
     if (tid < 27 && ln < R.end) s_nb[nb ^ 1][tid] = T.nbr[kNbrStride * ln + tid];
     __builtin_amdgcn_s_waitcnt(0);  // this lane's tile loads have landed
     __syncthreads();                // and everyone's
@@ -484,116 +511,178 @@ __device__ __forceinline__ int32_t halo_leaf(const int32_t* __restrict__ nrow, i
   return nrow[((x >> lgS) + 1) * 9 + ((y >> lgS) + 1) * 3 + ((z >> lgS) + 1)];
 }
 
-// Level 1 (4^3 cells per leaf): one wave per leaf, four leaves per workgroup. The wave stages its
-// leaf's 6^3 tile and computes all 4 cell tiles x both output-channel tiles, so every B operand
+// Level 1 (4^3 cells per leaf): one wave per leaf. Persistent workgroups walk a strided run of
+// groups of four leaves, one per wave; each wave's 6^3 tile (and, for c5, the 4^3 tile of g2
+// parents) arrives by direct global-to-LDS loads, every address formed before the first load is
+// issued, and the wave computes all 4 cell tiles x both output-channel tiles, so every B operand
 // is read once from LDS and every A operand once per leaf.
 //   MODE 0 (c1):      tile = p1 (16 ch); h1 = relu(conv) to ya; p2 = pool(h1) to yb (level 2)
 //   MODE 1 (c5, u1):  tile = h1 + up(g2) (xa = h1, xb = g2); h5 = relu(conv) kept in LDS;
 //                     g1 = u1(h5) to yb (16 ch)
 constexpr int kT1 = 6, kT13 = kT1 * kT1 * kT1;
+// tile-cell offset of tap t (0..26; offsets (t/9, t/3%3, t%3) - 1) in a 6^3 tile; 0 for padding
+constexpr int l1_tap(int t) { return t < 27 ? ((t / 9) * kT1 + (t / 3) % 3) * kT1 + t % 3 : 0; }
 template <int MODE>
-__global__ __launch_bounds__(kTpb) void k_pc_level1(DevTopo T, const __bf16* __restrict__ xa,
+struct L1Shape {
+  static constexpr int CIN = MODE == 0 ? 16 : 32, CH = CIN / 8;   // 16-byte chunks per cell
+  static constexpr int NCH = kT13 * CH, IT = (NCH + 63) / 64;     // tile chunks, per lane
+  static constexpr int NPCH = MODE == 0 ? 0 : 64 * CH, PIT = NPCH / 64;  // parent chunks, per lane
+  static constexpr int TILE = kT13 * CIN, WAVE = TILE + 64 * CIN * (MODE == 0 ? 0 : 1);  // bf16 per wave
+};
+template <int MODE>
+__global__ __launch_bounds__(kTpb, 2) void k_pc_level1(DevTopo T, const __bf16* __restrict__ xa,
                                                     const __bf16* __restrict__ xb, const bf16x8* __restrict__ w,
                                                     const bf16x8* __restrict__ w2, __bf16* __restrict__ ya,
                                                     __bf16* __restrict__ yb) {
-  constexpr int CIN = MODE == 0 ? 16 : 32, MT = 2, NK = (27 * CIN + 31) / 32, R = CIN + 8;
-  __shared__ __attribute__((aligned(16))) __bf16 s_t[4][kT13 * R];
+  using Sh = L1Shape<MODE>;
+  constexpr int CIN = Sh::CIN, CH = Sh::CH, MT = 2, NK = (27 * CIN + 31) / 32;
+  __shared__ __attribute__((aligned(16))) __bf16 s_t[4][Sh::WAVE];
+  __shared__ int32_t s_nb[2][4][27];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, j = lane & 15;
-  const int32_t grp = leaf_of_block();
-  if (grp * 4 >= T.L) return;  // uniform per workgroup
-  const int32_t l = grp * 4 + wave;
-  const bool live = l < T.L;
+  const int32_t ngroups = (T.L + 3) / 4;
+  const LeafRun R = leaf_run(ngroups);
+  if (R.first >= R.end) return;  // uniform per workgroup
   __bf16* S = s_t[wave];
-  if (live) {
-    const int32_t* nrow = T.nbr + kNbrStride * l;
-    for (int e = lane; e < kT13 * (CIN / 8); e += 64) {
-      const int c = e / (CIN / 8), part = e % (CIN / 8);
-      int lx, ly, lz;
-      const int32_t nl = halo_leaf(nrow, 2, c / 36, (c / 6) % 6, c % 6, lx, ly, lz);
-      bf16x8 o = zero8();
-      if (nl >= 0) {
-        const int32_t cell = (nl << 6) | (lx << 4) | (ly << 2) | lz;
-        const bf16x8 va = *reinterpret_cast<const bf16x8*>(xa + (int64_t)cell * CIN + 8 * part);
-        if (MODE == 0) {
-          o = va;
-        } else {
-          const int32_t pc = (nl << 3) | ((lx >> 1) << 2) | ((ly >> 1) << 1) | (lz >> 1);
-          const bf16x8 vb = *reinterpret_cast<const bf16x8*>(xb + (int64_t)pc * CIN + 8 * part);
+  const __bf16* P = s_t[wave] + Sh::TILE;
+  // chunk descriptors (the same for every leaf): slot (bits 0-4), local level-1 cell (5-10) or
+  // level-2 cell (parents), parent tile cell (11-16), chunk part (17-18)
+  int desc[Sh::IT + Sh::PIT];
+#pragma unroll
+  for (int it = 0; it < Sh::IT; ++it) {
+    const int e = min(lane + 64 * it, Sh::NCH - 1), c = e / CH;
+    const int x = c / 36 - 1, y = (c / 6) % 6 - 1, z = c % 6 - 1;
+    const int slot = ((x >> 2) + 1) * 9 + ((y >> 2) + 1) * 3 + ((z >> 2) + 1);
+    const int loc = ((x & 3) << 4) | ((y & 3) << 2) | (z & 3);
+    const int par = (((x >> 1) + 1) * 4 + ((y >> 1) + 1)) * 4 + ((z >> 1) + 1);
+    desc[it] = slot | (loc << 5) | (par << 11) | ((e % CH) << 17);
+  }
+#pragma unroll
+  for (int it = 0; it < Sh::PIT; ++it) {
+    const int e = lane + 64 * it, pc = e / CH;
+    const int x = (pc >> 4) - 1, y = ((pc >> 2) & 3) - 1, z = (pc & 3) - 1;
+    const int slot = ((x >> 1) + 1) * 9 + ((y >> 1) + 1) * 3 + ((z >> 1) + 1);
+    const int loc = ((x & 1) << 2) | ((y & 1) << 1) | (z & 1);
+    desc[Sh::IT + it] = slot | (loc << 5) | ((e % CH) << 17);
+  }
+  const int rowc = ((j >> 2) * kT1 + (j & 3)) * CIN;  // cell 16 t + j = (t, j >> 2, j & 3)
+  if (lane < 27 && R.first * 4 + wave < T.L) s_nb[0][wave][lane] = T.nbr[kNbrStride * (R.first * 4 + wave) + lane];
+  int nb = 0;
+  for (int32_t gi = R.first; gi < R.end; gi += R.step, nb ^= 1) {
+    const int32_t l = gi * 4 + wave, gn = gi + R.step;
+    const bool live = l < T.L;
+    __syncthreads();  // neighbour rows visible; the previous group's outputs are read
+    if (live) {
+      const __bf16* src[Sh::IT + Sh::PIT];
+#pragma unroll
+      for (int it = 0; it < Sh::IT + Sh::PIT; ++it) {
+        int d = desc[it];
+        opaque_v(d);  // derived values per leaf, not held across the loop
+        const int32_t nl = s_nb[nb][wave][d & 31];
+        const int loc = (d >> 5) & 63, part = (d >> 17) & 3;
+        src[it] = nl < 0 ? g_pc_zero
+                         : (it < Sh::IT ? xa + ((int64_t)nl * 64 + loc) * CIN : xb + ((int64_t)nl * 8 + loc) * CIN) +
+                               8 * part;
+      }
+#pragma unroll
+      for (int it = 0; it < Sh::IT; ++it)
+        if (lane + 64 * it < Sh::NCH)
+          lds_load16(src[it], S + 512 * it);
+#pragma unroll
+      for (int it = 0; it < Sh::PIT; ++it)
+        lds_load16(src[Sh::IT + it], S + Sh::TILE + 512 * it);
+    }
+    if (lane < 27 && gn < R.end && gn * 4 + wave < T.L)
+      s_nb[nb ^ 1][wave][lane] = T.nbr[kNbrStride * (gn * 4 + wave) + lane];
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (MODE == 1 && live) {
+      // s1 = h1 + up(g2) in place
+#pragma unroll 2
+      for (int it = 0; it < Sh::IT; ++it) {
+        const int e = lane + 64 * it;
+        if (e < Sh::NCH) {
+          const int par = (desc[it] >> 11) & 63;
+          const bf16x8 va = *reinterpret_cast<const bf16x8*>(S + e * 8);
+          const bf16x8 vb = *reinterpret_cast<const bf16x8*>(P + par * CIN + 8 * (e % CH));
+          bf16x8 o;
 #pragma unroll
           for (int q = 0; q < 8; ++q) o[q] = (__bf16)((float)va[q] + 1.0f * (float)vb[q]);
+          *reinterpret_cast<bf16x8*>(S + e * 8) = o;
         }
       }
-      *reinterpret_cast<bf16x8*>(S + c * R + 8 * part) = o;
     }
-  }
-  __syncthreads();
-  floatx4 acc[4][MT];
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int m = 0; m < MT; ++m) acc[t][m] = floatx4{0.f, 0.f, 0.f, 0.f};
-  const int cy = j >> 2, cz = j & 3;  // cell 16 t + j has cx = t
-  if (live) {
-    for (int kc = 0; kc < NK; ++kc) {
-      const int kb = 32 * kc + 8 * g;
-      const int tap = kb / CIN, ci0 = kb % CIN;
-      const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
-      bf16x8 a[MT];
-#pragma unroll
-      for (int m = 0; m < MT; ++m) a[m] = w[(kc * MT + m) * 64 + lane];
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        bf16x8 b = zero8();
-        if (tap < 27) b = *reinterpret_cast<const bf16x8*>(S + (((t + dx) * kT1 + cy + dy) * kT1 + cz + dz) * R + ci0);
-#pragma unroll
-        for (int m = 0; m < MT; ++m) acc[t][m] = mfma(a[m], b, acc[t][m]);
-      }
-    }
-  }
-  __syncthreads();  // the tile region now holds the 64 x 32 relu'd outputs (cell * 32 + channel)
-  if (live) {
+    if (MODE == 1) __syncthreads();
+    floatx4 acc[4][MT];
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
-      for (int m = 0; m < MT; ++m) {
-        bf16x4 o;
+      for (int m = 0; m < MT; ++m) acc[t][m] = floatx4{0.f, 0.f, 0.f, 0.f};
+    if (live) {
+      const bf16x8* wl = w;
+      opaque_s(wl);  // weights are re-read from the cache per leaf, not held in registers
 #pragma unroll
-        for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][m][v], 0.f);
-        const int cell = 16 * t + j;
-        if (MODE == 0) *reinterpret_cast<bf16x4*>(ya + ((int64_t)l * 64 + cell) * 32 + 16 * m + 4 * g) = o;
-        *reinterpret_cast<bf16x4*>(S + cell * 32 + 16 * m + 4 * g) = o;
+      for (int kc = 0; kc < NK; ++kc) {
+        // CIN 32: k-step kc is tap kc, channels 8 g..; CIN 16: taps 2 kc (g < 2) and 2 kc + 1 (g >= 2)
+        const int tlo = CIN == 32 ? kc : 2 * kc, thi = CIN == 32 ? kc : 2 * kc + 1;
+        const int off = (g < 2 ? l1_tap(tlo) : l1_tap(thi)) * CIN + (CIN == 32 ? 8 * g : 8 * (g & 1));
+        const bool padk = thi >= 27 && g >= 2;
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) a[m] = wl[(kc * MT + m) * 64 + lane];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          bf16x8 b = *reinterpret_cast<const bf16x8*>(S + rowc + 36 * CIN * t + (padk ? 0 : off));
+          if (padk) b = zero8();
+#pragma unroll
+          for (int m = 0; m < MT; ++m) acc[t][m] = mfma(a[m], b, acc[t][m]);
+        }
       }
-  }
-  __syncthreads();
-  if (!live) return;
-  if (MODE == 0) {
-    // p2: 8 coarse cells x 8 groups of 4 channels, children in k_pc_pool's order
-    const int cc = lane >> 3, q4 = (lane & 7) * 4;
-    const int cx = cc >> 2, ccy = (cc >> 1) & 1, ccz = cc & 1;
-    float sum[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int fc = ((2 * cx + (k >> 2)) << 4) | ((2 * ccy + ((k >> 1) & 1)) << 2) | (2 * ccz + (k & 1));
-      const bf16x4 v = *reinterpret_cast<const bf16x4*>(S + fc * 32 + q4);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) sum[q] += (float)v[q];
     }
-    bf16x4 o;
+    __syncthreads();  // the tile region now holds the 64 x 32 relu'd outputs (cell * 32 + channel)
+    if (live) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = (__bf16)(sum[q] * 0.125f);
-    *reinterpret_cast<bf16x4*>(yb + ((int64_t)l * 8 + cc) * 32 + q4) = o;
-  } else {
-    // g1 = u1(h5): K = 32 (one MFMA step), 16 output channels
-    const bf16x8 a = w2[lane];
+      for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + (16 * t + j) * 32 + 8 * g);
-      const floatx4 r = mfma(a, b, floatx4{0.f, 0.f, 0.f, 0.f});
+        for (int m = 0; m < MT; ++m) {
+          bf16x4 o;
+#pragma unroll
+          for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][m][v], 0.f);
+          const int cell = 16 * t + j;
+          if (MODE == 0) *reinterpret_cast<bf16x4*>(ya + ((int64_t)l * 64 + cell) * 32 + 16 * m + 4 * g) = o;
+          *reinterpret_cast<bf16x4*>(S + cell * 32 + 16 * m + 4 * g) = o;
+        }
+    }
+    __syncthreads();
+    if (!live) continue;
+    if (MODE == 0) {
+      // p2: 8 coarse cells x 8 groups of 4 channels, children in k_pc_pool's order
+      const int cc = lane >> 3, q4 = (lane & 7) * 4;
+      const int cx = cc >> 2, ccy = (cc >> 1) & 1, ccz = cc & 1;
+      float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int fc = ((2 * cx + (k >> 2)) << 4) | ((2 * ccy + ((k >> 1) & 1)) << 2) | (2 * ccz + (k & 1));
+        const bf16x4 v = *reinterpret_cast<const bf16x4*>(S + fc * 32 + q4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum[q] += (float)v[q];
+      }
       bf16x4 o;
 #pragma unroll
-      for (int v = 0; v < 4; ++v) o[v] = (__bf16)r[v];
-      *reinterpret_cast<bf16x4*>(yb + ((int64_t)l * 64 + 16 * t + j) * 16 + 4 * g) = o;
+      for (int q = 0; q < 4; ++q) o[q] = (__bf16)(sum[q] * 0.125f);
+      *reinterpret_cast<bf16x4*>(yb + ((int64_t)l * 8 + cc) * 32 + q4) = o;
+    } else {
+      // g1 = u1(h5): K = 32 (one MFMA step), 16 output channels
+      const bf16x8 a = w2[lane];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + (16 * t + j) * 32 + 8 * g);
+        const floatx4 r = mfma(a, b, floatx4{0.f, 0.f, 0.f, 0.f});
+        bf16x4 o;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (__bf16)r[v];
+        *reinterpret_cast<bf16x4*>(yb + ((int64_t)l * 64 + 16 * t + j) * 16 + 4 * g) = o;
+      }
     }
   }
 }
@@ -634,17 +723,16 @@ __global__ __launch_bounds__(kTpb) void k_pc_level2(DevTopo T, const __bf16* __r
   floatx4 acc[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  // lane's tile row for cell tile t: rowc + 2 kT23 t; a k-step covers one tap (CIN 32) or half a tap (CIN 64)
+  const int rowc = (sj * kT23 + (cx * kT2 + cy) * kT2 + cz) * R + 8 * g;
+#pragma unroll 6
   for (int kc = 0; kc < NK; ++kc) {
-    const int kb = 32 * kc + 8 * g;
-    const int tap = kb / CIN, ci0 = kb % CIN;
-    const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;
+    const int tap = CIN == 32 ? kc : kc >> 1, ci0 = CIN == 32 ? 0 : 32 * (kc & 1);
+    const int off = ((tap / 9 * kT2 + (tap / 3) % 3) * kT2 + tap % 3) * R + ci0;
     const bf16x8 a = w[(kc * MT + m) * 64 + lane];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      bf16x8 b = zero8();
-      if (tap < 27)
-        b = *reinterpret_cast<const bf16x8*>(
-            S + ((2 * t + sj) * kT23 + ((cx + dx) * kT2 + cy + dy) * kT2 + cz + dz) * R + ci0);
+      const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + rowc + 2 * kT23 * R * t + off);
       acc[t] = mfma(a, b, acc[t]);
     }
   }
@@ -1095,13 +1183,13 @@ void launch_pc_output_fused(const DevTopo& T, const __bf16* h0, const __bf16* g1
 
 void launch_pc_level1(const DevTopo& T, int mode, const __bf16* xa, const __bf16* xb, const __bf16* w,
                       const __bf16* w2, __bf16* ya, __bf16* yb, hipStream_t s) {
-  const unsigned g = grid_for_leaves((T.L + 3) / 4);
+  const int32_t groups = (T.L + 3) / 4;  // four leaves per workgroup pass
   const bf16x8* wv = reinterpret_cast<const bf16x8*>(w);
   const bf16x8* w2v = reinterpret_cast<const bf16x8*>(w2);
   if (mode == 0)
-    k_pc_level1<0><<<g, kTpb, 0, s>>>(T, xa, xb, wv, w2v, ya, yb);
+    k_pc_level1<0><<<persistent_groups(groups, 4), kTpb, 0, s>>>(T, xa, xb, wv, w2v, ya, yb);
   else
-    k_pc_level1<1><<<g, kTpb, 0, s>>>(T, xa, xb, wv, w2v, ya, yb);
+    k_pc_level1<1><<<persistent_groups(groups, 2), kTpb, 0, s>>>(T, xa, xb, wv, w2v, ya, yb);
   HNS_HIP(hipGetLastError());
 }
 
