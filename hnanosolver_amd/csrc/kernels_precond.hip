@@ -283,26 +283,39 @@ __device__ __forceinline__ int32_t tile_leaf(const int32_t* s_nb, int c, int& lx
 }
 
 // h0 = relu(conv3(r; c0)) written for the leaf, and p1 = pool(h0) from the same workgroup.
+__device__ __attribute__((aligned(16))) __bf16 g_pc_zero[8];  // zero-initialised
+typedef __attribute__((address_space(3))) void lds_void;
+// 16 (4) bytes per lane from global memory straight into LDS at dst + 16 (4) * lane; dst is
+// wave-uniform. The loads count against vmcnt; an LDS access issued behind one waits for it.
+__device__ __forceinline__ void lds_load16(const void* src, void* dst) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void*)dst, 16, 0, 0);
+}
+__device__ __forceinline__ void lds_load4(const void* src, void* dst) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void*)dst, 4, 0, 0);
+}
 // Same arithmetic as k_pc_input followed by k_pc_pool (same MFMA K order, same child order).
 __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* __restrict__ r,
                                                         const bf16x8* __restrict__ w, __bf16* __restrict__ h0,
                                                         __bf16* __restrict__ p1) {
-  __shared__ int32_t s_nb[27];
-  __shared__ __bf16 s_r[kT03];  // r rounded to bf16, as the MFMA operand takes it
+  __shared__ int32_t s_nb[2][27];
+  __shared__ __attribute__((aligned(16))) float s_r[kT03 + 24];  // 1000 values, padded to whole waves
   __shared__ __attribute__((aligned(16))) __bf16 s_h[kLeafVox * 16];
-  const int32_t l = leaf_of_block();
-  if (l >= T.L) return;
-  if (threadIdx.x < 27) s_nb[threadIdx.x] = T.nbr[kNbrStride * l + threadIdx.x];
-  __syncthreads();
-  const int32_t half = T.L * kHalfLeaf;
-  for (int c = threadIdx.x; c < kT03; c += kTpb) {
-    int lx, ly, lz;
-    const int32_t nl = tile_leaf(s_nb, c, lx, ly, lz);
-    s_r[c] = (__bf16)(nl < 0 ? 0.f : r[((lx + ly + lz) & 1) * half + nl * kHalfLeaf + cs_local(lx, ly, lz)]);
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, j = lane & 15;
+  const LeafRun R = leaf_run(T.L);
+  if (R.first >= R.end) return;  // uniform per workgroup
+  const int32_t half = T.L * kHalfLeaf;
+  // r tile cells e = tid + 256 it: neighbour slot (bits 0-4), colour (5), colour-split local offset (6-13)
+  constexpr int kIt = (kT03 + kTpb - 1) / kTpb;
+  int desc[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int c = min(tid + kTpb * it, kT03 - 1);
+    const int x = c / 100 - 1, y = (c / 10) % 10 - 1, z = c % 10 - 1;
+    const int slot = ((x >> 3) + 1) * 9 + ((y >> 3) + 1) * 3 + ((z >> 3) + 1);
+    const int lx = x & 7, ly = y & 7, lz = z & 7;
+    desc[it] = slot | (((lx + ly + lz) & 1) << 5) | (cs_local(lx, ly, lz) << 6);
+  }
   const bf16x8 a = w[lane];
   // taps 8 g + v of this lane: tile-cell offsets relative to the output cell's (0,0,0) neighbour
   int off[8];
@@ -312,37 +325,58 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
     off[v] = tile0(tap / 9, (tap / 3) % 3, tap % 3);
   }
   const int nvalid = min(8, 27 - 8 * g);  // taps beyond 26 are padding (zero)
-  for (int tile = wave; tile < kLeafVox / 16; tile += 4) {
-    const int cell = 16 * tile + j;
-    const int base = tile0(cell >> 6, (cell >> 3) & 7, cell & 7);
-    bf16x8 b;
+  if (tid < 27) s_nb[0][tid] = T.nbr[kNbrStride * R.first + tid];
+  int nb = 0;
+  for (int32_t l = R.first; l < R.end; l += R.step, nb ^= 1) {
+    const int32_t ln = l + R.step;
+    __syncthreads();  // neighbour row visible; the previous leaf's tiles are no longer read
+    const float* src[kIt];
 #pragma unroll
-    for (int v = 0; v < 8; ++v) b[v] = v < nvalid ? s_r[base + off[v]] : (__bf16)0.0f;
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    acc = mfma(a, b, acc);
-    bf16x4 o;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[v], 0.f);
-    *reinterpret_cast<bf16x4*>(h0 + ((int64_t)l * kLeafVox + cell) * 16 + 4 * g) = o;
-    *reinterpret_cast<bf16x4*>(s_h + cell * 16 + 4 * g) = o;
-  }
-  __syncthreads();
-  // p1: 64 coarse cells x 4 groups of 4 channels, one per lane, children in k_pc_pool's order
-  {
-    const int cc = threadIdx.x >> 2, q4 = (threadIdx.x & 3) * 4;
-    const int cx = cc >> 4, cy = (cc >> 2) & 3, cz = cc & 3;
-    float sum[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int fc = ((2 * cx + (k >> 2)) << 6) | ((2 * cy + ((k >> 1) & 1)) << 3) | (2 * cz + (k & 1));
-      const bf16x4 v = *reinterpret_cast<const bf16x4*>(s_h + fc * 16 + q4);
-#pragma unroll
-      for (int q = 0; q < 4; ++q) sum[q] += (float)v[q];
+    for (int it = 0; it < kIt; ++it) {
+      int d = desc[it];
+      opaque_v(d);
+      const int32_t nl = s_nb[nb][d & 31];
+      src[it] = nl < 0 ? reinterpret_cast<const float*>(g_pc_zero)
+                       : r + ((d >> 5) & 1) * half + nl * kHalfLeaf + ((d >> 6) & 255);
     }
-    bf16x4 o;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) o[q] = (__bf16)(sum[q] * 0.125f);
-    *reinterpret_cast<bf16x4*>(p1 + ((int64_t)l * 64 + cc) * 16 + q4) = o;
+    for (int it = 0; it < kIt; ++it)
+      if (kTpb * it + 64 * wave < kT03) lds_load4(src[it], s_r + kTpb * it + 64 * wave);
+    if (tid < 27 && ln < R.end) s_nb[nb ^ 1][tid] = T.nbr[kNbrStride * ln + tid];
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int tile = wave; tile < kLeafVox / 16; tile += 4) {
+      const int cell = 16 * tile + j;
+      const int base = tile0(cell >> 6, (cell >> 3) & 7, cell & 7);
+      bf16x8 b;
+#pragma unroll
+      for (int v = 0; v < 8; ++v) b[v] = v < nvalid ? (__bf16)s_r[base + off[v]] : (__bf16)0.0f;
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = mfma(a, b, acc);
+      bf16x4 o;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[v], 0.f);
+      *reinterpret_cast<bf16x4*>(h0 + ((int64_t)l * kLeafVox + cell) * 16 + 4 * g) = o;
+      *reinterpret_cast<bf16x4*>(s_h + cell * 16 + 4 * g) = o;
+    }
+    __syncthreads();
+    // p1: 64 coarse cells x 4 groups of 4 channels, one per lane, children in k_pc_pool's order
+    {
+      const int cc = tid >> 2, q4 = (tid & 3) * 4;
+      const int cx = cc >> 4, cy = (cc >> 2) & 3, cz = cc & 3;
+      float sum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int fc = ((2 * cx + (k >> 2)) << 6) | ((2 * cy + ((k >> 1) & 1)) << 3) | (2 * cz + (k & 1));
+        const bf16x4 v = *reinterpret_cast<const bf16x4*>(s_h + fc * 16 + q4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sum[q] += (float)v[q];
+      }
+      bf16x4 o;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = (__bf16)(sum[q] * 0.125f);
+      *reinterpret_cast<bf16x4*>(p1 + ((int64_t)l * 64 + cc) * 16 + q4) = o;
+    }
   }
 }
 
@@ -355,12 +389,6 @@ __global__ __launch_bounds__(kTpb) void k_pc_input_pool(DevTopo T, const float* 
 // waits for its loads the others on the compute unit keep the matrix cores busy. The neighbour row
 // of the next leaf is staged in LDS one leaf ahead, so no tile load waits on a lookup. Absent
 // neighbour leaves load a zero chunk.
-__device__ __attribute__((aligned(16))) __bf16 g_pc_zero[8];  // zero-initialised
-typedef __attribute__((address_space(3))) void lds_void;
-// 16 bytes per lane from global memory straight into LDS at dst + 16 * lane (dst wave-uniform)
-__device__ __forceinline__ void lds_load16(const void* src, void* dst) {
-  __builtin_amdgcn_global_load_lds(src, (lds_void*)dst, 16, 0, 0);
-}
 constexpr int kOutTileB = kT03 * 32;                     // s0 tile: 1000 cells x 16 bf16
 constexpr int kOutParB = 216 * 32;                       // g1 parent tile: 6^3 level-1 cells x 16 bf16
 constexpr int kOutBufB = kOutTileB + kOutParB;
@@ -1170,7 +1198,7 @@ void launch_pc_output(const DevTopo& T, const __bf16* x, const __bf16* w, const 
 }
 
 void launch_pc_input_pool(const DevTopo& T, const float* r_cs, const __bf16* w, __bf16* h0, __bf16* p1, hipStream_t s) {
-  k_pc_input_pool<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, reinterpret_cast<const bf16x8*>(w), h0, p1);
+  k_pc_input_pool<<<persistent_groups(T.L, 7), kTpb, 0, s>>>(T, r_cs, reinterpret_cast<const bf16x8*>(w), h0, p1);
   HNS_HIP(hipGetLastError());
 }
 
