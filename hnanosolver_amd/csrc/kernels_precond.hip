@@ -715,91 +715,122 @@ __global__ __launch_bounds__(kTpb, 2) void k_pc_level1(DevTopo T, const __bf16* 
   }
 }
 
-// Level 2 (2^3 cells per leaf): a workgroup takes 8 consecutive leaves and stages their 4^3 tiles;
-// wave w computes output-channel tile w for all 64 cells (4 cell tiles of 2 leaves each), so the
-// weights of a layer are read once per 8 leaves.
+// Level 2 (2^3 cells per leaf): persistent workgroups walk a strided run of groups of 8
+// consecutive leaves; a group's 4^3 tiles arrive by direct global-to-LDS loads (addresses formed
+// before the first load is issued); wave w computes output-channel tile w for all 64 cells
+// (4 cell tiles of 2 leaves each), so the weights of a layer are read once per 8 leaves.
 //   MODE 0 (c2):      tile = p2 (32 ch); h2 = relu(conv) to y (64 ch)
 //   MODE 1 (c3, u2):  tile = h2 (64 ch); h3 = relu(conv) kept in LDS; g2 = u2(h3) to y (32 ch)
 constexpr int kT2 = 4, kT23 = kT2 * kT2 * kT2;
 template <int MODE>
-__global__ __launch_bounds__(kTpb) void k_pc_level2(DevTopo T, const __bf16* __restrict__ x,
-                                                    const bf16x8* __restrict__ w, const bf16x8* __restrict__ w2,
-                                                    __bf16* __restrict__ y) {
-  constexpr int CIN = MODE == 0 ? 32 : 64, MT = 4, NK = (27 * CIN + 31) / 32, R = CIN + 8;
+__global__ __launch_bounds__(kTpb, 2) void k_pc_level2(DevTopo T, const __bf16* __restrict__ x,
+                                                       const bf16x8* __restrict__ w, const bf16x8* __restrict__ w2,
+                                                       __bf16* __restrict__ y) {
+  constexpr int CIN = MODE == 0 ? 32 : 64, CH = CIN / 8, MT = 4, NK = (27 * CIN + 31) / 32, R = CIN;
+  constexpr int NCH = 8 * kT23 * CH, IT = NCH / kTpb;  // 16-byte chunks per group, per lane
   __shared__ __attribute__((aligned(16))) __bf16 S[8 * kT23 * R];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ int32_t s_nb[2][8 * 27];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int g = lane >> 4, j = lane & 15;
-  const int32_t l0 = leaf_of_block() * 8;
-  if (l0 >= T.L) return;  // uniform per workgroup
-  const int nlive = min(8, T.L - l0);
-  for (int e = threadIdx.x; e < 8 * kT23 * (CIN / 8); e += kTpb) {
-    const int slot = e / (kT23 * (CIN / 8)), rem = e % (kT23 * (CIN / 8));
-    const int c = rem / (CIN / 8), part = rem % (CIN / 8);
-    bf16x8 o = zero8();
-    if (slot < nlive) {
-      int lx, ly, lz;
-      const int32_t nl = halo_leaf(T.nbr + kNbrStride * (l0 + slot), 1, c >> 4, (c >> 2) & 3, c & 3, lx, ly, lz);
-      if (nl >= 0)
-        o = *reinterpret_cast<const bf16x8*>(x + (int64_t)((nl << 3) | (lx << 2) | (ly << 1) | lz) * CIN + 8 * part);
-    }
-    *reinterpret_cast<bf16x8*>(S + (slot * kT23 + c) * R + 8 * part) = o;
-  }
-  __syncthreads();
-  // lane j of cell tile t: leaf slot 2 t + (j >> 3), local cell j & 7
-  const int sj = j >> 3, cj = j & 7, cx = cj >> 2, cy = (cj >> 1) & 1, cz = cj & 1;
-  const int m = wave;
-  floatx4 acc[4];
+  const int32_t ngroups = (T.L + 7) / 8;
+  const LeafRun RG = leaf_run(ngroups);
+  if (RG.first >= RG.end) return;  // uniform per workgroup
+  // chunk e = tid + 256 it: leaf slot (bits 0-2), neighbour slot (3-7), local cell (8-10), part (11-13)
+  int desc[IT];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
-  // lane's tile row for cell tile t: rowc + 2 kT23 t; a k-step covers one tap (CIN 32) or half a tap (CIN 64)
-  const int rowc = (sj * kT23 + (cx * kT2 + cy) * kT2 + cz) * R + 8 * g;
+  for (int it = 0; it < IT; ++it) {
+    const int e = tid + kTpb * it;
+    const int ls = e / (kT23 * CH), rem = e % (kT23 * CH), c = rem / CH, part = rem % CH;
+    const int xx = (c >> 4) - 1, yy = ((c >> 2) & 3) - 1, zz = (c & 3) - 1;
+    const int slot = ((xx >> 1) + 1) * 9 + ((yy >> 1) + 1) * 3 + ((zz >> 1) + 1);
+    const int loc = ((xx & 1) << 2) | ((yy & 1) << 1) | (zz & 1);
+    desc[it] = ls | (slot << 3) | (loc << 8) | (part << 11);
+  }
+  auto load_rows = [&](int32_t gi, int b) {
+    if (tid < 8 * 27) {
+      const int32_t l = gi * 8 + tid / 27;
+      s_nb[b][tid] = l < T.L ? T.nbr[kNbrStride * l + tid % 27] : -1;  // slots past the last leaf stage zeros
+    }
+  };
+  load_rows(RG.first, 0);
+  int nb = 0;
+  for (int32_t gi = RG.first; gi < RG.end; gi += RG.step, nb ^= 1) {
+    const int32_t l0 = gi * 8, gn = gi + RG.step;
+    const int nlive = min(8, T.L - l0);
+    __syncthreads();  // neighbour rows visible; the previous group's tiles are no longer read
+    {
+      const __bf16* src[IT];
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        int d = desc[it];
+        opaque_v(d);
+        const int32_t nl = s_nb[nb][(d & 7) * 27 + ((d >> 3) & 31)];
+        src[it] = nl < 0 ? g_pc_zero : x + ((int64_t)nl * 8 + ((d >> 8) & 7)) * CIN + 8 * ((d >> 11) & 7);
+      }
+#pragma unroll
+      for (int it = 0; it < IT; ++it) lds_load16(src[it], S + (kTpb * it + 64 * wave) * 8);
+    }
+    if (gn < RG.end) load_rows(gn, nb ^ 1);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // lane j of cell tile t: leaf slot 2 t + (j >> 3), local cell j & 7
+    const int sj = j >> 3, cj = j & 7, cx = cj >> 2, cy = (cj >> 1) & 1, cz = cj & 1;
+    const int m = wave;
+    floatx4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // lane's tile row for cell tile t: rowc + 2 kT23 t; a k-step covers one tap (CIN 32) or half a tap (CIN 64)
+    const int rowc = (sj * kT23 + (cx * kT2 + cy) * kT2 + cz) * R + 8 * g;
+    const bf16x8* wl = w;
+    opaque_s(wl);  // weights are re-read from the cache per group, not held in registers
 #pragma unroll 6
-  for (int kc = 0; kc < NK; ++kc) {
-    const int tap = CIN == 32 ? kc : kc >> 1, ci0 = CIN == 32 ? 0 : 32 * (kc & 1);
-    const int off = ((tap / 9 * kT2 + (tap / 3) % 3) * kT2 + tap % 3) * R + ci0;
-    const bf16x8 a = w[(kc * MT + m) * 64 + lane];
+    for (int kc = 0; kc < NK; ++kc) {
+      const int tap = CIN == 32 ? kc : kc >> 1, ci0 = CIN == 32 ? 0 : 32 * (kc & 1);
+      const int off = ((tap / 9 * kT2 + (tap / 3) % 3) * kT2 + tap % 3) * R + ci0;
+      const bf16x8 a = wl[(kc * MT + m) * 64 + lane];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + rowc + 2 * kT23 * R * t + off);
-      acc[t] = mfma(a, b, acc[t]);
+      for (int t = 0; t < 4; ++t) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + rowc + 2 * kT23 * R * t + off);
+        acc[t] = mfma(a, b, acc[t]);
+      }
     }
-  }
-  if (MODE == 0) {
+    if (MODE == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        if (2 * t + sj >= nlive) continue;
+        bf16x4 o;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][v], 0.f);
+        *reinterpret_cast<bf16x4*>(y + ((int64_t)(l0 + 2 * t + sj) * 8 + cj) * 64 + 16 * m + 4 * g) = o;
+      }
+      continue;
+    }
+    __syncthreads();  // the tile region now holds h3 for the 64 cells (cell * 64 + channel)
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      if (2 * t + sj >= nlive) continue;
       bf16x4 o;
 #pragma unroll
       for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][v], 0.f);
-      *reinterpret_cast<bf16x4*>(y + ((int64_t)(l0 + 2 * t + sj) * 8 + cj) * 64 + 16 * m + 4 * g) = o;
+      *reinterpret_cast<bf16x4*>(S + ((2 * t + sj) * 8 + cj) * 64 + 16 * m + 4 * g) = o;
     }
-    return;
-  }
-  __syncthreads();  // the tile region now holds h3 for the 64 cells (cell * 64 + channel)
+    __syncthreads();
+    // g2 = u2(h3): K = 64 (two MFMA steps), 32 output channels; wave w: output tile w & 1, cell tiles 2 (w >> 1) + {0, 1}
+    const int m2 = wave & 1;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    bf16x4 o;
+    for (int tt = 0; tt < 2; ++tt) {
+      const int t = 2 * (wave >> 1) + tt;
+      floatx4 r = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] = (__bf16)fmaxf(acc[t][v], 0.f);
-    *reinterpret_cast<bf16x4*>(S + ((2 * t + sj) * 8 + cj) * 64 + 16 * m + 4 * g) = o;
-  }
-  __syncthreads();
-  // g2 = u2(h3): K = 64 (two MFMA steps), 32 output channels; wave w: output tile w & 1, cell tiles 2 (w >> 1) + {0, 1}
-  const int m2 = wave & 1;
+      for (int kc = 0; kc < 2; ++kc) {
+        const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + ((2 * t + sj) * 8 + cj) * 64 + 32 * kc + 8 * g);
+        r = mfma(w2[(kc * 2 + m2) * 64 + lane], b, r);
+      }
+      if (2 * t + sj >= nlive) continue;
+      bf16x4 o;
 #pragma unroll
-  for (int tt = 0; tt < 2; ++tt) {
-    const int t = 2 * (wave >> 1) + tt;
-    floatx4 r = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int kc = 0; kc < 2; ++kc) {
-      const bf16x8 b = *reinterpret_cast<const bf16x8*>(S + ((2 * t + sj) * 8 + cj) * 64 + 32 * kc + 8 * g);
-      r = mfma(w2[(kc * 2 + m2) * 64 + lane], b, r);
+      for (int v = 0; v < 4; ++v) o[v] = (__bf16)r[v];
+      *reinterpret_cast<bf16x4*>(y + ((int64_t)(l0 + 2 * t + sj) * 8 + cj) * 32 + 16 * m2 + 4 * g) = o;
     }
-    if (2 * t + sj >= nlive) continue;
-    bf16x4 o;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) o[v] = (__bf16)r[v];
-    *reinterpret_cast<bf16x4*>(y + ((int64_t)(l0 + 2 * t + sj) * 8 + cj) * 32 + 16 * m2 + 4 * g) = o;
   }
 }
 
@@ -1205,7 +1236,7 @@ void launch_pc_input_pool(const DevTopo& T, const float* r_cs, const __bf16* w, 
 void launch_pc_output_fused(const DevTopo& T, const __bf16* h0, const __bf16* g1, const __bf16* w,
                             const float* params_out, float* z_cs, hipStream_t s) {
   k_pc_output_fused<<<persistent_groups(T.L, 4), kTpb, 0, s>>>(T, h0, g1, reinterpret_cast<const bf16x8*>(w), params_out,
-                                                          z_cs);
+                                                               z_cs);
   HNS_HIP(hipGetLastError());
 }
 
@@ -1223,13 +1254,13 @@ void launch_pc_level1(const DevTopo& T, int mode, const __bf16* xa, const __bf16
 
 void launch_pc_level2(const DevTopo& T, int mode, const __bf16* x, const __bf16* w, const __bf16* w2, __bf16* y,
                       hipStream_t s) {
-  const unsigned g = grid_for_leaves((T.L + 7) / 8);
+  const int32_t groups = (T.L + 7) / 8;  // eight leaves per workgroup pass
   const bf16x8* wv = reinterpret_cast<const bf16x8*>(w);
   const bf16x8* w2v = reinterpret_cast<const bf16x8*>(w2);
   if (mode == 0)
-    k_pc_level2<0><<<g, kTpb, 0, s>>>(T, x, wv, w2v, y);
+    k_pc_level2<0><<<persistent_groups(groups, 4), kTpb, 0, s>>>(T, x, wv, w2v, y);
   else
-    k_pc_level2<1><<<g, kTpb, 0, s>>>(T, x, wv, w2v, y);
+    k_pc_level2<1><<<persistent_groups(groups, 2), kTpb, 0, s>>>(T, x, wv, w2v, y);
   HNS_HIP(hipGetLastError());
 }
 
