@@ -44,4 +44,17 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 // Partials of r.z and q.z (q of the previous iteration) after the preconditioner wrote z.
 void launch_cg_rz(int32_t L, const float* r, const float* z, const float* q, const CgScalars& S, hipStream_t s);
 
+// Flexible CG with truncation kFcgDepth (Notay's FCG(m)): the directions d_i and products
+// q_i = A d_i of the last kFcgDepth iterations stay in the rings Dr, Qr (kFcgDepth vectors of
+// L * 512 each, slot i mod kFcgDepth), sigma_i = d_i . q_i in S.rho[i]. One iteration `it` after
+// z = M(r): stop test, d = z - sum_i (z.q_i / sigma_i) d_i, q = A d, alpha = (d.r)/(d.q),
+// x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
+constexpr int kFcgDepth = 8;
+// stop test, new direction and its product (three launches)
+void launch_fcg_direction(const DevTopo& T, int32_t it, const float* r, const float* z, float* Dr, float* Qr,
+                          double* zq, const CgScalars& S, hipStream_t s);
+// x += alpha d, r -= alpha q
+void launch_fcg_update(int32_t L, int32_t it, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
+                       hipStream_t s);
+
 }  // namespace hns

@@ -279,6 +279,189 @@ __global__ __launch_bounds__(kTpb) void k_cg_rz(int32_t L, const float* __restri
   }
 }
 
+// ---- flexible CG with truncation m (Notay's FCG(m)), the learned-preconditioner solve ----------
+// Iteration k, after the host has applied z = M(r): the ring keeps the last m directions d_i and
+// products q_i = A d_i (slot i mod m) with sigma_i = d_i . q_i in S.rho[i].
+//   dots:   stop test on r . r (as the CG product); partials of z . q_i for the k' = min(k, m)
+//           ring entries (i = k-1, ..., k-k'), one row of zq per entry
+//   dir:    d_k = z - sum_i (z . q_i / sigma_i) d_i into slot k mod m; partials of d_k . r
+//   spmv:   q_k = A d_k into slot k mod m; partials of d_k . q_k
+//   update: sigma_k = d_k . q_k, alpha_k = (d_k . r) / sigma_k; x += alpha_k d_k, r -= alpha_k q_k;
+//           partials of r . r
+// The d_k . r and d_k . q_k partials use the pa rows of r.z and q.z (unused by this solver).
+__device__ __forceinline__ int fcg_slot(int32_t i) { return (int)(i % kFcgDepth); }
+
+template <int N>
+__device__ inline void block_store_rows(double (&v)[N], double* s_red, double* rows) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) s_red[k * (kTpb / 64) + (threadIdx.x >> 6)] = v[k];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kTpb / 64; ++i) t += s_red[threadIdx.x * (kTpb / 64) + i];
+    rows[threadIdx.x * kCgMaxGroups + blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
+                                                   const float* __restrict__ Qr, double* __restrict__ zq,
+                                                   CgScalars S) {
+  __shared__ double s_red[kFcgDepth * (kTpb / 64)];
+  if (S.flag[0]) return;
+  const int G = (int)gridDim.x;
+  double rr1[1];
+  reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rr1);
+  const double rr = rr1[0];
+  const double bb = it == 0 ? rr : S.bb[0];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (rr <= S.tol2 * bb) {  // identical decision in every workgroup
+    if (lead) {
+      if (it == 0) S.bb[0] = bb;
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+    return;
+  }
+  if (lead && it == 0) S.bb[0] = bb;
+  const int np = min(it, kFcgDepth);
+  if (np == 0) return;
+  const float* q[kFcgDepth];
+#pragma unroll
+  for (int k = 0; k < kFcgDepth; ++k) q[k] = Qr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
+  double acc[kFcgDepth];
+#pragma unroll
+  for (int k = 0; k < kFcgDepth; ++k) acc[k] = 0.0;
+  const LeafRun R = leaf_run(L);
+  const int32_t half = L * kHalfLeaf;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
+      const double zv = (double)z[i];
+#pragma unroll
+      for (int k = 0; k < kFcgDepth; ++k)
+        if (k < np) acc[k] += zv * (double)q[k][i];
+    }
+  }
+  block_store_rows(acc, s_red, zq);
+}
+
+__global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
+                                                  const float* __restrict__ r, float* __restrict__ Dr,
+                                                  const double* __restrict__ zq, CgScalars S) {
+  __shared__ double s_red[kFcgDepth * (kTpb / 64)];
+  if (S.flag[0]) return;
+  const int G = (int)gridDim.x;
+  const int np = min(it, kFcgDepth);
+  double zqk[kFcgDepth];
+  reduce_partial_rows(zq, G, s_red, zqk);
+  float bf[kFcgDepth];
+  const float* dk[kFcgDepth];
+#pragma unroll
+  for (int k = 0; k < kFcgDepth; ++k) {
+    bf[k] = k < np ? (float)(-zqk[k] / S.rho[it - 1 - k]) : 0.f;
+    dk[k] = Dr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
+  }
+  float* dout = Dr + (int64_t)fcg_slot(it) * n;  // the slot of d_{it-m}, read first by the same lane
+  const LeafRun R = leaf_run(L);
+  const int32_t half = L * kHalfLeaf;
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
+      float d = z[i];
+#pragma unroll
+      for (int k = 0; k < kFcgDepth; ++k)
+        if (k < np) d = fmaf(bf[k], dk[k][i], d);
+      dout[i] = d;
+      acc += (double)d * (double)r[i];
+    }
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[kSlotRZ * kCgMaxGroups + blockIdx.x] = tt;
+}
+
+__global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int32_t it, int64_t n, const float* __restrict__ Dr,
+                                                   float* __restrict__ Qr, CgScalars S) {
+  __shared__ double s_red[kTpb / 64];
+  __shared__ float s_tile[2][kPairTile];
+  if (S.flag[0]) return;
+  const float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
+  float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const LeafRun R = leaf_run(T.L);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  auto get = [&](int c, int32_t i) { return d[c * half + i]; };
+  double acc = 0.0;
+  int buf = 0;
+  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
+    float* St = s_tile[buf];
+    const FaceRow Fr = face_row(T, l);
+    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
+    const float d0 = d[i0], d1 = d[i1];
+    pair_tile_halo(Fr, get, St);
+    St[c0 * kHalfLeaf + t] = d0;
+    St[(c0 ^ 1) * kHalfLeaf + t] = d1;
+    __syncthreads();
+    float s0, s1;
+    pair_tile_sums(St, t, d0, d1, s0, s1);
+    const float q0 = 6.0f * d0 - s0, q1 = 6.0f * d1 - s1;
+    q[i0] = q0;
+    q[i1] = q1;
+    acc += (double)d0 * (double)q0 + (double)d1 * (double)q1;
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[kSlotQZ * kCgMaxGroups + blockIdx.x] = tt;
+}
+
+__global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int32_t it, int64_t n, float* __restrict__ x,
+                                                     float* __restrict__ r, const float* __restrict__ Dr,
+                                                     const float* __restrict__ Qr, CgScalars S) {
+  __shared__ double s_red[2 * (kTpb / 64)];
+  if (S.flag[0]) return;
+  double rows[2];  // d . r, d . q
+  reduce_partial_rows(S.pa + kSlotRZ * kCgMaxGroups, (int)gridDim.x, s_red, rows);
+  const double sigma = rows[1];
+  if (!(sigma > 0.0)) {  // breakdown (A is SPD; only round-off gets here); same in every workgroup
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+    return;
+  }
+  const double alpha_d = rows[0] / sigma;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.rho[it] = sigma;
+    S.alpha[it] = alpha_d;
+  }
+  const float alpha = (float)alpha_d;
+  const float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
+  const float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const LeafRun R = leaf_run(L);
+  const int32_t half = L * kHalfLeaf;
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
+      x[i] = fmaf(alpha, d[i], x[i]);
+      const float rv = fmaf(-alpha, q[i], r[i]);
+      r[i] = rv;
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
+}
+
 }  // namespace
 
 void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, const CgScalars& S, hipStream_t s) {
@@ -305,6 +488,23 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 }
 void launch_cg_rz(int32_t L, const float* r, const float* z, const float* q, const CgScalars& S, hipStream_t s) {
   k_cg_rz<<<cg_groups(L), kTpb, 0, s>>>(L, r, z, q, S);
+  HNS_HIP(hipGetLastError());
+}
+
+void launch_fcg_direction(const DevTopo& T, int32_t it, const float* r, const float* z, float* Dr, float* Qr,
+                          double* zq, const CgScalars& S, hipStream_t s) {
+  const unsigned g = cg_groups(T.L);
+  const int64_t n = (int64_t)T.L * kLeafVox;
+  k_fcg_dots<<<g, kTpb, 0, s>>>(T.L, it, n, z, Qr, zq, S);
+  HNS_HIP(hipGetLastError());
+  k_fcg_dir<<<g, kTpb, 0, s>>>(T.L, it, n, z, r, Dr, zq, S);
+  HNS_HIP(hipGetLastError());
+  k_fcg_spmv<<<g, kTpb, 0, s>>>(T, it, n, Dr, Qr, S);
+  HNS_HIP(hipGetLastError());
+}
+void launch_fcg_update(int32_t L, int32_t it, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
+                       hipStream_t s) {
+  k_fcg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, (int64_t)L * kLeafVox, x, r, Dr, Qr, S);
   HNS_HIP(hipGetLastError());
 }
 

@@ -14,7 +14,8 @@ namespace hns {
 
 struct CgState {
   DBuf<float> r, d0, d1, q, z;
-  DBuf<double> rho, alpha, pa, pb, bb;
+  DBuf<float> dring, qring;  // flexible CG: kFcgDepth directions and products
+  DBuf<double> rho, alpha, pa, pb, bb, zq;
   DBuf<int> flag;
   int* h_flag = nullptr;
   int cap = 0;
@@ -172,7 +173,12 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   // The device stops at the first iteration whose recurrence residual passes the test. The stop
   // is accepted only when that residual was just replaced by b - A x; otherwise the residual is
   // replaced and the same iteration is tested again (and continued if it fails).
-  if (pre && !G.z.p) G.z.alloc(N_);
+  if (pre && !G.z.p) {
+    G.z.alloc(N_);
+    G.dring.alloc((size_t)kFcgDepth * N_);
+    G.qring.alloc((size_t)kFcgDepth * N_);
+    G.zq.alloc((size_t)kFcgDepth * kCgMaxGroups);
+  }
   // a preconditioned iteration costs far more than a sync: check the stop flag every 4
   const int batch0 = pre ? 4 : 8, batch_max = pre ? 4 : 128;
   int it = 0, batch = batch0, replaced_at = 0;
@@ -186,16 +192,23 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         tm.end(s);
         replaced_at = it;
       }
-      float* d_in = (it & 1) ? G.d1.p : G.d0.p;
-      float* d_out = (it & 1) ? G.d0.p : G.d1.p;
       if (pre) {
+        // flexible CG with truncation kFcgDepth on z = M(r)
         tm.begin(HNS_K_PRECOND, s);
         r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
-        launch_cg_rz(T.L, G.r.p, G.z.p, G.q.p, S, s);
         tm.end(s);
+        tm.begin(HNS_K_CG_PRODUCT, s);
+        launch_fcg_direction(T, it, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+        tm.end(s);
+        tm.begin(HNS_K_CG_UPDATE, s);
+        launch_fcg_update(T.L, it, p, G.r.p, G.dring.p, G.qring.p, S, s);
+        tm.end(s);
+        continue;
       }
+      float* d_in = (it & 1) ? G.d1.p : G.d0.p;
+      float* d_out = (it & 1) ? G.d0.p : G.d1.p;
       tm.begin(HNS_K_CG_PRODUCT, s);
-      launch_cg_product(T, it, it == 0, G.r.p, pre ? G.z.p : nullptr, d_in, d_out, G.q.p, S, s);
+      launch_cg_product(T, it, it == 0, G.r.p, nullptr, d_in, d_out, G.q.p, S, s);
       tm.end(s);
       tm.begin(HNS_K_CG_UPDATE, s);
       launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
