@@ -120,7 +120,9 @@ def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrumen
         if solver == "rbgs":
             hns.advance(ctx, K, sc.dt, params)
             return None
-        return hns.advance(ctx, K, sc.dt, params, solver=solver, tol=TOL, max_iters=200000,
+        # the learned solve falls back to plain RBGS if its flexible CG misses tol within 2000 iterations
+        return hns.advance(ctx, K, sc.dt, params, solver=solver, tol=TOL,
+                           max_iters=2000 if solver == "pcg_learned" else 200000,
                            check_every=100 if solver == "rbgs_tol" else 0)
 
     for _ in range(warmup):

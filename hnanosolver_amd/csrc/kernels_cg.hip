@@ -20,6 +20,8 @@
 //
 // Per iteration and voxel: product reads r (or z), d (8 B) and writes d, q (8 B); update reads
 // x, d, r, q (16 B) and writes x, r (8 B): 40 B.
+#include <algorithm>
+
 #include "hns_cg.hpp"
 
 namespace hns {
@@ -309,6 +311,8 @@ __device__ inline void block_store_rows(double (&v)[N], double* s_red, double* r
   }
 }
 
+// NP = min(it, kFcgDepth) ring entries in use (a template argument, so every load is unconditional)
+template <int NP>
 __global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
                                                    const float* __restrict__ Qr, double* __restrict__ zq,
                                                    CgScalars S) {
@@ -329,8 +333,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_
     return;
   }
   if (lead && it == 0) S.bb[0] = bb;
-  const int np = min(it, kFcgDepth);
-  if (np == 0) return;
+  if (NP == 0) return;
   const float* q[kFcgDepth];
 #pragma unroll
   for (int k = 0; k < kFcgDepth; ++k) q[k] = Qr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
@@ -345,27 +348,26 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_
       const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
       const double zv = (double)z[i];
 #pragma unroll
-      for (int k = 0; k < kFcgDepth; ++k)
-        if (k < np) acc[k] += zv * (double)q[k][i];
+      for (int k = 0; k < NP; ++k) acc[k] += zv * (double)q[k][i];
     }
   }
   block_store_rows(acc, s_red, zq);
 }
 
+template <int NP>
 __global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
                                                   const float* __restrict__ r, float* __restrict__ Dr,
                                                   const double* __restrict__ zq, CgScalars S) {
   __shared__ double s_red[kFcgDepth * (kTpb / 64)];
   if (S.flag[0]) return;
   const int G = (int)gridDim.x;
-  const int np = min(it, kFcgDepth);
-  double zqk[kFcgDepth];
-  reduce_partial_rows(zq, G, s_red, zqk);
-  float bf[kFcgDepth];
-  const float* dk[kFcgDepth];
+  double zqk[NP > 0 ? NP : 1];
+  if (NP > 0) reduce_partial_rows(zq, G, s_red, zqk);
+  float bf[NP > 0 ? NP : 1];
+  const float* dk[NP > 0 ? NP : 1];
 #pragma unroll
-  for (int k = 0; k < kFcgDepth; ++k) {
-    bf[k] = k < np ? (float)(-zqk[k] / S.rho[it - 1 - k]) : 0.f;
+  for (int k = 0; k < NP; ++k) {
+    bf[k] = (float)(-zqk[k] / S.rho[it - 1 - k]);
     dk[k] = Dr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
   }
   float* dout = Dr + (int64_t)fcg_slot(it) * n;  // the slot of d_{it-m}, read first by the same lane
@@ -378,8 +380,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int32_t it, int64_t
       const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
       float d = z[i];
 #pragma unroll
-      for (int k = 0; k < kFcgDepth; ++k)
-        if (k < np) d = fmaf(bf[k], dk[k][i], d);
+      for (int k = 0; k < NP; ++k) d = fmaf(bf[k], dk[k][i], d);
       dout[i] = d;
       acc += (double)d * (double)r[i];
     }
@@ -495,10 +496,26 @@ void launch_fcg_direction(const DevTopo& T, int32_t it, const float* r, const fl
                           double* zq, const CgScalars& S, hipStream_t s) {
   const unsigned g = cg_groups(T.L);
   const int64_t n = (int64_t)T.L * kLeafVox;
-  k_fcg_dots<<<g, kTpb, 0, s>>>(T.L, it, n, z, Qr, zq, S);
-  HNS_HIP(hipGetLastError());
-  k_fcg_dir<<<g, kTpb, 0, s>>>(T.L, it, n, z, r, Dr, zq, S);
-  HNS_HIP(hipGetLastError());
+  static_assert(kFcgDepth == 8, "dispatch below covers ring fills 0..8");
+  switch (std::min<int32_t>(it, kFcgDepth)) {
+#define HNS_FCG_CASE(NP)                                                   \
+  case NP:                                                                 \
+    k_fcg_dots<NP><<<g, kTpb, 0, s>>>(T.L, it, n, z, Qr, zq, S);           \
+    HNS_HIP(hipGetLastError());                                            \
+    k_fcg_dir<NP><<<g, kTpb, 0, s>>>(T.L, it, n, z, r, Dr, zq, S);         \
+    HNS_HIP(hipGetLastError());                                            \
+    break;
+    HNS_FCG_CASE(0)
+    HNS_FCG_CASE(1)
+    HNS_FCG_CASE(2)
+    HNS_FCG_CASE(3)
+    HNS_FCG_CASE(4)
+    HNS_FCG_CASE(5)
+    HNS_FCG_CASE(6)
+    HNS_FCG_CASE(7)
+    HNS_FCG_CASE(8)
+#undef HNS_FCG_CASE
+  }
   k_fcg_spmv<<<g, kTpb, 0, s>>>(T, it, n, Dr, Qr, S);
   HNS_HIP(hipGetLastError());
 }
