@@ -12,10 +12,10 @@ inline unsigned cg_groups(int32_t L) {
   return g < (unsigned)kCgMaxGroups ? g : (unsigned)kCgMaxGroups;
 }
 
-// Device scalars of one solve. rho[k] = r_k . r_k (CG) or r_k . z_k (PCG), alpha[k] the step
+// Device scalars of one solve. rho[k] = r_k . r_k (CG) or sigma_k = d_k . A d_k (flexible CG), alpha[k] the step
 // length of iteration k; flag[0] = stop, flag[1] = iterations done when stopped.
 struct CgScalars {
-  double* pa;     // kCgSlots rows of kCgMaxGroups partials (row = slot), written by init/replace/update/rz
+  double* pa;     // kCgSlots rows of kCgMaxGroups partials (row = slot), written by init/replace/update and FCG
   double* pb;     // one partial per workgroup (d . q), written by the product kernel
   double* rho;    // max_iters + 1 entries
   double* alpha;  // max_iters + 1 entries
@@ -23,8 +23,8 @@ struct CgScalars {
   double tol2;    // stop when r.r <= tol2 * ||b||^2
   double* bb;     // ||b||^2
 };
-// pa slots: r.r, r.z, q_{k-1}.z
-enum CgSlot { kSlotRR = 0, kSlotRZ = 1, kSlotQZ = 2, kCgSlots = 4 };
+// pa slots: r.r, and for the flexible CG d.r and d.q
+enum CgSlot { kSlotRR = 0, kSlotDR = 1, kSlotDQ = 2, kCgSlots = 4 };
 // the recurrence residual is replaced by b - A x every this many iterations
 constexpr int kResidualReplace = 64;
 
@@ -35,14 +35,11 @@ void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, c
 void launch_cg_replace(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
                        bool force, hipStream_t s);
 // One CG iteration `it` in two launches. Product: d_out = r + beta d_in (beta = 0 when `fresh`),
-// q = A d_out. Update: x += alpha d_out, r -= alpha q. With z != nullptr it is a flexible
-// preconditioned iteration (z = M r in place and launch_cg_rz run on it): d_out = z + beta d_in.
-void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* z, const float* d_in,
-                       float* d_out, float* q, const CgScalars& S, hipStream_t s);
+// q = A d_out. Update: x += alpha d_out, r -= alpha q.
+void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* d_in, float* d_out,
+                       float* q, const CgScalars& S, hipStream_t s);
 void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d, const float* q, const CgScalars& S,
                       hipStream_t s);
-// Partials of r.z and q.z (q of the previous iteration) after the preconditioner wrote z.
-void launch_cg_rz(int32_t L, const float* r, const float* z, const float* q, const CgScalars& S, hipStream_t s);
 
 // Flexible CG with truncation kFcgDepth (Notay's FCG(m)): the directions d_i and products
 // q_i = A d_i of the last kFcgDepth iterations stay in the rings Dr, Qr (kFcgDepth vectors of

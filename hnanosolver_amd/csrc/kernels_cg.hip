@@ -15,10 +15,10 @@
 // residual is replaced by b - A x every kResidualReplace iterations and before a stop is
 // accepted, so FP32 drift of the recurrence cannot end the solve early.
 //
-// Flexible PCG (Notay's truncated FCG(1)): beta_k = -alpha_{k-1} (q_{k-1} . z_k) / (r_{k-1} . z_{k-1}),
-// which is the Polak-Ribiere form z_k . (r_k - r_{k-1}) / (r_{k-1} . z_{k-1}) without keeping r_{k-1}.
+// The preconditioned solve is flexible CG with truncation kFcgDepth (Notay's FCG(m); below), since
+// the learned preconditioner is nonlinear.
 //
-// Per iteration and voxel: product reads r (or z), d (8 B) and writes d, q (8 B); update reads
+// Per CG iteration and voxel: product reads r, d (8 B) and writes d, q (8 B); update reads
 // x, d, r, q (16 B) and writes x, r (8 B): 40 B.
 #include <algorithm>
 
@@ -143,23 +143,21 @@ __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __r
   if (force) clear_flags(S);
 }
 
-// Iteration `it` (0-based), product half: reduce r.r (and r.z, q_{k-1}.z) from pa; stop when
-// converged; d_out = s + beta d_in (s = r, or z when PRE) at the voxel pair and its neighbours;
+// Iteration `it` (0-based), product half: reduce r.r from pa; stop when converged;
+// d_out = r + beta d_in at the voxel pair and its neighbours;
 // q = A d_out; partials of d_out . q into pb. `fresh` restarts the direction (beta = 0) and, at
 // it == 0, records ||b||^2. d_in and d_out are distinct buffers, so no workgroup reads a value
 // another one has already replaced.
-template <bool PRE>
 __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fresh, const float* __restrict__ src,
                                                   const float* __restrict__ d_in, float* __restrict__ d_out,
                                                   float* __restrict__ q, CgScalars S) {
-  __shared__ double s_red[3 * (kTpb / 64)];
+  __shared__ double s_red[kTpb / 64];
   __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
   const int G = (int)gridDim.x;
-  static_assert(kSlotRR == 0 && kSlotRZ == 1 && kSlotQZ == 2, "slot rows read together");
-  double rows[PRE ? 3 : 1];
-  reduce_partial_rows(S.pa, G, s_red, rows);
-  const double rr = rows[kSlotRR];
+  double rows[1];
+  reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rows);
+  const double rr = rows[0];
   const double bb = it == 0 ? rr : S.bb[0];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (rr <= S.tol2 * bb) {  // identical decision in every workgroup
@@ -170,16 +168,10 @@ __global__ __launch_bounds__(kTpb) void k_cg_spmv(DevTopo T, int32_t it, int fre
     }
     return;
   }
-  double gam = rr, beta = 0.0;
-  if (PRE) {
-    gam = rows[PRE ? kSlotRZ : 0];
-    if (!fresh) beta = -S.alpha[it - 1] * rows[PRE ? kSlotQZ : 0] / S.rho[it - 1];
-  } else if (!fresh) {
-    beta = rr / S.rho[it - 1];
-  }
+  const double beta = fresh ? 0.0 : rr / S.rho[it - 1];
   if (lead) {
     if (it == 0) S.bb[0] = bb;
-    S.rho[it] = gam;
+    S.rho[it] = rr;
   }
   const float bf = (float)beta;
   const LeafRun R = leaf_run(T.L);
@@ -256,31 +248,6 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
   if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
 }
 
-// Partials of r.z and q.z (q of the previous iteration) into pa slots kSlotRZ, kSlotQZ.
-__global__ __launch_bounds__(kTpb) void k_cg_rz(int32_t L, const float* __restrict__ r, const float* __restrict__ z,
-                                                const float* __restrict__ q, CgScalars S) {
-  __shared__ double s_red[kTpb / 64];
-  if (S.flag[0]) return;
-  const LeafRun R = leaf_run(L);
-  const int32_t half = L * kHalfLeaf;
-  double a = 0.0, b = 0.0;
-  for (int32_t l = R.first; l < R.end; l += R.step) {
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
-      const double zv = (double)z[i];
-      a += (double)r[i] * zv;
-      b += (double)q[i] * zv;
-    }
-  }
-  const double ta = block_sum_all(a, s_red);
-  const double tb = block_sum_all(b, s_red);
-  if (threadIdx.x == 0) {
-    S.pa[kSlotRZ * kCgMaxGroups + blockIdx.x] = ta;
-    S.pa[kSlotQZ * kCgMaxGroups + blockIdx.x] = tb;
-  }
-}
-
 // ---- flexible CG with truncation m (Notay's FCG(m)), the learned-preconditioner solve ----------
 // Iteration k, after the host has applied z = M(r): the ring keeps the last m directions d_i and
 // products q_i = A d_i (slot i mod m) with sigma_i = d_i . q_i in S.rho[i].
@@ -290,7 +257,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_rz(int32_t L, const float* __restri
 //   spmv:   q_k = A d_k into slot k mod m; partials of d_k . q_k
 //   update: sigma_k = d_k . q_k, alpha_k = (d_k . r) / sigma_k; x += alpha_k d_k, r -= alpha_k q_k;
 //           partials of r . r
-// The d_k . r and d_k . q_k partials use the pa rows of r.z and q.z (unused by this solver).
+// The d_k . r and d_k . q_k partials go to the pa rows kSlotDR and kSlotDQ.
 __device__ __forceinline__ int fcg_slot(int32_t i) { return (int)(i % kFcgDepth); }
 
 template <int N>
@@ -386,7 +353,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int32_t it, int64_t
     }
   }
   const double tt = block_sum_all(acc, s_red);
-  if (threadIdx.x == 0) S.pa[kSlotRZ * kCgMaxGroups + blockIdx.x] = tt;
+  if (threadIdx.x == 0) S.pa[kSlotDR * kCgMaxGroups + blockIdx.x] = tt;
 }
 
 __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int32_t it, int64_t n, const float* __restrict__ Dr,
@@ -420,7 +387,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int32_t it, int64_
     acc += (double)d0 * (double)q0 + (double)d1 * (double)q1;
   }
   const double tt = block_sum_all(acc, s_red);
-  if (threadIdx.x == 0) S.pa[kSlotQZ * kCgMaxGroups + blockIdx.x] = tt;
+  if (threadIdx.x == 0) S.pa[kSlotDQ * kCgMaxGroups + blockIdx.x] = tt;
 }
 
 __global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int32_t it, int64_t n, float* __restrict__ x,
@@ -429,7 +396,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int32_t it, int6
   __shared__ double s_red[2 * (kTpb / 64)];
   if (S.flag[0]) return;
   double rows[2];  // d . r, d . q
-  reduce_partial_rows(S.pa + kSlotRZ * kCgMaxGroups, (int)gridDim.x, s_red, rows);
+  reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rows);
   const double sigma = rows[1];
   if (!(sigma > 0.0)) {  // breakdown (A is SPD; only round-off gets here); same in every workgroup
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -474,12 +441,9 @@ void launch_cg_replace(const DevTopo& T, const float* div, float dx, const float
   k_cg_replace<<<cg_groups(T.L), kTpb, 0, s>>>(T, div, dx * dx, x, r, S, force ? 1 : 0);
   HNS_HIP(hipGetLastError());
 }
-void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* z, const float* d_in,
-                       float* d_out, float* q, const CgScalars& S, hipStream_t s) {
-  if (z)
-    k_cg_spmv<true><<<cg_groups(T.L), kTpb, 0, s>>>(T, it, fresh ? 1 : 0, z, d_in, d_out, q, S);
-  else
-    k_cg_spmv<false><<<cg_groups(T.L), kTpb, 0, s>>>(T, it, fresh ? 1 : 0, r, d_in, d_out, q, S);
+void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r, const float* d_in, float* d_out,
+                       float* q, const CgScalars& S, hipStream_t s) {
+  k_cg_spmv<<<cg_groups(T.L), kTpb, 0, s>>>(T, it, fresh ? 1 : 0, r, d_in, d_out, q, S);
   HNS_HIP(hipGetLastError());
 }
 void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d, const float* q, const CgScalars& S,
@@ -487,11 +451,6 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
   k_cg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, x, r, d, q, S);
   HNS_HIP(hipGetLastError());
 }
-void launch_cg_rz(int32_t L, const float* r, const float* z, const float* q, const CgScalars& S, hipStream_t s) {
-  k_cg_rz<<<cg_groups(L), kTpb, 0, s>>>(L, r, z, q, S);
-  HNS_HIP(hipGetLastError());
-}
-
 void launch_fcg_direction(const DevTopo& T, int32_t it, const float* r, const float* z, float* Dr, float* Qr,
                           double* zq, const CgScalars& S, hipStream_t s) {
   const unsigned g = cg_groups(T.L);

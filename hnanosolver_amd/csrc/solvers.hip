@@ -208,7 +208,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       float* d_in = (it & 1) ? G.d1.p : G.d0.p;
       float* d_out = (it & 1) ? G.d0.p : G.d1.p;
       tm.begin(HNS_K_CG_PRODUCT, s);
-      launch_cg_product(T, it, it == 0, G.r.p, nullptr, d_in, d_out, G.q.p, S, s);
+      launch_cg_product(T, it, it == 0, G.r.p, d_in, d_out, G.q.p, S, s);
       tm.end(s);
       tm.begin(HNS_K_CG_UPDATE, s);
       launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
