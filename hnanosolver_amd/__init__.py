@@ -32,7 +32,7 @@ STATUS = {0: "HNS_OK", -1: "HNS_ERR_INVALID_ARGUMENT", -2: "HNS_ERR_EMPTY", -3: 
           -8: "HNS_ERR_OUT_OF_MEMORY", -9: "HNS_ERR_NOT_FOUND", -10: "HNS_ERR_NO_DEVICE",
           -11: "HNS_ERR_NOT_CONVERGED"}
 HNS_FLOAT, HNS_VEC3F = 1, 3
-SOLVERS = {"rbgs": 0, "rbgs_tol": 1, "cg": 2, "pcg_learned": 3}
+SOLVERS = {"rbgs": 0, "rbgs_tol": 1, "cg": 2, "pcg_learned": 3, "mgcg": 4}
 KERNEL_CLASSES = ["advect_velocity", "vorticity_forces", "divergence_combustion", "rbgs", "residual", "gradient",
                   "advect_scalars", "cg_product", "cg_update", "precond", "other"]
 

@@ -47,8 +47,9 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 // z = M(r): stop test, d = z - sum_i (z.q_i / sigma_i) d_i, q = A d, alpha = (d.r)/(d.q),
 // x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
 constexpr int kFcgDepth = 8;
-// stop test, new direction and its product (three launches)
-void launch_fcg_direction(const DevTopo& T, int32_t it, const float* r, const float* z, float* Dr, float* Qr,
+// stop test, new direction and its product (three launches), orthogonalised against the last
+// min(it, depth) directions (1 <= depth <= kFcgDepth)
+void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,
                           double* zq, const CgScalars& S, hipStream_t s);
 // x += alpha d, r -= alpha q
 void launch_fcg_update(int32_t L, int32_t it, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,

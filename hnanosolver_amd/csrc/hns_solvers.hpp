@@ -110,6 +110,7 @@ class Solvers {
   double* h_red_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   CgState* cg_ = nullptr;
+  struct Multigrid* mg_ = nullptr;
   void rbgs_iters(const DevTopo& T, const float* div, float* p, int first_it, int n, const SolveRequest& r, Timing& tm,
                   hipStream_t s);
   SolveResult solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,

@@ -451,12 +451,13 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
   k_cg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, x, r, d, q, S);
   HNS_HIP(hipGetLastError());
 }
-void launch_fcg_direction(const DevTopo& T, int32_t it, const float* r, const float* z, float* Dr, float* Qr,
+void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,
                           double* zq, const CgScalars& S, hipStream_t s) {
+  if (depth < 1 || depth > kFcgDepth) fail(HNS_ERR_INVALID_ARGUMENT, "flexible CG truncation out of range");
   const unsigned g = cg_groups(T.L);
   const int64_t n = (int64_t)T.L * kLeafVox;
   static_assert(kFcgDepth == 8, "dispatch below covers ring fills 0..8");
-  switch (std::min<int32_t>(it, kFcgDepth)) {
+  switch (std::min<int32_t>(it, depth)) {
 #define HNS_FCG_CASE(NP)                                                   \
   case NP:                                                                 \
     k_fcg_dots<NP><<<g, kTpb, 0, s>>>(T.L, it, n, z, Qr, zq, S);           \

@@ -12,6 +12,7 @@
 #include <cstdlib>
 
 #include "hns_internal.hpp"
+#include "hns_mg.hpp"
 
 namespace hns {
 namespace {
@@ -271,6 +272,12 @@ int rbgs_lpw() {
   return v;
 }
 }  // namespace
+
+// The sweep on A u = b with A = 6 I - (face neighbours): div = b and dx^2 = -1 make the SOR target
+// (sum - div dx^2) / 6 exactly (sum + b) / 6; omega = 1 (Gauss-Seidel).
+void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, hipStream_t s) {
+  launch_rbgs_lpw<1>(T, b_cs, u_cs, -1.0f, colour, 1.0f, first, s);
+}
 
 void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,
                  hipStream_t s) {

@@ -7,6 +7,7 @@
 #include <cstring>
 
 #include "hns_cg.hpp"
+#include "hns_mg.hpp"
 #include "hns_precond.hpp"
 #include "hns_solvers.hpp"
 
@@ -31,6 +32,7 @@ Solvers::~Solvers() {
   if (e0_) (void)hipEventDestroy(e0_);
   if (e1_) (void)hipEventDestroy(e1_);
   delete cg_;
+  delete mg_;
 }
 
 void Solvers::init(int32_t L, size_t N) {
@@ -120,6 +122,7 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
     }
     case HNS_SOLVER_CG:
     case HNS_SOLVER_PCG_LEARNED:
+    case HNS_SOLVER_MGCG:
       res = solve_cg(T, div, p, r, tm, s);
       break;
     default:
@@ -137,8 +140,13 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
 
 SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
                               hipStream_t s) {
-  const bool pre = r.solver == HNS_SOLVER_PCG_LEARNED;
-  if (pre && (!r.pc || !r.pcw)) fail(HNS_ERR_INVALID_ARGUMENT, "no learned preconditioner is set");
+  // flexible CG on a preconditioner: the learned network (truncation kFcgDepth) or the multigrid
+  // V-cycle (nearly symmetric and linear: truncation kMgFcgDepth)
+  const bool learned = r.solver == HNS_SOLVER_PCG_LEARNED, mg = r.solver == HNS_SOLVER_MGCG;
+  const bool pre = learned || mg;
+  const int depth = learned ? kFcgDepth : kMgFcgDepth;
+  if (learned && (!r.pc || !r.pcw)) fail(HNS_ERR_INVALID_ARGUMENT, "no learned preconditioner is set");
+  if (mg && !mg_) mg_ = new Multigrid();
   const int cap = r.max_iters > 0 ? r.max_iters : 100000;
   if (!cg_) cg_ = new CgState();
   CgState& G = *cg_;
@@ -193,12 +201,13 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         replaced_at = it;
       }
       if (pre) {
-        // flexible CG with truncation kFcgDepth on z = M(r)
+        // flexible CG on z = M(r)
         tm.begin(HNS_K_PRECOND, s);
-        r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
+        if (learned) r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
+        else mg_->apply(T, G.r.p, G.z.p, s);
         tm.end(s);
         tm.begin(HNS_K_CG_PRODUCT, s);
-        launch_fcg_direction(T, it, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+        launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
         tm.end(s);
         tm.begin(HNS_K_CG_UPDATE, s);
         launch_fcg_update(T.L, it, p, G.r.p, G.dring.p, G.qring.p, S, s);

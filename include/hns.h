@@ -135,6 +135,7 @@ hns_status hns_advance(hns_context* ctx, int32_t iteration, float dt, const hns_
 #define HNS_SOLVER_RBGS_TOL 1    /* RBGS until ||div - L p|| / ||div|| <= tol (or max_iters) */
 #define HNS_SOLVER_CG 2          /* conjugate gradients on the same operator, to tol */
 #define HNS_SOLVER_PCG_LEARNED 3 /* flexible PCG with the learned preconditioner, to tol */
+#define HNS_SOLVER_MGCG 4        /* CG preconditioned by a geometric multigrid V-cycle, to tol */
 typedef struct hns_solve_options {
   uint32_t struct_size;  /* sizeof(hns_solve_options) */
   int32_t solver;        /* HNS_SOLVER_* */
