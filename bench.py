@@ -38,6 +38,10 @@ KERNEL_BYTES = {
     "cg_update": 24.0,
 }
 KERNEL_NAMES = {"rbgs": "k_rbgs", "cg_product": "k_cg_spmv", "cg_update": "k_cg_update"}
+# Multigrid-preconditioned CG, compulsory bytes per voxel and iteration: level 0 of the V-cycle
+# (4 colour sweeps 32 B, residual + restriction 8 B, prolongation 9 B), the coarser levels (~8 B),
+# and the flexible CG with truncation 1 (z.q 8 B, direction 16 B, product 8 B, update 24 B)
+MGCG_BYTES_PER_ITERATION = 32.0 + 8.0 + 9.0 + 8.0 + 8.0 + 16.0 + 8.0 + 24.0
 
 
 def step_bytes(solve_bytes: float, m_scalars: int) -> float:
@@ -183,8 +187,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fire256_8m")
-    ap.add_argument("--legs", default="cg,pcg_learned,rbgs,rbgs_tol",
-                    help="solver legs: cg, pcg_learned, rbgs (fixed K), rbgs_tol")
+    ap.add_argument("--legs", default="mgcg,cg,pcg_learned,rbgs,rbgs_tol",
+                    help="solver legs: mgcg, cg, pcg_learned, rbgs (fixed K), rbgs_tol")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -219,12 +223,18 @@ def main(argv=None):
         legs[solver] = run_leg(hns, torch, sc, data, topo, params, solver,
                                steps=min(a.steps, 3) if slow else a.steps, warmup=min(a.warmup, 1) if slow else a.warmup,
                                instrument=solver != "rbgs_tol", K=K)
-    tol_legs = [s for s in legs if s in ("cg", "pcg_learned")]
+    tol_legs = [s for s in legs if s in ("mgcg", "cg", "pcg_learned")]
     head = min(tol_legs, key=lambda s: legs[s]["ms_per_step"]) if tol_legs else next(iter(legs))
     L = legs[head]
     value = L["voxel_steps_per_s"]
+    kern_leg = L
     if head == "rbgs":
         kern, solve_b = "rbgs", 2 * KERNEL_BYTES["rbgs"] * K
+    elif head == "mgcg":
+        # the V-cycle's dominant kernel is the level-0 red-black sweep (4 per iteration), the same
+        # k_rbgs launch the rbgs leg times one by one
+        kern, kern_leg = "rbgs", legs.get("rbgs", L)
+        solve_b = MGCG_BYTES_PER_ITERATION * L["iterations_per_step"]
     else:
         kern = max(("cg_product", "cg_update"), key=lambda k: L["kernels"][k]["ms_per_step"])
         solve_b = (KERNEL_BYTES["cg_product"] + KERNEL_BYTES["cg_update"]) * L["iterations_per_step"]
@@ -236,7 +246,7 @@ def main(argv=None):
         "config": {"workload": sc.name, "voxels": sc.n, "leaves": sc.n_leaves, "dt": sc.dt,
                    "voxel_size": sc.voxel_size, "advected_scalars": L["m_scalars"], "solver": head,
                    "tol": TOL if head != "rbgs" else None, "notes": sc.notes},
-        "roofline": roofline(L, kern, sc.n, sc.name),
+        "roofline": roofline(kern_leg, kern, sc.n, sc.name),
         "step_roofline": {"bytes_per_voxel_step": step_b, "achieved_GBs": step_b * value / 1e9,
                           "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
         "legs": legs,
@@ -248,8 +258,9 @@ def main(argv=None):
             out["speedup_%s_vs_plain_rbgs_to_tol" % s] = legs["rbgs_tol"]["ms_per_step"] / legs[s]["ms_per_step"]
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(sc, "cg" if head != "rbgs" else "rbgs", L["iterations_per_step"],
-                                           a.cpu_budget)
+        # the oracle runs CG (it has no multigrid), at the GPU CG leg's iteration count when there is one
+        cpu_iters = legs["cg"]["iterations_per_step"] if "cg" in legs else L["iterations_per_step"]
+        out["cpu_baseline"] = cpu_baseline(sc, "cg" if head != "rbgs" else "rbgs", cpu_iters, a.cpu_budget)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

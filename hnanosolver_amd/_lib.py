@@ -101,6 +101,7 @@ _SIGS = [
     ("hnsk_precond_adam", C.c_int32, [C.c_int64, _VP, _VP, _VP, _VP, C.c_float, C.c_float, C.c_float, C.c_float,
                                       C.c_int32, _VP]),
     ("hnsk_poisson_apply", C.c_int32, [_VP, _VP, _VP, C.c_float, C.c_float, _VP, _VP, _VP]),
+    ("hnsk_mg_apply", C.c_int32, [_VP, _VP, _VP, C.c_int32, _VP]),
     ("hnsk_pressure_solve_learned", C.c_int32, [_VP, _VP, _VP, _VP, C.c_float, C.POINTER(hns_solve_options),
                                                 C.POINTER(hns_solve_stats), _VP]),
     ("hnsk_diff_forward", C.c_int32, [_VP, _VP, _VP, C.c_int32, C.c_float, C.c_float, C.c_int32, C.c_float,

@@ -13,6 +13,7 @@
 
 #include "hns_adjoint.hpp"
 #include "hns_internal.hpp"
+#include "hns_mg.hpp"
 #include "hns_precond.hpp"
 #include "hns_solvers.hpp"
 
@@ -1054,6 +1055,15 @@ hns_status hnsk_poisson_apply(const hns_topology* topo, const float* x_cs, const
     const DevTopo T = topo_of(topo);
     if (!x_cs || !y_cs || !out_cs || !work || out_cs == x_cs) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
     launch_pc_poisson(T, x_cs, y_cs, a, b, out_cs, work + 4, work, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
+hns_status hnsk_mg_apply(const hns_topology* topo, const float* r_cs, float* z_cs, int32_t graph, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!r_cs || !z_cs || r_cs == z_cs) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    topo->solvers().multigrid().apply(T, r_cs, z_cs, static_cast<hipStream_t>(stream), graph != 0);
     return HNS_OK;
   });
 }

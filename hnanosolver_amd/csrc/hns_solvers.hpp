@@ -100,6 +100,8 @@ class Solvers {
                     hipStream_t s);
   // ||div - L p|| / ||div|| (0 when div == 0); synchronises s.
   double rel_residual(const DevTopo& T, const float* div, const float* p, float dx, Timing& tm, hipStream_t s);
+  // the multigrid V-cycle of the HNS_SOLVER_MGCG solves (made on first use)
+  struct Multigrid& multigrid();
 
  private:
   int32_t L_ = 0;

@@ -11,9 +11,29 @@
 // neighbour, so level l carries 4^l (6 u - sum of face neighbours), zero outside active leaves as at
 // level 0. Smoothing is red-black Gauss-Seidel on each level's colouring by the parity of the global
 // level coordinate (at level 3, the leaf's).
+#include <cstdlib>
+
 #include "hns_mg.hpp"
 
 namespace hns {
+
+const MgShape& mg_shape() {
+  static const MgShape v = [] {
+    MgShape m;
+    auto env = [](const char* k, int lo, int hi, int d) {
+      const char* e = std::getenv(k);
+      const int x = e ? std::atoi(e) : d;
+      return x >= lo && x <= hi ? x : d;
+    };
+    m.coarse = env("HNS_MG_COARSE", 1, 256, m.coarse);
+    m.smooth0 = env("HNS_MG_SMOOTH0", 1, 8, m.smooth0);
+    m.smooth = env("HNS_MG_SMOOTH", 1, 8, m.smooth);
+    m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
+    return m;
+  }();
+  return v;
+}
+
 namespace {
 
 constexpr int kTpb = 256;
@@ -172,6 +192,10 @@ void smooth(const DevTopo& T, int lgS, const float* b, float* u, float scale, in
 
 void Multigrid::ensure(int32_t leaves) {
   if (L >= leaves && b[1].p) return;
+  if (exec_) {  // the captured V-cycle refers to the buffers about to be replaced
+    HNS_HIP(hipGraphExecDestroy(exec_));
+    exec_ = nullptr;
+  }
   L = leaves;
   for (int l = 1; l < kMgLevels; ++l) {
     const size_t n = (size_t)leaves << (3 * (3 - l));
@@ -180,18 +204,20 @@ void Multigrid::ensure(int32_t leaves) {
   }
 }
 
-void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s) {
-  ensure(T.L);
-  // level 0: z = 0, one red-black pre-smoothing iteration (ω = 1, A z = r)
-  launch_rbgs_rhs(T, r_cs, z_cs, 0, true, s);
-  launch_rbgs_rhs(T, r_cs, z_cs, 1, true, s);
+void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s) {
+  const MgShape& M = mg_shape();
+  // level 0: z = 0, red-black pre-smoothing (Gauss-Seidel on A z = r)
+  for (int k = 0; k < M.smooth0; ++k) {
+    launch_rbgs_rhs(T, r_cs, z_cs, 0, k == 0, s);
+    launch_rbgs_rhs(T, r_cs, z_cs, 1, k == 0, s);
+  }
   k_mg0_restrict<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, z_cs, b[1].p);
   HNS_HIP(hipGetLastError());
-  // descend: levels 1, 2 pre-smooth and restrict; level 3 (one cell per leaf) smooths kMgCoarseIters times
+  // descend: levels 1, 2 pre-smooth and restrict; level 3 (one cell per leaf) is smoothed M.coarse times
   for (int l = 1; l < kMgLevels; ++l) {
     const int lgS = 3 - l;
     const float scale = (float)(1 << (2 * l));
-    const int iters = l == kMgLevels - 1 ? kMgCoarseIters : 1;
+    const int iters = l == kMgLevels - 1 ? M.coarse : M.smooth;
     for (int k = 0; k < iters; ++k) {
       smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, s);
       smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, s);
@@ -207,13 +233,55 @@ void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStrea
     const float scale = (float)(1 << (2 * l));
     k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p);
     HNS_HIP(hipGetLastError());
-    smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, s);
-    smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, s);
+    for (int k = 0; k < M.smooth; ++k) {
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, s);
+    }
   }
   k_mg0_prolong<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, z_cs, u[1].p);
   HNS_HIP(hipGetLastError());
-  launch_rbgs_rhs(T, r_cs, z_cs, 1, false, s);
-  launch_rbgs_rhs(T, r_cs, z_cs, 0, false, s);
+  for (int k = 0; k < M.smooth0; ++k) {
+    launch_rbgs_rhs(T, r_cs, z_cs, 1, false, s);
+    launch_rbgs_rhs(T, r_cs, z_cs, 0, false, s);
+  }
+}
+
+void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s, bool graph) {
+  ensure(T.L);
+  if (!graph) {
+    launch_all(T, r_cs, z_cs, s);
+    return;
+  }
+  if (!exec_ || key_nbr_ != T.nbr || key_L_ != T.L || key_r_ != r_cs || key_z_ != z_cs) {
+    if (exec_) {
+      HNS_HIP(hipGraphExecDestroy(exec_));
+      exec_ = nullptr;
+    }
+    if (!cap_) HNS_HIP(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    HNS_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_all(T, r_cs, z_cs, cap_);
+    } catch (...) {
+      (void)hipStreamEndCapture(cap_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HNS_HIP(hipStreamEndCapture(cap_, &g));
+    const hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HNS_HIP(e);
+    key_nbr_ = T.nbr;
+    key_L_ = T.L;
+    key_r_ = r_cs;
+    key_z_ = z_cs;
+  }
+  HNS_HIP(hipGraphLaunch(exec_, s));
+}
+
+Multigrid::~Multigrid() {
+  if (exec_) (void)hipGraphExecDestroy(exec_);
+  if (cap_) (void)hipStreamDestroy(cap_);
 }
 
 }  // namespace hns

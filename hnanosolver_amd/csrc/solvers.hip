@@ -27,6 +27,11 @@ struct CgState {
 
 Solvers::Solvers() = default;
 
+Multigrid& Solvers::multigrid() {
+  if (!mg_) mg_ = new Multigrid();
+  return *mg_;
+}
+
 Solvers::~Solvers() {
   if (h_red_) (void)hipHostFree(h_red_);
   if (e0_) (void)hipEventDestroy(e0_);
@@ -141,12 +146,12 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
 SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
                               hipStream_t s) {
   // flexible CG on a preconditioner: the learned network (truncation kFcgDepth) or the multigrid
-  // V-cycle (nearly symmetric and linear: truncation kMgFcgDepth)
+  // V-cycle (linear and nearly symmetric: a short truncation suffices)
   const bool learned = r.solver == HNS_SOLVER_PCG_LEARNED, mg = r.solver == HNS_SOLVER_MGCG;
   const bool pre = learned || mg;
-  const int depth = learned ? kFcgDepth : kMgFcgDepth;
+  const int depth = learned ? kFcgDepth : mg_shape().depth;
   if (learned && (!r.pc || !r.pcw)) fail(HNS_ERR_INVALID_ARGUMENT, "no learned preconditioner is set");
-  if (mg && !mg_) mg_ = new Multigrid();
+  if (mg) multigrid();
   const int cap = r.max_iters > 0 ? r.max_iters : 100000;
   if (!cg_) cg_ = new CgState();
   CgState& G = *cg_;

@@ -109,3 +109,12 @@ def aos_to_soa(x, out, asynchronous=False):
 def soa_to_aos(x, out, asynchronous=False):
     n = x.numel() // 3
     _check(lib().hnsk_soa_to_aos(n, _ptr(x), _ptr(out), _stream(), int(asynchronous)))
+
+
+def mg_apply(topo, r_cs, z_cs, graph=True, asynchronous=False):
+    """z = V(r): one multigrid V-cycle of the HNS_SOLVER_MGCG preconditioner (colour-split device
+    vectors); graph replays the captured V-cycle, otherwise its kernels are launched one by one."""
+    _check(lib().hnsk_mg_apply(topo.handle, _ptr(r_cs), _ptr(z_cs), int(bool(graph)), _stream()))
+    if not asynchronous:
+        import torch
+        torch.cuda.synchronize()
