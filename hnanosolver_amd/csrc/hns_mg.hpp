@@ -8,12 +8,14 @@ namespace hns {
 // levels 0 (voxels) .. 3 (one cell per leaf)
 constexpr int kMgLevels = 4;
 
-// V-cycle shape: red-black Gauss-Seidel iterations on the coarsest level, pre- and post-smoothing
-// iterations on level 0 and on levels 1-2, and the flexible CG's truncation around the V-cycle.
-// Defaults are the measured best at 8 M voxels; HNS_MG_COARSE, HNS_MG_SMOOTH0, HNS_MG_SMOOTH and
-// HNS_MG_DEPTH override them (read once per process).
+// V-cycle shape: red-black SOR iterations on the coarsest level and their over-relaxation, pre- and
+// post-smoothing Gauss-Seidel iterations on level 0 and on levels 1-2, and the flexible CG's
+// truncation around the V-cycle. Defaults are the measured best at 8 M voxels; HNS_MG_COARSE,
+// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH and HNS_MG_DEPTH override them (read
+// once per process).
 struct MgShape {
-  int coarse = 32, smooth0 = 1, smooth = 1, depth = 1;
+  int coarse = 12, smooth0 = 1, smooth = 1, depth = 1;
+  float coarse_omega = 1.85f;
 };
 const MgShape& mg_shape();
 

@@ -29,6 +29,7 @@ const MgShape& mg_shape() {
     m.smooth0 = env("HNS_MG_SMOOTH0", 1, 8, m.smooth0);
     m.smooth = env("HNS_MG_SMOOTH", 1, 8, m.smooth);
     m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
+    m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
     return m;
   }();
   return v;
@@ -65,12 +66,12 @@ __device__ __forceinline__ float mg_nsum(const DevTopo& T, int lgS, const float*
          mg_at(u, mg_face<2, 1>(T, lgS, leaf, x, y, z)) + mg_at(u, mg_face<2, -1>(T, lgS, leaf, x, y, z));
 }
 
-// One Gauss-Seidel colour sweep at level lgS >= 0 (leaf-dense): cells of colour COLOUR solve
-// s (6 u - sum) = b for u. FIRST: the level's u starts at zero (colour 0 reads no neighbours; both
-// colours take u_old = 0).
+// One SOR colour sweep at level lgS >= 0 (leaf-dense): cells of colour COLOUR move towards the
+// solution of s (6 u - sum) = b by omega times the Gauss-Seidel step. FIRST: the level's u starts
+// at zero (colour 0 reads no neighbours; both colours take u_old = 0).
 template <int COLOUR, bool FIRST>
 __global__ __launch_bounds__(kTpb) void k_mg_smooth(DevTopo T, int lgS, const float* __restrict__ b,
-                                                    float* __restrict__ u, float inv_s) {
+                                                    float* __restrict__ u, float inv_s, float omega) {
   const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
   const int32_t ncell = T.L << (3 * lgS);
   if (c >= ncell) return;
@@ -88,7 +89,8 @@ __global__ __launch_bounds__(kTpb) void k_mg_smooth(DevTopo T, int lgS, const fl
     return;
   }
   const float s = (FIRST && COLOUR == 0) ? 0.f : mg_nsum(T, lgS, u, leaf, x, y, z);
-  u[c] = (b[c] * inv_s + s) * (1.0f / 6.0f);
+  const float gs = (b[c] * inv_s + s) * (1.0f / 6.0f);
+  u[c] = FIRST ? omega * gs : fmaf(omega, gs - u[c], u[c]);  // SOR; omega = 1 is Gauss-Seidel
 }
 
 // Restriction of the level-lgS residual b - s (6 u - sum) (leaf-dense) to level lgS + 1: one lane
@@ -175,15 +177,16 @@ __global__ __launch_bounds__(kTpb) void k_mg0_prolong(int32_t L, float* __restri
 
 inline unsigned cells_grid(int32_t L, int lgS) { return (unsigned)((((int64_t)L << (3 * lgS)) + kTpb - 1) / kTpb); }
 
-void smooth(const DevTopo& T, int lgS, const float* b, float* u, float scale, int colour, bool first, hipStream_t s) {
+void smooth(const DevTopo& T, int lgS, const float* b, float* u, float scale, int colour, bool first, float omega,
+            hipStream_t s) {
   const unsigned g = cells_grid(T.L, lgS);
   const float inv = 1.0f / scale;
   if (colour == 0) {
-    if (first) k_mg_smooth<0, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv);
-    else k_mg_smooth<0, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv);
+    if (first) k_mg_smooth<0, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
+    else k_mg_smooth<0, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
   } else {
-    if (first) k_mg_smooth<1, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv);
-    else k_mg_smooth<1, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv);
+    if (first) k_mg_smooth<1, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
+    else k_mg_smooth<1, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
   }
   HNS_HIP(hipGetLastError());
 }
@@ -218,9 +221,10 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, hip
     const int lgS = 3 - l;
     const float scale = (float)(1 << (2 * l));
     const int iters = l == kMgLevels - 1 ? M.coarse : M.smooth;
+    const float omega = l == kMgLevels - 1 ? M.coarse_omega : 1.0f;
     for (int k = 0; k < iters; ++k) {
-      smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, s);
-      smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, omega, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, omega, s);
     }
     if (l + 1 < kMgLevels) {
       k_mg_restrict<<<cells_grid(T.L, lgS - 1), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, b[l + 1].p);
@@ -234,8 +238,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, hip
     k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p);
     HNS_HIP(hipGetLastError());
     for (int k = 0; k < M.smooth; ++k) {
-      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, s);
-      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, 1.0f, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, 1.0f, s);
     }
   }
   k_mg0_prolong<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, z_cs, u[1].p);
