@@ -246,7 +246,7 @@ def main(argv=None):
         "config": {"workload": sc.name, "voxels": sc.n, "leaves": sc.n_leaves, "dt": sc.dt,
                    "voxel_size": sc.voxel_size, "advected_scalars": L["m_scalars"], "solver": head,
                    "tol": TOL if head != "rbgs" else None, "notes": sc.notes},
-        "roofline": roofline(kern_leg, kern, sc.n, sc.name),
+        "roofline": roofline(kern_leg, kern, sc.n, sc.name) if kern in kern_leg.get("kernels", {}) else None,
         "step_roofline": {"bytes_per_voxel_step": step_b, "achieved_GBs": step_b * value / 1e9,
                           "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
         "legs": legs,

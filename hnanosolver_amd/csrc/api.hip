@@ -1059,11 +1059,15 @@ hns_status hnsk_poisson_apply(const hns_topology* topo, const float* x_cs, const
   });
 }
 
-hns_status hnsk_mg_apply(const hns_topology* topo, const float* r_cs, float* z_cs, int32_t graph, void* stream) {
+hns_status hnsk_mg_apply(const hns_topology* topo, const float* r_cs, float* z_cs, int32_t flags, void* stream) {
   return guard([&] {
     const DevTopo T = topo_of(topo);
-    if (!r_cs || !z_cs || r_cs == z_cs) fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
-    topo->solvers().multigrid().apply(T, r_cs, z_cs, static_cast<hipStream_t>(stream), graph != 0);
+    if (!r_cs || !z_cs || r_cs == z_cs || (flags & ~(HNS_MG_GRAPH | HNS_MG_COARSE_LAUNCHES | HNS_MG_UNFUSED)))
+      fail(HNS_ERR_INVALID_ARGUMENT, "bad arguments");
+    const unsigned variant = ((flags & HNS_MG_COARSE_LAUNCHES) ? kMgCoarseLaunches : 0u) |
+                             ((flags & HNS_MG_UNFUSED) ? kMgUnfused : 0u);
+    topo->solvers().multigrid().apply(T, r_cs, z_cs, static_cast<hipStream_t>(stream), (flags & HNS_MG_GRAPH) != 0,
+                                      nullptr, variant);
     return HNS_OK;
   });
 }

@@ -43,11 +43,14 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 
 // Flexible CG with truncation kFcgDepth (Notay's FCG(m)): the directions d_i and products
 // q_i = A d_i of the last kFcgDepth iterations stay in the rings Dr, Qr (kFcgDepth vectors of
-// L * 512 each, slot i mod kFcgDepth), sigma_i = d_i . q_i in S.rho[i]. One iteration `it` after
-// z = M(r): stop test, d = z - sum_i (z.q_i / sigma_i) d_i, q = A d, alpha = (d.r)/(d.q),
+// L * 512 each, slot i mod kFcgDepth), sigma_i = d_i . q_i in S.rho[i]. One iteration `it`: stop
+// test, z = M(r), d = z - sum_i (z.q_i / sigma_i) d_i, q = A d, alpha = (d.r)/(d.q),
 // x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
 constexpr int kFcgDepth = 8;
-// stop test, new direction and its product (three launches), orthogonalised against the last
+// stop test of iteration `it` on the r.r partials (one workgroup), launched before z = M(r):
+// once it raises S.flag, the preconditioner's kernels and the rest of the iteration return at once
+void launch_fcg_test(int32_t L, int32_t it, const CgScalars& S, hipStream_t s);
+// new direction and its product (three launches), orthogonalised against the last
 // min(it, depth) directions (1 <= depth <= kFcgDepth)
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,
                           double* zq, const CgScalars& S, hipStream_t s);

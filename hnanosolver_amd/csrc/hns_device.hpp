@@ -298,4 +298,28 @@ __device__ inline float trilinear_s(const LeafView& V, const float* __restrict__
   return lerp(y0, y1, tx);
 }
 
+// The SOR update of redBlackGaussSeidelUpdate (Kernel.cu:487-488), shared by every red-black
+// kernel so they produce identical bits: pGS = (sum6 - div dx^2) / 6 with the reference's
+// 0.166666667f.
+__device__ __forceinline__ float sor_update(float s, float dv, float pOld, float dx2, float omega) {
+  const float pGS = (s - dv * dx2) * 0.166666667f;
+  return pOld + omega * (pGS - pOld);
+}
+
+// Empty device asm that makes a value opaque to the optimiser, so what is derived from it is
+// recomputed where it is used instead of being hoisted into registers across a loop (v: a
+// per-lane value, s: a wave-uniform one). The host pass of a template kernel sees nothing.
+template <class T>
+__device__ __forceinline__ void opaque_v(T& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+v"(v));
+#endif
+}
+template <class T>
+__device__ __forceinline__ void opaque_s(T& v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm volatile("" : "+s"(v));
+#endif
+}
+
 }  // namespace hns

@@ -8,42 +8,67 @@ namespace hns {
 // levels 0 (voxels) .. 3 (one cell per leaf)
 constexpr int kMgLevels = 4;
 
-// V-cycle shape: red-black SOR iterations on the coarsest level and their over-relaxation, pre- and
+// V-cycle shape: red-black SOR iterations on the coarsest level (one workgroup for up to 8192
+// leaves of each colour, else one launch per colour sweep) and their over-relaxation, pre- and
 // post-smoothing Gauss-Seidel iterations on level 0 and on levels 1-2, and the flexible CG's
-// truncation around the V-cycle. Defaults are the measured best at 8 M voxels; HNS_MG_COARSE,
-// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH and HNS_MG_DEPTH override them (read
-// once per process).
+// truncation around the V-cycle, and the scale of every coarse-grid correction (piecewise-constant
+// prolongation under-corrects smooth errors; over-correcting by 1.6 took the 8 M-voxel fire solve
+// from 16 to 12 iterations). Defaults are the measured best at 8 M voxels; HNS_MG_COARSE,
+// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH and HNS_MG_OVER (x100)
+// override them (read once per process).
 struct MgShape {
   int coarse = 12, smooth0 = 1, smooth = 1, depth = 1;
   float coarse_omega = 1.85f;
+  float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
 };
 const MgShape& mg_shape();
 
 // The V-cycle is ~35 small launches, most of them on the coarse levels; it is captured once into a
 // HIP graph per (topology, r, z) and replayed (the captured kernels take only these pointers, and
 // none records a memset), which removes most of the per-launch cost.
+// V-cycle variants that compute the same bits as the default (Multigrid::apply's `variant` bits)
+constexpr unsigned kMgCoarseLaunches = 1;  // coarsest level as one launch per colour sweep
+constexpr unsigned kMgUnfused = 2;         // levels 0-2 as separate sweep, restriction and prolongation launches
+
 struct Multigrid {
   int32_t L = 0;
   DBuf<float> b[kMgLevels], u[kMgLevels];  // levels 1..3 (leaf-dense); level 0 uses the caller's vectors
+  DBuf<float> zt;                          // level-0 scratch (colour-split): pre-smoothed z of the fused path
+  DBuf<float> ut[kMgLevels];               // levels 1..2: pre-smoothed u of the fused path
   void ensure(int32_t leaves);
   // z = V(r) on colour-split level-0 vectors: approximately A^-1 r for A = 6 I - (face neighbours).
   // graph: replay the captured V-cycle (captured on first use for these pointers); otherwise launch
-  // the kernels one by one. Both run the same kernels on the same data.
-  void apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s, bool graph = true);
+  // the kernels one by one. Both run the same kernels on the same data. stop: a device flag (null:
+  // none); every kernel of the V-cycle returns at once while it is non-zero (the solve has ended).
+  // variant: kMg* bits selecting an equivalent kernel sequence (same bits).
+  void apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s, bool graph = true,
+             const int* stop = nullptr, unsigned variant = 0);
   ~Multigrid();
 
  private:
-  void launch_all(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s);
+  void launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
+                  hipStream_t s);
+  // colour-sorted slot layout of the coarsest level for its one-workgroup solve, built on the host
+  // once per topology (synchronises s); coarse_ch_ = 0 when the leaves do not fit one workgroup
+  void prepare_coarse(const DevTopo& T, hipStream_t s);
+  DBuf<int32_t> cperm_;  // slot -> leaf (-1: padding)
+  DBuf<uint32_t> cnb_;   // three planes of packed 16-bit face-neighbour slots
+  int coarse_ch_ = 0;
+  const int32_t* ckey_nbr_ = nullptr;
+  int32_t ckey_L_ = 0;
   hipStream_t cap_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
   const int32_t* key_nbr_ = nullptr;
   int32_t key_L_ = 0;
   const float* key_r_ = nullptr;
   float* key_z_ = nullptr;
+  const int* key_stop_ = nullptr;
+  unsigned key_variant_ = 0;
 };
 
 // One red-black Gauss-Seidel colour sweep (ω = 1) on A u = b, colour-split (the RBGS kernel with
-// the right-hand side taken as b); `first`: u starts at zero.
-void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, hipStream_t s);
+// the right-hand side taken as b); `first`: u starts at zero; returns at once while *stop != 0.
+void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, const int* stop,
+                     hipStream_t s);
 
 }  // namespace hns

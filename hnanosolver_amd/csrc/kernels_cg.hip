@@ -278,29 +278,32 @@ __device__ inline void block_store_rows(double (&v)[N], double* s_red, double* r
   }
 }
 
+// Stop test of flexible-CG iteration `it`, before its preconditioner runs: one workgroup reduces
+// the G partials of r.r (G = the CG grid) and raises the stop flag, on which every later kernel of
+// the iteration, the preconditioner's included, returns at once. Iteration 0 records ||b||^2.
+__global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int32_t it, CgScalars S) {
+  __shared__ double s_red[kTpb / 64];
+  if (S.flag[0]) return;
+  double rr1[1];
+  reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rr1);
+  const double rr = rr1[0];
+  const double bb = it == 0 ? rr : S.bb[0];
+  if (threadIdx.x == 0) {
+    if (it == 0) S.bb[0] = bb;
+    if (rr <= S.tol2 * bb) {
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+  }
+}
+
 // NP = min(it, kFcgDepth) ring entries in use (a template argument, so every load is unconditional)
 template <int NP>
 __global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
                                                    const float* __restrict__ Qr, double* __restrict__ zq,
                                                    CgScalars S) {
   __shared__ double s_red[kFcgDepth * (kTpb / 64)];
-  if (S.flag[0]) return;
-  const int G = (int)gridDim.x;
-  double rr1[1];
-  reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rr1);
-  const double rr = rr1[0];
-  const double bb = it == 0 ? rr : S.bb[0];
-  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  if (rr <= S.tol2 * bb) {  // identical decision in every workgroup
-    if (lead) {
-      if (it == 0) S.bb[0] = bb;
-      S.flag[1] = it;
-      S.flag[0] = 1;
-    }
-    return;
-  }
-  if (lead && it == 0) S.bb[0] = bb;
-  if (NP == 0) return;
+  if (NP == 0 || S.flag[0]) return;
   const float* q[kFcgDepth];
 #pragma unroll
   for (int k = 0; k < kFcgDepth; ++k) q[k] = Qr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
@@ -449,6 +452,10 @@ void launch_cg_product(const DevTopo& T, int32_t it, bool fresh, const float* r,
 void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d, const float* q, const CgScalars& S,
                       hipStream_t s) {
   k_cg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, x, r, d, q, S);
+  HNS_HIP(hipGetLastError());
+}
+void launch_fcg_test(int32_t L, int32_t it, const CgScalars& S, hipStream_t s) {
+  k_fcg_test<<<1, kTpb, 0, s>>>((int)cg_groups(L), it, S);
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,

@@ -11,7 +11,9 @@
 // neighbour, so level l carries 4^l (6 u - sum of face neighbours), zero outside active leaves as at
 // level 0. Smoothing is red-black Gauss-Seidel on each level's colouring by the parity of the global
 // level coordinate (at level 3, the leaf's).
+#include <algorithm>
 #include <cstdlib>
+#include <vector>
 
 #include "hns_mg.hpp"
 
@@ -30,6 +32,7 @@ const MgShape& mg_shape() {
     m.smooth = env("HNS_MG_SMOOTH", 1, 8, m.smooth);
     m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
     m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
+    m.over = 0.01f * (float)env("HNS_MG_OVER", 50, 300, (int)(100.0f * m.over + 0.5f));
     return m;
   }();
   return v;
@@ -38,6 +41,8 @@ const MgShape& mg_shape() {
 namespace {
 
 constexpr int kTpb = 256;
+
+
 
 // Face neighbour of level cell (leaf, x, y, z) along AX by DIR (edge S = 1 << lgS): its index, or -1.
 template <int AX, int DIR>
@@ -66,58 +71,154 @@ __device__ __forceinline__ float mg_nsum(const DevTopo& T, int lgS, const float*
          mg_at(u, mg_face<2, 1>(T, lgS, leaf, x, y, z)) + mg_at(u, mg_face<2, -1>(T, lgS, leaf, x, y, z));
 }
 
-// One SOR colour sweep at level lgS >= 0 (leaf-dense): cells of colour COLOUR move towards the
-// solution of s (6 u - sum) = b by omega times the Gauss-Seidel step. FIRST: the level's u starts
-// at zero (colour 0 reads no neighbours; both colours take u_old = 0).
+// One SOR colour sweep at level lgS >= 1 (leaf-dense, S = 1 << lgS even): cells of colour COLOUR
+// move towards the solution of s (6 u - sum) = b by omega times the Gauss-Seidel step. One lane per
+// z-pair of cells, updating the pair's cell of this colour (the colour is the parity of the local
+// coordinate: leaf origins are even at every level). FIRST: the level's u starts at zero (colour 0
+// reads no neighbours and also zeroes the pair's colour-1 cell; both colours take u_old = 0).
+// stop: the solve's stop flag (null: none); once raised, the kernel returns at once.
 template <int COLOUR, bool FIRST>
 __global__ __launch_bounds__(kTpb) void k_mg_smooth(DevTopo T, int lgS, const float* __restrict__ b,
-                                                    float* __restrict__ u, float inv_s, float omega) {
-  const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
-  const int32_t ncell = T.L << (3 * lgS);
-  if (c >= ncell) return;
+                                                    float* __restrict__ u, float inv_s, float omega,
+                                                    const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int32_t h = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
+  if (h >= ((T.L << (3 * lgS)) >> 1)) return;
   const int m = (1 << lgS) - 1;
-  const int32_t leaf = c >> (3 * lgS);
-  const int x = (c >> (2 * lgS)) & m, y = (c >> lgS) & m, z = c & m;
-  // colour of the global level coordinate: the local parity for S even; the leaf's parity at S = 1
-  int lp = 0;
-  if (lgS == 0) {
-    const int4 o = T.origins[leaf];
-    lp = ((o.x + o.y + o.z) >> 3) & 1;
-  }
-  if (((lp + x + y + z) & 1) != COLOUR) {
-    if (FIRST && COLOUR == 0) u[c] = 0.f;  // the other colour's first sweep reads these as neighbours
-    return;
-  }
+  const int32_t c0 = h << 1;
+  const int32_t leaf = c0 >> (3 * lgS);
+  const int x = (c0 >> (2 * lgS)) & m, y = (c0 >> lgS) & m;
+  const int par = (x + y + COLOUR) & 1;
+  const int z = (c0 & m) | par;
+  const int32_t c = c0 | par;
+  if (FIRST && COLOUR == 0) u[c ^ 1] = 0.f;  // the colour-1 cell, read as a neighbour by nobody yet
   const float s = (FIRST && COLOUR == 0) ? 0.f : mg_nsum(T, lgS, u, leaf, x, y, z);
   const float gs = (b[c] * inv_s + s) * (1.0f / 6.0f);
   u[c] = FIRST ? omega * gs : fmaf(omega, gs - u[c], u[c]);  // SOR; omega = 1 is Gauss-Seidel
 }
 
-// Restriction of the level-lgS residual b - s (6 u - sum) (leaf-dense) to level lgS + 1: one lane
-// per coarse cell sums its 8 children's residuals.
+// The same SOR colour sweep on level 3 (one cell per leaf; the colour is the leaf's parity), one
+// launch per colour: the path for topologies too large for k_mg_coarse.
+template <int COLOUR, bool FIRST>
+__global__ __launch_bounds__(kTpb) void k_mg_leaf_sweep(DevTopo T, const float* __restrict__ b, float* __restrict__ u,
+                                                        float inv_s, float omega, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
+  if (c >= T.L) return;
+  const int4 o = T.origins[c];
+  if ((((o.x + o.y + o.z) >> 3) & 1) != COLOUR) {
+    if (FIRST && COLOUR == 0) u[c] = 0.f;
+    return;
+  }
+  const float s = (FIRST && COLOUR == 0) ? 0.f : mg_nsum(T, 0, u, c, 0, 0, 0);
+  const float gs = (b[c] * inv_s + s) * (1.0f / 6.0f);
+  u[c] = FIRST ? omega * gs : fmaf(omega, gs - u[c], u[c]);
+}
+
+// Level 3 solved by `iters` red-black SOR iterations in one workgroup. The leaves are laid out in
+// colour-sorted slots (Multigrid::prepare_coarse): colour 0 in slots [0, H), colour 1 in [H, 2H),
+// H = kCoarseTpb * CH, unused slots padding; slot 2H is a zero that absent neighbours point at.
+// u lives in LDS by slot; lane t owns slots t + k * kCoarseTpb (k < CH: colour 0, else colour 1)
+// and keeps their scaled right-hand sides, old values and face-neighbour slots (16-bit pairs) in
+// registers, so a colour sweep is 6 * CH LDS reads per lane, a barrier, and no branches (a padding
+// slot reads only the zero slot and stays 0). Identical arithmetic, in the same order, to `iters`
+// pairs of k_mg_leaf_sweep launches.
+constexpr int kCoarseTpb = 1024;
+constexpr int kCoarseMaxCH = 8;
+template <int CH>
+__global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restrict__ perm,
+                                                          const uint32_t* __restrict__ nbp, const float* __restrict__ b,
+                                                          float* __restrict__ u, float inv_s, float omega, int iters,
+                                                          const int* __restrict__ stop) {
+  constexpr int C = 2 * CH, H = kCoarseTpb * CH;
+  __shared__ float s_u[2 * H + 1];
+  if (stop && *stop) return;
+  const int t = threadIdx.x;
+  float bi[C], uo[C];
+  uint32_t nb[C][3];
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int j = t + k * kCoarseTpb;
+    const int32_t leaf = perm[j];
+    nb[k][0] = nbp[j];          // +x | -x << 16
+    nb[k][1] = nbp[2 * H + j];  // +y | -y << 16
+    nb[k][2] = nbp[4 * H + j];  // +z | -z << 16
+    bi[k] = leaf >= 0 ? b[leaf] * inv_s : 0.f;
+    uo[k] = 0.f;
+  }
+  for (int j = t; j <= 2 * H; j += kCoarseTpb) s_u[j] = 0.f;
+  __syncthreads();
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int colour = 0; colour < 2; ++colour) {
+      // new values of this colour's slots first (they read only the other colour), then the stores
+      float v[CH];
+#pragma unroll
+      for (int kk = 0; kk < CH; ++kk) {
+        const int k = colour * CH + kk;
+        float sum = 0.f;
+        if (!(it == 0 && colour == 0)) {
+          uint32_t n0 = nb[k][0], n1 = nb[k][1], n2 = nb[k][2];
+          opaque_v(n0);  // unpack here, not once for all iterations into registers
+          opaque_v(n1);
+          opaque_v(n2);
+          sum = s_u[n0 & 0xffff] + s_u[n0 >> 16] + s_u[n1 & 0xffff] + s_u[n1 >> 16] + s_u[n2 & 0xffff] +
+                s_u[n2 >> 16];
+        }
+        const float gs = (bi[k] + sum) * (1.0f / 6.0f);
+        v[kk] = it == 0 ? omega * gs : fmaf(omega, gs - uo[k], uo[k]);
+      }
+#pragma unroll
+      for (int kk = 0; kk < CH; ++kk) {
+        const int k = colour * CH + kk;
+        uo[k] = v[kk];
+        s_u[t + k * kCoarseTpb] = v[kk];
+      }
+      __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    const int32_t leaf = perm[t + k * kCoarseTpb];
+    if (leaf >= 0) u[leaf] = uo[k];
+  }
+}
+
+// Restriction of the level-lgS residual b - s (6 u - sum) (leaf-dense, lgS = 1 or 2) to level
+// lgS + 1: one lane per fine cell forms its residual in LDS, then one lane per coarse cell sums
+// its 8 children (a workgroup holds whole leaves: 256 is a multiple of S^3).
 __global__ __launch_bounds__(kTpb) void k_mg_restrict(DevTopo T, int lgS, const float* __restrict__ b,
-                                                      const float* __restrict__ u, float s,
-                                                      float* __restrict__ bc) {
-  const int32_t cc = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
-  const int lgC = lgS - 1;
-  const int32_t ncoarse = T.L << (3 * lgC);
-  if (cc >= ncoarse) return;
-  const int mc = (1 << lgC) - 1;
-  const int32_t leaf = cc >> (3 * lgC);
+                                                      const float* __restrict__ u, float s, float over,
+                                                      float* __restrict__ bc, const int* __restrict__ stop) {
+  __shared__ float s_r[kTpb];
+  if (stop && *stop) return;
+  const int32_t ncell = T.L << (3 * lgS);
+  const int32_t f = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
+  const int m = (1 << lgS) - 1;
+  if (f < ncell) {
+    const int32_t leaf = f >> (3 * lgS);
+    const int x = (f >> (2 * lgS)) & m, y = (f >> lgS) & m, z = f & m;
+    s_r[threadIdx.x] = b[f] - s * (6.0f * u[f] - mg_nsum(T, lgS, u, leaf, x, y, z));
+  }
+  __syncthreads();
+  const int lgC = lgS - 1, mc = (1 << lgC) - 1;
+  const int32_t cc = (int32_t)(blockIdx.x * (kTpb / 8) + threadIdx.x);
+  if (threadIdx.x >= kTpb / 8 || cc >= (T.L << (3 * lgC))) return;
+  const int lb = threadIdx.x >> (3 * lgC);  // leaf within the workgroup
   const int X = (cc >> (2 * lgC)) & mc, Y = (cc >> lgC) & mc, Z = cc & mc;
   float sum = 0.f;
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k & 1);
-    const int32_t f = (leaf << (3 * lgS)) | (((x << lgS) | y) << lgS) | z;
-    sum += b[f] - s * (6.0f * u[f] - mg_nsum(T, lgS, u, leaf, x, y, z));
+    sum += s_r[(lb << (3 * lgS)) | (((x << lgS) | y) << lgS) | z];
   }
-  bc[cc] = sum;
+  bc[cc] = over * sum;
 }
 
 // u (level lgS, leaf-dense) += its parent's value in uc (level lgS + 1)
 __global__ __launch_bounds__(kTpb) void k_mg_prolong(int32_t L, int lgS, float* __restrict__ u,
-                                                     const float* __restrict__ uc) {
+                                                     const float* __restrict__ uc, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
   if (c >= (L << (3 * lgS))) return;
   const int m = (1 << lgS) - 1, lgC = lgS - 1;
@@ -129,9 +230,11 @@ __global__ __launch_bounds__(kTpb) void k_mg_prolong(int32_t L, int lgS, float* 
 // Level 0 (colour-split): residual b - (6 u - sum) of the leaf's voxel pairs staged in LDS, then
 // restricted to the leaf's 64 level-1 cells.
 __global__ __launch_bounds__(kTpb) void k_mg0_restrict(DevTopo T, const float* __restrict__ b,
-                                                       const float* __restrict__ u, float* __restrict__ b1) {
+                                                       const float* __restrict__ u, float over,
+                                                       float* __restrict__ b1, const int* __restrict__ stop) {
   __shared__ float s_tile[kPairTile];
   __shared__ float s_r[kLeafVox];
+  if (stop && *stop) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const FaceRow Fr = face_row(T, l);
@@ -157,12 +260,14 @@ __global__ __launch_bounds__(kTpb) void k_mg0_restrict(DevTopo T, const float* _
 #pragma unroll
     for (int k = 0; k < 8; ++k)
       sum += s_r[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
-    b1[l * 64 + t] = sum;
+    b1[l * 64 + t] = over * sum;
   }
 }
 
 // Level 0 (colour-split) u += parent value in u1 (level 1, leaf-dense)
-__global__ __launch_bounds__(kTpb) void k_mg0_prolong(int32_t L, float* __restrict__ u, const float* __restrict__ u1) {
+__global__ __launch_bounds__(kTpb) void k_mg0_prolong(int32_t L, float* __restrict__ u, const float* __restrict__ u1,
+                                                      const int* __restrict__ stop) {
+  if (stop && *stop) return;
   const int32_t l = leaf_of_block();
   if (l >= L) return;
   const int32_t half = L * kHalfLeaf;
@@ -175,18 +280,446 @@ __global__ __launch_bounds__(kTpb) void k_mg0_prolong(int32_t L, float* __restri
   }
 }
 
+// ---- level 0 fused: one workgroup per leaf on a 12^3 LDS tile ------------------------------
+// Tile coordinates 0..11 are the leaf's local coordinates -2..9, index (x * 12 + y) * 12 + z; the
+// colour of a tile position is (x + y + z) & 1 (leaf origins are multiples of 8). A red-black
+// iteration of the core needs, of the ring around it, the face slabs at depth 1 (both colours) and
+// depth 2 and the depth-1 edges (colour 0 of those suffices; both are staged); corners and deeper
+// edges are never read and are not staged. Values of absent leaves are 0 (the Dirichlet boundary).
+constexpr int kT12 = 12;
+__device__ __forceinline__ int t12(int x, int y, int z) { return (x * kT12 + y) * kT12 + z; }
+
+struct TilePos {
+  int32_t nl;      // leaf holding the position (-1: absent)
+  int vx, vy, vz;  // local coordinates in that leaf
+  int tx, ty, tz;  // tile coordinates
+};
+// face slab entry e in [0, 768): face e >> 7 (-x, +x, -y, +y, -z, +z), depth ((e >> 6) & 1) + 1,
+// in-face coordinates ((e >> 3) & 7, e & 7) in x, y, z order
+__device__ __forceinline__ TilePos face_pos(const FaceRow& F, int e) {
+  const int f = e >> 7, d = (e >> 6) & 1, a = (e >> 3) & 7, b = e & 7;
+  TilePos p;
+  switch (f) {
+    case 0: p = {F.xm, 7 - d, a, b, 1 - d, a + 2, b + 2}; break;
+    case 1: p = {F.xp, d, a, b, 10 + d, a + 2, b + 2}; break;
+    case 2: p = {F.ym, a, 7 - d, b, a + 2, 1 - d, b + 2}; break;
+    case 3: p = {F.yp, a, d, b, a + 2, 10 + d, b + 2}; break;
+    case 4: p = {F.zm, a, b, 7 - d, a + 2, b + 2, 1 - d}; break;
+    default: p = {F.zp, a, b, d, a + 2, b + 2, 10 + d}; break;
+  }
+  return p;
+}
+// depth-1 edge entry e in [0, 96): edge g = e >> 3 (0-3 along z, 4-7 along y, 8-11 along x; bits
+// 0 and 1 of g the signs of the two stepped axes), position e & 7 along it
+__device__ __forceinline__ TilePos edge_pos(const DevTopo& T, int32_t l, int e) {
+  const int g = e >> 3, w = e & 7, s0 = (g & 1) ? 1 : -1, s1 = (g & 2) ? 1 : -1;
+  int d[3];
+  if (g < 4) {
+    d[0] = s0; d[1] = s1; d[2] = 0;
+  } else if (g < 8) {
+    d[0] = s0; d[1] = 0; d[2] = s1;
+  } else {
+    d[0] = 0; d[1] = s0; d[2] = s1;
+  }
+  int v[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = d[k] == 0 ? w : (d[k] > 0 ? 0 : 7);
+  TilePos p;
+  p.nl = T.nbr[kNbrStride * l + (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1)];
+  p.vx = v[0]; p.vy = v[1]; p.vz = v[2];
+  p.tx = v[0] + 2 + 8 * d[0]; p.ty = v[1] + 2 + 8 * d[1]; p.tz = v[2] + 2 + 8 * d[2];
+  return p;
+}
+__device__ __forceinline__ float t12_sum(const float* S, int x, int y, int z) {  // +x, -x, +y, -y, +z, -z
+  return S[t12(x + 1, y, z)] + S[t12(x - 1, y, z)] + S[t12(x, y + 1, z)] + S[t12(x, y - 1, z)] +
+         S[t12(x, y, z + 1)] + S[t12(x, y, z - 1)];
+}
+// colour-1 positions updated ahead of the core's colour 0: the core's (one per lane) and those of
+// the depth-1 face slabs (lanes < 192: face t >> 5, 32 positions each)
+__device__ __forceinline__ bool ring_c1(const FaceRow& F, int t, int& tx, int& ty, int& tz) {
+  const int f = t >> 5, a = (t >> 2) & 7, bh = t & 3;
+  const TilePos p = face_pos(F, (f << 7) | (a << 3) | (2 * bh));
+  tx = p.tx; ty = p.ty; tz = p.tz;
+  if (((tx + ty + tz) & 1) == 0) {  // step to the colour-1 neighbour along the slab's second axis
+    if (f < 4) ++tz; else ++ty;
+  }
+  return p.nl >= 0;
+}
+
+// Pre-smoothing and restriction of the V-cycle's level 0 in one pass: from z = 0, one red-black
+// Gauss-Seidel iteration on A z = b (colour 0 then colour 1, A = 6 I - neighbours, the arithmetic of
+// k_rbgs with dx2 = -1, omega = 1), the residual b - A z and its restriction to level 1 (times
+// `over`), z written to zout. The tile holds b (s_b) and z (s_z): z0 is a function of b alone, z1 of
+// the ring's z0, and the core's residual needs z of both colours on the depth-1 slabs, so the same
+// values as the separate sweeps come out bit for bit.
+__global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b,
+                                                  float* __restrict__ zout, float over, float* __restrict__ b1,
+                                                  const int* __restrict__ stop) {
+  __shared__ float s_b[kT12 * kT12 * kT12];
+  __shared__ float s_z[kT12 * kT12 * kT12];
+  __shared__ float s_r[kLeafVox];
+  if (stop && *stop) return;
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const FaceRow Fr = face_row(T, l);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const float dx2 = -1.0f;
+  auto put = [&](int tx, int ty, int tz, float bv) {
+    const int i = t12(tx, ty, tz);
+    s_b[i] = bv;
+    s_z[i] = ((tx + ty + tz) & 1) == 0 ? sor_update(0.f, bv, 0.f, dx2, 1.0f) : 0.f;
+  };
+  const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
+  const int c0 = (x + y) & 1;
+  {
+    const float b0 = b[c0 * half + l * kHalfLeaf + t], b1v = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
+    put(x + 2, y + 2, 2 * zz + 2, b0);
+    put(x + 2, y + 2, 2 * zz + 3, b1v);
+  }
+  for (int e = t; e < 768; e += kTpb) {
+    const TilePos p = face_pos(Fr, e);
+    put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[((p.vx + p.vy + p.vz) & 1) * half + p.nl * kHalfLeaf +
+                                           cs_local(p.vx, p.vy, p.vz)]);
+  }
+  if (t < 96) {
+    const TilePos p = edge_pos(T, l, t);
+    put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[((p.vx + p.vy + p.vz) & 1) * half + p.nl * kHalfLeaf +
+                                           cs_local(p.vx, p.vy, p.vz)]);
+  }
+  __syncthreads();
+  // colour 1 on the core and the depth-1 slabs (positions in absent leaves stay 0)
+  {
+    const int zc1 = 2 * zz + (c0 ^ 1);  // the lane's colour-1 voxel of the core
+    const int i = t12(x + 2, y + 2, zc1 + 2);
+    s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, 1.0f);
+  }
+  if (t < 192) {
+    int tx, ty, tz;
+    if (ring_c1(Fr, t, tx, ty, tz)) {
+      const int i = t12(tx, ty, tz);
+      s_z[i] = sor_update(t12_sum(s_z, tx, ty, tz), s_b[i], 0.f, dx2, 1.0f);
+    }
+  }
+  __syncthreads();
+  // residual of the core's voxel pair, restriction, and z of the core
+  {
+    const int i0 = t12(x + 2, y + 2, 2 * zz + 2), i1 = i0 + 1;
+    const float u0 = s_z[i0], u1 = s_z[i1];
+    const float sp0 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 2), sp1 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 3);
+    s_r[(x << 6) | (y << 3) | (2 * zz)] = s_b[i0] - (6.0f * u0 - sp0);
+    s_r[(x << 6) | (y << 3) | (2 * zz + 1)] = s_b[i1] - (6.0f * u1 - sp1);
+    zout[c0 * half + l * kHalfLeaf + t] = u0;
+    zout[(c0 ^ 1) * half + l * kHalfLeaf + t] = u1;
+  }
+  __syncthreads();
+  if (t < 64) {
+    const int X = t >> 4, Y = (t >> 2) & 3, Z = t & 3;
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      sum += s_r[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
+    b1[l * 64 + t] = over * sum;
+  }
+}
+
+// Prolongation and post-smoothing of the V-cycle's level 0 in one pass: zin + (parent value in u1)
+// staged on the tile, then one red-black iteration on A z = b in the reverse order (colour 1 on the
+// core and the depth-1 slabs, then colour 0 on the core), the core written to zout (distinct from
+// zin: other workgroups read zin's halo). Bitwise the result of k_mg0_prolong and two k_rbgs sweeps.
+__global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
+                                                   const float* __restrict__ zin, const float* __restrict__ u1,
+                                                   float* __restrict__ zout, const int* __restrict__ stop) {
+  __shared__ float s_b[kT12 * kT12 * kT12];
+  __shared__ float s_z[kT12 * kT12 * kT12];
+  if (stop && *stop) return;
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  const FaceRow Fr = face_row(T, l);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const float dx2 = -1.0f;
+  auto zval = [&](int32_t nl, int vx, int vy, int vz) {
+    if (nl < 0) return 0.f;
+    const float z = zin[((vx + vy + vz) & 1) * half + nl * kHalfLeaf + cs_local(vx, vy, vz)];
+    return z + u1[nl * 64 + (((vx >> 1) << 4) | ((vy >> 1) << 2) | (vz >> 1))];
+  };
+  auto bval = [&](int32_t nl, int vx, int vy, int vz) {
+    return nl < 0 ? 0.f : b[((vx + vy + vz) & 1) * half + nl * kHalfLeaf + cs_local(vx, vy, vz)];
+  };
+  const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
+  const int c0 = (x + y) & 1;
+  {
+    const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
+    s_z[i0] = zval(l, x, y, 2 * zz);
+    s_z[i0 + 1] = zval(l, x, y, 2 * zz + 1);
+    s_b[i0] = b[c0 * half + l * kHalfLeaf + t];
+    s_b[i0 + 1] = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
+  }
+  for (int e = t; e < 768; e += kTpb) {
+    const TilePos p = face_pos(Fr, e);
+    const int i = t12(p.tx, p.ty, p.tz);
+    s_z[i] = zval(p.nl, p.vx, p.vy, p.vz);
+    if (((e >> 6) & 1) == 0) s_b[i] = bval(p.nl, p.vx, p.vy, p.vz);  // depth 1
+  }
+  if (t < 96) {
+    const TilePos p = edge_pos(T, l, t);
+    s_z[t12(p.tx, p.ty, p.tz)] = zval(p.nl, p.vx, p.vy, p.vz);
+  }
+  __syncthreads();
+  // colour 1 on the core and the depth-1 slabs
+  {
+    const int zc1 = 2 * zz + (c0 ^ 1);
+    const int i = t12(x + 2, y + 2, zc1 + 2);
+    s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], s_z[i], dx2, 1.0f);
+  }
+  if (t < 192) {
+    int tx, ty, tz;
+    if (ring_c1(Fr, t, tx, ty, tz)) {
+      const int i = t12(tx, ty, tz);
+      s_z[i] = sor_update(t12_sum(s_z, tx, ty, tz), s_b[i], s_z[i], dx2, 1.0f);
+    }
+  }
+  __syncthreads();
+  // colour 0 on the core, then the core out
+  const int zc0 = 2 * zz + c0;
+  const int i = t12(x + 2, y + 2, zc0 + 2);
+  const float v0 = sor_update(t12_sum(s_z, x + 2, y + 2, zc0 + 2), s_b[i], s_z[i], dx2, 1.0f);
+  const float v1 = s_z[t12(x + 2, y + 2, 2 * zz + (c0 ^ 1) + 2)];
+  zout[l * kHalfLeaf + t] = v0;          // colour 0 plane: the lane's colour-0 voxel
+  zout[half + l * kHalfLeaf + t] = v1;   // colour 1 plane
+}
+
+// ---- levels 1 and 2 fused: the same two passes on leaf-dense levels ---------------------------
+// S = 1 << LG cells per leaf axis (4 at level 1, 2 at level 2), one lane per cell, 256 / S^3 leaves
+// per workgroup, each on its own (S + 4)^3 tile (local coordinates -2..S+1). Arithmetic of
+// k_mg_smooth (omega = 1), k_mg_restrict and k_mg_prolong, in the same order.
+template <int LG>
+struct LTile {
+  static constexpr int S = 1 << LG, E = S + 4, N = E * E * E, C = S * S * S, LPW = kTpb / C;
+  __device__ static int at(int x, int y, int z) { return (x * E + y) * E + z; }
+  __device__ static int32_t cell(int32_t leaf, int x, int y, int z) {
+    return (leaf << (3 * LG)) | (((x << LG) | y) << LG) | z;
+  }
+  // face slab entry e in [0, 12 S^2): face e >> (2 LG + 1), depth ((e >> 2 LG) & 1) + 1
+  __device__ static TilePos face(const FaceRow& F, int e) {
+    const int f = e >> (2 * LG + 1), d = (e >> (2 * LG)) & 1, a = (e >> LG) & (S - 1), b = e & (S - 1);
+    TilePos p;
+    switch (f) {
+      case 0: p = {F.xm, S - 1 - d, a, b, 1 - d, a + 2, b + 2}; break;
+      case 1: p = {F.xp, d, a, b, S + 2 + d, a + 2, b + 2}; break;
+      case 2: p = {F.ym, a, S - 1 - d, b, a + 2, 1 - d, b + 2}; break;
+      case 3: p = {F.yp, a, d, b, a + 2, S + 2 + d, b + 2}; break;
+      case 4: p = {F.zm, a, b, S - 1 - d, a + 2, b + 2, 1 - d}; break;
+      default: p = {F.zp, a, b, d, a + 2, b + 2, S + 2 + d}; break;
+    }
+    return p;
+  }
+  // depth-1 edge entry e in [0, 12 S), as edge_pos
+  __device__ static TilePos edge(const DevTopo& T, int32_t l, int e) {
+    const int g = e >> LG, w = e & (S - 1), s0 = (g & 1) ? 1 : -1, s1 = (g & 2) ? 1 : -1;
+    int d[3];
+    if (g < 4) {
+      d[0] = s0; d[1] = s1; d[2] = 0;
+    } else if (g < 8) {
+      d[0] = s0; d[1] = 0; d[2] = s1;
+    } else {
+      d[0] = 0; d[1] = s0; d[2] = s1;
+    }
+    int v[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[k] = d[k] == 0 ? w : (d[k] > 0 ? 0 : S - 1);
+    TilePos p;
+    p.nl = T.nbr[kNbrStride * l + (d[0] + 1) * 9 + (d[1] + 1) * 3 + (d[2] + 1)];
+    p.vx = v[0]; p.vy = v[1]; p.vz = v[2];
+    p.tx = v[0] + 2 + S * d[0]; p.ty = v[1] + 2 + S * d[1]; p.tz = v[2] + 2 + S * d[2];
+    return p;
+  }
+  __device__ static float nsum(const float* A, int x, int y, int z) {  // +x, -x, +y, -y, +z, -z
+    return A[at(x + 1, y, z)] + A[at(x - 1, y, z)] + A[at(x, y + 1, z)] + A[at(x, y - 1, z)] + A[at(x, y, z + 1)] +
+           A[at(x, y, z - 1)];
+  }
+};
+
+// Level LG's colour-1 positions ahead of the core's colour 0: the core's (the lane's cell, if of
+// colour 1) and the depth-1 face slabs' (6 S^2 / 2 of them, entries lane, lane + C, ...).
+template <int LG, class F>
+__device__ __forceinline__ void ltile_c1(const FaceRow& Fr, int lane, int cx, int cy, int cz, F&& update) {
+  using Tl = LTile<LG>;
+  if (((cx + cy + cz) & 1) == 1) update(cx + 2, cy + 2, cz + 2);
+  for (int e = lane; e < 3 * Tl::S * Tl::S; e += Tl::C) {
+    const int f = e / (Tl::S * Tl::S / 2), j = e % (Tl::S * Tl::S / 2);
+    const int a = j / (Tl::S / 2), bh = j % (Tl::S / 2);
+    const TilePos p = Tl::face(Fr, (f << (2 * LG + 1)) | (a << LG) | (2 * bh));
+    if (p.nl < 0) continue;
+    int tx = p.tx, ty = p.ty, tz = p.tz;
+    if (((tx + ty + tz) & 1) == 0) {
+      if (f < 4) ++tz; else ++ty;
+    }
+    update(tx, ty, tz);
+  }
+}
+
+template <int LG>
+__global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __restrict__ b, float* __restrict__ zout,
+                                                  float scale, float over, float* __restrict__ bc,
+                                                  const int* __restrict__ stop) {
+  using Tl = LTile<LG>;
+  __shared__ float s_b[Tl::LPW][Tl::N];
+  __shared__ float s_z[Tl::LPW][Tl::N];
+  __shared__ float s_r[kTpb];
+  if (stop && *stop) return;
+  const int lw = threadIdx.x >> (3 * LG), lane = threadIdx.x & (Tl::C - 1);
+  const int32_t l = (int32_t)blockIdx.x * Tl::LPW + lw;
+  const bool live = l < T.L;
+  const float inv_s = 1.0f / scale;
+  const int cx = lane >> (2 * LG), cy = (lane >> LG) & (Tl::S - 1), cz = lane & (Tl::S - 1);
+  float* B = s_b[lw];
+  float* Z = s_z[lw];
+  FaceRow Fr{};
+  if (live) {
+    Fr = face_row(T, l);
+    auto put = [&](int tx, int ty, int tz, float bv) {
+      const int i = Tl::at(tx, ty, tz);
+      B[i] = bv;
+      Z[i] = ((tx + ty + tz) & 1) == 0 ? 1.0f * ((bv * inv_s + 0.f) * (1.0f / 6.0f)) : 0.f;
+    };
+    put(cx + 2, cy + 2, cz + 2, b[Tl::cell(l, cx, cy, cz)]);
+    for (int e = lane; e < 12 * Tl::S * Tl::S; e += Tl::C) {
+      const TilePos p = Tl::face(Fr, e);
+      put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[Tl::cell(p.nl, p.vx, p.vy, p.vz)]);
+    }
+    for (int e = lane; e < 12 * Tl::S; e += Tl::C) {
+      const TilePos p = Tl::edge(T, l, e);
+      put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[Tl::cell(p.nl, p.vx, p.vy, p.vz)]);
+    }
+  }
+  __syncthreads();
+  if (live)
+    ltile_c1<LG>(Fr, lane, cx, cy, cz, [&](int tx, int ty, int tz) {
+      const int i = Tl::at(tx, ty, tz);
+      const float gs = (B[i] * inv_s + Tl::nsum(Z, tx, ty, tz)) * (1.0f / 6.0f);
+      Z[i] = 1.0f * gs;
+    });
+  __syncthreads();
+  if (live) {
+    const int i = Tl::at(cx + 2, cy + 2, cz + 2);
+    const float u = Z[i];
+    s_r[threadIdx.x] = B[i] - scale * (6.0f * u - Tl::nsum(Z, cx + 2, cy + 2, cz + 2));
+    zout[Tl::cell(l, cx, cy, cz)] = u;
+  }
+  __syncthreads();
+  // one lane per coarse cell: the leaf's S^3 / 8 children sums
+  constexpr int CC = Tl::C / 8;
+  if (!live || lane >= CC) return;
+  const int lgC = LG - 1, mc = (1 << lgC) - 1;
+  const int X = lane >> (2 * lgC), Y = (lane >> lgC) & mc, Z0 = lane & mc;
+  float sum = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z0 + (k & 1);
+    sum += s_r[(lw << (3 * LG)) | (((x << LG) | y) << LG) | z];
+  }
+  bc[((int32_t)l << (3 * lgC)) | lane] = over * sum;
+}
+
+template <int LG>
+__global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __restrict__ b,
+                                                   const float* __restrict__ zin, const float* __restrict__ uc,
+                                                   float* __restrict__ zout, float scale,
+                                                   const int* __restrict__ stop) {
+  using Tl = LTile<LG>;
+  __shared__ float s_b[Tl::LPW][Tl::N];
+  __shared__ float s_z[Tl::LPW][Tl::N];
+  if (stop && *stop) return;
+  const int lw = threadIdx.x >> (3 * LG), lane = threadIdx.x & (Tl::C - 1);
+  const int32_t l = (int32_t)blockIdx.x * Tl::LPW + lw;
+  const bool live = l < T.L;
+  const float inv_s = 1.0f / scale;
+  const int cx = lane >> (2 * LG), cy = (lane >> LG) & (Tl::S - 1), cz = lane & (Tl::S - 1);
+  constexpr int lgC = LG - 1;
+  float* B = s_b[lw];
+  float* Z = s_z[lw];
+  auto zval = [&](int32_t nl, int vx, int vy, int vz) {
+    if (nl < 0) return 0.f;
+    return zin[Tl::cell(nl, vx, vy, vz)] +
+           uc[(nl << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
+  };
+  FaceRow Fr{};
+  if (live) {
+    Fr = face_row(T, l);
+    const int i = Tl::at(cx + 2, cy + 2, cz + 2);
+    Z[i] = zval(l, cx, cy, cz);
+    B[i] = b[Tl::cell(l, cx, cy, cz)];
+    for (int e = lane; e < 12 * Tl::S * Tl::S; e += Tl::C) {
+      const TilePos p = Tl::face(Fr, e);
+      const int j = Tl::at(p.tx, p.ty, p.tz);
+      Z[j] = zval(p.nl, p.vx, p.vy, p.vz);
+      if (((e >> (2 * LG)) & 1) == 0) B[j] = p.nl < 0 ? 0.f : b[Tl::cell(p.nl, p.vx, p.vy, p.vz)];
+    }
+    for (int e = lane; e < 12 * Tl::S; e += Tl::C) {
+      const TilePos p = Tl::edge(T, l, e);
+      Z[Tl::at(p.tx, p.ty, p.tz)] = zval(p.nl, p.vx, p.vy, p.vz);
+    }
+  }
+  __syncthreads();
+  auto sweep = [&](int tx, int ty, int tz) {
+    const int i = Tl::at(tx, ty, tz);
+    const float gs = (B[i] * inv_s + Tl::nsum(Z, tx, ty, tz)) * (1.0f / 6.0f);
+    Z[i] = fmaf(1.0f, gs - Z[i], Z[i]);
+  };
+  if (live) ltile_c1<LG>(Fr, lane, cx, cy, cz, sweep);
+  __syncthreads();
+  if (!live) return;
+  if (((cx + cy + cz) & 1) == 0) sweep(cx + 2, cy + 2, cz + 2);
+  zout[Tl::cell(l, cx, cy, cz)] = Z[Tl::at(cx + 2, cy + 2, cz + 2)];
+}
+
 inline unsigned cells_grid(int32_t L, int lgS) { return (unsigned)((((int64_t)L << (3 * lgS)) + kTpb - 1) / kTpb); }
 
+// one colour sweep on level lgS >= 1 (one lane per z-pair) or on level 3 (lgS = 0, one lane per leaf)
 void smooth(const DevTopo& T, int lgS, const float* b, float* u, float scale, int colour, bool first, float omega,
-            hipStream_t s) {
-  const unsigned g = cells_grid(T.L, lgS);
+            const int* stop, hipStream_t s) {
   const float inv = 1.0f / scale;
-  if (colour == 0) {
-    if (first) k_mg_smooth<0, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
-    else k_mg_smooth<0, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
+  if (lgS == 0) {
+    const unsigned g = cells_grid(T.L, 0);
+    if (colour == 0) {
+      if (first) k_mg_leaf_sweep<0, true><<<g, kTpb, 0, s>>>(T, b, u, inv, omega, stop);
+      else k_mg_leaf_sweep<0, false><<<g, kTpb, 0, s>>>(T, b, u, inv, omega, stop);
+    } else {
+      if (first) k_mg_leaf_sweep<1, true><<<g, kTpb, 0, s>>>(T, b, u, inv, omega, stop);
+      else k_mg_leaf_sweep<1, false><<<g, kTpb, 0, s>>>(T, b, u, inv, omega, stop);
+    }
   } else {
-    if (first) k_mg_smooth<1, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
-    else k_mg_smooth<1, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega);
+    const unsigned g = (unsigned)(((((int64_t)T.L << (3 * lgS)) >> 1) + kTpb - 1) / kTpb);
+    if (colour == 0) {
+      if (first) k_mg_smooth<0, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega, stop);
+      else k_mg_smooth<0, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega, stop);
+    } else {
+      if (first) k_mg_smooth<1, true><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega, stop);
+      else k_mg_smooth<1, false><<<g, kTpb, 0, s>>>(T, lgS, b, u, inv, omega, stop);
+    }
+  }
+  HNS_HIP(hipGetLastError());
+}
+
+// `iters` red-black SOR iterations on level 3: one workgroup on the colour-sorted slots when
+// prepare_coarse found a layout (ch > 0), else one launch per colour sweep
+void coarse_solve(const DevTopo& T, int ch, const int32_t* perm, const uint32_t* nbp, const float* b, float* u,
+                  float scale, int iters, float omega, const int* stop, bool launches, hipStream_t s) {
+  const float inv = 1.0f / scale;
+  if (ch == 0 || launches) {
+    for (int k = 0; k < iters; ++k) {
+      smooth(T, 0, b, u, scale, 0, k == 0, omega, stop, s);
+      smooth(T, 0, b, u, scale, 1, k == 0, omega, stop, s);
+    }
+    return;
+  }
+  switch (ch) {
+    case 1: k_mg_coarse<1><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
+    case 2: k_mg_coarse<2><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
+    case 4: k_mg_coarse<4><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
+    default: k_mg_coarse<8><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
   }
   HNS_HIP(hipGetLastError());
 }
@@ -200,63 +733,144 @@ void Multigrid::ensure(int32_t leaves) {
     exec_ = nullptr;
   }
   L = leaves;
+  zt.alloc((size_t)leaves * kLeafVox);
   for (int l = 1; l < kMgLevels; ++l) {
     const size_t n = (size_t)leaves << (3 * (3 - l));
     b[l].alloc(n);
     u[l].alloc(n);
+    if (l < kMgLevels - 1) ut[l].alloc(n);
   }
 }
 
-void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s) {
-  const MgShape& M = mg_shape();
-  // level 0: z = 0, red-black pre-smoothing (Gauss-Seidel on A z = r)
-  for (int k = 0; k < M.smooth0; ++k) {
-    launch_rbgs_rhs(T, r_cs, z_cs, 0, k == 0, s);
-    launch_rbgs_rhs(T, r_cs, z_cs, 1, k == 0, s);
+void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
+  if (ckey_nbr_ == T.nbr && ckey_L_ == T.L) return;
+  if (exec_) {  // a captured V-cycle may refer to the layout about to be replaced
+    HNS_HIP(hipGraphExecDestroy(exec_));
+    exec_ = nullptr;
   }
-  k_mg0_restrict<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, z_cs, b[1].p);
-  HNS_HIP(hipGetLastError());
-  // descend: levels 1, 2 pre-smooth and restrict; level 3 (one cell per leaf) is smoothed M.coarse times
-  for (int l = 1; l < kMgLevels; ++l) {
+  coarse_ch_ = 0;
+  ckey_nbr_ = T.nbr;
+  ckey_L_ = T.L;
+  const int32_t L = T.L;
+  if (L <= 0 || L > 2 * kCoarseTpb * kCoarseMaxCH) return;
+  std::vector<int4> org((size_t)L);
+  std::vector<int32_t> nbr((size_t)L * kNbrStride);
+  HNS_HIP(hipStreamSynchronize(s));
+  HNS_HIP(hipMemcpy(org.data(), T.origins, org.size() * sizeof(int4), hipMemcpyDeviceToHost));
+  HNS_HIP(hipMemcpy(nbr.data(), T.nbr, nbr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  std::vector<int32_t> slot((size_t)L);
+  int32_t n[2] = {0, 0};
+  for (int32_t l = 0; l < L; ++l) {
+    const int c = ((org[l].x + org[l].y + org[l].z) >> 3) & 1;
+    slot[l] = c * 0x40000000 + n[c]++;  // colour tag, fixed up below
+  }
+  int ch = 1;
+  while (ch < kCoarseMaxCH && ch * kCoarseTpb < std::max(n[0], n[1])) ch *= 2;
+  if (ch * kCoarseTpb < std::max(n[0], n[1])) return;  // colours too unbalanced for one workgroup
+  const int32_t H = ch * kCoarseTpb;
+  for (int32_t l = 0; l < L; ++l) slot[l] = slot[l] >= 0x40000000 ? H + (slot[l] - 0x40000000) : slot[l];
+  std::vector<int32_t> perm((size_t)2 * H, -1);
+  std::vector<uint32_t> nbp((size_t)6 * H, (uint32_t)(2 * H) | ((uint32_t)(2 * H) << 16));
+  static const int kFace[6] = {22, 4, 16, 10, 14, 12};  // +x, -x, +y, -y, +z, -z
+  for (int32_t l = 0; l < L; ++l) {
+    const int32_t j = slot[l];
+    perm[j] = l;
+    uint32_t f[6];
+    for (int q = 0; q < 6; ++q) {
+      const int32_t nl = nbr[(size_t)l * kNbrStride + kFace[q]];
+      f[q] = nl < 0 ? (uint32_t)(2 * H) : (uint32_t)slot[nl];
+    }
+    for (int a = 0; a < 3; ++a) nbp[(size_t)2 * a * H + j] = f[2 * a] | (f[2 * a + 1] << 16);
+  }
+  cperm_.alloc(perm.size());
+  cnb_.alloc(nbp.size());
+  HNS_HIP(hipMemcpy(cperm_.p, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  HNS_HIP(hipMemcpy(cnb_.p, nbp.data(), nbp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  coarse_ch_ = ch;
+}
+
+void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
+                           hipStream_t s) {
+  const MgShape& M = mg_shape();
+  const bool fused = M.smooth0 == 1 && !(variant & kMgUnfused);
+  if (fused) {
+    // level 0 pre-smoothing and restriction in one pass, into the scratch level-0 vector
+    k_mg0_pre<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, stop);
+    HNS_HIP(hipGetLastError());
+  } else {
+    // level 0: z = 0, red-black pre-smoothing (Gauss-Seidel on A z = r)
+    for (int k = 0; k < M.smooth0; ++k) {
+      launch_rbgs_rhs(T, r_cs, z_cs, 0, k == 0, stop, s);
+      launch_rbgs_rhs(T, r_cs, z_cs, 1, k == 0, stop, s);
+    }
+    k_mg0_restrict<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, z_cs, M.over, b[1].p, stop);
+    HNS_HIP(hipGetLastError());
+  }
+  // descend: levels 1, 2 pre-smooth and restrict
+  const bool fused_l = M.smooth == 1 && !(variant & kMgUnfused);
+  for (int l = 1; l < kMgLevels - 1; ++l) {
     const int lgS = 3 - l;
     const float scale = (float)(1 << (2 * l));
-    const int iters = l == kMgLevels - 1 ? M.coarse : M.smooth;
-    const float omega = l == kMgLevels - 1 ? M.coarse_omega : 1.0f;
-    for (int k = 0; k < iters; ++k) {
-      smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, omega, s);
-      smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, omega, s);
-    }
-    if (l + 1 < kMgLevels) {
-      k_mg_restrict<<<cells_grid(T.L, lgS - 1), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, b[l + 1].p);
+    if (fused_l) {
+      const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
+      if (lgS == 2) k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over, b[l + 1].p, stop);
+      else k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over, b[l + 1].p, stop);
       HNS_HIP(hipGetLastError());
+      continue;
     }
+    for (int k = 0; k < M.smooth; ++k) {
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, 1.0f, stop, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, 1.0f, stop, s);
+    }
+    k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, M.over, b[l + 1].p,
+                                                        stop);
+    HNS_HIP(hipGetLastError());
   }
+  // level 3 (one cell per leaf): M.coarse over-relaxed red-black iterations
+  coarse_solve(T, coarse_ch_, cperm_.p, cnb_.p, b[kMgLevels - 1].p, u[kMgLevels - 1].p,
+               (float)(1 << (2 * (kMgLevels - 1))), M.coarse, M.coarse_omega, stop, (variant & kMgCoarseLaunches) != 0,
+               s);
   // ascend: prolong and post-smooth in the reverse colour order
   for (int l = kMgLevels - 2; l >= 1; --l) {
     const int lgS = 3 - l;
     const float scale = (float)(1 << (2 * l));
-    k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p);
+    if (fused_l) {
+      const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
+      if (lgS == 2) k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, stop);
+      else k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, stop);
+      HNS_HIP(hipGetLastError());
+      continue;
+    }
+    k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p, stop);
     HNS_HIP(hipGetLastError());
     for (int k = 0; k < M.smooth; ++k) {
-      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, 1.0f, s);
-      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, 1.0f, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, 1.0f, stop, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, 1.0f, stop, s);
     }
   }
-  k_mg0_prolong<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, z_cs, u[1].p);
+  if (fused) {
+    k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, stop);
+    HNS_HIP(hipGetLastError());
+    return;
+  }
+  k_mg0_prolong<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, z_cs, u[1].p, stop);
   HNS_HIP(hipGetLastError());
   for (int k = 0; k < M.smooth0; ++k) {
-    launch_rbgs_rhs(T, r_cs, z_cs, 1, false, s);
-    launch_rbgs_rhs(T, r_cs, z_cs, 0, false, s);
+    launch_rbgs_rhs(T, r_cs, z_cs, 1, false, stop, s);
+    launch_rbgs_rhs(T, r_cs, z_cs, 0, false, stop, s);
   }
 }
 
-void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s, bool graph) {
+void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStream_t s, bool graph, const int* stop,
+                      unsigned variant) {
   ensure(T.L);
+  prepare_coarse(T, s);
   if (!graph) {
-    launch_all(T, r_cs, z_cs, s);
+    launch_all(T, r_cs, z_cs, stop, variant, s);
     return;
   }
-  if (!exec_ || key_nbr_ != T.nbr || key_L_ != T.L || key_r_ != r_cs || key_z_ != z_cs) {
+  if (!exec_ || key_nbr_ != T.nbr || key_L_ != T.L || key_r_ != r_cs || key_z_ != z_cs || key_stop_ != stop ||
+      key_variant_ != variant) {
     if (exec_) {
       HNS_HIP(hipGraphExecDestroy(exec_));
       exec_ = nullptr;
@@ -265,7 +879,7 @@ void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStrea
     hipGraph_t g = nullptr;
     HNS_HIP(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
     try {
-      launch_all(T, r_cs, z_cs, cap_);
+      launch_all(T, r_cs, z_cs, stop, variant, cap_);
     } catch (...) {
       (void)hipStreamEndCapture(cap_, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -279,6 +893,8 @@ void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStrea
     key_L_ = T.L;
     key_r_ = r_cs;
     key_z_ = z_cs;
+    key_stop_ = stop;
+    key_variant_ = variant;
   }
   HNS_HIP(hipGraphLaunch(exec_, s));
 }

@@ -58,21 +58,6 @@ constexpr int kTpb = 256;
 __device__ __forceinline__ floatx4 mfma(const bf16x8& a, const bf16x8& b, const floatx4& c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
-// Empty device asm that makes a value opaque to the optimiser, so what is derived from it is
-// recomputed where it is used instead of being hoisted into registers across a loop (v: a
-// per-lane value, s: a wave-uniform one). The host pass of a template kernel sees nothing.
-template <class T>
-__device__ __forceinline__ void opaque_v(T& v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  asm volatile("" : "+v"(v));
-#endif
-}
-template <class T>
-__device__ __forceinline__ void opaque_s(T& v) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  asm volatile("" : "+s"(v));
-#endif
-}
 __device__ __forceinline__ bf16x8 zero8() {
   bf16x8 z;
 #pragma unroll

@@ -19,13 +19,6 @@ namespace {
 
 constexpr int kNbXm = 4, kNbXp = 22, kNbYm = 10, kNbYp = 16, kNbZm = 12, kNbZp = 14;  // (dx+1)*9+(dy+1)*3+(dz+1)
 
-// The SOR update of redBlackGaussSeidelUpdate (Kernel.cu:487-488), shared by both kernels so they
-// produce identical bits: pGS = (sum6 - div dx^2) / 6 with the reference's 0.166666667f.
-__device__ __forceinline__ float sor_update(float s, float dv, float pOld, float dx2, float omega) {
-  const float pGS = (s - dv * dx2) * 0.166666667f;
-  return pOld + omega * (pGS - pOld);
-}
-
 // LPW leaves per workgroup (256 lanes each); the workgroup group index is XCD-mapped. The other
 // colour's plane of the leaf and the 6 half-faces of it that the sweep reads across the leaf
 // faces (32 voxels each: x faces indexed (y, z>>1), y faces (x, z>>1), z faces (x, y>>1)) are
@@ -33,8 +26,9 @@ __device__ __forceinline__ float sor_update(float s, float dv, float pOld, float
 constexpr int kRbTile = kHalfLeaf + 6 * 32;
 template <int COLOUR, bool FIRST, int LPW>
 __global__ __launch_bounds__(256 * LPW) void k_rbgs(DevTopo T, const float* __restrict__ div, float* __restrict__ p,
-                                                    float dx2, float omega) {
+                                                    float dx2, float omega, const int* __restrict__ stop) {
   __shared__ float s_o[LPW][kRbTile];
+  if (stop && *stop) return;  // a multigrid V-cycle of a solve that has stopped
   const int32_t lw = LPW > 1 ? (int32_t)(threadIdx.x >> 8) : 0;
   const int32_t l = leaf_of_block() * LPW + lw;
   const bool live = l < T.L;  // a partially filled workgroup still reaches the barrier
@@ -252,15 +246,15 @@ __global__ __launch_bounds__(1024) void k_reduce_pairs(const double* __restrict_
 namespace {
 template <int LPW>
 void launch_rbgs_lpw(const DevTopo& T, const float* div_cs, float* p_cs, float dx2, int colour, float omega, bool first,
-                     hipStream_t s) {
+                     hipStream_t s, const int* stop = nullptr) {
   const unsigned g = grid_for_leaves((T.L + LPW - 1) / LPW);
   const unsigned b = 256 * LPW;
   if (colour == 0) {
-    if (first) k_rbgs<0, true, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
-    else k_rbgs<0, false, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+    if (first) k_rbgs<0, true, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega, stop);
+    else k_rbgs<0, false, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega, stop);
   } else {
-    if (first) k_rbgs<1, true, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
-    else k_rbgs<1, false, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega);
+    if (first) k_rbgs<1, true, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega, stop);
+    else k_rbgs<1, false, LPW><<<g, b, 0, s>>>(T, div_cs, p_cs, dx2, omega, stop);
   }
 }
 int rbgs_lpw() {
@@ -275,8 +269,10 @@ int rbgs_lpw() {
 
 // The sweep on A u = b with A = 6 I - (face neighbours): div = b and dx^2 = -1 make the SOR target
 // (sum - div dx^2) / 6 exactly (sum + b) / 6; omega = 1 (Gauss-Seidel).
-void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, hipStream_t s) {
-  launch_rbgs_lpw<1>(T, b_cs, u_cs, -1.0f, colour, 1.0f, first, s);
+void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, const int* stop,
+                     hipStream_t s) {
+  launch_rbgs_lpw<1>(T, b_cs, u_cs, -1.0f, colour, 1.0f, first, s, stop);
+  HNS_HIP(hipGetLastError());
 }
 
 void launch_rbgs(const DevTopo& T, const float* div_cs, float* p_cs, float dx, int colour, float omega, bool first,

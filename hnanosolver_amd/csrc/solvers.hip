@@ -206,10 +206,13 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         replaced_at = it;
       }
       if (pre) {
-        // flexible CG on z = M(r)
+        // flexible CG on z = M(r), after the stop test (a stopped solve's V-cycle returns at once)
+        tm.begin(HNS_K_CG_UPDATE, s);
+        launch_fcg_test(T.L, it, S, s);
+        tm.end(s);
         tm.begin(HNS_K_PRECOND, s);
         if (learned) r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
-        else mg_->apply(T, G.r.p, G.z.p, s);
+        else mg_->apply(T, G.r.p, G.z.p, s, true, S.flag);
         tm.end(s);
         tm.begin(HNS_K_CG_PRODUCT, s);
         launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
@@ -241,6 +244,18 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       tm.end(s);
       replaced_at = it;
       batch = batch0;
+      if (pre) {
+        // re-test the stop on the replaced residual alone before paying for another iteration
+        tm.begin(HNS_K_CG_UPDATE, s);
+        launch_fcg_test(T.L, it, S, s);
+        tm.end(s);
+        HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+        HNS_HIP(hipStreamSynchronize(s));
+        if (G.h_flag[0]) {
+          stopped = true;
+          break;
+        }
+      }
       continue;
     }
     batch = std::min(batch * 2, batch_max);

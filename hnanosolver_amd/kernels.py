@@ -111,10 +111,13 @@ def soa_to_aos(x, out, asynchronous=False):
     _check(lib().hnsk_soa_to_aos(n, _ptr(x), _ptr(out), _stream(), int(asynchronous)))
 
 
-def mg_apply(topo, r_cs, z_cs, graph=True, asynchronous=False):
+def mg_apply(topo, r_cs, z_cs, graph=True, coarse_launches=False, unfused=False, asynchronous=False):
     """z = V(r): one multigrid V-cycle of the HNS_SOLVER_MGCG preconditioner (colour-split device
-    vectors); graph replays the captured V-cycle, otherwise its kernels are launched one by one."""
-    _check(lib().hnsk_mg_apply(topo.handle, _ptr(r_cs), _ptr(z_cs), int(bool(graph)), _stream()))
+    vectors); graph replays the captured V-cycle, otherwise its kernels are launched one by one;
+    coarse_launches runs the coarsest level as one launch per colour sweep, unfused level 0 as
+    separate sweeps, restriction and prolongation (same bits)."""
+    flags = (1 if graph else 0) | (2 if coarse_launches else 0) | (4 if unfused else 0)
+    _check(lib().hnsk_mg_apply(topo.handle, _ptr(r_cs), _ptr(z_cs), flags, _stream()))
     if not asynchronous:
         import torch
         torch.cuda.synchronize()

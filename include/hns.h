@@ -301,11 +301,16 @@ hns_status hnsk_precond_adam(int64_t n, float* params, const float* grad, float*
 hns_status hnsk_poisson_apply(const hns_topology* topo, const float* x_cs, const float* y_cs, float a, float b,
                               float* out_cs, double* work, void* stream);
 /* z = V(r): one geometric multigrid V-cycle of the HNS_SOLVER_MGCG preconditioner on colour-split
- * vectors (approximately A^-1 r, A as above). graph != 0 replays the V-cycle captured in a HIP
- * graph, graph == 0 launches its kernels one by one; both compute the same. No reference
- * counterpart (the reference runs a fixed RBGS count, HNanoSolver.cu:364-373). Calls sharing a
- * topology must not overlap. */
-hns_status hnsk_mg_apply(const hns_topology* topo, const float* r_cs, float* z_cs, int32_t graph, void* stream);
+ * vectors (approximately A^-1 r, A as above). flags: HNS_MG_GRAPH replays the V-cycle captured in
+ * a HIP graph (else its kernels are launched one by one); HNS_MG_COARSE_LAUNCHES runs the coarsest
+ * level as one launch per colour sweep instead of one workgroup; HNS_MG_UNFUSED runs level 0 as
+ * separate sweep, restriction and prolongation launches instead of two fused tile passes. All
+ * combinations compute the same bits. No reference counterpart (the reference runs a fixed RBGS count, HNanoSolver.cu:364-373).
+ * Calls sharing a topology must not overlap. */
+#define HNS_MG_GRAPH 1
+#define HNS_MG_COARSE_LAUNCHES 2
+#define HNS_MG_UNFUSED 4
+hns_status hnsk_mg_apply(const hns_topology* topo, const float* r_cs, float* z_cs, int32_t flags, void* stream);
 /* ---- reverse mode of the differentiable step (no reference counterpart; the reference is
  *      forward only). Step: u1 = advect_vector(u) (Kernel.cu:349-426), div = divergence(u1),
  *      p = K RBGS iterations from 0 (HNanoSolver.cu:364-373), u2 = u1 - grad p, rho' =

@@ -94,3 +94,31 @@ def test_vcycle_graph_replays_match_eager_launches(hip):
             print("  trial %d replay %d: differing elements %d of %d" % (trial, rep, n, sc.n))
             assert n == 0
             K.mg_apply(topo, r, torch.empty_like(r), graph=False)  # eager work between replays
+
+
+@pytest.mark.parametrize("shape,keep", [((10, 8, 9), 0.7), ((16, 16, 12), 0.7), ((24, 24, 24), 0.72),
+                                        ((32, 32, 16), 1.0), ((32, 32, 17), 0.95)])
+def test_vcycle_variants_match(hip, shape, keep):
+    """Equivalent V-cycle kernel sequences give bitwise the same z: the coarsest level solved inside
+    one workgroup (LDS-resident, up to 8192 leaves per colour: 1, 4 and 8 slots per lane here, and
+    exactly 16384 leaves) or by per-sweep launches (above 16384 leaves both take this path), level 0
+    as two fused 12^3-tile passes or as separate sweeps, restriction and prolongation, eager or
+    replayed from a graph."""
+    torch = hip
+    rng = np.random.default_rng(sum(shape))
+    nx, ny, nz = shape
+    org = [(8 * x, 8 * y, 8 * z) for x in range(nx) for y in range(ny) for z in range(nz) if rng.random() < keep]
+    coords = canonical_coords(np.array(org))
+    _case("v-cycle variants, %d leaves" % (len(coords) // 512))
+    topo = _topo(coords, 0.1)
+    r = torch.from_numpy(rng.standard_normal(len(coords)).astype(np.float32)).cuda()
+    ref = torch.empty_like(r)
+    K.mg_apply(topo, r, ref, graph=False, coarse_launches=True, unfused=True)
+    assert bool(torch.isfinite(ref).all())
+    for graph, coarse, unfused in [(False, False, False), (False, True, False), (False, False, True), (True, False, False)]:
+        z = torch.empty_like(r)
+        K.mg_apply(topo, r, z, graph=graph, coarse_launches=coarse, unfused=unfused)
+        n = int((z != ref).sum())
+        print("  graph %d coarse-launches %d unfused %d: differing elements %d of %d" % (
+            graph, coarse, unfused, n, len(coords)))
+        assert n == 0
