@@ -297,16 +297,17 @@ struct TilePos {
 // face slab entry e in [0, 768): face e >> 7 (-x, +x, -y, +y, -z, +z), depth ((e >> 6) & 1) + 1,
 // in-face coordinates ((e >> 3) & 7, e & 7) in x, y, z order
 __device__ __forceinline__ TilePos face_pos(const FaceRow& F, int e) {
-  const int f = e >> 7, d = (e >> 6) & 1, a = (e >> 3) & 7, b = e & 7;
+  // selects, not a switch: the staging loops stay one basic block, so all their loads overlap
+  const int f = e >> 7, ax = f >> 1, pos = f & 1, d = (e >> 6) & 1, a = (e >> 3) & 7, b = e & 7;
+  const int vd = pos ? d : 7 - d, td = pos ? 10 + d : 1 - d;
   TilePos p;
-  switch (f) {
-    case 0: p = {F.xm, 7 - d, a, b, 1 - d, a + 2, b + 2}; break;
-    case 1: p = {F.xp, d, a, b, 10 + d, a + 2, b + 2}; break;
-    case 2: p = {F.ym, a, 7 - d, b, a + 2, 1 - d, b + 2}; break;
-    case 3: p = {F.yp, a, d, b, a + 2, 10 + d, b + 2}; break;
-    case 4: p = {F.zm, a, b, 7 - d, a + 2, b + 2, 1 - d}; break;
-    default: p = {F.zp, a, b, d, a + 2, b + 2, 10 + d}; break;
-  }
+  p.nl = f == 0 ? F.xm : f == 1 ? F.xp : f == 2 ? F.ym : f == 3 ? F.yp : f == 4 ? F.zm : F.zp;
+  p.vx = ax == 0 ? vd : a;
+  p.vy = ax == 0 ? a : (ax == 1 ? vd : b);
+  p.vz = ax == 2 ? vd : b;
+  p.tx = ax == 0 ? td : a + 2;
+  p.ty = ax == 0 ? a + 2 : (ax == 1 ? td : b + 2);
+  p.tz = ax == 2 ? td : b + 2;
   return p;
 }
 // depth-1 edge entry e in [0, 96): edge g = e >> 3 (0-3 along z, 4-7 along y, 8-11 along x; bits
@@ -372,21 +373,25 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
   };
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
-  {
-    const float b0 = b[c0 * half + l * kHalfLeaf + t], b1v = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
-    put(x + 2, y + 2, 2 * zz + 2, b0);
-    put(x + 2, y + 2, 2 * zz + 3, b1v);
+  // every load issued before the first LDS store (absent leaves load the own leaf, then read 0)
+  auto at = [&](const TilePos& p) {
+    return ((p.vx + p.vy + p.vz) & 1) * half + (p.nl < 0 ? l : p.nl) * kHalfLeaf + cs_local(p.vx, p.vy, p.vz);
+  };
+  const TilePos ep = edge_pos(T, l, t < 96 ? t : 0);
+  TilePos fp[3];
+  float fv[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    fp[k] = face_pos(Fr, t + k * kTpb);
+    fv[k] = b[at(fp[k])];
   }
-  for (int e = t; e < 768; e += kTpb) {
-    const TilePos p = face_pos(Fr, e);
-    put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[((p.vx + p.vy + p.vz) & 1) * half + p.nl * kHalfLeaf +
-                                           cs_local(p.vx, p.vy, p.vz)]);
-  }
-  if (t < 96) {
-    const TilePos p = edge_pos(T, l, t);
-    put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[((p.vx + p.vy + p.vz) & 1) * half + p.nl * kHalfLeaf +
-                                           cs_local(p.vx, p.vy, p.vz)]);
-  }
+  const float b0 = b[c0 * half + l * kHalfLeaf + t], b1v = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
+  const float ev = b[at(ep)];
+  put(x + 2, y + 2, 2 * zz + 2, b0);
+  put(x + 2, y + 2, 2 * zz + 3, b1v);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) put(fp[k].tx, fp[k].ty, fp[k].tz, fp[k].nl < 0 ? 0.f : fv[k]);
+  if (t < 96) put(ep.tx, ep.ty, ep.tz, ep.nl < 0 ? 0.f : ev);
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs (positions in absent leaves stay 0)
   {
@@ -439,33 +444,44 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const float dx2 = -1.0f;
-  auto zval = [&](int32_t nl, int vx, int vy, int vz) {
-    if (nl < 0) return 0.f;
-    const float z = zin[((vx + vy + vz) & 1) * half + nl * kHalfLeaf + cs_local(vx, vy, vz)];
-    return z + u1[nl * 64 + (((vx >> 1) << 4) | ((vy >> 1) << 2) | (vz >> 1))];
+  // every load issued before the first LDS store (absent leaves load the own leaf, then read 0);
+  // a staged z is zin plus its parent's value in u1
+  auto at = [&](const TilePos& p) {
+    return ((p.vx + p.vy + p.vz) & 1) * half + (p.nl < 0 ? l : p.nl) * kHalfLeaf + cs_local(p.vx, p.vy, p.vz);
   };
-  auto bval = [&](int32_t nl, int vx, int vy, int vz) {
-    return nl < 0 ? 0.f : b[((vx + vy + vz) & 1) * half + nl * kHalfLeaf + cs_local(vx, vy, vz)];
+  auto parent = [&](const TilePos& p) {
+    return (p.nl < 0 ? l : p.nl) * 64 + (((p.vx >> 1) << 4) | ((p.vy >> 1) << 2) | (p.vz >> 1));
   };
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
+  const TilePos ep = edge_pos(T, l, t < 96 ? t : 0);
+  TilePos fp[3];
+  float fz[3], fu[3], fb[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    fp[k] = face_pos(Fr, t + k * kTpb);
+    fz[k] = zin[at(fp[k])];
+    fu[k] = u1[parent(fp[k])];
+    fb[k] = b[at(fp[k])];  // used at depth 1
+  }
+  const float z0 = zin[c0 * half + l * kHalfLeaf + t], z1 = zin[(c0 ^ 1) * half + l * kHalfLeaf + t];
+  const float up = u1[l * 64 + ((x >> 1) << 4) + ((y >> 1) << 2) + zz];  // the pair's parent (z >> 1 = zz)
+  const float b0 = b[c0 * half + l * kHalfLeaf + t], b1v = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
+  const float ez = zin[at(ep)], eu = u1[parent(ep)];
   {
     const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
-    s_z[i0] = zval(l, x, y, 2 * zz);
-    s_z[i0 + 1] = zval(l, x, y, 2 * zz + 1);
-    s_b[i0] = b[c0 * half + l * kHalfLeaf + t];
-    s_b[i0 + 1] = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
+    s_z[i0] = z0 + up;
+    s_z[i0 + 1] = z1 + up;
+    s_b[i0] = b0;
+    s_b[i0 + 1] = b1v;
   }
-  for (int e = t; e < 768; e += kTpb) {
-    const TilePos p = face_pos(Fr, e);
-    const int i = t12(p.tx, p.ty, p.tz);
-    s_z[i] = zval(p.nl, p.vx, p.vy, p.vz);
-    if (((e >> 6) & 1) == 0) s_b[i] = bval(p.nl, p.vx, p.vy, p.vz);  // depth 1
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int i = t12(fp[k].tx, fp[k].ty, fp[k].tz);
+    s_z[i] = fp[k].nl < 0 ? 0.f : fz[k] + fu[k];
+    s_b[i] = fp[k].nl < 0 ? 0.f : fb[k];  // read at depth 1 only
   }
-  if (t < 96) {
-    const TilePos p = edge_pos(T, l, t);
-    s_z[t12(p.tx, p.ty, p.tz)] = zval(p.nl, p.vx, p.vy, p.vz);
-  }
+  if (t < 96) s_z[t12(ep.tx, ep.ty, ep.tz)] = ep.nl < 0 ? 0.f : ez + eu;
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs
   {
@@ -497,22 +513,24 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
 template <int LG>
 struct LTile {
   static constexpr int S = 1 << LG, E = S + 4, N = E * E * E, C = S * S * S, LPW = kTpb / C;
+  static constexpr int NF = 12 * S * S / C, NE = (12 * S + C - 1) / C;  // face / edge entries per lane
   __device__ static int at(int x, int y, int z) { return (x * E + y) * E + z; }
   __device__ static int32_t cell(int32_t leaf, int x, int y, int z) {
     return (leaf << (3 * LG)) | (((x << LG) | y) << LG) | z;
   }
   // face slab entry e in [0, 12 S^2): face e >> (2 LG + 1), depth ((e >> 2 LG) & 1) + 1
   __device__ static TilePos face(const FaceRow& F, int e) {
-    const int f = e >> (2 * LG + 1), d = (e >> (2 * LG)) & 1, a = (e >> LG) & (S - 1), b = e & (S - 1);
+    const int f = e >> (2 * LG + 1), ax = f >> 1, pos = f & 1, d = (e >> (2 * LG)) & 1;
+    const int a = (e >> LG) & (S - 1), b = e & (S - 1);
+    const int vd = pos ? d : S - 1 - d, td = pos ? S + 2 + d : 1 - d;
     TilePos p;
-    switch (f) {
-      case 0: p = {F.xm, S - 1 - d, a, b, 1 - d, a + 2, b + 2}; break;
-      case 1: p = {F.xp, d, a, b, S + 2 + d, a + 2, b + 2}; break;
-      case 2: p = {F.ym, a, S - 1 - d, b, a + 2, 1 - d, b + 2}; break;
-      case 3: p = {F.yp, a, d, b, a + 2, S + 2 + d, b + 2}; break;
-      case 4: p = {F.zm, a, b, S - 1 - d, a + 2, b + 2, 1 - d}; break;
-      default: p = {F.zp, a, b, d, a + 2, b + 2, S + 2 + d}; break;
-    }
+    p.nl = f == 0 ? F.xm : f == 1 ? F.xp : f == 2 ? F.ym : f == 3 ? F.yp : f == 4 ? F.zm : F.zp;
+    p.vx = ax == 0 ? vd : a;
+    p.vy = ax == 0 ? a : (ax == 1 ? vd : b);
+    p.vz = ax == 2 ? vd : b;
+    p.tx = ax == 0 ? td : a + 2;
+    p.ty = ax == 0 ? a + 2 : (ax == 1 ? td : b + 2);
+    p.tz = ax == 2 ? td : b + 2;
     return p;
   }
   // depth-1 edge entry e in [0, 12 S), as edge_pos
@@ -584,15 +602,25 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
       B[i] = bv;
       Z[i] = ((tx + ty + tz) & 1) == 0 ? 1.0f * ((bv * inv_s + 0.f) * (1.0f / 6.0f)) : 0.f;
     };
-    put(cx + 2, cy + 2, cz + 2, b[Tl::cell(l, cx, cy, cz)]);
-    for (int e = lane; e < 12 * Tl::S * Tl::S; e += Tl::C) {
-      const TilePos p = Tl::face(Fr, e);
-      put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[Tl::cell(p.nl, p.vx, p.vy, p.vz)]);
+    // every load issued before the first LDS store (absent leaves load the own leaf, then read 0)
+    TilePos fp[Tl::NF], ep[Tl::NE];
+    float fv[Tl::NF], ev[Tl::NE];
+#pragma unroll
+    for (int k = 0; k < Tl::NE; ++k) ep[k] = Tl::edge(T, l, std::min(lane + k * Tl::C, 12 * Tl::S - 1));
+#pragma unroll
+    for (int k = 0; k < Tl::NF; ++k) {
+      fp[k] = Tl::face(Fr, lane + k * Tl::C);
+      fv[k] = b[Tl::cell(fp[k].nl < 0 ? l : fp[k].nl, fp[k].vx, fp[k].vy, fp[k].vz)];
     }
-    for (int e = lane; e < 12 * Tl::S; e += Tl::C) {
-      const TilePos p = Tl::edge(T, l, e);
-      put(p.tx, p.ty, p.tz, p.nl < 0 ? 0.f : b[Tl::cell(p.nl, p.vx, p.vy, p.vz)]);
-    }
+    const float bv = b[Tl::cell(l, cx, cy, cz)];
+#pragma unroll
+    for (int k = 0; k < Tl::NE; ++k) ev[k] = b[Tl::cell(ep[k].nl < 0 ? l : ep[k].nl, ep[k].vx, ep[k].vy, ep[k].vz)];
+    put(cx + 2, cy + 2, cz + 2, bv);
+#pragma unroll
+    for (int k = 0; k < Tl::NF; ++k) put(fp[k].tx, fp[k].ty, fp[k].tz, fp[k].nl < 0 ? 0.f : fv[k]);
+#pragma unroll
+    for (int k = 0; k < Tl::NE; ++k)
+      if (lane + k * Tl::C < 12 * Tl::S) put(ep[k].tx, ep[k].ty, ep[k].tz, ep[k].nl < 0 ? 0.f : ev[k]);
   }
   __syncthreads();
   if (live)
@@ -640,27 +668,41 @@ __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __res
   constexpr int lgC = LG - 1;
   float* B = s_b[lw];
   float* Z = s_z[lw];
-  auto zval = [&](int32_t nl, int vx, int vy, int vz) {
-    if (nl < 0) return 0.f;
-    return zin[Tl::cell(nl, vx, vy, vz)] +
-           uc[(nl << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
+  auto zval = [&](int32_t nl, int vx, int vy, int vz) {  // absent leaves load the own leaf, then read 0
+    const int32_t src = nl < 0 ? l : nl;
+    const float v = zin[Tl::cell(src, vx, vy, vz)] +
+                    uc[(src << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
+    return nl < 0 ? 0.f : v;
   };
   FaceRow Fr{};
   if (live) {
     Fr = face_row(T, l);
+    // every load issued before the first LDS store
+    TilePos fp[Tl::NF], ep[Tl::NE];
+    float fz[Tl::NF], fb[Tl::NF], ez[Tl::NE];
+#pragma unroll
+    for (int k = 0; k < Tl::NE; ++k) ep[k] = Tl::edge(T, l, std::min(lane + k * Tl::C, 12 * Tl::S - 1));
+#pragma unroll
+    for (int k = 0; k < Tl::NF; ++k) {
+      fp[k] = Tl::face(Fr, lane + k * Tl::C);
+      fz[k] = zval(fp[k].nl, fp[k].vx, fp[k].vy, fp[k].vz);
+      fb[k] = b[Tl::cell(fp[k].nl < 0 ? l : fp[k].nl, fp[k].vx, fp[k].vy, fp[k].vz)];
+    }
+    const float zc = zval(l, cx, cy, cz), bc0 = b[Tl::cell(l, cx, cy, cz)];
+#pragma unroll
+    for (int k = 0; k < Tl::NE; ++k) ez[k] = zval(ep[k].nl, ep[k].vx, ep[k].vy, ep[k].vz);
     const int i = Tl::at(cx + 2, cy + 2, cz + 2);
-    Z[i] = zval(l, cx, cy, cz);
-    B[i] = b[Tl::cell(l, cx, cy, cz)];
-    for (int e = lane; e < 12 * Tl::S * Tl::S; e += Tl::C) {
-      const TilePos p = Tl::face(Fr, e);
-      const int j = Tl::at(p.tx, p.ty, p.tz);
-      Z[j] = zval(p.nl, p.vx, p.vy, p.vz);
-      if (((e >> (2 * LG)) & 1) == 0) B[j] = p.nl < 0 ? 0.f : b[Tl::cell(p.nl, p.vx, p.vy, p.vz)];
+    Z[i] = zc;
+    B[i] = bc0;
+#pragma unroll
+    for (int k = 0; k < Tl::NF; ++k) {
+      const int j = Tl::at(fp[k].tx, fp[k].ty, fp[k].tz);
+      Z[j] = fz[k];
+      B[j] = fp[k].nl < 0 ? 0.f : fb[k];  // read at depth 1 only
     }
-    for (int e = lane; e < 12 * Tl::S; e += Tl::C) {
-      const TilePos p = Tl::edge(T, l, e);
-      Z[Tl::at(p.tx, p.ty, p.tz)] = zval(p.nl, p.vx, p.vy, p.vz);
-    }
+#pragma unroll
+    for (int k = 0; k < Tl::NE; ++k)
+      if (lane + k * Tl::C < 12 * Tl::S) Z[Tl::at(ep[k].tx, ep[k].ty, ep[k].tz)] = ez[k];
   }
   __syncthreads();
   auto sweep = [&](int tx, int ty, int tz) {
