@@ -19,7 +19,8 @@ struct CgScalars {
   double* pb;     // one partial per workgroup (d . q), written by the product kernel
   double* rho;    // max_iters + 1 entries
   double* alpha;  // max_iters + 1 entries
-  int* flag;      // 2 entries
+  int* flag;      // 2 entries: stop raised, stop iteration
+  int* it;        // flexible CG: the device's iteration counter (0 after init, advanced by the test)
   double tol2;    // stop when r.r <= tol2 * ||b||^2
   double* bb;     // ||b||^2
 };
@@ -47,15 +48,21 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 // test, z = M(r), d = z - sum_i (z.q_i / sigma_i) d_i, q = A d, alpha = (d.r)/(d.q),
 // x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
 constexpr int kFcgDepth = 8;
-// stop test of iteration `it` on the r.r partials (one workgroup), launched before z = M(r):
-// once it raises S.flag, the preconditioner's kernels and the rest of the iteration return at once
-void launch_fcg_test(int32_t L, int32_t it, const CgScalars& S, hipStream_t s);
+// iterations between stop-flag reads of a flexible-CG solve (and per replayed multigrid graph)
+constexpr int kFcgBatch = 4;
+// The flexible-CG kernels take the iteration number from *S.it, so one iteration's launches are the
+// same for every iteration (they can be replayed from a graph). Once S.flag is raised, the
+// preconditioner's kernels and the rest of the iteration return at once.
+// Stop test on the r.r partials (one workgroup): advance = true after an update (the next
+// iteration's test; it moves *S.it on), false before iteration 0 (it records ||b||^2) and after a
+// residual replacement (the same iteration tested again).
+void launch_fcg_test(int32_t L, bool advance, const CgScalars& S, hipStream_t s);
 // new direction and its product (three launches), orthogonalised against the last
-// min(it, depth) directions (1 <= depth <= kFcgDepth)
+// min(it, depth) directions (1 <= depth <= kFcgDepth); `it` (the host's count) selects the kernels
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,
                           double* zq, const CgScalars& S, hipStream_t s);
 // x += alpha d, r -= alpha q
-void launch_fcg_update(int32_t L, int32_t it, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
+void launch_fcg_update(int32_t L, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
                        hipStream_t s);
 
 }  // namespace hns

@@ -36,6 +36,16 @@ struct Multigrid {
   DBuf<float> zt;                          // level-0 scratch (colour-split): pre-smoothed z of the fused path
   DBuf<float> ut[kMgLevels];               // levels 1..2: pre-smoothed u of the fused path
   void ensure(int32_t leaves);
+  // buffers and coarse layout for topology T (synchronises s when the layout is built)
+  void prepare(const DevTopo& T, hipStream_t s) {
+    ensure(T.L);
+    prepare_coarse(T, s);
+  }
+  // the V-cycle's kernels launched on s (for capture into a caller's graph; prepare first)
+  void enqueue(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, hipStream_t s) {
+    launch_all(T, r_cs, z_cs, stop, 0u, s);
+  }
+  uint64_t generation = 0;  // advanced whenever buffers or the coarse layout change
   // z = V(r) on colour-split level-0 vectors: approximately A^-1 r for A = 6 I - (face neighbours).
   // graph: replay the captured V-cycle (captured on first use for these pointers); otherwise launch
   // the kernels one by one. Both run the same kernels on the same data. stop: a device flag (null:

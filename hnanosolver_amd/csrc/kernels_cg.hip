@@ -103,6 +103,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __rest
   const double t = block_sum_all(acc, s_red);
   if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = t;
   clear_flags(S);
+  if (blockIdx.x == 0 && threadIdx.x == 0) *S.it = 0;
 }
 
 // Residual replacement: r = b - A x (evaluated in FP64, stored FP32); partials of r.r into pa
@@ -278,17 +279,20 @@ __device__ inline void block_store_rows(double (&v)[N], double* s_red, double* r
   }
 }
 
-// Stop test of flexible-CG iteration `it`, before its preconditioner runs: one workgroup reduces
-// the G partials of r.r (G = the CG grid) and raises the stop flag, on which every later kernel of
-// the iteration, the preconditioner's included, returns at once. Iteration 0 records ||b||^2.
-__global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int32_t it, CgScalars S) {
+// Stop test of flexible-CG iteration it, before its preconditioner runs: one workgroup reduces the
+// G partials of r.r (G = the CG grid) and raises the stop flag, on which every later kernel of the
+// iteration, the preconditioner's included, returns at once. advance: first move the device
+// iteration counter on (the test that follows an update); iteration 0 records ||b||^2.
+__global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int advance, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
   if (S.flag[0]) return;
+  const int32_t it = *S.it + advance;
   double rr1[1];
-  reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rr1);
+  reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rr1);  // every lane has read *S.it
   const double rr = rr1[0];
   const double bb = it == 0 ? rr : S.bb[0];
   if (threadIdx.x == 0) {
+    if (advance) *S.it = it;
     if (it == 0) S.bb[0] = bb;
     if (rr <= S.tol2 * bb) {
       S.flag[1] = it;
@@ -299,11 +303,12 @@ __global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int32_t it, CgScalars 
 
 // NP = min(it, kFcgDepth) ring entries in use (a template argument, so every load is unconditional)
 template <int NP>
-__global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
+__global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int64_t n, const float* __restrict__ z,
                                                    const float* __restrict__ Qr, double* __restrict__ zq,
                                                    CgScalars S) {
   __shared__ double s_red[kFcgDepth * (kTpb / 64)];
   if (NP == 0 || S.flag[0]) return;
+  const int32_t it = *S.it;
   const float* q[kFcgDepth];
 #pragma unroll
   for (int k = 0; k < kFcgDepth; ++k) q[k] = Qr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
@@ -325,11 +330,12 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int32_t it, int64_
 }
 
 template <int NP>
-__global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int32_t it, int64_t n, const float* __restrict__ z,
+__global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int64_t n, const float* __restrict__ z,
                                                   const float* __restrict__ r, float* __restrict__ Dr,
                                                   const double* __restrict__ zq, CgScalars S) {
   __shared__ double s_red[kFcgDepth * (kTpb / 64)];
   if (S.flag[0]) return;
+  const int32_t it = *S.it;
   const int G = (int)gridDim.x;
   double zqk[NP > 0 ? NP : 1];
   if (NP > 0) reduce_partial_rows(zq, G, s_red, zqk);
@@ -359,11 +365,12 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int32_t it, int64_t
   if (threadIdx.x == 0) S.pa[kSlotDR * kCgMaxGroups + blockIdx.x] = tt;
 }
 
-__global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int32_t it, int64_t n, const float* __restrict__ Dr,
+__global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int64_t n, const float* __restrict__ Dr,
                                                    float* __restrict__ Qr, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
   __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
+  const int32_t it = *S.it;
   const float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
   float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
   const LeafRun R = leaf_run(T.L);
@@ -393,11 +400,12 @@ __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int32_t it, int64_
   if (threadIdx.x == 0) S.pa[kSlotDQ * kCgMaxGroups + blockIdx.x] = tt;
 }
 
-__global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int32_t it, int64_t n, float* __restrict__ x,
+__global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int64_t n, float* __restrict__ x,
                                                      float* __restrict__ r, const float* __restrict__ Dr,
                                                      const float* __restrict__ Qr, CgScalars S) {
   __shared__ double s_red[2 * (kTpb / 64)];
   if (S.flag[0]) return;
+  const int32_t it = *S.it;
   double rows[2];  // d . r, d . q
   reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rows);
   const double sigma = rows[1];
@@ -454,8 +462,8 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
   k_cg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, x, r, d, q, S);
   HNS_HIP(hipGetLastError());
 }
-void launch_fcg_test(int32_t L, int32_t it, const CgScalars& S, hipStream_t s) {
-  k_fcg_test<<<1, kTpb, 0, s>>>((int)cg_groups(L), it, S);
+void launch_fcg_test(int32_t L, bool advance, const CgScalars& S, hipStream_t s) {
+  k_fcg_test<<<1, kTpb, 0, s>>>((int)cg_groups(L), advance ? 1 : 0, S);
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,
@@ -467,9 +475,9 @@ void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* 
   switch (std::min<int32_t>(it, depth)) {
 #define HNS_FCG_CASE(NP)                                                   \
   case NP:                                                                 \
-    k_fcg_dots<NP><<<g, kTpb, 0, s>>>(T.L, it, n, z, Qr, zq, S);           \
+    k_fcg_dots<NP><<<g, kTpb, 0, s>>>(T.L, n, z, Qr, zq, S);               \
     HNS_HIP(hipGetLastError());                                            \
-    k_fcg_dir<NP><<<g, kTpb, 0, s>>>(T.L, it, n, z, r, Dr, zq, S);         \
+    k_fcg_dir<NP><<<g, kTpb, 0, s>>>(T.L, n, z, r, Dr, zq, S);             \
     HNS_HIP(hipGetLastError());                                            \
     break;
     HNS_FCG_CASE(0)
@@ -483,12 +491,12 @@ void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* 
     HNS_FCG_CASE(8)
 #undef HNS_FCG_CASE
   }
-  k_fcg_spmv<<<g, kTpb, 0, s>>>(T, it, n, Dr, Qr, S);
+  k_fcg_spmv<<<g, kTpb, 0, s>>>(T, n, Dr, Qr, S);
   HNS_HIP(hipGetLastError());
 }
-void launch_fcg_update(int32_t L, int32_t it, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
+void launch_fcg_update(int32_t L, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
                        hipStream_t s) {
-  k_fcg_update<<<cg_groups(L), kTpb, 0, s>>>(L, it, (int64_t)L * kLeafVox, x, r, Dr, Qr, S);
+  k_fcg_update<<<cg_groups(L), kTpb, 0, s>>>(L, (int64_t)L * kLeafVox, x, r, Dr, Qr, S);
   HNS_HIP(hipGetLastError());
 }
 

@@ -775,6 +775,7 @@ void Multigrid::ensure(int32_t leaves) {
     exec_ = nullptr;
   }
   L = leaves;
+  ++generation;
   zt.alloc((size_t)leaves * kLeafVox);
   for (int l = 1; l < kMgLevels; ++l) {
     const size_t n = (size_t)leaves << (3 * (3 - l));
@@ -791,6 +792,7 @@ void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
     exec_ = nullptr;
   }
   coarse_ch_ = 0;
+  ++generation;
   ckey_nbr_ = T.nbr;
   ckey_L_ = T.L;
   const int32_t L = T.L;

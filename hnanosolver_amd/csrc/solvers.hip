@@ -17,13 +17,63 @@ struct CgState {
   DBuf<float> r, d0, d1, q, z;
   DBuf<float> dring, qring;  // flexible CG: kFcgDepth directions and products
   DBuf<double> rho, alpha, pa, pb, bb, zq;
-  DBuf<int> flag;
+  DBuf<int> flag;  // stop raised, stop iteration, device iteration counter
   int* h_flag = nullptr;
   int cap = 0;
+  // kFcgBatch multigrid-preconditioned iterations captured in one graph, and what it was made for
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t batch_exec = nullptr;
+  const int32_t* key_nbr = nullptr;
+  int32_t key_L = 0;
+  const float* key_p = nullptr;
+  uint64_t key_gen = 0;
+  int key_depth = 0;
   ~CgState() {
+    if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (h_flag) (void)hipHostFree(h_flag);
   }
 };
+
+// kFcgBatch flexible-CG iterations on the multigrid V-cycle (iteration >= depth, so every
+// iteration's launches are the same), replayed from one graph made on first use for these buffers
+void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth, hipStream_t s) {
+  CgState& G = *cg_;
+  Multigrid& M = *mg_;
+  M.prepare(T, s);
+  if (!G.batch_exec || G.key_nbr != T.nbr || G.key_L != T.L || G.key_p != p || G.key_gen != M.generation ||
+      G.key_depth != depth) {
+    if (G.batch_exec) {
+      HNS_HIP(hipGraphExecDestroy(G.batch_exec));
+      G.batch_exec = nullptr;
+    }
+    if (!G.cap_stream) HNS_HIP(hipStreamCreateWithFlags(&G.cap_stream, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    HNS_HIP(hipStreamBeginCapture(G.cap_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < kFcgBatch; ++k) {
+        M.enqueue(T, G.r.p, G.z.p, S.flag, G.cap_stream);
+        launch_fcg_direction(T, depth, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, G.cap_stream);
+        launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, G.cap_stream);
+        launch_fcg_test(T.L, true, S, G.cap_stream);
+      }
+    } catch (...) {
+      (void)hipStreamEndCapture(G.cap_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HNS_HIP(hipStreamEndCapture(G.cap_stream, &g));
+    const hipError_t e = hipGraphInstantiate(&G.batch_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HNS_HIP(e);
+    G.key_nbr = T.nbr;
+    G.key_L = T.L;
+    G.key_p = p;
+    G.key_gen = M.generation;
+    G.key_depth = depth;
+  }
+  HNS_HIP(hipGraphLaunch(G.batch_exec, s));
+}
 
 Solvers::Solvers() = default;
 
@@ -164,7 +214,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.pa.alloc((size_t)kCgMaxGroups * kCgSlots);
     G.pb.alloc(kCgMaxGroups);
     G.bb.alloc(1);
-    G.flag.alloc(2);
+    G.flag.alloc(3);
     HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 2 * sizeof(int), hipHostMallocDefault));
   }
   if (G.cap < cap) {
@@ -178,6 +228,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.rho = G.rho.p;
   S.alpha = G.alpha.p;
   S.flag = G.flag.p;
+  S.it = G.flag.p + 2;
   S.bb = G.bb.p;
   S.tol2 = (double)r.tol * (double)r.tol;
   tm.begin(HNS_K_CG_UPDATE, s);
@@ -192,24 +243,38 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.qring.alloc((size_t)kFcgDepth * N_);
     G.zq.alloc((size_t)kFcgDepth * kCgMaxGroups);
   }
-  // a preconditioned iteration costs far more than a sync: check the stop flag every 4
-  const int batch0 = pre ? 4 : 8, batch_max = pre ? 4 : 128;
+  // a preconditioned iteration costs far more than a sync: check the stop flag every kFcgBatch
+  // iterations, in batches aligned to multiples of kFcgBatch (the multigrid solve replays whole
+  // aligned batches from one graph)
+  const int batch0 = pre ? kFcgBatch : 8, batch_max = pre ? kFcgBatch : 128;
   int it = 0, batch = batch0, replaced_at = 0;
   bool stopped = false;
   while (it < cap) {
-    const int nb = std::min(batch, cap - it);
-    for (int k = 0; k < nb; ++k, ++it) {
-      if (it > 0 && it % kResidualReplace == 0) {
+    int nb = std::min(batch, cap - it);
+    if (pre) nb = std::min(nb, kFcgBatch - it % kFcgBatch);
+    for (int k = 0; k < nb;) {
+      if (it > 0 && it % kResidualReplace == 0 && replaced_at != it) {
         tm.begin(HNS_K_CG_UPDATE, s);
         launch_cg_replace(T, div, r.dx, p, G.r.p, S, false, s);
+        if (pre) launch_fcg_test(T.L, false, S, s);  // test the residual just replaced
         tm.end(s);
         replaced_at = it;
       }
-      if (pre) {
-        // flexible CG on z = M(r), after the stop test (a stopped solve's V-cycle returns at once)
-        tm.begin(HNS_K_CG_UPDATE, s);
-        launch_fcg_test(T.L, it, S, s);
+      if (pre && mg && it >= depth && it % kFcgBatch == 0 && nb - k >= kFcgBatch) {
+        tm.begin(HNS_K_PRECOND, s);
+        mg_batch(T, p, S, depth, s);
         tm.end(s);
+        it += kFcgBatch;
+        k += kFcgBatch;
+        continue;
+      }
+      if (pre) {
+        // flexible CG on z = M(r); the stop test of iteration it > 0 ran after the last update
+        if (it == 0) {
+          tm.begin(HNS_K_CG_UPDATE, s);
+          launch_fcg_test(T.L, false, S, s);
+          tm.end(s);
+        }
         tm.begin(HNS_K_PRECOND, s);
         if (learned) r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
         else mg_->apply(T, G.r.p, G.z.p, s, true, S.flag);
@@ -218,18 +283,21 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
         tm.end(s);
         tm.begin(HNS_K_CG_UPDATE, s);
-        launch_fcg_update(T.L, it, p, G.r.p, G.dring.p, G.qring.p, S, s);
+        launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, s);
+        launch_fcg_test(T.L, true, S, s);
         tm.end(s);
-        continue;
+      } else {
+        float* d_in = (it & 1) ? G.d1.p : G.d0.p;
+        float* d_out = (it & 1) ? G.d0.p : G.d1.p;
+        tm.begin(HNS_K_CG_PRODUCT, s);
+        launch_cg_product(T, it, it == 0, G.r.p, d_in, d_out, G.q.p, S, s);
+        tm.end(s);
+        tm.begin(HNS_K_CG_UPDATE, s);
+        launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
+        tm.end(s);
       }
-      float* d_in = (it & 1) ? G.d1.p : G.d0.p;
-      float* d_out = (it & 1) ? G.d0.p : G.d1.p;
-      tm.begin(HNS_K_CG_PRODUCT, s);
-      launch_cg_product(T, it, it == 0, G.r.p, d_in, d_out, G.q.p, S, s);
-      tm.end(s);
-      tm.begin(HNS_K_CG_UPDATE, s);
-      launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
-      tm.end(s);
+      ++it;
+      ++k;
     }
     HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HNS_HIP(hipStreamSynchronize(s));
@@ -247,7 +315,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       if (pre) {
         // re-test the stop on the replaced residual alone before paying for another iteration
         tm.begin(HNS_K_CG_UPDATE, s);
-        launch_fcg_test(T.L, it, S, s);
+        launch_fcg_test(T.L, false, S, s);
         tm.end(s);
         HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
         HNS_HIP(hipStreamSynchronize(s));
