@@ -12,6 +12,70 @@ inline unsigned cg_groups(int32_t L) {
   return g < (unsigned)kCgMaxGroups ? g : (unsigned)kCgMaxGroups;
 }
 
+// ---- device helpers shared by the CG kernels and the multigrid pass that feeds flexible CG -------
+// (256-lane workgroups; partial rows are kCgMaxGroups doubles, one entry per workgroup)
+constexpr int kCgTpb = 256;
+// Sum of a 256-lane value over the workgroup; the total is returned to every lane.
+__device__ inline double block_sum_all(double v, double* s_red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kCgTpb / 64; ++i) t += s_red[i];
+  __syncthreads();
+  return t;
+}
+
+// Sums of N rows of partials (row k at p + k * kCgMaxGroups, G entries each), one pass and one
+// barrier pair for all rows, each in the same fixed order as reduce_partials; all lanes get them.
+template <int N>
+__device__ inline void reduce_partial_rows(const double* __restrict__ p, int G, double* s_red, double (&out)[N]) {
+  double v[N];
+#pragma unroll
+  for (int k = 0; k < N; ++k) v[k] = 0.0;
+  for (int i = threadIdx.x; i < G; i += kCgTpb)
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] += p[k * kCgMaxGroups + i];
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) s_red[k * (kCgTpb / 64) + (threadIdx.x >> 6)] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < N; ++k) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kCgTpb / 64; ++i) t += s_red[k * (kCgTpb / 64) + i];
+    out[k] = t;
+  }
+  __syncthreads();
+}
+
+
+template <int N>
+__device__ inline void block_store_rows(double (&v)[N], double* s_red, double* rows) {
+#pragma unroll
+  for (int k = 0; k < N; ++k)
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < N; ++k) s_red[k * (kCgTpb / 64) + (threadIdx.x >> 6)] = v[k];
+  __syncthreads();
+  if (threadIdx.x < N) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < kCgTpb / 64; ++i) t += s_red[threadIdx.x * (kCgTpb / 64) + i];
+    rows[threadIdx.x * kCgMaxGroups + blockIdx.x] = t;
+  }
+}
+
+
 // Device scalars of one solve. rho[k] = r_k . r_k (CG) or sigma_k = d_k . A d_k (flexible CG), alpha[k] the step
 // length of iteration k; flag[0] = stop, flag[1] = iterations done when stopped.
 struct CgScalars {
@@ -46,8 +110,14 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 // q_i = A d_i of the last kFcgDepth iterations stay in the rings Dr, Qr (kFcgDepth vectors of
 // L * 512 each, slot i mod kFcgDepth), sigma_i = d_i . q_i in S.rho[i]. One iteration `it`: stop
 // test, z = M(r), d = z - sum_i (z.q_i / sigma_i) d_i, q = A d, alpha = (d.r)/(d.q),
+// Truncation 1 on the multigrid V-cycle (whose last pass leaves z . q_prev in zq row 0 and z . r in
+// pa row kSlotDR): d = z + beta d_prev and q = A d in one launch (first: iteration 0, d = z), partials
+// of d . q; the update then takes alpha = (z . r) / (d . q).
+void launch_fcg1_dirspmv(const DevTopo& T, bool first, const float* z, float* Dr, float* Qr, const double* zq,
+                         const CgScalars& S, hipStream_t s);
 // x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
 constexpr int kFcgDepth = 8;
+__device__ __forceinline__ int fcg_slot(int32_t i) { return (int)(i % kFcgDepth); }
 // iterations between stop-flag reads of a flexible-CG solve (and per replayed multigrid graph)
 constexpr int kFcgBatch = 4;
 // The flexible-CG kernels take the iteration number from *S.it, so one iteration's launches are the

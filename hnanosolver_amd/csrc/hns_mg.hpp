@@ -30,6 +30,17 @@ const MgShape& mg_shape();
 constexpr unsigned kMgCoarseLaunches = 1;  // coarsest level as one launch per colour sweep
 constexpr unsigned kMgUnfused = 2;         // levels 0-2 as separate sweep, restriction and prolongation launches
 
+// What the last level-0 pass leaves for flexible CG with truncation 1 (kernels_cg.hip): per
+// workgroup (cg_groups(L) of them) partials of z . r into zr and of z . q_prev into zq, q_prev the
+// product of the previous iteration in the ring Qr (slot (*it - 1) mod kFcgDepth, n floats each).
+struct McgHook {
+  double* zr;
+  double* zq;
+  const float* Qr;
+  int64_t n;
+  const int* it;
+};
+
 struct Multigrid {
   int32_t L = 0;
   DBuf<float> b[kMgLevels], u[kMgLevels];  // levels 1..3 (leaf-dense); level 0 uses the caller's vectors
@@ -42,8 +53,10 @@ struct Multigrid {
     prepare_coarse(T, s);
   }
   // the V-cycle's kernels launched on s (for capture into a caller's graph; prepare first)
-  void enqueue(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, hipStream_t s) {
-    launch_all(T, r_cs, z_cs, stop, 0u, s);
+  // hook: also leave the flexible-CG partials (null: none)
+  void enqueue(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, hipStream_t s,
+               const McgHook* hook = nullptr) {
+    launch_all(T, r_cs, z_cs, stop, 0u, s, hook);
   }
   uint64_t generation = 0;  // advanced whenever buffers or the coarse layout change
   // z = V(r) on colour-split level-0 vectors: approximately A^-1 r for A = 6 I - (face neighbours).
@@ -57,7 +70,7 @@ struct Multigrid {
 
  private:
   void launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
-                  hipStream_t s);
+                  hipStream_t s, const McgHook* hook = nullptr);
   // colour-sorted slot layout of the coarsest level for its one-workgroup solve, built on the host
   // once per topology (synchronises s); coarse_ch_ = 0 when the leaves do not fit one workgroup
   void prepare_coarse(const DevTopo& T, hipStream_t s);

@@ -29,47 +29,6 @@ namespace {
 
 constexpr int kTpb = 256;
 
-// Sum of a 256-lane value over the workgroup; the total is returned to every lane.
-__device__ inline double block_sum_all(double v, double* s_red) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = v;
-  __syncthreads();
-  double t = 0.0;
-#pragma unroll
-  for (int i = 0; i < kTpb / 64; ++i) t += s_red[i];
-  __syncthreads();
-  return t;
-}
-
-// Sums of N rows of partials (row k at p + k * kCgMaxGroups, G entries each), one pass and one
-// barrier pair for all rows, each in the same fixed order as reduce_partials; all lanes get them.
-template <int N>
-__device__ inline void reduce_partial_rows(const double* __restrict__ p, int G, double* s_red, double (&out)[N]) {
-  double v[N];
-#pragma unroll
-  for (int k = 0; k < N; ++k) v[k] = 0.0;
-  for (int i = threadIdx.x; i < G; i += kTpb)
-#pragma unroll
-    for (int k = 0; k < N; ++k) v[k] += p[k * kCgMaxGroups + i];
-#pragma unroll
-  for (int k = 0; k < N; ++k)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < N; ++k) s_red[k * (kTpb / 64) + (threadIdx.x >> 6)] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < kTpb / 64; ++i) t += s_red[k * (kTpb / 64) + i];
-    out[k] = t;
-  }
-  __syncthreads();
-}
-
 // Sum of the G partials p[0..G), fixed order, returned to all lanes.
 __device__ inline double reduce_partials(const double* __restrict__ p, int G, double* s_red) {
   double v = 0.0;
@@ -259,26 +218,6 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
 //   update: sigma_k = d_k . q_k, alpha_k = (d_k . r) / sigma_k; x += alpha_k d_k, r -= alpha_k q_k;
 //           partials of r . r
 // The d_k . r and d_k . q_k partials go to the pa rows kSlotDR and kSlotDQ.
-__device__ __forceinline__ int fcg_slot(int32_t i) { return (int)(i % kFcgDepth); }
-
-template <int N>
-__device__ inline void block_store_rows(double (&v)[N], double* s_red, double* rows) {
-#pragma unroll
-  for (int k = 0; k < N; ++k)
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
-  if ((threadIdx.x & 63) == 0)
-#pragma unroll
-    for (int k = 0; k < N; ++k) s_red[k * (kTpb / 64) + (threadIdx.x >> 6)] = v[k];
-  __syncthreads();
-  if (threadIdx.x < N) {
-    double t = 0.0;
-#pragma unroll
-    for (int i = 0; i < kTpb / 64; ++i) t += s_red[threadIdx.x * (kTpb / 64) + i];
-    rows[threadIdx.x * kCgMaxGroups + blockIdx.x] = t;
-  }
-}
-
 // Stop test of flexible-CG iteration it, before its preconditioner runs: one workgroup reduces the
 // G partials of r.r (G = the CG grid) and raises the stop flag, on which every later kernel of the
 // iteration, the preconditioner's included, returns at once. advance: first move the device
@@ -400,6 +339,61 @@ __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int64_t n, const f
   if (threadIdx.x == 0) S.pa[kSlotDQ * kCgMaxGroups + blockIdx.x] = tt;
 }
 
+// Flexible CG with truncation 1 on the multigrid V-cycle, direction and product in one pass:
+// beta = -(z . q_prev) / sigma_prev from the partials the V-cycle's last pass left in zq row 0
+// (NP = 1; NP = 0 at iteration 0: d = z), d = z + beta d_prev formed for the leaf's voxel pairs and
+// its face halos (the same fmaf as k_fcg_dir), q = A d, d and q into their ring slots, partials of
+// d . q. The numerator of alpha, z . r, is also left by the V-cycle (pa row kSlotDR).
+template <int NP>
+__global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, const float* __restrict__ z,
+                                                       float* __restrict__ Dr, float* __restrict__ Qr,
+                                                       const double* __restrict__ zq, CgScalars S) {
+  __shared__ double s_red[kTpb / 64];
+  __shared__ float s_tile[2][kPairTile];
+  if (S.flag[0]) return;
+  const int32_t it = *S.it;
+  float beta = 0.f;
+  const float* __restrict__ dp = z;
+  if (NP > 0) {
+    double zq1[1];
+    reduce_partial_rows(zq, (int)gridDim.x, s_red, zq1);
+    beta = (float)(-zq1[0] / S.rho[it - 1]);
+    dp = Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n;
+  }
+  float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
+  float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const LeafRun R = leaf_run(T.L);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  auto get = [&](int c, int32_t i) {
+    const int32_t j = c * half + i;
+    return NP > 0 ? fmaf(beta, dp[j], z[j]) : z[j];
+  };
+  double acc = 0.0;
+  int buf = 0;
+  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
+    float* St = s_tile[buf];
+    const FaceRow Fr = face_row(T, l);
+    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
+    const float d0 = get(c0, l * kHalfLeaf + t), d1 = get(c0 ^ 1, l * kHalfLeaf + t);
+    pair_tile_halo(Fr, get, St);
+    St[c0 * kHalfLeaf + t] = d0;
+    St[(c0 ^ 1) * kHalfLeaf + t] = d1;
+    __syncthreads();
+    float s0, s1;
+    pair_tile_sums(St, t, d0, d1, s0, s1);
+    const float q0 = 6.0f * d0 - s0, q1 = 6.0f * d1 - s1;
+    d[i0] = d0;
+    d[i1] = d1;
+    q[i0] = q0;
+    q[i1] = q1;
+    acc += (double)d0 * (double)q0 + (double)d1 * (double)q1;
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[kSlotDQ * kCgMaxGroups + blockIdx.x] = tt;
+}
+
 __global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int64_t n, float* __restrict__ x,
                                                      float* __restrict__ r, const float* __restrict__ Dr,
                                                      const float* __restrict__ Qr, CgScalars S) {
@@ -492,6 +486,14 @@ void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* 
 #undef HNS_FCG_CASE
   }
   k_fcg_spmv<<<g, kTpb, 0, s>>>(T, n, Dr, Qr, S);
+  HNS_HIP(hipGetLastError());
+}
+void launch_fcg1_dirspmv(const DevTopo& T, bool first, const float* z, float* Dr, float* Qr, const double* zq,
+                         const CgScalars& S, hipStream_t s) {
+  const unsigned g = cg_groups(T.L);
+  const int64_t n = (int64_t)T.L * kLeafVox;
+  if (first) k_fcg1_dirspmv<0><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
+  else k_fcg1_dirspmv<1><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg_update(int32_t L, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,

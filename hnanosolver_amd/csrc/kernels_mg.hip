@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <vector>
 
+#include "hns_cg.hpp"
 #include "hns_mg.hpp"
 
 namespace hns {
@@ -428,18 +429,15 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
   }
 }
 
-// Prolongation and post-smoothing of the V-cycle's level 0 in one pass: zin + (parent value in u1)
+// Prolongation and post-smoothing of the V-cycle's level 0 for leaf l: zin + (parent value in u1)
 // staged on the tile, then one red-black iteration on A z = b in the reverse order (colour 1 on the
 // core and the depth-1 slabs, then colour 0 on the core), the core written to zout (distinct from
 // zin: other workgroups read zin's halo). Bitwise the result of k_mg0_prolong and two k_rbgs sweeps.
-__global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
-                                                   const float* __restrict__ zin, const float* __restrict__ u1,
-                                                   float* __restrict__ zout, const int* __restrict__ stop) {
-  __shared__ float s_b[kT12 * kT12 * kT12];
-  __shared__ float s_z[kT12 * kT12 * kT12];
-  if (stop && *stop) return;
-  const int32_t l = leaf_of_block();
-  if (l >= T.L) return;
+// Returns the lane's final colour-0 / colour-1 values (v0, v1) and their right-hand sides (r0, r1).
+__device__ __forceinline__ void mg0_post_leaf(const DevTopo& T, int32_t l, const float* __restrict__ b,
+                                              const float* __restrict__ zin, const float* __restrict__ u1,
+                                              float* __restrict__ zout, float* s_b, float* s_z, float& v0,
+                                              float& v1, float& r0, float& r1) {
   const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
@@ -499,11 +497,58 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
   __syncthreads();
   // colour 0 on the core, then the core out
   const int zc0 = 2 * zz + c0;
-  const int i = t12(x + 2, y + 2, zc0 + 2);
-  const float v0 = sor_update(t12_sum(s_z, x + 2, y + 2, zc0 + 2), s_b[i], s_z[i], dx2, 1.0f);
-  const float v1 = s_z[t12(x + 2, y + 2, 2 * zz + (c0 ^ 1) + 2)];
-  zout[l * kHalfLeaf + t] = v0;          // colour 0 plane: the lane's colour-0 voxel
-  zout[half + l * kHalfLeaf + t] = v1;   // colour 1 plane
+  const int i = t12(x + 2, y + 2, zc0 + 2), i1 = t12(x + 2, y + 2, 2 * zz + (c0 ^ 1) + 2);
+  v0 = sor_update(t12_sum(s_z, x + 2, y + 2, zc0 + 2), s_b[i], s_z[i], dx2, 1.0f);
+  v1 = s_z[i1];
+  r0 = s_b[i];
+  r1 = s_b[i1];
+  zout[l * kHalfLeaf + t] = v0;         // colour 0 plane: the lane's colour-0 voxel
+  zout[half + l * kHalfLeaf + t] = v1;  // colour 1 plane
+}
+
+// One workgroup per leaf (the V-cycle on its own).
+__global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
+                                                   const float* __restrict__ zin, const float* __restrict__ u1,
+                                                   float* __restrict__ zout, const int* __restrict__ stop) {
+  __shared__ float s_b[kT12 * kT12 * kT12];
+  __shared__ float s_z[kT12 * kT12 * kT12];
+  if (stop && *stop) return;
+  const int32_t l = leaf_of_block();
+  if (l >= T.L) return;
+  float v0, v1, r0, r1;
+  mg0_post_leaf(T, l, b, zin, u1, zout, s_b, s_z, v0, v1, r0, r1);
+}
+
+// Inside flexible CG (truncation 1): the same pass over a strided leaf run per workgroup
+// (cg_groups(L) of them), also leaving the partials of z . r (b = r) and, from iteration 1, of
+// z . q_prev for the direction and step length.
+__global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* __restrict__ b,
+                                                       const float* __restrict__ zin, const float* __restrict__ u1,
+                                                       float* __restrict__ zout, McgHook H,
+                                                       const int* __restrict__ stop) {
+  __shared__ float s_b[kT12 * kT12 * kT12];
+  __shared__ float s_z[kT12 * kT12 * kT12];
+  __shared__ double s_red[kTpb / 64];
+  if (stop && *stop) return;
+  const int32_t it = *H.it;
+  const int32_t half = T.L * kHalfLeaf;
+  const float* __restrict__ qp = H.Qr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * H.n;
+  const LeafRun R = leaf_run(T.L);
+  double zr = 0.0, zq = 0.0;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+    float v0, v1, r0, r1;
+    mg0_post_leaf(T, l, b, zin, u1, zout, s_b, s_z, v0, v1, r0, r1);
+    zr += (double)v0 * (double)r0 + (double)v1 * (double)r1;
+    if (it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
+                      (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
+    __syncthreads();  // the tile is restaged for the next leaf
+  }
+  const double tr = block_sum_all(zr, s_red);
+  const double tq = block_sum_all(zq, s_red);
+  if (threadIdx.x == 0) {
+    H.zr[blockIdx.x] = tr;
+    H.zq[blockIdx.x] = tq;
+  }
 }
 
 // ---- levels 1 and 2 fused: the same two passes on leaf-dense levels ---------------------------
@@ -834,9 +879,10 @@ void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
 }
 
 void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
-                           hipStream_t s) {
+                           hipStream_t s, const McgHook* hook) {
   const MgShape& M = mg_shape();
   const bool fused = M.smooth0 == 1 && !(variant & kMgUnfused);
+  if (hook && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the flexible-CG pass needs the fused level-0 V-cycle");
   if (fused) {
     // level 0 pre-smoothing and restriction in one pass, into the scratch level-0 vector
     k_mg0_pre<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, stop);
@@ -893,7 +939,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     }
   }
   if (fused) {
-    k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, stop);
+    if (hook) k_mg0_post_fcg<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, *hook, stop);
+    else k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, stop);
     HNS_HIP(hipGetLastError());
     return;
   }

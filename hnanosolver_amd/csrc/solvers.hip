@@ -35,6 +35,23 @@ struct CgState {
   }
 };
 
+// One flexible-CG iteration on the multigrid V-cycle, launched on s (it: the host's count, which
+// only selects kernels). Truncation 1 (the default): the V-cycle's last pass leaves z . r and
+// z . q_prev, then direction and product in one launch; otherwise the general FCG(depth) kernels.
+void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int depth, int it, hipStream_t s) {
+  CgState& G = *cg_;
+  if (depth == 1 && mg_shape().smooth0 == 1) {
+    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, G.zq.p, G.qring.p, (int64_t)N_, S.it};
+    mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H);
+    launch_fcg1_dirspmv(T, it == 0, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+  } else {
+    mg_->enqueue(T, G.r.p, G.z.p, S.flag, s);
+    launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+  }
+  launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, s);
+  launch_fcg_test(T.L, true, S, s);
+}
+
 // kFcgBatch flexible-CG iterations on the multigrid V-cycle (iteration >= depth, so every
 // iteration's launches are the same), replayed from one graph made on first use for these buffers
 void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth, hipStream_t s) {
@@ -51,12 +68,7 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
     hipGraph_t g = nullptr;
     HNS_HIP(hipStreamBeginCapture(G.cap_stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int k = 0; k < kFcgBatch; ++k) {
-        M.enqueue(T, G.r.p, G.z.p, S.flag, G.cap_stream);
-        launch_fcg_direction(T, depth, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, G.cap_stream);
-        launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, G.cap_stream);
-        launch_fcg_test(T.L, true, S, G.cap_stream);
-      }
+      for (int k = 0; k < kFcgBatch; ++k) mg_iteration(T, p, S, depth, depth, G.cap_stream);
     } catch (...) {
       (void)hipStreamEndCapture(G.cap_stream, &g);
       if (g) (void)hipGraphDestroy(g);
@@ -275,9 +287,17 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
           launch_fcg_test(T.L, false, S, s);
           tm.end(s);
         }
+        if (mg) {
+          tm.begin(HNS_K_PRECOND, s);
+          mg_->prepare(T, s);
+          mg_iteration(T, p, S, depth, it, s);
+          tm.end(s);
+          ++it;
+          ++k;
+          continue;
+        }
         tm.begin(HNS_K_PRECOND, s);
-        if (learned) r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
-        else mg_->apply(T, G.r.p, G.z.p, s, true, S.flag);
+        r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
         tm.end(s);
         tm.begin(HNS_K_CG_PRODUCT, s);
         launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
