@@ -336,6 +336,9 @@ __device__ __forceinline__ float t12_sum(const float* S, int x, int y, int z) { 
   return S[t12(x + 1, y, z)] + S[t12(x - 1, y, z)] + S[t12(x, y + 1, z)] + S[t12(x, y - 1, z)] +
          S[t12(x, y, z + 1)] + S[t12(x, y, z - 1)];
 }
+__device__ __forceinline__ float t12_sumi(const float* S, int i) {  // the same sum at tile index i
+  return S[i + kT12 * kT12] + S[i - kT12 * kT12] + S[i + kT12] + S[i - kT12] + S[i + 1] + S[i - 1];
+}
 // colour-1 positions updated ahead of the core's colour 0: the core's (one per lane) and those of
 // the depth-1 face slabs (lanes < 192: face t >> 5, 32 positions each)
 __device__ __forceinline__ bool ring_c1(const FaceRow& F, int t, int& tx, int& ty, int& tz) {
@@ -346,6 +349,63 @@ __device__ __forceinline__ bool ring_c1(const FaceRow& F, int t, int& tx, int& t
     if (f < 4) ++tz; else ++ty;
   }
   return p.nl >= 0;
+}
+
+// Leaf-independent staging positions of the level-0 tile, one entry per lane slot, built at compile
+// time so the kernels decode a table word instead of recomputing positions (the tile passes are
+// bound by integer instruction issue). Face entry e (as face_pos): bits 0-10 tile index, 11-18
+// colour-split local index in the neighbour, 19 its colour, 20-25 its level-1 parent cell. Edge
+// entry (as edge_pos): the same, plus the 27-slot neighbour index in bits 26-30. Ring entry (as
+// ring_c1, lanes < 192): tile index of the colour-1 slab position.
+struct Tab12 {
+  uint32_t face[768];
+  uint32_t edge[96];
+  uint32_t ring[192];
+};
+constexpr uint32_t tab_word(int tx, int ty, int tz, int vx, int vy, int vz) {
+  return (uint32_t)((tx * kT12 + ty) * kT12 + tz) | ((uint32_t)((vx << 5) | (vy << 2) | (vz >> 1)) << 11) |
+         ((uint32_t)((vx + vy + vz) & 1) << 19) | ((uint32_t)(((vx >> 1) << 4) | ((vy >> 1) << 2) | (vz >> 1)) << 20);
+}
+constexpr Tab12 make_tab12() {
+  Tab12 T{};
+  for (int e = 0; e < 768; ++e) {
+    const int f = e >> 7, ax = f >> 1, pos = f & 1, d = (e >> 6) & 1, a = (e >> 3) & 7, b = e & 7;
+    const int vd = pos ? d : 7 - d, td = pos ? 10 + d : 1 - d;
+    const int vx = ax == 0 ? vd : a, vy = ax == 0 ? a : (ax == 1 ? vd : b), vz = ax == 2 ? vd : b;
+    const int tx = ax == 0 ? td : a + 2, ty = ax == 0 ? a + 2 : (ax == 1 ? td : b + 2), tz = ax == 2 ? td : b + 2;
+    T.face[e] = tab_word(tx, ty, tz, vx, vy, vz);
+    if (d == 0 && (b & 1) == 0 && (e & 127) < 64) {  // ring: the colour-1 position of the z (or y) pair
+      const int r = (f << 5) | (a << 2) | (b >> 1);
+      int rz = tz, ry = ty;
+      if (((tx + ty + tz) & 1) == 0) {
+        if (f < 4) ++rz; else ++ry;
+      }
+      T.ring[r] = (uint32_t)((tx * kT12 + ry) * kT12 + rz);
+    }
+  }
+  for (int e = 0; e < 96; ++e) {
+    const int g = e >> 3, w = e & 7, s0 = (g & 1) ? 1 : -1, s1 = (g & 2) ? 1 : -1;
+    const int d0 = g < 8 ? s0 : 0, d1 = g < 4 ? s1 : (g < 8 ? 0 : s0), d2 = g < 4 ? 0 : s1;
+    const int vx = d0 == 0 ? w : (d0 > 0 ? 0 : 7), vy = d1 == 0 ? w : (d1 > 0 ? 0 : 7), vz = d2 == 0 ? w : (d2 > 0 ? 0 : 7);
+    T.edge[e] = tab_word(vx + 2 + 8 * d0, vy + 2 + 8 * d1, vz + 2 + 8 * d2, vx, vy, vz) |
+                ((uint32_t)((d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1)) << 26);
+  }
+  return T;
+}
+__constant__ Tab12 kTab12 = make_tab12();
+// table word -> tile index / colour-split index in leaf nl / its level-1 parent in leaf nl
+__device__ __forceinline__ int tw_tile(uint32_t w) { return (int)(w & 2047u); }
+__device__ __forceinline__ uint32_t tw_cs(uint32_t w, uint32_t nl, uint32_t half) {
+  return ((w >> 19) & 1u) * half + nl * (uint32_t)kHalfLeaf + ((w >> 11) & 255u);
+}
+__device__ __forceinline__ uint32_t tw_parent(uint32_t w, uint32_t nl) { return nl * 64u + ((w >> 20) & 63u); }
+// element i of a device vector through a 32-bit byte offset (every vector here is below 4 GiB)
+__device__ __forceinline__ float ld32(const float* __restrict__ p, uint32_t i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + (i << 2));
+}
+// face f (uniform per wave) of the face row
+__device__ __forceinline__ int32_t face_nl(const FaceRow& F, int f) {
+  return f == 0 ? F.xm : f == 1 ? F.xp : f == 2 ? F.ym : f == 3 ? F.yp : f == 4 ? F.zm : F.zp;
 }
 
 // Pre-smoothing and restriction of the V-cycle's level 0 in one pass: from z = 0, one red-black
@@ -367,32 +427,36 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const float dx2 = -1.0f;
-  auto put = [&](int tx, int ty, int tz, float bv) {
-    const int i = t12(tx, ty, tz);
-    s_b[i] = bv;
-    s_z[i] = ((tx + ty + tz) & 1) == 0 ? sor_update(0.f, bv, 0.f, dx2, 1.0f) : 0.f;
-  };
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
+  const uint32_t uh = (uint32_t)half, ul = (uint32_t)l;
   // every load issued before the first LDS store (absent leaves load the own leaf, then read 0)
-  auto at = [&](const TilePos& p) {
-    return ((p.vx + p.vy + p.vz) & 1) * half + (p.nl < 0 ? l : p.nl) * kHalfLeaf + cs_local(p.vx, p.vy, p.vz);
-  };
-  const TilePos ep = edge_pos(T, l, t < 96 ? t : 0);
-  TilePos fp[3];
+  const uint32_t ew = kTab12.edge[t < 96 ? t : 0];
+  const int32_t enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
+  uint32_t fw[3];
+  int32_t fnl[3];
   float fv[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    fp[k] = face_pos(Fr, t + k * kTpb);
-    fv[k] = b[at(fp[k])];
+    fw[k] = kTab12.face[t + k * kTpb];
+    fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
+    fv[k] = ld32(b, tw_cs(fw[k], fnl[k] < 0 ? ul : (uint32_t)fnl[k], uh));
   }
-  const float b0 = b[c0 * half + l * kHalfLeaf + t], b1v = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
-  const float ev = b[at(ep)];
-  put(x + 2, y + 2, 2 * zz + 2, b0);
-  put(x + 2, y + 2, 2 * zz + 3, b1v);
+  const float b0 = ld32(b, (uint32_t)(c0 * half) + ul * kHalfLeaf + t);
+  const float b1v = ld32(b, (uint32_t)((c0 ^ 1) * half) + ul * kHalfLeaf + t);
+  const float ev = ld32(b, tw_cs(ew, enl < 0 ? ul : (uint32_t)enl, uh));
+  auto puti = [&](int i, int col, float bv) {
+    s_b[i] = bv;
+    s_z[i] = col == 0 ? sor_update(0.f, bv, 0.f, dx2, 1.0f) : 0.f;
+  };
+  {
+    const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
+    puti(i0, c0, b0);
+    puti(i0 + 1, c0 ^ 1, b1v);
+  }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) put(fp[k].tx, fp[k].ty, fp[k].tz, fp[k].nl < 0 ? 0.f : fv[k]);
-  if (t < 96) put(ep.tx, ep.ty, ep.tz, ep.nl < 0 ? 0.f : ev);
+  for (int k = 0; k < 3; ++k) puti(tw_tile(fw[k]), (int)((fw[k] >> 19) & 1u), fnl[k] < 0 ? 0.f : fv[k]);
+  if (t < 96) puti(tw_tile(ew), (int)((ew >> 19) & 1u), enl < 0 ? 0.f : ev);
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs (positions in absent leaves stay 0)
   {
@@ -400,12 +464,9 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
     const int i = t12(x + 2, y + 2, zc1 + 2);
     s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, 1.0f);
   }
-  if (t < 192) {
-    int tx, ty, tz;
-    if (ring_c1(Fr, t, tx, ty, tz)) {
-      const int i = t12(tx, ty, tz);
-      s_z[i] = sor_update(t12_sum(s_z, tx, ty, tz), s_b[i], 0.f, dx2, 1.0f);
-    }
+  if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
+    const int i = (int)kTab12.ring[t];
+    s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], 0.f, dx2, 1.0f);
   }
   __syncthreads();
   // residual of the core's voxel pair, restriction, and z of the core
@@ -444,28 +505,30 @@ __device__ __forceinline__ void mg0_post_leaf(const DevTopo& T, int32_t l, const
   const float dx2 = -1.0f;
   // every load issued before the first LDS store (absent leaves load the own leaf, then read 0);
   // a staged z is zin plus its parent's value in u1
-  auto at = [&](const TilePos& p) {
-    return ((p.vx + p.vy + p.vz) & 1) * half + (p.nl < 0 ? l : p.nl) * kHalfLeaf + cs_local(p.vx, p.vy, p.vz);
-  };
-  auto parent = [&](const TilePos& p) {
-    return (p.nl < 0 ? l : p.nl) * 64 + (((p.vx >> 1) << 4) | ((p.vy >> 1) << 2) | (p.vz >> 1));
-  };
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
-  const TilePos ep = edge_pos(T, l, t < 96 ? t : 0);
-  TilePos fp[3];
+  const uint32_t uh = (uint32_t)half, ul = (uint32_t)l;
+  const uint32_t ew = kTab12.edge[t < 96 ? t : 0];
+  const int32_t enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
+  uint32_t fw[3];
+  int32_t fnl[3];
   float fz[3], fu[3], fb[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    fp[k] = face_pos(Fr, t + k * kTpb);
-    fz[k] = zin[at(fp[k])];
-    fu[k] = u1[parent(fp[k])];
-    fb[k] = b[at(fp[k])];  // used at depth 1
+    fw[k] = kTab12.face[t + k * kTpb];
+    fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
+    const uint32_t src = fnl[k] < 0 ? ul : (uint32_t)fnl[k];
+    const uint32_t i = tw_cs(fw[k], src, uh);
+    fz[k] = ld32(zin, i);
+    fu[k] = ld32(u1, tw_parent(fw[k], src));
+    fb[k] = ld32(b, i);  // used at depth 1
   }
-  const float z0 = zin[c0 * half + l * kHalfLeaf + t], z1 = zin[(c0 ^ 1) * half + l * kHalfLeaf + t];
-  const float up = u1[l * 64 + ((x >> 1) << 4) + ((y >> 1) << 2) + zz];  // the pair's parent (z >> 1 = zz)
-  const float b0 = b[c0 * half + l * kHalfLeaf + t], b1v = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
-  const float ez = zin[at(ep)], eu = u1[parent(ep)];
+  const uint32_t i0g = (uint32_t)(c0 * half) + ul * kHalfLeaf + t, i1g = (uint32_t)((c0 ^ 1) * half) + ul * kHalfLeaf + t;
+  const float z0 = ld32(zin, i0g), z1 = ld32(zin, i1g);
+  const float up = ld32(u1, ul * 64u + (uint32_t)(((x >> 1) << 4) + ((y >> 1) << 2) + zz));  // the pair's parent
+  const float b0 = ld32(b, i0g), b1v = ld32(b, i1g);
+  const uint32_t esrc = enl < 0 ? ul : (uint32_t)enl;
+  const float ez = ld32(zin, tw_cs(ew, esrc, uh)), eu = ld32(u1, tw_parent(ew, esrc));
   {
     const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
     s_z[i0] = z0 + up;
@@ -475,11 +538,11 @@ __device__ __forceinline__ void mg0_post_leaf(const DevTopo& T, int32_t l, const
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const int i = t12(fp[k].tx, fp[k].ty, fp[k].tz);
-    s_z[i] = fp[k].nl < 0 ? 0.f : fz[k] + fu[k];
-    s_b[i] = fp[k].nl < 0 ? 0.f : fb[k];  // read at depth 1 only
+    const int i = tw_tile(fw[k]);
+    s_z[i] = fnl[k] < 0 ? 0.f : fz[k] + fu[k];
+    s_b[i] = fnl[k] < 0 ? 0.f : fb[k];  // read at depth 1 only
   }
-  if (t < 96) s_z[t12(ep.tx, ep.ty, ep.tz)] = ep.nl < 0 ? 0.f : ez + eu;
+  if (t < 96) s_z[tw_tile(ew)] = enl < 0 ? 0.f : ez + eu;
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs
   {
@@ -487,12 +550,9 @@ __device__ __forceinline__ void mg0_post_leaf(const DevTopo& T, int32_t l, const
     const int i = t12(x + 2, y + 2, zc1 + 2);
     s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], s_z[i], dx2, 1.0f);
   }
-  if (t < 192) {
-    int tx, ty, tz;
-    if (ring_c1(Fr, t, tx, ty, tz)) {
-      const int i = t12(tx, ty, tz);
-      s_z[i] = sor_update(t12_sum(s_z, tx, ty, tz), s_b[i], s_z[i], dx2, 1.0f);
-    }
+  if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
+    const int i = (int)kTab12.ring[t];
+    s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], s_z[i], dx2, 1.0f);
   }
   __syncthreads();
   // colour 0 on the core, then the core out
