@@ -199,30 +199,30 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   const int32_t N = T.L * kLeafVox;
   const Vel U{u, u + N, u + 2 * N};
   {
-    // advect_scalars reads element 0 on a miss (velocity and fields alike)
-    const float mu0 = u[0], mu1 = u[N], mu2 = u[2 * N];
-    float mf[FieldSet::kMax];
+    // advect_scalars reads element 0 on a miss (velocity and fields alike): an absent cell loads
+    // element 0 of each plane. Every load of the tile is issued before the first LDS store.
+    constexpr int kIt = (kE3 + 511) / 512;
+    float v[kIt][3 + FieldSet::kMax];
 #pragma unroll
-    for (int s = 0; s < FieldSet::kMax; ++s) mf[s] = s < F.m ? F.in[s][0] : 0.f;
-    for (int c = threadIdx.x; c < kE3; c += 512) {
+    for (int it = 0; it < kIt; ++it) {
+      const int c = min((int)threadIdx.x + it * 512, kE3 - 1);
       const int tx = c / (kE * kE), ty = (c / kE) % kE, tz = c % kE;
       const int lx = tx - kH, ly = ty - kH, lz = tz - kH;
       const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
-      if (nl >= 0) {
-        const int32_t g = nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
-        s_u[c] = u[g];
-        s_u[kE3 + c] = u[N + g];
-        s_u[2 * kE3 + c] = u[2 * N + g];
+      const int32_t g = nl < 0 ? 0 : nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+      v[it][0] = u[g];
+      v[it][1] = u[N + g];
+      v[it][2] = u[2 * N + g];
 #pragma unroll
-        for (int s = 0; s < FieldSet::kMax; ++s)
-          if (s < F.m) s_u[(3 + s) * kE3 + c] = F.in[s][g];
-      } else {
-        s_u[c] = mu0;
-        s_u[kE3 + c] = mu1;
-        s_u[2 * kE3 + c] = mu2;
+      for (int s = 0; s < FieldSet::kMax; ++s) v[it][3 + s] = s < F.m ? F.in[s][g] : 0.f;
+    }
 #pragma unroll
-        for (int s = 0; s < FieldSet::kMax; ++s)
-          if (s < F.m) s_u[(3 + s) * kE3 + c] = mf[s];
+    for (int it = 0; it < kIt; ++it) {
+      const int c = (int)threadIdx.x + it * 512;
+      if (c < kE3) {
+#pragma unroll
+        for (int k = 0; k < 3 + FieldSet::kMax; ++k)
+          if (k < 3 + F.m) s_u[k * kE3 + c] = v[it][k];
       }
     }
     __syncthreads();
@@ -239,27 +239,47 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   Interp8T bd;
   const bool bt = setup_interp_tile(V, bx, by, bz, bd);
   float vfx = 0.f, vfy = 0.f, vfz = 0.f;
+  // waves whose traces all stay in the tile (the common case) take a path without global reads
+  if (__all(bt)) {
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const float3 v = bt ? tvec(s_u, bd.idx[q]) : f3(U.x[bd.idx[q]], U.y[bd.idx[q]], U.z[bd.idx[q]]);
-    vfx = fmaf(bd.w[q], v.x, vfx);
-    vfy = fmaf(bd.w[q], v.y, vfy);
-    vfz = fmaf(bd.w[q], v.z, vfz);
+    for (int q = 0; q < 8; ++q) {
+      const float3 v = tvec(s_u, bd.idx[q]);
+      vfx = fmaf(bd.w[q], v.x, vfx);
+      vfy = fmaf(bd.w[q], v.y, vfy);
+      vfz = fmaf(bd.w[q], v.z, vfz);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const float3 v = bt ? tvec(s_u, bd.idx[q]) : f3(U.x[bd.idx[q]], U.y[bd.idx[q]], U.z[bd.idx[q]]);
+      vfx = fmaf(bd.w[q], v.x, vfx);
+      vfy = fmaf(bd.w[q], v.y, vfy);
+      vfz = fmaf(bd.w[q], v.z, vfz);
+    }
   }
   float fx = bx + sdt * vfx, fy = by + sdt * vfy, fz = bz + sdt * vfz;
   if (sdf && trilinear_s(V, sdf, fx, fy, fz) < 0.f) { fx = bx; fy = by; fz = bz; }
   Interp8T fd;
   const bool ft = setup_interp_tile(V, fx, fy, fz, fd);
   const int nct[6] = {ct - kE * kE, ct + kE * kE, ct - kE, ct + kE, ct - 1, ct + 1};
+  const bool wave_tile = __all(bt && ft);
   for (int s = 0; s < F.m; ++s) {
     const float* __restrict__ d = F.in[s];
     const float* s_f = s_dyn + (3 + s) * kE3;
     const float phiOrig = s_f[ct];
     float phiF = 0.f, phiB = 0.f;
+    if (wave_tile) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      phiF = fmaf(bt ? s_f[bd.idx[q]] : d[bd.idx[q]], bd.w[q], phiF);
-      phiB = fmaf(ft ? s_f[fd.idx[q]] : d[fd.idx[q]], fd.w[q], phiB);
+      for (int q = 0; q < 8; ++q) {
+        phiF = fmaf(s_f[bd.idx[q]], bd.w[q], phiF);
+        phiB = fmaf(s_f[fd.idx[q]], fd.w[q], phiB);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        phiF = fmaf(bt ? s_f[bd.idx[q]] : d[bd.idx[q]], bd.w[q], phiF);
+        phiB = fmaf(ft ? s_f[fd.idx[q]] : d[fd.idx[q]], fd.w[q], phiB);
+      }
     }
     const float corr = fmaf(0.5f, phiOrig - phiB, phiF);
     float mn = phiOrig, mx = phiOrig;
