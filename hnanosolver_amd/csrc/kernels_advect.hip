@@ -28,17 +28,27 @@ __device__ __forceinline__ int tix(int x, int y, int z) { return (x * kE + y) * 
 template <int NC>
 __device__ inline void fill_tile(float* __restrict__ s, const float* __restrict__ f, int32_t stride,
                                  const int32_t* s_nb, int l, const float* miss) {
-  for (int c = threadIdx.x; c < kE3; c += 512) {
+  // every load issued before the first LDS store (an absent cell loads element 0, then takes miss)
+  constexpr int kIt = (kE3 + 511) / 512;
+  float v[kIt][NC];
+  bool hit[kIt];
+#pragma unroll
+  for (int it = 0; it < kIt; ++it) {
+    const int c = min((int)threadIdx.x + it * 512, kE3 - 1);
     const int tx = c / (kE * kE), ty = (c / kE) % kE, tz = c % kE;
     const int lx = tx - kH, ly = ty - kH, lz = tz - kH;
     const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
-    if (nl >= 0) {
-      const int32_t g = nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+    hit[it] = nl >= 0;
+    const int32_t g = nl < 0 ? 0 : nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
 #pragma unroll
-      for (int k = 0; k < NC; ++k) s[k * kE3 + c] = f[k * stride + g];
-    } else {
+    for (int k = 0; k < NC; ++k) v[it][k] = f[k * stride + g];
+  }
 #pragma unroll
-      for (int k = 0; k < NC; ++k) s[k * kE3 + c] = miss[k];
+  for (int it = 0; it < kIt; ++it) {
+    const int c = (int)threadIdx.x + it * 512;
+    if (c < kE3) {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) s[k * kE3 + c] = hit[it] ? v[it][k] : miss[k];
     }
   }
   (void)l;
