@@ -133,9 +133,10 @@ __global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restr
                                                           const int* __restrict__ stop) {
   constexpr int C = 2 * CH, H = kCoarseTpb * CH;
   __shared__ float s_u[2 * H + 1];
+  __shared__ float s_bi[2 * H];  // scaled right-hand sides by slot (in LDS to leave registers for loads)
   if (stop && *stop) return;
   const int t = threadIdx.x;
-  float bi[C], uo[C];
+  float uo[C];
   uint32_t nb[C][3];
 #pragma unroll
   for (int k = 0; k < C; ++k) {
@@ -144,7 +145,7 @@ __global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restr
     nb[k][0] = nbp[j];          // +x | -x << 16
     nb[k][1] = nbp[2 * H + j];  // +y | -y << 16
     nb[k][2] = nbp[4 * H + j];  // +z | -z << 16
-    bi[k] = leaf >= 0 ? b[leaf] * inv_s : 0.f;
+    s_bi[j] = leaf >= 0 ? b[leaf] * inv_s : 0.f;
     uo[k] = 0.f;
   }
   for (int j = t; j <= 2 * H; j += kCoarseTpb) s_u[j] = 0.f;
@@ -152,22 +153,38 @@ __global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restr
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
     for (int colour = 0; colour < 2; ++colour) {
-      // new values of this colour's slots first (they read only the other colour), then the stores
+      // new values of this colour's slots first (they read only the other colour), then the stores;
+      // the reads of NG slots are issued before their sums
+      constexpr int NG = CH < 2 ? CH : 2;
       float v[CH];
 #pragma unroll
-      for (int kk = 0; kk < CH; ++kk) {
-        const int k = colour * CH + kk;
-        float sum = 0.f;
-        if (!(it == 0 && colour == 0)) {
-          uint32_t n0 = nb[k][0], n1 = nb[k][1], n2 = nb[k][2];
-          opaque_v(n0);  // unpack here, not once for all iterations into registers
-          opaque_v(n1);
-          opaque_v(n2);
-          sum = s_u[n0 & 0xffff] + s_u[n0 >> 16] + s_u[n1 & 0xffff] + s_u[n1 >> 16] + s_u[n2 & 0xffff] +
-                s_u[n2 >> 16];
+      for (int kk = 0; kk < CH; kk += NG) {
+        float a[NG][6], bb[NG];
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          const int k = colour * CH + kk + g;
+          bb[g] = s_bi[t + k * kCoarseTpb];
+          if (!(it == 0 && colour == 0)) {
+            uint32_t n0 = nb[k][0], n1 = nb[k][1], n2 = nb[k][2];
+            opaque_v(n0);  // unpack here, not once for all iterations into registers
+            opaque_v(n1);
+            opaque_v(n2);
+            a[g][0] = s_u[n0 & 0xffff];
+            a[g][1] = s_u[n0 >> 16];
+            a[g][2] = s_u[n1 & 0xffff];
+            a[g][3] = s_u[n1 >> 16];
+            a[g][4] = s_u[n2 & 0xffff];
+            a[g][5] = s_u[n2 >> 16];
+          }
         }
-        const float gs = (bi[k] + sum) * (1.0f / 6.0f);
-        v[kk] = it == 0 ? omega * gs : fmaf(omega, gs - uo[k], uo[k]);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+          const int k = colour * CH + kk + g;
+          float sum = 0.f;
+          if (!(it == 0 && colour == 0)) sum = a[g][0] + a[g][1] + a[g][2] + a[g][3] + a[g][4] + a[g][5];
+          const float gs = (bb[g] + sum) * (1.0f / 6.0f);
+          v[kk + g] = it == 0 ? omega * gs : fmaf(omega, gs - uo[k], uo[k]);
+        }
       }
 #pragma unroll
       for (int kk = 0; kk < CH; ++kk) {
