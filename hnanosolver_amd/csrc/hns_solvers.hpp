@@ -118,7 +118,8 @@ class Solvers {
   SolveResult solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
                        hipStream_t s);
   void mg_batch(const DevTopo& T, float* p, const struct CgScalars& S, int depth, hipStream_t s);
-  void mg_iteration(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int it, hipStream_t s);
+  void mg_iteration(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int it, hipStream_t s,
+                    Timing* tm = nullptr);
 };
 
 }  // namespace hns

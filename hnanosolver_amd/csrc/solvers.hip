@@ -38,18 +38,37 @@ struct CgState {
 // One flexible-CG iteration on the multigrid V-cycle, launched on s (it: the host's count, which
 // only selects kernels). Truncation 1 (the default): the V-cycle's last pass leaves z . r and
 // z . q_prev, then direction and product in one launch; otherwise the general FCG(depth) kernels.
-void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int depth, int it, hipStream_t s) {
+void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int depth, int it, hipStream_t s,
+                           Timing* tm) {
   CgState& G = *cg_;
+  auto begin = [&](int k) {
+    if (tm) tm->begin(k, s);
+  };
+  auto end = [&] {
+    if (tm) tm->end(s);
+  };
   if (depth == 1 && mg_shape().smooth0 == 1) {
     const McgHook H{S.pa + kSlotDR * kCgMaxGroups, G.zq.p, G.qring.p, (int64_t)N_, S.it};
+    begin(HNS_K_PRECOND);
     mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H);
+    end();
+    begin(HNS_K_CG_PRODUCT);
     launch_fcg1_dirspmv(T, it == 0, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+    end();
   } else {
+    begin(HNS_K_PRECOND);
     mg_->enqueue(T, G.r.p, G.z.p, S.flag, s);
+    end();
+    begin(HNS_K_CG_PRODUCT);
     launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+    end();
   }
+  begin(HNS_K_CG_UPDATE);
   launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, s);
+  end();
+  begin(HNS_K_OTHER);
   launch_fcg_test(T.L, true, S, s);
+  end();
 }
 
 // kFcgBatch flexible-CG iterations on the multigrid V-cycle (iteration >= depth, so every
@@ -243,7 +262,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.it = G.flag.p + 2;
   S.bb = G.bb.p;
   S.tol2 = (double)r.tol * (double)r.tol;
-  tm.begin(HNS_K_CG_UPDATE, s);
+  tm.begin(HNS_K_OTHER, s);
   launch_cg_init(T.L, div, r.dx, p, G.r.p, S, s);
   tm.end(s);
   // The device stops at the first iteration whose recurrence residual passes the test. The stop
@@ -266,13 +285,14 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     if (pre) nb = std::min(nb, kFcgBatch - it % kFcgBatch);
     for (int k = 0; k < nb;) {
       if (it > 0 && it % kResidualReplace == 0 && replaced_at != it) {
-        tm.begin(HNS_K_CG_UPDATE, s);
+        tm.begin(HNS_K_OTHER, s);
         launch_cg_replace(T, div, r.dx, p, G.r.p, S, false, s);
         if (pre) launch_fcg_test(T.L, false, S, s);  // test the residual just replaced
         tm.end(s);
         replaced_at = it;
       }
-      if (pre && mg && it >= depth && it % kFcgBatch == 0 && nb - k >= kFcgBatch) {
+      // (with per-kernel timing on, the iterations are launched one by one so events can bracket them)
+      if (pre && mg && !tm.enabled && it >= depth && it % kFcgBatch == 0 && nb - k >= kFcgBatch) {
         tm.begin(HNS_K_PRECOND, s);
         mg_batch(T, p, S, depth, s);
         tm.end(s);
@@ -283,15 +303,13 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       if (pre) {
         // flexible CG on z = M(r); the stop test of iteration it > 0 ran after the last update
         if (it == 0) {
-          tm.begin(HNS_K_CG_UPDATE, s);
+          tm.begin(HNS_K_OTHER, s);
           launch_fcg_test(T.L, false, S, s);
           tm.end(s);
         }
         if (mg) {
-          tm.begin(HNS_K_PRECOND, s);
           mg_->prepare(T, s);
-          mg_iteration(T, p, S, depth, it, s);
-          tm.end(s);
+          mg_iteration(T, p, S, depth, it, s, &tm);
           ++it;
           ++k;
           continue;
@@ -327,14 +345,14 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         stopped = true;
         break;
       }
-      tm.begin(HNS_K_CG_UPDATE, s);
+      tm.begin(HNS_K_OTHER, s);
       launch_cg_replace(T, div, r.dx, p, G.r.p, S, true, s);
       tm.end(s);
       replaced_at = it;
       batch = batch0;
       if (pre) {
         // re-test the stop on the replaced residual alone before paying for another iteration
-        tm.begin(HNS_K_CG_UPDATE, s);
+        tm.begin(HNS_K_OTHER, s);
         launch_fcg_test(T.L, false, S, s);
         tm.end(s);
         HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
