@@ -14,12 +14,13 @@ constexpr int kMgLevels = 4;
 // truncation around the V-cycle, and the scale of every coarse-grid correction (piecewise-constant
 // prolongation under-corrects smooth errors; over-correcting by 1.6 took the 8 M-voxel fire solve
 // from 16 to 12 iterations). Defaults are the measured best at 8 M voxels; HNS_MG_COARSE,
-// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH and HNS_MG_OVER (x100)
-// override them (read once per process).
+// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH, HNS_MG_OVER (x100) and
+// HNS_MG_OVER_DEEP (x100; default: HNS_MG_OVER) override them (read once per process).
 struct MgShape {
   int coarse = 12, smooth0 = 1, smooth = 1, depth = 1;
   float coarse_omega = 1.85f;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
+  float over_deep = 1.6f;  // the same below level 1 (restrictions of levels 1 and 2)
 };
 const MgShape& mg_shape();
 

@@ -23,6 +23,27 @@ constexpr int kE3 = kE * kE * kE;
 
 __device__ __forceinline__ int tix(int x, int y, int z) { return (x * kE + y) * kE + z; }
 
+// Tile cell c -> (27-slot of the leaf holding it) << 9 | its leaf-dense local index, built at
+// compile time so the fill decodes a table word instead of dividing c (the fills were a large
+// share of the kernels' integer instructions).
+struct FillTab {
+  uint32_t w[kE3];
+};
+constexpr FillTab make_fill_tab() {
+  FillTab t{};
+  for (int c = 0; c < kE3; ++c) {
+    const int lx = c / (kE * kE) - kH, ly = (c / kE) % kE - kH, lz = c % kE - kH;
+    const int slot = ((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1);
+    t.w[c] = ((uint32_t)slot << 9) | (uint32_t)(((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+  }
+  return t;
+}
+__constant__ FillTab kFill = make_fill_tab();
+// element i of a device vector through a 32-bit byte offset (every vector here is below 4 GiB)
+__device__ __forceinline__ float ld32(const float* __restrict__ p, uint32_t i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(p) + (i << 2));
+}
+
 // Fills NC planes of the tile from a leaf-dense field with plane stride `stride`; absent cells get
 // miss[c]. Ends with a barrier.
 template <int NC>
@@ -34,14 +55,12 @@ __device__ inline void fill_tile(float* __restrict__ s, const float* __restrict_
   bool hit[kIt];
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {
-    const int c = min((int)threadIdx.x + it * 512, kE3 - 1);
-    const int tx = c / (kE * kE), ty = (c / kE) % kE, tz = c % kE;
-    const int lx = tx - kH, ly = ty - kH, lz = tz - kH;
-    const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
+    const uint32_t w = kFill.w[min((int)threadIdx.x + it * 512, kE3 - 1)];
+    const int32_t nl = s_nb[w >> 9];
     hit[it] = nl >= 0;
-    const int32_t g = nl < 0 ? 0 : nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
+    const uint32_t g = nl < 0 ? 0u : ((uint32_t)nl << 9) + (w & 511u);
 #pragma unroll
-    for (int k = 0; k < NC; ++k) v[it][k] = f[k * stride + g];
+    for (int k = 0; k < NC; ++k) v[it][k] = ld32(f + (int64_t)k * stride, g);
   }
 #pragma unroll
   for (int it = 0; it < kIt; ++it) {

@@ -34,6 +34,7 @@ const MgShape& mg_shape() {
     m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
     m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
     m.over = 0.01f * (float)env("HNS_MG_OVER", 50, 300, (int)(100.0f * m.over + 0.5f));
+    m.over_deep = 0.01f * (float)env("HNS_MG_OVER_DEEP", 50, 300, (int)(100.0f * m.over + 0.5f));
     return m;
   }();
   return v;
@@ -980,8 +981,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     const float scale = (float)(1 << (2 * l));
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
-      if (lgS == 2) k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over, b[l + 1].p, stop);
-      else k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over, b[l + 1].p, stop);
+      if (lgS == 2) k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, stop);
+      else k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, stop);
       HNS_HIP(hipGetLastError());
       continue;
     }
@@ -989,7 +990,7 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
       smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, 1.0f, stop, s);
       smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, 1.0f, stop, s);
     }
-    k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, M.over, b[l + 1].p,
+    k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, M.over_deep, b[l + 1].p,
                                                         stop);
     HNS_HIP(hipGetLastError());
   }
