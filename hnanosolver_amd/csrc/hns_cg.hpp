@@ -83,6 +83,7 @@ struct CgScalars {
   double* pb;     // one partial per workgroup (d . q), written by the product kernel
   double* rho;    // max_iters + 1 entries
   double* alpha;  // max_iters + 1 entries
+  double* rz;     // multigrid PCG: r_k . z_k, max_iters + 1 entries
   int* flag;      // 2 entries: stop raised, stop iteration
   int* it;        // flexible CG: the device's iteration counter (0 after init, advanced by the test)
   double tol2;    // stop when r.r <= tol2 * ||b||^2
@@ -113,8 +114,10 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 // Truncation 1 on the multigrid V-cycle (whose last pass leaves z . q_prev in zq row 0 and z . r in
 // pa row kSlotDR): d = z + beta d_prev and q = A d in one launch (first: iteration 0, d = z), partials
 // of d . q; the update then takes alpha = (z . r) / (d . q).
+// pcg: Fletcher-Reeves beta = (z_k . r_k) / (z_{k-1} . r_{k-1}) from the z . r partials instead
+// (the V-cycle then leaves no z . q_prev)
 void launch_fcg1_dirspmv(const DevTopo& T, bool first, const float* z, float* Dr, float* Qr, const double* zq,
-                         const CgScalars& S, hipStream_t s);
+                         const CgScalars& S, hipStream_t s, bool pcg = false);
 // x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
 constexpr int kFcgDepth = 8;
 __device__ __forceinline__ int fcg_slot(int32_t i) { return (int)(i % kFcgDepth); }

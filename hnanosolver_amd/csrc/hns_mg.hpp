@@ -17,7 +17,7 @@ constexpr int kMgLevels = 4;
 // HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH, HNS_MG_OVER (x100) and
 // HNS_MG_OVER_DEEP (x100; default: HNS_MG_OVER) override them (read once per process).
 struct MgShape {
-  int coarse = 12, smooth0 = 1, smooth = 1, depth = 1;
+  int coarse = 12, smooth0 = 1, smooth = 1, depth = 1, pcg = 0;
   float coarse_omega = 1.85f;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
   float over_deep = 1.6f;  // the same below level 1 (restrictions of levels 1 and 2)
@@ -36,7 +36,7 @@ constexpr unsigned kMgUnfused = 2;         // levels 0-2 as separate sweep, rest
 // product of the previous iteration in the ring Qr (slot (*it - 1) mod kFcgDepth, n floats each).
 struct McgHook {
   double* zr;
-  double* zq;
+  double* zq;  // null: no z . q_prev (the PCG variant)
   const float* Qr;
   int64_t n;
   const int* it;

@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int64_t n, const f
 // (NP = 1; NP = 0 at iteration 0: d = z), d = z + beta d_prev formed for the leaf's voxel pairs and
 // its face halos (the same fmaf as k_fcg_dir), q = A d, d and q into their ring slots, partials of
 // d . q. The numerator of alpha, z . r, is also left by the V-cycle (pa row kSlotDR).
-template <int NP>
+template <int NP, bool PCG>
 __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, const float* __restrict__ z,
                                                        float* __restrict__ Dr, float* __restrict__ Qr,
                                                        const double* __restrict__ zq, CgScalars S) {
@@ -354,12 +354,17 @@ __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, con
   const int32_t it = *S.it;
   float beta = 0.f;
   const float* __restrict__ dp = z;
-  if (NP > 0) {
+  if (PCG) {
+    double rz1[1];
+    reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rz1);
+    if (blockIdx.x == 0 && threadIdx.x == 0) S.rz[it] = rz1[0];
+    if (NP > 0) beta = (float)(rz1[0] / S.rz[it - 1]);
+  } else if (NP > 0) {
     double zq1[1];
     reduce_partial_rows(zq, (int)gridDim.x, s_red, zq1);
     beta = (float)(-zq1[0] / S.rho[it - 1]);
-    dp = Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n;
   }
+  if (NP > 0) dp = Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n;
   float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
   float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
   const LeafRun R = leaf_run(T.L);
@@ -489,11 +494,16 @@ void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* 
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg1_dirspmv(const DevTopo& T, bool first, const float* z, float* Dr, float* Qr, const double* zq,
-                         const CgScalars& S, hipStream_t s) {
+                         const CgScalars& S, hipStream_t s, bool pcg) {
   const unsigned g = cg_groups(T.L);
   const int64_t n = (int64_t)T.L * kLeafVox;
-  if (first) k_fcg1_dirspmv<0><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
-  else k_fcg1_dirspmv<1><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
+  if (pcg) {
+    if (first) k_fcg1_dirspmv<0, true><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
+    else k_fcg1_dirspmv<1, true><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
+  } else {
+    if (first) k_fcg1_dirspmv<0, false><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
+    else k_fcg1_dirspmv<1, false><<<g, kTpb, 0, s>>>(T, n, z, Dr, Qr, zq, S);
+  }
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg_update(int32_t L, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,

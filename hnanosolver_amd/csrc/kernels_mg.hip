@@ -32,6 +32,7 @@ const MgShape& mg_shape() {
     m.smooth0 = env("HNS_MG_SMOOTH0", 1, 8, m.smooth0);
     m.smooth = env("HNS_MG_SMOOTH", 1, 8, m.smooth);
     m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
+    m.pcg = env("HNS_MG_PCG", 0, 1, m.pcg);
     m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
     m.over = 0.01f * (float)env("HNS_MG_OVER", 50, 300, (int)(100.0f * m.over + 0.5f));
     m.over_deep = 0.01f * (float)env("HNS_MG_OVER_DEEP", 50, 300, (int)(100.0f * m.over + 0.5f));
@@ -617,15 +618,15 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
     float v0, v1, r0, r1;
     mg0_post_leaf(T, l, b, zin, u1, zout, s_b, s_z, v0, v1, r0, r1);
     zr += (double)v0 * (double)r0 + (double)v1 * (double)r1;
-    if (it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
-                      (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
+    if (H.zq && it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
+                              (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
     __syncthreads();  // the tile is restaged for the next leaf
   }
   const double tr = block_sum_all(zr, s_red);
   const double tq = block_sum_all(zq, s_red);
   if (threadIdx.x == 0) {
     H.zr[blockIdx.x] = tr;
-    H.zq[blockIdx.x] = tq;
+    if (H.zq) H.zq[blockIdx.x] = tq;
   }
 }
 

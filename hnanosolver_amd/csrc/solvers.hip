@@ -16,7 +16,7 @@ namespace hns {
 struct CgState {
   DBuf<float> r, d0, d1, q, z;
   DBuf<float> dring, qring;  // flexible CG: kFcgDepth directions and products
-  DBuf<double> rho, alpha, pa, pb, bb, zq;
+  DBuf<double> rho, alpha, rz, pa, pb, bb, zq;
   DBuf<int> flag;  // stop raised, stop iteration, device iteration counter
   int* h_flag = nullptr;
   int cap = 0;
@@ -48,12 +48,13 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
     if (tm) tm->end(s);
   };
   if (depth == 1 && mg_shape().smooth0 == 1) {
-    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, G.zq.p, G.qring.p, (int64_t)N_, S.it};
+    const bool pcg = mg_shape().pcg != 0;
+    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, pcg ? nullptr : G.zq.p, G.qring.p, (int64_t)N_, S.it};
     begin(HNS_K_PRECOND);
     mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H);
     end();
     begin(HNS_K_CG_PRODUCT);
-    launch_fcg1_dirspmv(T, it == 0, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+    launch_fcg1_dirspmv(T, it == 0, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s, pcg);
     end();
   } else {
     begin(HNS_K_PRECOND);
@@ -251,6 +252,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   if (G.cap < cap) {
     G.rho.alloc((size_t)cap + 1);
     G.alpha.alloc((size_t)cap + 1);
+    G.rz.alloc((size_t)cap + 1);
     G.cap = cap;
   }
   CgScalars S;
@@ -258,6 +260,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.pb = G.pb.p;
   S.rho = G.rho.p;
   S.alpha = G.alpha.p;
+  S.rz = G.rz.p;
   S.flag = G.flag.p;
   S.it = G.flag.p + 2;
   S.bb = G.bb.p;
