@@ -433,6 +433,37 @@ __device__ __forceinline__ int32_t face_nl(const FaceRow& F, int f) {
 // `over`), z written to zout. The tile holds b (s_b) and z (s_z): z0 is a function of b alone, z1 of
 // the ring's z0, and the core's residual needs z of both colours on the depth-1 slabs, so the same
 // values as the separate sweeps come out bit for bit.
+// The global reads of one leaf's tile (absent leaves read the own leaf, then stage 0), issued a leaf
+// ahead in the persistent passes so their latency overlaps the previous leaf's LDS phases. The
+// staging positions themselves (table words) are the same for every leaf.
+struct Pre0In {
+  int32_t fnl[3], enl;
+  float fv[3], ev, b0, b1;
+};
+__device__ __forceinline__ void mg0_pre_load(const DevTopo& T, int32_t l, const float* __restrict__ b,
+                                             const uint32_t (&fw)[3], uint32_t ew, Pre0In& P) {
+  const FaceRow Fr = face_row(T, l);
+  const int t = threadIdx.x;
+  const uint32_t uh = (uint32_t)(T.L * kHalfLeaf), ul = (uint32_t)l;
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  P.enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    P.fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
+    P.fv[k] = ld32(b, tw_cs(fw[k], P.fnl[k] < 0 ? ul : (uint32_t)P.fnl[k], uh));
+  }
+  P.b0 = ld32(b, (uint32_t)c0 * uh + ul * kHalfLeaf + t);
+  P.b1 = ld32(b, (uint32_t)(c0 ^ 1) * uh + ul * kHalfLeaf + t);
+  P.ev = ld32(b, tw_cs(ew, P.enl < 0 ? ul : (uint32_t)P.enl, uh));
+}
+
+// Pre-smoothing and restriction of the V-cycle's level 0 in one pass per leaf: from z = 0, one
+// red-black Gauss-Seidel iteration on A z = b (colour 0 then colour 1, A = 6 I - neighbours, the
+// arithmetic of k_rbgs with dx2 = -1, omega = 1), the residual b - A z and its restriction to level
+// 1 (times `over`), z written to zout. The tile holds b (s_b) and z (s_z): z0 is a function of b
+// alone, z1 of the ring's z0, and the core's residual needs z of both colours on the depth-1 slabs,
+// so the same values as the separate sweeps come out bit for bit. Persistent: each workgroup walks
+// a strided leaf run.
 __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b,
                                                   float* __restrict__ zout, float over, float* __restrict__ b1,
                                                   const int* __restrict__ stop) {
@@ -440,72 +471,68 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
   __shared__ float s_z[kT12 * kT12 * kT12];
   __shared__ float s_r[kLeafVox];
   if (stop && *stop) return;
-  const int32_t l = leaf_of_block();
-  if (l >= T.L) return;
-  const FaceRow Fr = face_row(T, l);
+  const LeafRun R = leaf_run(T.L);
+  if (R.first >= R.end) return;
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const float dx2 = -1.0f;
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
-  const uint32_t uh = (uint32_t)half, ul = (uint32_t)l;
-  // every load issued before the first LDS store (absent leaves load the own leaf, then read 0)
   const uint32_t ew = kTab12.edge[t < 96 ? t : 0];
-  const int32_t enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
   uint32_t fw[3];
-  int32_t fnl[3];
-  float fv[3];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    fw[k] = kTab12.face[t + k * kTpb];
-    fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
-    fv[k] = ld32(b, tw_cs(fw[k], fnl[k] < 0 ? ul : (uint32_t)fnl[k], uh));
-  }
-  const float b0 = ld32(b, (uint32_t)(c0 * half) + ul * kHalfLeaf + t);
-  const float b1v = ld32(b, (uint32_t)((c0 ^ 1) * half) + ul * kHalfLeaf + t);
-  const float ev = ld32(b, tw_cs(ew, enl < 0 ? ul : (uint32_t)enl, uh));
+  for (int k = 0; k < 3; ++k) fw[k] = kTab12.face[t + k * kTpb];
   auto puti = [&](int i, int col, float bv) {
     s_b[i] = bv;
     s_z[i] = col == 0 ? sor_update(0.f, bv, 0.f, dx2, 1.0f) : 0.f;
   };
-  {
-    const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
-    puti(i0, c0, b0);
-    puti(i0 + 1, c0 ^ 1, b1v);
-  }
+  Pre0In cur;
+  mg0_pre_load(T, R.first, b, fw, ew, cur);
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+    Pre0In nxt = cur;
+    if (l + R.step < R.end) mg0_pre_load(T, l + R.step, b, fw, ew, nxt);
+    const FaceRow Fr = face_row(T, l);
+    {
+      const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
+      puti(i0, c0, cur.b0);
+      puti(i0 + 1, c0 ^ 1, cur.b1);
+    }
 #pragma unroll
-  for (int k = 0; k < 3; ++k) puti(tw_tile(fw[k]), (int)((fw[k] >> 19) & 1u), fnl[k] < 0 ? 0.f : fv[k]);
-  if (t < 96) puti(tw_tile(ew), (int)((ew >> 19) & 1u), enl < 0 ? 0.f : ev);
-  __syncthreads();
-  // colour 1 on the core and the depth-1 slabs (positions in absent leaves stay 0)
-  {
-    const int zc1 = 2 * zz + (c0 ^ 1);  // the lane's colour-1 voxel of the core
-    const int i = t12(x + 2, y + 2, zc1 + 2);
-    s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, 1.0f);
-  }
-  if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
-    const int i = (int)kTab12.ring[t];
-    s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], 0.f, dx2, 1.0f);
-  }
-  __syncthreads();
-  // residual of the core's voxel pair, restriction, and z of the core
-  {
-    const int i0 = t12(x + 2, y + 2, 2 * zz + 2), i1 = i0 + 1;
-    const float u0 = s_z[i0], u1 = s_z[i1];
-    const float sp0 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 2), sp1 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 3);
-    s_r[(x << 6) | (y << 3) | (2 * zz)] = s_b[i0] - (6.0f * u0 - sp0);
-    s_r[(x << 6) | (y << 3) | (2 * zz + 1)] = s_b[i1] - (6.0f * u1 - sp1);
-    zout[c0 * half + l * kHalfLeaf + t] = u0;
-    zout[(c0 ^ 1) * half + l * kHalfLeaf + t] = u1;
-  }
-  __syncthreads();
-  if (t < 64) {
-    const int X = t >> 4, Y = (t >> 2) & 3, Z = t & 3;
-    float sum = 0.f;
+    for (int k = 0; k < 3; ++k) puti(tw_tile(fw[k]), (int)((fw[k] >> 19) & 1u), cur.fnl[k] < 0 ? 0.f : cur.fv[k]);
+    if (t < 96) puti(tw_tile(ew), (int)((ew >> 19) & 1u), cur.enl < 0 ? 0.f : cur.ev);
+    __syncthreads();
+    // colour 1 on the core and the depth-1 slabs (positions in absent leaves stay 0)
+    {
+      const int zc1 = 2 * zz + (c0 ^ 1);  // the lane's colour-1 voxel of the core
+      const int i = t12(x + 2, y + 2, zc1 + 2);
+      s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, 1.0f);
+    }
+    if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
+      const int i = (int)kTab12.ring[t];
+      s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], 0.f, dx2, 1.0f);
+    }
+    __syncthreads();
+    // residual of the core's voxel pair, restriction, and z of the core
+    {
+      const int i0 = t12(x + 2, y + 2, 2 * zz + 2), i1 = i0 + 1;
+      const float u0 = s_z[i0], u1 = s_z[i1];
+      const float sp0 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 2), sp1 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 3);
+      s_r[(x << 6) | (y << 3) | (2 * zz)] = s_b[i0] - (6.0f * u0 - sp0);
+      s_r[(x << 6) | (y << 3) | (2 * zz + 1)] = s_b[i1] - (6.0f * u1 - sp1);
+      zout[c0 * half + l * kHalfLeaf + t] = u0;
+      zout[(c0 ^ 1) * half + l * kHalfLeaf + t] = u1;
+    }
+    __syncthreads();
+    if (t < 64) {
+      const int X = t >> 4, Y = (t >> 2) & 3, Z = t & 3;
+      float sum = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      sum += s_r[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
-    b1[l * 64 + t] = over * sum;
+      for (int k = 0; k < 8; ++k)
+        sum += s_r[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
+      b1[l * 64 + t] = over * sum;
+    }
+    __syncthreads();  // the tile and s_r are rewritten for the next leaf
+    cur = nxt;
   }
 }
 
@@ -513,55 +540,64 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
 // staged on the tile, then one red-black iteration on A z = b in the reverse order (colour 1 on the
 // core and the depth-1 slabs, then colour 0 on the core), the core written to zout (distinct from
 // zin: other workgroups read zin's halo). Bitwise the result of k_mg0_prolong and two k_rbgs sweeps.
-// Returns the lane's final colour-0 / colour-1 values (v0, v1) and their right-hand sides (r0, r1).
-__device__ __forceinline__ void mg0_post_leaf(const DevTopo& T, int32_t l, const float* __restrict__ b,
+// mg0_post_load issues the leaf's global reads; mg0_post_compute stages and updates the tile and
+// returns the lane's final colour-0 / colour-1 values (v0, v1) and their right-hand sides (r0, r1).
+struct Post0In {
+  int32_t fnl[3], enl;
+  float fz[3], fu[3], fb[3], ez, eu, z0, z1, up, b0, b1;
+};
+__device__ __forceinline__ void mg0_post_load(const DevTopo& T, int32_t l, const float* __restrict__ b,
                                               const float* __restrict__ zin, const float* __restrict__ u1,
-                                              float* __restrict__ zout, float* s_b, float* s_z, float& v0,
-                                              float& v1, float& r0, float& r1) {
+                                              const uint32_t (&fw)[3], uint32_t ew, Post0In& P) {
+  const FaceRow Fr = face_row(T, l);
+  const int t = threadIdx.x;
+  const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
+  const int c0 = (x + y) & 1;
+  const uint32_t uh = (uint32_t)(T.L * kHalfLeaf), ul = (uint32_t)l;
+  P.enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    P.fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
+    const uint32_t src = P.fnl[k] < 0 ? ul : (uint32_t)P.fnl[k];
+    const uint32_t i = tw_cs(fw[k], src, uh);
+    P.fz[k] = ld32(zin, i);
+    P.fu[k] = ld32(u1, tw_parent(fw[k], src));
+    P.fb[k] = ld32(b, i);  // used at depth 1
+  }
+  const uint32_t i0g = (uint32_t)c0 * uh + ul * kHalfLeaf + t, i1g = (uint32_t)(c0 ^ 1) * uh + ul * kHalfLeaf + t;
+  P.z0 = ld32(zin, i0g);
+  P.z1 = ld32(zin, i1g);
+  P.up = ld32(u1, ul * 64u + (uint32_t)(((x >> 1) << 4) + ((y >> 1) << 2) + zz));  // the pair's parent
+  P.b0 = ld32(b, i0g);
+  P.b1 = ld32(b, i1g);
+  const uint32_t esrc = P.enl < 0 ? ul : (uint32_t)P.enl;
+  P.ez = ld32(zin, tw_cs(ew, esrc, uh));
+  P.eu = ld32(u1, tw_parent(ew, esrc));
+}
+__device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, const Post0In& P,
+                                                 const uint32_t (&fw)[3], uint32_t ew, float* __restrict__ zout,
+                                                 float* s_b, float* s_z, float& v0, float& v1, float& r0,
+                                                 float& r1) {
   const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const float dx2 = -1.0f;
-  // every load issued before the first LDS store (absent leaves load the own leaf, then read 0);
-  // a staged z is zin plus its parent's value in u1
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
-  const uint32_t uh = (uint32_t)half, ul = (uint32_t)l;
-  const uint32_t ew = kTab12.edge[t < 96 ? t : 0];
-  const int32_t enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
-  uint32_t fw[3];
-  int32_t fnl[3];
-  float fz[3], fu[3], fb[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    fw[k] = kTab12.face[t + k * kTpb];
-    fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
-    const uint32_t src = fnl[k] < 0 ? ul : (uint32_t)fnl[k];
-    const uint32_t i = tw_cs(fw[k], src, uh);
-    fz[k] = ld32(zin, i);
-    fu[k] = ld32(u1, tw_parent(fw[k], src));
-    fb[k] = ld32(b, i);  // used at depth 1
-  }
-  const uint32_t i0g = (uint32_t)(c0 * half) + ul * kHalfLeaf + t, i1g = (uint32_t)((c0 ^ 1) * half) + ul * kHalfLeaf + t;
-  const float z0 = ld32(zin, i0g), z1 = ld32(zin, i1g);
-  const float up = ld32(u1, ul * 64u + (uint32_t)(((x >> 1) << 4) + ((y >> 1) << 2) + zz));  // the pair's parent
-  const float b0 = ld32(b, i0g), b1v = ld32(b, i1g);
-  const uint32_t esrc = enl < 0 ? ul : (uint32_t)enl;
-  const float ez = ld32(zin, tw_cs(ew, esrc, uh)), eu = ld32(u1, tw_parent(ew, esrc));
   {
     const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
-    s_z[i0] = z0 + up;
-    s_z[i0 + 1] = z1 + up;
-    s_b[i0] = b0;
-    s_b[i0 + 1] = b1v;
+    s_z[i0] = P.z0 + P.up;
+    s_z[i0 + 1] = P.z1 + P.up;
+    s_b[i0] = P.b0;
+    s_b[i0 + 1] = P.b1;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int i = tw_tile(fw[k]);
-    s_z[i] = fnl[k] < 0 ? 0.f : fz[k] + fu[k];
-    s_b[i] = fnl[k] < 0 ? 0.f : fb[k];  // read at depth 1 only
+    s_z[i] = P.fnl[k] < 0 ? 0.f : P.fz[k] + P.fu[k];
+    s_b[i] = P.fnl[k] < 0 ? 0.f : P.fb[k];  // read at depth 1 only
   }
-  if (t < 96) s_z[tw_tile(ew)] = enl < 0 ? 0.f : ez + eu;
+  if (t < 96) s_z[tw_tile(ew)] = P.enl < 0 ? 0.f : P.ez + P.eu;
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs
   {
@@ -584,6 +620,12 @@ __device__ __forceinline__ void mg0_post_leaf(const DevTopo& T, int32_t l, const
   zout[l * kHalfLeaf + t] = v0;         // colour 0 plane: the lane's colour-0 voxel
   zout[half + l * kHalfLeaf + t] = v1;  // colour 1 plane
 }
+__device__ __forceinline__ void lane_tables(uint32_t (&fw)[3], uint32_t& ew) {
+  const int t = threadIdx.x;
+  ew = kTab12.edge[t < 96 ? t : 0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) fw[k] = kTab12.face[t + k * kTpb];
+}
 
 // One workgroup per leaf (the V-cycle on its own).
 __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
@@ -594,8 +636,12 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
   if (stop && *stop) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
+  uint32_t fw[3], ew;
+  lane_tables(fw, ew);
+  Post0In P;
+  mg0_post_load(T, l, b, zin, u1, fw, ew, P);
   float v0, v1, r0, r1;
-  mg0_post_leaf(T, l, b, zin, u1, zout, s_b, s_z, v0, v1, r0, r1);
+  mg0_post_compute(T, l, P, fw, ew, zout, s_b, s_z, v0, v1, r0, r1);
 }
 
 // Inside flexible CG (truncation 1): the same pass over a strided leaf run per workgroup
@@ -614,9 +660,13 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
   const float* __restrict__ qp = H.Qr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * H.n;
   const LeafRun R = leaf_run(T.L);
   double zr = 0.0, zq = 0.0;
+  uint32_t fw[3], ew;
+  lane_tables(fw, ew);
   for (int32_t l = R.first; l < R.end; l += R.step) {
+    Post0In cur;  // (loading a leaf ahead here costs more in occupancy than it hides)
+    mg0_post_load(T, l, b, zin, u1, fw, ew, cur);
     float v0, v1, r0, r1;
-    mg0_post_leaf(T, l, b, zin, u1, zout, s_b, s_z, v0, v1, r0, r1);
+    mg0_post_compute(T, l, cur, fw, ew, zout, s_b, s_z, v0, v1, r0, r1);
     zr += (double)v0 * (double)r0 + (double)v1 * (double)r1;
     if (H.zq && it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
                               (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
@@ -964,7 +1014,7 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
   if (hook && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the flexible-CG pass needs the fused level-0 V-cycle");
   if (fused) {
     // level 0 pre-smoothing and restriction in one pass, into the scratch level-0 vector
-    k_mg0_pre<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, stop);
+    k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, stop);
     HNS_HIP(hipGetLastError());
   } else {
     // level 0: z = 0, red-black pre-smoothing (Gauss-Seidel on A z = r)
