@@ -347,23 +347,32 @@ __global__ __launch_bounds__(512) void k_vorticity_forces_lds(DevTopo T, const f
                                                               float gravity) {
   __shared__ int32_t s_nb[27];
   __shared__ float s_u[3 * kE3];
+  __shared__ float s_mag[10 * 10 * 10];  // |curl| on the tile's inner 10^3 (the leaf and a 1-voxel ring)
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const LeafView V = load_leaf(T, l, s_nb);
   const int32_t N = T.L * kLeafVox;
   const float zero[3] = {0.f, 0.f, 0.f};
   fill_tile<3>(s_u, u, N, s_nb, l, zero);
+  const float factor = 0.5f * inv_dx;
+  // each |curl| once (the gradient below reads six of them per voxel); the same function on the same
+  // tile values as evaluating it at the neighbour, so the bits do not change
+  for (int m = threadIdx.x; m < 1000; m += 512) {
+    const int mx = m / 100, my = (m / 10) % 10, mz = m % 10;
+    s_mag[m] = tile_curl_mag(s_u, tix(mx + 1, my + 1, mz + 1), factor);
+  }
+  __syncthreads();
   const int32_t n = threadIdx.x;
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
   const int32_t idx = l * kLeafVox + n;
   const int ct = tix(lx + kH, ly + kH, lz + kH);
-  const float factor = 0.5f * inv_dx;
   float3 v = tvec(s_u, ct);
   const float3 w = tile_curl(s_u, ct, factor);
-  const int sx = F * kE * kE, sy = F * kE, sz = F;  // F is +-1 here
-  const float mpX = tile_curl_mag(s_u, ct + sx, factor), mmX = tile_curl_mag(s_u, ct - sx, factor);
-  const float mpY = tile_curl_mag(s_u, ct + sy, factor), mmY = tile_curl_mag(s_u, ct - sy, factor);
-  const float mpZ = tile_curl_mag(s_u, ct + sz, factor), mmZ = tile_curl_mag(s_u, ct - sz, factor);
+  const int cm = ((lx + 1) * 10 + (ly + 1)) * 10 + (lz + 1);
+  const int sx = F * 100, sy = F * 10, sz = F;  // F is +-1 here
+  const float mpX = s_mag[cm + sx], mmX = s_mag[cm - sx];
+  const float mpY = s_mag[cm + sy], mmY = s_mag[cm - sy];
+  const float mpZ = s_mag[cm + sz], mmZ = s_mag[cm - sz];
   const float gx = (mpX - mmX) * 0.5f * inv_dx;
   const float gy = (mpY - mmY) * 0.5f * inv_dx;
   const float gz = (mpZ - mmZ) * 0.5f * inv_dx;
