@@ -14,13 +14,17 @@ constexpr int kMgLevels = 4;
 // truncation around the V-cycle, and the scale of every coarse-grid correction (piecewise-constant
 // prolongation under-corrects smooth errors; over-correcting by 1.6 took the 8 M-voxel fire solve
 // from 16 to 12 iterations). Defaults are the measured best at 8 M voxels; HNS_MG_COARSE,
-// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH, HNS_MG_OVER (x100) and
-// HNS_MG_OVER_DEEP (x100; default: HNS_MG_OVER) override them (read once per process).
+// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH, HNS_MG_OVER (x100),
+// HNS_MG_OVER_DEEP (x100; default: HNS_MG_OVER), HNS_MG_OMEGA0 and HNS_MG_OMEGAL (x100) and
+// HNS_MG_PCG (Fletcher-Reeves PCG instead of FCG(1)) override them (read once per process).
 struct MgShape {
   int coarse = 12, smooth0 = 1, smooth = 1, depth = 1, pcg = 0;
   float coarse_omega = 1.85f;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
   float over_deep = 1.6f;  // the same below level 1 (restrictions of levels 1 and 2)
+  // smoothing relaxation on level 0 and on levels 1-2 (over-relaxed red-black smoothing: 12 -> 10
+  // iterations at 8 M voxels; 10 held for 1.1-1.3 and 1.35-1.4)
+  float omega0 = 1.1f, omega_l = 1.4f;
 };
 const MgShape& mg_shape();
 
@@ -90,9 +94,9 @@ struct Multigrid {
   unsigned key_variant_ = 0;
 };
 
-// One red-black Gauss-Seidel colour sweep (ω = 1) on A u = b, colour-split (the RBGS kernel with
+// One red-black SOR colour sweep (relaxation omega) on A u = b, colour-split (the RBGS kernel with
 // the right-hand side taken as b); `first`: u starts at zero; returns at once while *stop != 0.
-void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, const int* stop,
-                     hipStream_t s);
+void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, float omega,
+                     const int* stop, hipStream_t s);
 
 }  // namespace hns

@@ -33,6 +33,8 @@ const MgShape& mg_shape() {
     m.smooth = env("HNS_MG_SMOOTH", 1, 8, m.smooth);
     m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
     m.pcg = env("HNS_MG_PCG", 0, 1, m.pcg);
+    m.omega0 = 0.01f * (float)env("HNS_MG_OMEGA0", 50, 199, (int)(100.0f * m.omega0 + 0.5f));
+    m.omega_l = 0.01f * (float)env("HNS_MG_OMEGAL", 50, 199, (int)(100.0f * m.omega_l + 0.5f));
     m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
     m.over = 0.01f * (float)env("HNS_MG_OVER", 50, 300, (int)(100.0f * m.over + 0.5f));
     m.over_deep = 0.01f * (float)env("HNS_MG_OVER_DEEP", 50, 300, (int)(100.0f * m.over + 0.5f));
@@ -466,7 +468,7 @@ __device__ __forceinline__ void mg0_pre_load(const DevTopo& T, int32_t l, const 
 // a strided leaf run.
 __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b,
                                                   float* __restrict__ zout, float over, float* __restrict__ b1,
-                                                  const int* __restrict__ stop) {
+                                                  float w, const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
   __shared__ float s_z[kT12 * kT12 * kT12];
   __shared__ float s_r[kLeafVox];
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
   for (int k = 0; k < 3; ++k) fw[k] = kTab12.face[t + k * kTpb];
   auto puti = [&](int i, int col, float bv) {
     s_b[i] = bv;
-    s_z[i] = col == 0 ? sor_update(0.f, bv, 0.f, dx2, 1.0f) : 0.f;
+    s_z[i] = col == 0 ? sor_update(0.f, bv, 0.f, dx2, w) : 0.f;
   };
   Pre0In cur;
   mg0_pre_load(T, R.first, b, fw, ew, cur);
@@ -505,11 +507,11 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
     {
       const int zc1 = 2 * zz + (c0 ^ 1);  // the lane's colour-1 voxel of the core
       const int i = t12(x + 2, y + 2, zc1 + 2);
-      s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, 1.0f);
+      s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, w);
     }
     if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
       const int i = (int)kTab12.ring[t];
-      s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], 0.f, dx2, 1.0f);
+      s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], 0.f, dx2, w);
     }
     __syncthreads();
     // residual of the core's voxel pair, restriction, and z of the core
@@ -576,7 +578,7 @@ __device__ __forceinline__ void mg0_post_load(const DevTopo& T, int32_t l, const
 }
 __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, const Post0In& P,
                                                  const uint32_t (&fw)[3], uint32_t ew, float* __restrict__ zout,
-                                                 float* s_b, float* s_z, float& v0, float& v1, float& r0,
+                                                 float* s_b, float* s_z, float w, float& v0, float& v1, float& r0,
                                                  float& r1) {
   const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
@@ -603,17 +605,17 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
   {
     const int zc1 = 2 * zz + (c0 ^ 1);
     const int i = t12(x + 2, y + 2, zc1 + 2);
-    s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], s_z[i], dx2, 1.0f);
+    s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], s_z[i], dx2, w);
   }
   if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
     const int i = (int)kTab12.ring[t];
-    s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], s_z[i], dx2, 1.0f);
+    s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], s_z[i], dx2, w);
   }
   __syncthreads();
   // colour 0 on the core, then the core out
   const int zc0 = 2 * zz + c0;
   const int i = t12(x + 2, y + 2, zc0 + 2), i1 = t12(x + 2, y + 2, 2 * zz + (c0 ^ 1) + 2);
-  v0 = sor_update(t12_sum(s_z, x + 2, y + 2, zc0 + 2), s_b[i], s_z[i], dx2, 1.0f);
+  v0 = sor_update(t12_sum(s_z, x + 2, y + 2, zc0 + 2), s_b[i], s_z[i], dx2, w);
   v1 = s_z[i1];
   r0 = s_b[i];
   r1 = s_b[i1];
@@ -630,7 +632,7 @@ __device__ __forceinline__ void lane_tables(uint32_t (&fw)[3], uint32_t& ew) {
 // One workgroup per leaf (the V-cycle on its own).
 __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
                                                    const float* __restrict__ zin, const float* __restrict__ u1,
-                                                   float* __restrict__ zout, const int* __restrict__ stop) {
+                                                   float* __restrict__ zout, float w, const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
   __shared__ float s_z[kT12 * kT12 * kT12];
   if (stop && *stop) return;
@@ -641,7 +643,7 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
   Post0In P;
   mg0_post_load(T, l, b, zin, u1, fw, ew, P);
   float v0, v1, r0, r1;
-  mg0_post_compute(T, l, P, fw, ew, zout, s_b, s_z, v0, v1, r0, r1);
+  mg0_post_compute(T, l, P, fw, ew, zout, s_b, s_z, w, v0, v1, r0, r1);
 }
 
 // Inside flexible CG (truncation 1): the same pass over a strided leaf run per workgroup
@@ -649,7 +651,7 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
 // z . q_prev for the direction and step length.
 __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* __restrict__ b,
                                                        const float* __restrict__ zin, const float* __restrict__ u1,
-                                                       float* __restrict__ zout, McgHook H,
+                                                       float* __restrict__ zout, McgHook H, float w,
                                                        const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
   __shared__ float s_z[kT12 * kT12 * kT12];
@@ -666,7 +668,7 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
     Post0In cur;  // (loading a leaf ahead here costs more in occupancy than it hides)
     mg0_post_load(T, l, b, zin, u1, fw, ew, cur);
     float v0, v1, r0, r1;
-    mg0_post_compute(T, l, cur, fw, ew, zout, s_b, s_z, v0, v1, r0, r1);
+    mg0_post_compute(T, l, cur, fw, ew, zout, s_b, s_z, w, v0, v1, r0, r1);
     zr += (double)v0 * (double)r0 + (double)v1 * (double)r1;
     if (H.zq && it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
                               (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
@@ -754,7 +756,7 @@ __device__ __forceinline__ void ltile_c1(const FaceRow& Fr, int lane, int cx, in
 
 template <int LG>
 __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __restrict__ b, float* __restrict__ zout,
-                                                  float scale, float over, float* __restrict__ bc,
+                                                  float scale, float over, float* __restrict__ bc, float w,
                                                   const int* __restrict__ stop) {
   using Tl = LTile<LG>;
   __shared__ float s_b[Tl::LPW][Tl::N];
@@ -774,7 +776,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
     auto put = [&](int tx, int ty, int tz, float bv) {
       const int i = Tl::at(tx, ty, tz);
       B[i] = bv;
-      Z[i] = ((tx + ty + tz) & 1) == 0 ? 1.0f * ((bv * inv_s + 0.f) * (1.0f / 6.0f)) : 0.f;
+      Z[i] = ((tx + ty + tz) & 1) == 0 ? w * ((bv * inv_s + 0.f) * (1.0f / 6.0f)) : 0.f;
     };
     // every load issued before the first LDS store (absent leaves load the own leaf, then read 0)
     TilePos fp[Tl::NF], ep[Tl::NE];
@@ -801,7 +803,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
     ltile_c1<LG>(Fr, lane, cx, cy, cz, [&](int tx, int ty, int tz) {
       const int i = Tl::at(tx, ty, tz);
       const float gs = (B[i] * inv_s + Tl::nsum(Z, tx, ty, tz)) * (1.0f / 6.0f);
-      Z[i] = 1.0f * gs;
+      Z[i] = w * gs;
     });
   __syncthreads();
   if (live) {
@@ -828,7 +830,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
 template <int LG>
 __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __restrict__ b,
                                                    const float* __restrict__ zin, const float* __restrict__ uc,
-                                                   float* __restrict__ zout, float scale,
+                                                   float* __restrict__ zout, float scale, float w,
                                                    const int* __restrict__ stop) {
   using Tl = LTile<LG>;
   __shared__ float s_b[Tl::LPW][Tl::N];
@@ -882,7 +884,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __res
   auto sweep = [&](int tx, int ty, int tz) {
     const int i = Tl::at(tx, ty, tz);
     const float gs = (B[i] * inv_s + Tl::nsum(Z, tx, ty, tz)) * (1.0f / 6.0f);
-    Z[i] = fmaf(1.0f, gs - Z[i], Z[i]);
+    Z[i] = fmaf(w, gs - Z[i], Z[i]);
   };
   if (live) ltile_c1<LG>(Fr, lane, cx, cy, cz, sweep);
   __syncthreads();
@@ -1014,13 +1016,13 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
   if (hook && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the flexible-CG pass needs the fused level-0 V-cycle");
   if (fused) {
     // level 0 pre-smoothing and restriction in one pass, into the scratch level-0 vector
-    k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, stop);
+    k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, M.omega0, stop);
     HNS_HIP(hipGetLastError());
   } else {
     // level 0: z = 0, red-black pre-smoothing (Gauss-Seidel on A z = r)
     for (int k = 0; k < M.smooth0; ++k) {
-      launch_rbgs_rhs(T, r_cs, z_cs, 0, k == 0, stop, s);
-      launch_rbgs_rhs(T, r_cs, z_cs, 1, k == 0, stop, s);
+      launch_rbgs_rhs(T, r_cs, z_cs, 0, k == 0, M.omega0, stop, s);
+      launch_rbgs_rhs(T, r_cs, z_cs, 1, k == 0, M.omega0, stop, s);
     }
     k_mg0_restrict<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, z_cs, M.over, b[1].p, stop);
     HNS_HIP(hipGetLastError());
@@ -1032,14 +1034,15 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     const float scale = (float)(1 << (2 * l));
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
-      if (lgS == 2) k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, stop);
-      else k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, stop);
+      if (lgS == 2)
+        k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, stop);
+      else k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, stop);
       HNS_HIP(hipGetLastError());
       continue;
     }
     for (int k = 0; k < M.smooth; ++k) {
-      smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, 1.0f, stop, s);
-      smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, 1.0f, stop, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, M.omega_l, stop, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, M.omega_l, stop, s);
     }
     k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, M.over_deep, b[l + 1].p,
                                                         stop);
@@ -1055,29 +1058,29 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     const float scale = (float)(1 << (2 * l));
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
-      if (lgS == 2) k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, stop);
-      else k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, stop);
+      if (lgS == 2) k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, stop);
+      else k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, stop);
       HNS_HIP(hipGetLastError());
       continue;
     }
     k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p, stop);
     HNS_HIP(hipGetLastError());
     for (int k = 0; k < M.smooth; ++k) {
-      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, 1.0f, stop, s);
-      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, 1.0f, stop, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, M.omega_l, stop, s);
+      smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, M.omega_l, stop, s);
     }
   }
   if (fused) {
-    if (hook) k_mg0_post_fcg<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, *hook, stop);
-    else k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, stop);
+    if (hook) k_mg0_post_fcg<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, *hook, M.omega0, stop);
+    else k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, M.omega0, stop);
     HNS_HIP(hipGetLastError());
     return;
   }
   k_mg0_prolong<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T.L, z_cs, u[1].p, stop);
   HNS_HIP(hipGetLastError());
   for (int k = 0; k < M.smooth0; ++k) {
-    launch_rbgs_rhs(T, r_cs, z_cs, 1, false, stop, s);
-    launch_rbgs_rhs(T, r_cs, z_cs, 0, false, stop, s);
+    launch_rbgs_rhs(T, r_cs, z_cs, 1, false, M.omega0, stop, s);
+    launch_rbgs_rhs(T, r_cs, z_cs, 0, false, M.omega0, stop, s);
   }
 }
 

@@ -269,9 +269,9 @@ int rbgs_lpw() {
 
 // The sweep on A u = b with A = 6 I - (face neighbours): div = b and dx^2 = -1 make the SOR target
 // (sum - div dx^2) / 6 exactly (sum + b) / 6; omega = 1 (Gauss-Seidel).
-void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, const int* stop,
-                     hipStream_t s) {
-  launch_rbgs_lpw<1>(T, b_cs, u_cs, -1.0f, colour, 1.0f, first, s, stop);
+void launch_rbgs_rhs(const DevTopo& T, const float* b_cs, float* u_cs, int colour, bool first, float omega,
+                     const int* stop, hipStream_t s) {
+  launch_rbgs_lpw<1>(T, b_cs, u_cs, -1.0f, colour, omega, first, s, stop);
   HNS_HIP(hipGetLastError());
 }
 
