@@ -86,6 +86,7 @@ struct CgScalars {
   double* rz;     // multigrid PCG: r_k . z_k, max_iters + 1 entries
   int* flag;      // 2 entries: stop raised, stop iteration
   int* it;        // flexible CG: the device's iteration counter (0 after init, advanced by the test)
+  double* last;   // flexible CG: r.r and ||b||^2 of the last stop test
   double tol2;    // stop when r.r <= tol2 * ||b||^2
   double* bb;     // ||b||^2
 };
@@ -130,6 +131,10 @@ constexpr int kFcgBatch = 4;
 // iteration's test; it moves *S.it on), false before iteration 0 (it records ||b||^2) and after a
 // residual replacement (the same iteration tested again).
 void launch_fcg_test(int32_t L, bool advance, const CgScalars& S, hipStream_t s);
+// If the stop was raised: r = b - A x (FP64-evaluated) and the same iteration tested again on it;
+// the flag then stands only if the replaced residual passes. Nothing when no stop was raised.
+void launch_fcg_retest(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
+                       hipStream_t s);
 // new direction and its product (three launches), orthogonalised against the last
 // min(it, depth) directions (1 <= depth <= kFcgDepth); `it` (the host's count) selects the kernels
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,

@@ -73,7 +73,8 @@ __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __r
                                                      CgScalars S, int force) {
   __shared__ double s_red[kTpb / 64];
   __shared__ double s_tile[2][kPairTile];
-  if (!force && S.flag[0]) return;
+  if (force == 0 && S.flag[0]) return;  // periodic: not once the solve has stopped
+  if (force == 2 && !S.flag[0]) return;  // re-test of a stop: only when one was raised
   const LeafRun R = leaf_run(T.L);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
@@ -100,7 +101,7 @@ __global__ __launch_bounds__(kTpb) void k_cg_replace(DevTopo T, const float* __r
   }
   const double tt = block_sum_all(acc, s_red);
   if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
-  if (force) clear_flags(S);
+  if (force == 1) clear_flags(S);
 }
 
 // Iteration `it` (0-based), product half: reduce r.r from pa; stop when converged;
@@ -222,15 +223,20 @@ __global__ __launch_bounds__(kTpb) void k_cg_update(int32_t L, int32_t it, float
 // G partials of r.r (G = the CG grid) and raises the stop flag, on which every later kernel of the
 // iteration, the preconditioner's included, returns at once. advance: first move the device
 // iteration counter on (the test that follows an update); iteration 0 records ||b||^2.
-__global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int advance, CgScalars S) {
+// retest: only after a stop was raised (and r replaced by b - A x since): the stop is lowered and
+// the same iteration tested again on the replaced residual.
+__global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int advance, int retest, CgScalars S) {
   __shared__ double s_red[kTpb / 64];
-  if (S.flag[0]) return;
+  if (retest ? !S.flag[0] : S.flag[0]) return;
   const int32_t it = *S.it + advance;
   double rr1[1];
   reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, G, s_red, rr1);  // every lane has read *S.it
   const double rr = rr1[0];
   const double bb = it == 0 ? rr : S.bb[0];
   if (threadIdx.x == 0) {
+    if (retest) S.flag[0] = 0;
+    S.last[0] = rr;
+    S.last[1] = bb;
     if (advance) *S.it = it;
     if (it == 0) S.bb[0] = bb;
     if (rr <= S.tol2 * bb) {
@@ -462,7 +468,14 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg_test(int32_t L, bool advance, const CgScalars& S, hipStream_t s) {
-  k_fcg_test<<<1, kTpb, 0, s>>>((int)cg_groups(L), advance ? 1 : 0, S);
+  k_fcg_test<<<1, kTpb, 0, s>>>((int)cg_groups(L), advance ? 1 : 0, 0, S);
+  HNS_HIP(hipGetLastError());
+}
+void launch_fcg_retest(const DevTopo& T, const float* div, float dx, const float* x, float* r, const CgScalars& S,
+                       hipStream_t s) {
+  k_cg_replace<<<cg_groups(T.L), kTpb, 0, s>>>(T, div, dx * dx, x, r, S, 2);
+  HNS_HIP(hipGetLastError());
+  k_fcg_test<<<1, kTpb, 0, s>>>((int)cg_groups(T.L), 0, 1, S);
   HNS_HIP(hipGetLastError());
 }
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,

@@ -16,9 +16,10 @@ namespace hns {
 struct CgState {
   DBuf<float> r, d0, d1, q, z;
   DBuf<float> dring, qring;  // flexible CG: kFcgDepth directions and products
-  DBuf<double> rho, alpha, rz, pa, pb, bb, zq;
+  DBuf<double> rho, alpha, rz, pa, pb, bb, zq, last;
   DBuf<int> flag;  // stop raised, stop iteration, device iteration counter
   int* h_flag = nullptr;
+  double* h_last = nullptr;
   int cap = 0;
   // kFcgBatch multigrid-preconditioned iterations captured in one graph, and what it was made for
   hipStream_t cap_stream = nullptr;
@@ -32,6 +33,7 @@ struct CgState {
     if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     if (h_flag) (void)hipHostFree(h_flag);
+    if (h_last) (void)hipHostFree(h_last);
   }
 };
 
@@ -247,7 +249,9 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.pb.alloc(kCgMaxGroups);
     G.bb.alloc(1);
     G.flag.alloc(3);
-    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 2 * sizeof(int), hipHostMallocDefault));
+    G.last.alloc(2);
+    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 3 * sizeof(int), hipHostMallocDefault));
+    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_last), 2 * sizeof(double), hipHostMallocDefault));
   }
   if (G.cap < cap) {
     G.rho.alloc((size_t)cap + 1);
@@ -263,6 +267,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.rz = G.rz.p;
   S.flag = G.flag.p;
   S.it = G.flag.p + 2;
+  S.last = G.last.p;
   S.bb = G.bb.p;
   S.tol2 = (double)r.tol * (double)r.tol;
   tm.begin(HNS_K_OTHER, s);
@@ -280,12 +285,14 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   // a preconditioned iteration costs far more than a sync: check the stop flag every kFcgBatch
   // iterations, in batches aligned to multiples of kFcgBatch (the multigrid solve replays whole
   // aligned batches from one graph)
-  const int batch0 = pre ? kFcgBatch : 8, batch_max = pre ? kFcgBatch : 128;
+  // (the multigrid solve's first read comes after 2 batches: it rarely stops before, and iterations
+  // past a stop return at once)
+  const int batch0 = mg ? 2 * kFcgBatch : (pre ? kFcgBatch : 8), batch_max = pre ? kFcgBatch : 128;
   int it = 0, batch = batch0, replaced_at = 0;
   bool stopped = false;
   while (it < cap) {
     int nb = std::min(batch, cap - it);
-    if (pre) nb = std::min(nb, kFcgBatch - it % kFcgBatch);
+    if (pre && it % kFcgBatch != 0) nb = std::min(nb, kFcgBatch - it % kFcgBatch);
     for (int k = 0; k < nb;) {
       if (it > 0 && it % kResidualReplace == 0 && replaced_at != it) {
         tm.begin(HNS_K_OTHER, s);
@@ -340,6 +347,23 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       ++it;
       ++k;
     }
+    if (pre) {
+      // a stop raised in the batch is re-tested on r = b - A x before the host looks, so one read
+      // decides: stopped, or continue from the device's iteration
+      tm.begin(HNS_K_OTHER, s);
+      launch_fcg_retest(T, div, r.dx, p, G.r.p, S, s);
+      tm.end(s);
+      HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+      HNS_HIP(hipMemcpyAsync(G.h_last, G.last.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+      HNS_HIP(hipStreamSynchronize(s));
+      if (G.h_flag[0]) {
+        stopped = true;
+        break;
+      }
+      it = G.h_flag[2];
+      batch = kFcgBatch;
+      continue;
+    }
     HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
     HNS_HIP(hipStreamSynchronize(s));
     if (G.h_flag[0]) {
@@ -353,25 +377,19 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       tm.end(s);
       replaced_at = it;
       batch = batch0;
-      if (pre) {
-        // re-test the stop on the replaced residual alone before paying for another iteration
-        tm.begin(HNS_K_OTHER, s);
-        launch_fcg_test(T.L, false, S, s);
-        tm.end(s);
-        HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-        HNS_HIP(hipStreamSynchronize(s));
-        if (G.h_flag[0]) {
-          stopped = true;
-          break;
-        }
-      }
       continue;
     }
     batch = std::min(batch * 2, batch_max);
   }
   SolveResult res;
   res.iterations = stopped ? G.h_flag[1] : it;
-  res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
+  if (pre && stopped) {
+    // the stop stands on r = b - A x evaluated in FP64 (its r.r and ||b||^2 came with the flag)
+    const double rr = G.h_last[0], bb = G.h_last[1];
+    res.rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : (rr > 0.0 ? INFINITY : 0.0);
+  } else {
+    res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
+  }
   // plain RBGS is the fallback of the learned preconditioner: continue from the PCG iterate
   if (pre && !(res.rel_residual <= (double)r.tol)) {
     const int chunk = 50, cap_fb = 200000;
