@@ -217,6 +217,8 @@ __device__ inline bool setup_interp_tile(const LeafView& V, float x, float y, fl
 
 // The velocity tile and all M field tiles are filled in one pass (dynamic LDS: (3 + M) 12^3
 // floats), so the loads of every tile are in flight together and there is one barrier.
+// M = F.m fields (a template argument: every field loop unrolls without run-time checks)
+template <int M>
 __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const float* __restrict__ u, FieldSet F,
                                                             float sdt, const float* __restrict__ sdf) {
   __shared__ int32_t s_nb[27];
@@ -231,7 +233,7 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
     // advect_scalars reads element 0 on a miss (velocity and fields alike): an absent cell loads
     // element 0 of each plane. Every load of the tile is issued before the first LDS store.
     constexpr int kIt = (kE3 + 511) / 512;
-    float v[kIt][3 + FieldSet::kMax];
+    float v[kIt][3 + M];
 #pragma unroll
     for (int it = 0; it < kIt; ++it) {
       const int c = min((int)threadIdx.x + it * 512, kE3 - 1);
@@ -243,15 +245,14 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
       v[it][1] = u[N + g];
       v[it][2] = u[2 * N + g];
 #pragma unroll
-      for (int s = 0; s < FieldSet::kMax; ++s) v[it][3 + s] = s < F.m ? F.in[s][g] : 0.f;
+      for (int s = 0; s < M; ++s) v[it][3 + s] = F.in[s][g];
     }
 #pragma unroll
     for (int it = 0; it < kIt; ++it) {
       const int c = (int)threadIdx.x + it * 512;
       if (c < kE3) {
 #pragma unroll
-        for (int k = 0; k < 3 + FieldSet::kMax; ++k)
-          if (k < 3 + F.m) s_u[k * kE3 + c] = v[it][k];
+        for (int k = 0; k < 3 + M; ++k) s_u[k * kE3 + c] = v[it][k];
       }
     }
     __syncthreads();
@@ -292,7 +293,8 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   const bool ft = setup_interp_tile(V, fx, fy, fz, fd);
   const int nct[6] = {ct - kE * kE, ct + kE * kE, ct - kE, ct + kE, ct - 1, ct + 1};
   const bool wave_tile = __all(bt && ft);
-  for (int s = 0; s < F.m; ++s) {
+#pragma unroll
+  for (int s = 0; s < M; ++s) {
     const float* __restrict__ d = F.in[s];
     const float* s_f = s_dyn + (3 + s) * kE3;
     const float phiOrig = s_f[ct];
@@ -417,7 +419,15 @@ void launch_advect_scalars_lds(const DevTopo& T, const float* u, const FieldSet&
       g.out[q] = f.out[b + q];
     }
     const size_t lds = (size_t)(3 + g.m) * kE3 * sizeof(float);
-    k_advect_scalars_lds<<<grid_for_leaves(T.L), 512, lds, s>>>(T, u, g, dt * inv_dx, sdf);
+    const unsigned grid = grid_for_leaves(T.L);
+    const float sdt = dt * inv_dx;
+    switch (g.m) {
+      case 1: k_advect_scalars_lds<1><<<grid, 512, lds, s>>>(T, u, g, sdt, sdf); break;
+      case 2: k_advect_scalars_lds<2><<<grid, 512, lds, s>>>(T, u, g, sdt, sdf); break;
+      case 3: k_advect_scalars_lds<3><<<grid, 512, lds, s>>>(T, u, g, sdt, sdf); break;
+      case 4: k_advect_scalars_lds<4><<<grid, 512, lds, s>>>(T, u, g, sdt, sdf); break;
+      default: k_advect_scalars_lds<5><<<grid, 512, lds, s>>>(T, u, g, sdt, sdf); break;
+    }
     HNS_HIP(hipGetLastError());
   }
 }
