@@ -122,3 +122,39 @@ def test_vcycle_variants_match(hip, shape, keep):
         print("  graph %d coarse-launches %d unfused %d: differing elements %d of %d" % (
             graph, coarse, unfused, n, len(coords)))
         assert n == 0
+
+
+def test_mgcg_step_graph_batches_match_eager_iterations(hip):
+    """The fire step with the multigrid solve: iterations replayed four at a time from a captured graph
+    (timing off) and launched one by one (timing on, so events can bracket them) give bitwise the
+    same state, over steps that reach the graph batches."""
+    sc = scenes.control64()
+    _case("mgcg step, graph batches vs eager, control64 N=%d" % sc.n)
+
+    def run(timing):
+        d = hns.GridIndexedData(sc.coords)
+        for k, v in sc.fields.items():
+            d.add_value_block(k, v.copy())
+        d.add_value_block("velocity", sc.velocity.copy())
+        topo = hns.create_index_grid(d, sc.voxel_size)
+        ctx = hns.ensure_context(None, d, topo, sc.voxel_size)
+        ctx.set_timing(timing)
+        p = hns.make_params(**sc.params)
+        its = [hns.advance(ctx, 0, sc.dt, p, solver="mgcg", tol=1e-5).iterations for _ in range(2)]
+        out = hns.GridIndexedData(sc.coords)
+        names = [k for k in sc.fields if ctx.has_field(k)]
+        for k in names:
+            out.add_value_block(k, np.zeros(sc.n, np.float32))
+        out.add_value_block("velocity", np.zeros((sc.n, 3), np.float32))
+        hns.download_for_output(ctx, out)
+        hns.destroy_context(ctx)
+        return its, {k: out[k].copy() for k in names + ["velocity"]}
+
+    its_g, a = run(False)
+    its_e, b = run(True)
+    print("  iterations per step: graph %s, eager %s" % (its_g, its_e))
+    assert its_g == its_e and min(its_g) >= 5  # iterations 4-7 come from the graph
+    for k in a:
+        n = int(np.sum(a[k] != b[k]))
+        print("  %-12s differing elements %d of %d" % (k, n, a[k].size))
+        assert n == 0
