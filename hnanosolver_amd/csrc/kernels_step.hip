@@ -361,6 +361,7 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
                                                            const float* __restrict__ p, float* __restrict__ out,
                                                            float inv_dx, const float* __restrict__ sdf) {
   __shared__ int32_t s_nb[27];
+  __shared__ float s_p[10 * 10 * 10];  // p of the leaf (local -1..8 at 0..9) with its six face halos
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const LeafView V = load_leaf(T, l, s_nb);
@@ -370,9 +371,34 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
   const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
   const int32_t idx = l * kLeafVox + n;
   const FaceRow Fr = face_row(T, l);
-  const float pxp = load_s(p, face_cs<0, 1>(Fr, lx, ly, lz, half)), pxm = load_s(p, face_cs<0, -1>(Fr, lx, ly, lz, half));
-  const float pyp = load_s(p, face_cs<1, 1>(Fr, lx, ly, lz, half)), pym = load_s(p, face_cs<1, -1>(Fr, lx, ly, lz, half));
-  const float pzp = load_s(p, face_cs<2, 1>(Fr, lx, ly, lz, half)), pzm = load_s(p, face_cs<2, -1>(Fr, lx, ly, lz, half));
+  auto at = [](int a, int b, int c) { return (a * 10 + b) * 10 + c; };
+  {
+    // the lane's own voxel, and lanes < 384 one face-halo voxel each (absent leaves read 0)
+    const float own = p[((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz)];
+    float hv = 0.f;
+    int hi = 0;
+    if (n < 384) {
+      const int f = n >> 6, a = (n >> 3) & 7, b = n & 7;
+      int32_t nl;
+      int vx, vy, vz;
+      switch (f) {
+        case 0: nl = Fr.xm; vx = 7; vy = a; vz = b; hi = at(0, a + 1, b + 1); break;
+        case 1: nl = Fr.xp; vx = 0; vy = a; vz = b; hi = at(9, a + 1, b + 1); break;
+        case 2: nl = Fr.ym; vx = a; vy = 7; vz = b; hi = at(a + 1, 0, b + 1); break;
+        case 3: nl = Fr.yp; vx = a; vy = 0; vz = b; hi = at(a + 1, 9, b + 1); break;
+        case 4: nl = Fr.zm; vx = a; vy = b; vz = 7; hi = at(a + 1, b + 1, 0); break;
+        default: nl = Fr.zp; vx = a; vy = b; vz = 0; hi = at(a + 1, b + 1, 9); break;
+      }
+      if (nl >= 0) hv = p[((vx + vy + vz) & 1) * half + nl * kHalfLeaf + cs_local(vx, vy, vz)];
+    }
+    s_p[at(lx + 1, ly + 1, lz + 1)] = own;
+    if (n < 384) s_p[hi] = hv;
+  }
+  __syncthreads();
+  const int c = at(lx + 1, ly + 1, lz + 1);
+  const float pxp = s_p[c + 100], pxm = s_p[c - 100];
+  const float pyp = s_p[c + 10], pym = s_p[c - 10];
+  const float pzp = s_p[c + 1], pzm = s_p[c - 1];
   const float s = 0.5f * inv_dx;
   float3 v = f3(u[idx] - s * (pxp - pxm), u[N + idx] - s * (pyp - pym), u[2 * N + idx] - s * (pzp - pzm));
   if (sdf) {
