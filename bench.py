@@ -38,14 +38,16 @@ KERNEL_BYTES = {
     "cg_update": 24.0,
 }
 KERNEL_NAMES = {"rbgs": "k_rbgs", "cg_product": "k_cg_spmv", "cg_update": "k_cg_update",
-                "fcg_update": "k_fcg_update"}
-# the flexible-CG update of the multigrid solve: read x, d, r, q, write x, r (its CG_UPDATE class)
-KERNEL_BYTES["fcg_update"] = 24.0
+                "mgcg_update": "k_fcg_rupdate"}
+# the residual update of the multigrid solve: read r, q, write r (its CG_UPDATE class; the solve's x
+# update rides on the coarse-level launch of the next V-cycle)
+KERNEL_BYTES["mgcg_update"] = 12.0
 # Multigrid-preconditioned CG, compulsory bytes per voxel and iteration: the V-cycle's fused level-0
 # passes (pre: read r, write z and the level-1 right-hand side, 8.5 B; post: read z, its parents, r
 # and the previous product q, write z, 16.5 B), the coarser levels (~3 B), direction and product
-# (read z, d_prev, write d, q: 16 B) and the update (24 B)
-MGCG_BYTES_PER_ITERATION = 8.5 + 16.5 + 3.0 + 16.0 + 24.0
+# (read z, d_prev, write d, q: 16 B), the residual update (12 B) and the x update (read x, d, write
+# x: 12 B, in the coarse-level launch)
+MGCG_BYTES_PER_ITERATION = 8.5 + 16.5 + 3.0 + 16.0 + 12.0 + 12.0
 
 
 def step_bytes(solve_bytes: float, m_scalars: int) -> float:
@@ -235,10 +237,11 @@ def main(argv=None):
     if head == "rbgs":
         kern, solve_b = "rbgs", 2 * KERNEL_BYTES["rbgs"] * K
     elif head == "mgcg":
-        # the solve's largest HBM-bound kernel: the flexible-CG update (the instrumented steps launch
-        # the iterations one by one so events bracket it; the timed steps replay graphs). The coarse
-        # level's one-workgroup solve takes longer but moves almost no bytes.
-        kern, timing_class = "fcg_update", "cg_update"
+        # the solve's HBM-bound kernel that the instrumented steps bracket alone (they launch the
+        # iterations one by one so events bracket it; the timed steps replay graphs): the residual
+        # update. The V-cycle's kernels are timed as one class; the coarse level's one-workgroup
+        # solve takes longest but is bound by LDS latency.
+        kern, timing_class = "mgcg_update", "cg_update"
         solve_b = MGCG_BYTES_PER_ITERATION * L["iterations_per_step"]
     else:
         kern = max(("cg_product", "cg_update"), key=lambda k: L["kernels"][k]["ms_per_step"])

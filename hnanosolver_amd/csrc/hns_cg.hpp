@@ -89,6 +89,10 @@ struct CgScalars {
   double* last;   // flexible CG: r.r and ||b||^2 of the last stop test
   double tol2;    // stop when r.r <= tol2 * ||b||^2
   double* bb;     // ||b||^2
+  // flexible CG on the multigrid V-cycle with the x update deferred (null: not deferred): the
+  // iteration k whose V-cycle last applied the pending x += alpha_{k-1} d_{k-1} (so every update
+  // before k is in x), or the iteration a flush completed x at
+  int* xdone = nullptr;
 };
 // pa slots: r.r, and for the flexible CG d.r and d.q
 enum CgSlot { kSlotRR = 0, kSlotDR = 1, kSlotDQ = 2, kCgSlots = 4 };
@@ -139,8 +143,17 @@ void launch_fcg_retest(const DevTopo& T, const float* div, float dx, const float
 // min(it, depth) directions (1 <= depth <= kFcgDepth); `it` (the host's count) selects the kernels
 void launch_fcg_direction(const DevTopo& T, int32_t it, int depth, const float* r, const float* z, float* Dr, float* Qr,
                           double* zq, const CgScalars& S, hipStream_t s);
-// x += alpha d, r -= alpha q
+// x += alpha d, r -= alpha q (x null: r only, the x update deferred to the next V-cycle)
 void launch_fcg_update(int32_t L, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
                        hipStream_t s);
+// Deferred x update (S.xdone set): iteration k's V-cycle applies x += alpha_{k-1} d_{k-1} on the
+// CUs its one-workgroup coarse solve leaves idle, and its product kernel then records *S.xdone = k.
+// Where x must be complete before that, the flush applies the pending update (k = *S.it, when
+// k >= 1 and *S.xdone < k); the next advance-free stop test then records *S.xdone = k, so the next
+// V-cycle does not apply it again. when: kFlushRunning (a periodic residual replacement: only while
+// no stop is raised; its test follows), kFlushStopped (before a stop re-test: only once a stop is
+// raised; the re-test follows), kFlushEnd (after the solve: always; nothing follows).
+enum FcgFlush { kFlushRunning = 0, kFlushStopped = 1, kFlushEnd = 2 };
+void launch_fcg_xflush(int32_t L, float* x, const float* Dr, const CgScalars& S, FcgFlush when, hipStream_t s);
 
 }  // namespace hns

@@ -28,22 +28,36 @@ struct MgShape {
 };
 const MgShape& mg_shape();
 
-// The V-cycle is ~35 small launches, most of them on the coarse levels; it is captured once into a
-// HIP graph per (topology, r, z) and replayed (the captured kernels take only these pointers, and
-// none records a memset), which removes most of the per-launch cost.
+// The default V-cycle is 7 launches (one fused pre- and post-smoothing pass per level 0-2 and the
+// one-workgroup coarse solve); the unfused variant ~35, most of them on the coarse levels. It is
+// captured once into a HIP graph per (topology, r, z, stop, variant) and replayed (the captured
+// kernels take only these pointers, and none records a memset), which removes most of the
+// per-launch cost.
 // V-cycle variants that compute the same bits as the default (Multigrid::apply's `variant` bits)
 constexpr unsigned kMgCoarseLaunches = 1;  // coarsest level as one launch per colour sweep
 constexpr unsigned kMgUnfused = 2;         // levels 0-2 as separate sweep, restriction and prolongation launches
 
+// The deferred x update of the flexible CG (x null: none): the coarse-level launch's extra
+// workgroups apply x += alpha_{k-1} d_{k-1} (alpha[k - 1], direction ring slot (k - 1) mod
+// kFcgDepth of Dr, n floats each) when k = *it >= 1 and *xdone < k, while its first workgroup
+// solves the coarsest level. Needs the one-workgroup coarse solve (Multigrid::coarse_fused).
+struct McgXUpdate {
+  float* x;
+  const float* Dr;
+  const double* alpha;
+  const int* xdone;
+};
 // What the last level-0 pass leaves for flexible CG with truncation 1 (kernels_cg.hip): per
 // workgroup (cg_groups(L) of them) partials of z . r into zr and of z . q_prev into zq, q_prev the
 // product of the previous iteration in the ring Qr (slot (*it - 1) mod kFcgDepth, n floats each).
+// xu: the deferred x update, if any.
 struct McgHook {
   double* zr;
   double* zq;  // null: no z . q_prev (the PCG variant)
   const float* Qr;
   int64_t n;
   const int* it;
+  McgXUpdate xu;
 };
 
 struct Multigrid {
@@ -64,6 +78,8 @@ struct Multigrid {
     launch_all(T, r_cs, z_cs, stop, 0u, s, hook);
   }
   uint64_t generation = 0;  // advanced whenever buffers or the coarse layout change
+  // the coarsest level is solved by one workgroup (after prepare for the topology)
+  bool coarse_fused() const { return coarse_ch_ > 0; }
   // z = V(r) on colour-split level-0 vectors: approximately A^-1 r for A = 6 I - (face neighbours).
   // graph: replay the captured V-cycle (captured on first use for these pointers); otherwise launch
   // the kernels one by one. Both run the same kernels on the same data. stop: a device flag (null:

@@ -62,7 +62,10 @@ __global__ __launch_bounds__(kTpb) void k_cg_init(int32_t L, const float* __rest
   const double t = block_sum_all(acc, s_red);
   if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = t;
   clear_flags(S);
-  if (blockIdx.x == 0 && threadIdx.x == 0) *S.it = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *S.it = 0;
+    if (S.xdone) *S.xdone = 0;
+  }
 }
 
 // Residual replacement: r = b - A x (evaluated in FP64, stored FP32); partials of r.r into pa
@@ -238,6 +241,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_test(int G, int advance, int retes
     S.last[0] = rr;
     S.last[1] = bb;
     if (advance) *S.it = it;
+    else if (S.xdone) *S.xdone = it;  // after a flush (or before iteration 0): x holds every update
     if (it == 0) S.bb[0] = bb;
     if (rr <= S.tol2 * bb) {
       S.flag[1] = it;
@@ -358,6 +362,8 @@ __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, con
   __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
   const int32_t it = *S.it;
+  // this iteration's V-cycle has applied the deferred x update of iteration it - 1
+  if (S.xdone && blockIdx.x == 0 && threadIdx.x == 0) *S.xdone = it;
   float beta = 0.f;
   const float* __restrict__ dp = z;
   if (PCG) {
@@ -446,6 +452,98 @@ __global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int64_t n, float
   if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
 }
 
+// k_fcg_update without the x update (deferred: S.alpha[it] and the direction ring keep what it
+// needs): r -= alpha q, partials of r . r. With half the loads of the full update, the loads of two
+// leaves are kept in flight: the first pair's before the partials reduce, the next pair's while a
+// pair is stored. The arithmetic and the order of the r . r sums are the full update's.
+__global__ __launch_bounds__(kTpb) void k_fcg_rupdate(int32_t L, int64_t n, float* __restrict__ r,
+                                                      const float* __restrict__ Qr, CgScalars S) {
+  __shared__ double s_red[2 * (kTpb / 64)];
+  if (S.flag[0]) return;
+  const int32_t it = *S.it;
+  const float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const LeafRun R = leaf_run(L);
+  const int32_t half = L * kHalfLeaf;
+  auto at = [&](int32_t leaf, int c) { return c * half + leaf * kHalfLeaf + (int32_t)threadIdx.x; };
+  // leaves l and l + R.step (those below R.end): element 2 j + c is leaf l + j R.step, colour half c
+  auto load = [&](int32_t l, float* rv, float* qv) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int32_t lj = l + j * R.step;
+      if (lj < R.end) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          rv[2 * j + c] = r[at(lj, c)];
+          qv[2 * j + c] = q[at(lj, c)];
+        }
+      }
+    }
+  };
+  float r0[4], q0[4];
+  if (R.first < R.end) load(R.first, r0, q0);
+  double rows[2];  // d . r, d . q
+  reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rows);
+  const double sigma = rows[1];
+  if (!(sigma > 0.0)) {  // breakdown, as in k_fcg_update
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+    return;
+  }
+  const double alpha_d = rows[0] / sigma;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.rho[it] = sigma;
+    S.alpha[it] = alpha_d;
+  }
+  const float alpha = (float)alpha_d;
+  double acc = 0.0;
+  for (int32_t l = R.first; l < R.end; l += 2 * R.step) {
+    float r1[4], q1[4];
+    if (l + 2 * R.step < R.end) load(l + 2 * R.step, r1, q1);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int32_t lj = l + j * R.step;
+      if (lj < R.end) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const float rv = fmaf(-alpha, q0[2 * j + c], r0[2 * j + c]);
+          r[at(lj, c)] = rv;
+          acc += (double)rv * (double)rv;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      r0[e] = r1[e];
+      q0[e] = q1[e];
+    }
+  }
+  const double tt = block_sum_all(acc, s_red);
+  if (threadIdx.x == 0) S.pa[kSlotRR * kCgMaxGroups + blockIdx.x] = tt;
+}
+
+// The pending deferred x update (`when`: an FcgFlush): x += alpha_{k-1} d_{k-1}, k = *S.it, when
+// k >= 1 and *S.xdone < k, with k_fcg_update's arithmetic
+__global__ __launch_bounds__(kTpb) void k_fcg_xflush(int32_t L, int64_t n, float* __restrict__ x,
+                                                     const float* __restrict__ Dr, CgScalars S, int when) {
+  if (when == kFlushRunning && S.flag[0]) return;
+  if (when == kFlushStopped && !S.flag[0]) return;
+  const int32_t k = *S.it;
+  if (k < 1 || *S.xdone >= k) return;
+  const float alpha = (float)S.alpha[k - 1];
+  const float* __restrict__ d = Dr + (int64_t)fcg_slot(k - 1) * n;
+  const LeafRun R = leaf_run(L);
+  const int32_t half = L * kHalfLeaf;
+  for (int32_t l = R.first; l < R.end; l += R.step) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const int32_t i = c * half + l * kHalfLeaf + threadIdx.x;
+      x[i] = fmaf(alpha, d[i], x[i]);
+    }
+  }
+}
+
 }  // namespace
 
 void launch_cg_init(int32_t L, const float* div, float dx, float* x, float* r, const CgScalars& S, hipStream_t s) {
@@ -521,7 +619,13 @@ void launch_fcg1_dirspmv(const DevTopo& T, bool first, const float* z, float* Dr
 }
 void launch_fcg_update(int32_t L, float* x, float* r, const float* Dr, const float* Qr, const CgScalars& S,
                        hipStream_t s) {
-  k_fcg_update<<<cg_groups(L), kTpb, 0, s>>>(L, (int64_t)L * kLeafVox, x, r, Dr, Qr, S);
+  if (x) k_fcg_update<<<cg_groups(L), kTpb, 0, s>>>(L, (int64_t)L * kLeafVox, x, r, Dr, Qr, S);
+  else k_fcg_rupdate<<<cg_groups(L), kTpb, 0, s>>>(L, (int64_t)L * kLeafVox, r, Qr, S);
+  HNS_HIP(hipGetLastError());
+}
+void launch_fcg_xflush(int32_t L, float* x, const float* Dr, const CgScalars& S, FcgFlush when, hipStream_t s) {
+  if (!S.xdone) fail(HNS_ERR_INVALID_ARGUMENT, "x flush without a deferred x update");
+  k_fcg_xflush<<<cg_groups(L), kTpb, 0, s>>>(L, (int64_t)L * kLeafVox, x, Dr, S, (int)when);
   HNS_HIP(hipGetLastError());
 }
 

@@ -130,15 +130,54 @@ __global__ __launch_bounds__(kTpb) void k_mg_leaf_sweep(DevTopo T, const float* 
 // pairs of k_mg_leaf_sweep launches.
 constexpr int kCoarseTpb = 1024;
 constexpr int kCoarseMaxCH = 8;
+// at most this many extra workgroups carry the deferred x update beside the coarse solve (each on a
+// CU of its own: it holds the solve's LDS allocation). Measured at 8 M voxels (the solve ~50 us,
+// the update ~100 MB): 96 workgroups that first sleep ~13 us (so the solve's initial loads meet an
+// idle memory system) beat 255 that start at once by 2 % of the fire step.
+constexpr int kCoarseSideGroups = 96;
+
+// Workgroup g of G extra workgroups of the coarse launch: the flexible CG's deferred
+// x += alpha_{k-1} d_{k-1} (k = *it; McgXUpdate) over n floats, k_fcg_update's fmaf, four loads of
+// each vector in flight per lane, after SLEEPS * 127 * 64 idle cycles
+template <int SLEEPS>
+__device__ void coarse_side_xupdate(const McgXUpdate& X, const int* __restrict__ it, int64_t n, int g, int G) {
+#pragma unroll
+  for (int j = 0; j < SLEEPS; ++j) __builtin_amdgcn_s_sleep(127);
+  const int32_t k = *it;
+  if (k < 1 || *X.xdone >= k) return;
+  const float alpha = (float)X.alpha[k - 1];
+  const float* __restrict__ d = X.Dr + (int64_t)fcg_slot(k - 1) * n;
+  float* __restrict__ x = X.x;
+  const int64_t stride = (int64_t)G * kCoarseTpb;
+  int64_t i = (int64_t)g * kCoarseTpb + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const float d0 = d[i], d1 = d[i + stride], d2 = d[i + 2 * stride], d3 = d[i + 3 * stride];
+    const float x0 = x[i], x1 = x[i + stride], x2 = x[i + 2 * stride], x3 = x[i + 3 * stride];
+    x[i] = fmaf(alpha, d0, x0);
+    x[i + stride] = fmaf(alpha, d1, x1);
+    x[i + 2 * stride] = fmaf(alpha, d2, x2);
+    x[i + 3 * stride] = fmaf(alpha, d3, x3);
+  }
+  for (; i < n; i += stride) x[i] = fmaf(alpha, d[i], x[i]);
+}
+
+// Workgroup 0 solves the coarsest level; workgroups 1.. (launched only with X.x set) apply the
+// deferred x update, on CUs the solve leaves idle.
 template <int CH>
 __global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restrict__ perm,
                                                           const uint32_t* __restrict__ nbp, const float* __restrict__ b,
                                                           float* __restrict__ u, float inv_s, float omega, int iters,
-                                                          const int* __restrict__ stop) {
+                                                          const int* __restrict__ stop, McgXUpdate X,
+                                                          const int* __restrict__ it_dev, int64_t n) {
   constexpr int C = 2 * CH, H = kCoarseTpb * CH;
   __shared__ float s_u[2 * H + 1];
   __shared__ float s_bi[2 * H];  // scaled right-hand sides by slot (in LDS to leave registers for loads)
   if (stop && *stop) return;
+  if (blockIdx.x > 0) {
+    // the solve's length grows with CH, and so does the sleep
+    coarse_side_xupdate<CH / 2>(X, it_dev, n, (int)blockIdx.x - 1, (int)gridDim.x - 1);
+    return;
+  }
   const int t = threadIdx.x;
   float uo[C];
   uint32_t nb[C][3];
@@ -924,8 +963,16 @@ void smooth(const DevTopo& T, int lgS, const float* b, float* u, float scale, in
 // `iters` red-black SOR iterations on level 3: one workgroup on the colour-sorted slots when
 // prepare_coarse found a layout (ch > 0), else one launch per colour sweep
 void coarse_solve(const DevTopo& T, int ch, const int32_t* perm, const uint32_t* nbp, const float* b, float* u,
-                  float scale, int iters, float omega, const int* stop, bool launches, hipStream_t s) {
+                  float scale, int iters, float omega, const int* stop, bool launches, const McgHook* hook,
+                  hipStream_t s) {
   const float inv = 1.0f / scale;
+  const McgXUpdate X = hook ? hook->xu : McgXUpdate{nullptr, nullptr, nullptr, nullptr};
+  if (X.x && (ch == 0 || launches))
+    fail(HNS_ERR_INVALID_ARGUMENT, "the deferred x update needs the one-workgroup coarse solve");
+  const int64_t n = hook ? hook->n : 0;
+  const int* it_dev = hook ? hook->it : nullptr;
+  const int side =
+      X.x ? (int)std::min<int64_t>(kCoarseSideGroups, (n + 4 * kCoarseTpb - 1) / (4 * kCoarseTpb)) : 0;
   if (ch == 0 || launches) {
     for (int k = 0; k < iters; ++k) {
       smooth(T, 0, b, u, scale, 0, k == 0, omega, stop, s);
@@ -934,10 +981,18 @@ void coarse_solve(const DevTopo& T, int ch, const int32_t* perm, const uint32_t*
     return;
   }
   switch (ch) {
-    case 1: k_mg_coarse<1><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
-    case 2: k_mg_coarse<2><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
-    case 4: k_mg_coarse<4><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
-    default: k_mg_coarse<8><<<1, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop); break;
+    case 1:
+      k_mg_coarse<1><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
+      break;
+    case 2:
+      k_mg_coarse<2><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
+      break;
+    case 4:
+      k_mg_coarse<4><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
+      break;
+    default:
+      k_mg_coarse<8><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
+      break;
   }
   HNS_HIP(hipGetLastError());
 }
@@ -1051,7 +1106,7 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
   // level 3 (one cell per leaf): M.coarse over-relaxed red-black iterations
   coarse_solve(T, coarse_ch_, cperm_.p, cnb_.p, b[kMgLevels - 1].p, u[kMgLevels - 1].p,
                (float)(1 << (2 * (kMgLevels - 1))), M.coarse, M.coarse_omega, stop, (variant & kMgCoarseLaunches) != 0,
-               s);
+               hook, s);
   // ascend: prolong and post-smooth in the reverse colour order
   for (int l = kMgLevels - 2; l >= 1; --l) {
     const int lgS = 3 - l;

@@ -17,7 +17,7 @@ struct CgState {
   DBuf<float> r, d0, d1, q, z;
   DBuf<float> dring, qring;  // flexible CG: kFcgDepth directions and products
   DBuf<double> rho, alpha, rz, pa, pb, bb, zq, last;
-  DBuf<int> flag;  // stop raised, stop iteration, device iteration counter
+  DBuf<int> flag;  // stop raised, stop iteration, device iteration counter, deferred-x-update mark
   int* h_flag = nullptr;
   double* h_last = nullptr;
   int cap = 0;
@@ -51,7 +51,9 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
   };
   if (depth == 1 && mg_shape().smooth0 == 1) {
     const bool pcg = mg_shape().pcg != 0;
-    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, pcg ? nullptr : G.zq.p, G.qring.p, (int64_t)N_, S.it};
+    // with S.xdone set, x += alpha d of the previous iteration rides on the coarse-level launch
+    const McgXUpdate X{S.xdone ? p : nullptr, G.dring.p, S.alpha, S.xdone};
+    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, pcg ? nullptr : G.zq.p, G.qring.p, (int64_t)N_, S.it, X};
     begin(HNS_K_PRECOND);
     mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H);
     end();
@@ -67,7 +69,7 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
     end();
   }
   begin(HNS_K_CG_UPDATE);
-  launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, s);
+  launch_fcg_update(T.L, S.xdone ? nullptr : p, G.r.p, G.dring.p, G.qring.p, S, s);
   end();
   begin(HNS_K_OTHER);
   launch_fcg_test(T.L, true, S, s);
@@ -248,7 +250,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.pa.alloc((size_t)kCgMaxGroups * kCgSlots);
     G.pb.alloc(kCgMaxGroups);
     G.bb.alloc(1);
-    G.flag.alloc(3);
+    G.flag.alloc(4);
     G.last.alloc(2);
     HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 3 * sizeof(int), hipHostMallocDefault));
     HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_last), 2 * sizeof(double), hipHostMallocDefault));
@@ -270,6 +272,12 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.last = G.last.p;
   S.bb = G.bb.p;
   S.tol2 = (double)r.tol * (double)r.tol;
+  // FCG(1) on the fused V-cycle with a one-workgroup coarse solve: the x update is deferred into
+  // the next V-cycle's coarse-level launch (flushed where x must be complete)
+  if (mg && depth == 1 && mg_shape().smooth0 == 1) {
+    mg_->prepare(T, s);
+    if (mg_->coarse_fused()) S.xdone = G.flag.p + 3;
+  }
   tm.begin(HNS_K_OTHER, s);
   launch_cg_init(T.L, div, r.dx, p, G.r.p, S, s);
   tm.end(s);
@@ -287,7 +295,11 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   // aligned batches from one graph)
   // (the multigrid solve's first read comes after 2 batches: it rarely stops before, and iterations
   // past a stop return at once)
-  const int batch0 = mg ? 2 * kFcgBatch : (pre ? kFcgBatch : 8), batch_max = pre ? kFcgBatch : 128;
+  // (with per-kernel timing on, the flag is read after every iteration, so no launch past the stop
+  // returning at once is timed: the timed launches are the ones that do the work, and the device
+  // computes the same as with batches)
+  const int batch0 = tm.enabled ? 1 : (mg ? 2 * kFcgBatch : (pre ? kFcgBatch : 8));
+  const int batch_max = tm.enabled ? 1 : (pre ? kFcgBatch : 128);
   int it = 0, batch = batch0, replaced_at = 0;
   bool stopped = false;
   while (it < cap) {
@@ -296,6 +308,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     for (int k = 0; k < nb;) {
       if (it > 0 && it % kResidualReplace == 0 && replaced_at != it) {
         tm.begin(HNS_K_OTHER, s);
+        if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushRunning, s);
         launch_cg_replace(T, div, r.dx, p, G.r.p, S, false, s);
         if (pre) launch_fcg_test(T.L, false, S, s);  // test the residual just replaced
         tm.end(s);
@@ -351,6 +364,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
       // a stop raised in the batch is re-tested on r = b - A x before the host looks, so one read
       // decides: stopped, or continue from the device's iteration
       tm.begin(HNS_K_OTHER, s);
+      if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushStopped, s);
       launch_fcg_retest(T, div, r.dx, p, G.r.p, S, s);
       tm.end(s);
       HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -361,7 +375,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
         break;
       }
       it = G.h_flag[2];
-      batch = kFcgBatch;
+      batch = batch_max;
       continue;
     }
     HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -381,6 +395,11 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     }
     batch = std::min(batch * 2, batch_max);
   }
+  if (S.xdone) {  // x complete before anything reads it (nothing pending after a stop: it was flushed)
+    tm.begin(HNS_K_OTHER, s);
+    launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushEnd, s);
+    tm.end(s);
+  }
   SolveResult res;
   res.iterations = stopped ? G.h_flag[1] : it;
   if (pre && stopped) {
@@ -390,8 +409,9 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   } else {
     res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
   }
-  // plain RBGS is the fallback of the learned preconditioner: continue from the PCG iterate
-  if (pre && !(res.rel_residual <= (double)r.tol)) {
+  // plain RBGS is the fallback of the learned preconditioner (an untrained network can stall the
+  // solve): continue from the PCG iterate. The multigrid solve stops at max_iters like CG.
+  if (learned && !(res.rel_residual <= (double)r.tol)) {
     const int chunk = 50, cap_fb = 200000;
     while (!(res.rel_residual <= (double)r.tol) && res.fallback < cap_fb) {
       rbgs_iters(T, div, p, 1, chunk, r, tm, s);

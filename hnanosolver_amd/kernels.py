@@ -11,6 +11,8 @@ from typing import Sequence
 
 from . import _check, lib
 
+HNS_ERR_NOT_CONVERGED = -11  # a to-tolerance solve hit max_iters (results still written)
+
 
 def _ptr(t):
     if t is None:
@@ -66,13 +68,16 @@ def residual(topo, p_cs, div_cs, dx):
 
 
 def pressure_solve(topo, div_cs, p_cs, dx, omega=1.0, iterations=0, solver="rbgs", tol=1e-5, max_iters=100000,
-                   check_every=0):
-    """Whole pressure solve from p = 0; returns hns_solve_stats (raises unless converged)."""
+                   check_every=0, allow_unconverged=False):
+    """Whole pressure solve from p = 0; returns hns_solve_stats. Raises unless converged, or with
+    allow_unconverged returns the stats of a solve that hit max_iters first (p holds its result)."""
     from . import SOLVERS, hns_solve_options, hns_solve_stats
     o = hns_solve_options(C.sizeof(hns_solve_options), SOLVERS[solver], float(tol), int(max_iters), int(check_every))
     s = hns_solve_stats(C.sizeof(hns_solve_stats), 0, -1.0, 0.0)
-    _check(lib().hnsk_pressure_solve(topo.handle, _ptr(div_cs), _ptr(p_cs), dx, omega, int(iterations), C.byref(o),
-                                     C.byref(s), _stream()))
+    st = lib().hnsk_pressure_solve(topo.handle, _ptr(div_cs), _ptr(p_cs), dx, omega, int(iterations), C.byref(o),
+                                   C.byref(s), _stream())
+    if not (allow_unconverged and st == HNS_ERR_NOT_CONVERGED):
+        _check(st)
     return s
 
 

@@ -126,8 +126,8 @@ def test_vcycle_variants_match(hip, shape, keep):
 
 def test_mgcg_step_graph_batches_match_eager_iterations(hip):
     """The fire step with the multigrid solve: iterations replayed four at a time from a captured graph
-    (timing off) and launched one by one (timing on, so events can bracket them) give bitwise the
-    same state, over steps that reach the graph batches."""
+    (timing off) and launched one by one with the stop flag read after each (timing on, so events can
+    bracket them) give bitwise the same state, over steps that reach the graph batches."""
     sc = scenes.control64()
     _case("mgcg step, graph batches vs eager, control64 N=%d" % sc.n)
 
@@ -158,3 +158,55 @@ def test_mgcg_step_graph_batches_match_eager_iterations(hip):
         n = int(np.sum(a[k] != b[k]))
         print("  %-12s differing elements %d of %d" % (k, n, a[k].size))
         assert n == 0
+
+
+def test_mgcg_capped_solves_apply_every_update(hip):
+    """Solves ended by max_iters (no stop; the multigrid solve has no fallback): the x update of an
+    iteration is applied by the next iteration's coarse-level launch, the last one by the flush after
+    the solve. Each iteration cuts the residual several times over, so the oracle's residual of the
+    returned pressure falls by a clear factor from cap to cap (a missing last update would leave it
+    where the previous cap did), and the solver's own residual of the same pressure agrees."""
+    torch = hip
+    sc = scenes.control64()
+    _case("mgcg capped at 1..7 iterations, control64 N=%d" % sc.n)
+    topo = _topo(sc.coords, sc.voxel_size)
+    div = P.scene_divergence(sc, topo)
+    d0 = _dense(torch, topo, div)
+    o = Oracle(sc.coords, np.float64)
+    rels = []
+    for cap in range(1, 8):
+        p = torch.empty_like(div)
+        st = K.pressure_solve(topo, div, p, sc.voxel_size, solver="mgcg", tol=1e-9, max_iters=cap,
+                              allow_unconverged=True)
+        r2, d2 = o.residual(_dense(torch, topo, p), d0, sc.voxel_size)
+        rel = float(np.sqrt(r2 / d2))
+        print("  cap %d: %d iterations, rel %.4e (oracle %.4e), fallback %d" % (
+            cap, st.iterations, st.rel_residual, rel, st.fallback_iterations))
+        assert st.iterations == cap and st.fallback_iterations == 0
+        assert abs(st.rel_residual - rel) <= 1e-3 * rel + 1e-9
+        rels.append(rel)
+    for a, b in zip(rels, rels[1:]):
+        assert b < 0.7 * a
+
+
+def test_mgcg_long_solve_through_replacements(hip):
+    """A tolerance below what FP32 iterates reach: the solve runs to its cap through rejected stops (the
+    recurrence residual passes, the replaced b - A x does not) and a periodic residual replacement at
+    iteration 64, each preceded by completing the deferred x update. The pressure it returns sits at
+    the FP32 floor, and the solver's residual of it agrees with the oracle's."""
+    torch = hip
+    sc = scenes.smoke64()
+    _case("mgcg to 1e-9 capped at 72 iterations, smoke64 N=%d" % sc.n)
+    topo = _topo(sc.coords, sc.voxel_size)
+    div = P.scene_divergence(sc, topo)
+    p = torch.empty_like(div)
+    st = K.pressure_solve(topo, div, p, sc.voxel_size, solver="mgcg", tol=1e-9, max_iters=72,
+                          allow_unconverged=True)
+    r2, d2 = Oracle(sc.coords, np.float64).residual(_dense(torch, topo, p), _dense(torch, topo, div),
+                                                     sc.voxel_size)
+    rel = float(np.sqrt(r2 / d2))
+    print("  %d iterations, rel %.4e (oracle %.4e), fallback %d" % (
+        st.iterations, st.rel_residual, rel, st.fallback_iterations))
+    assert st.iterations == 72 and st.fallback_iterations == 0
+    assert rel < 1e-5
+    assert abs(st.rel_residual - rel) <= 1e-2 * rel + 1e-9
