@@ -83,7 +83,7 @@ struct CgScalars {
   double* pb;     // one partial per workgroup (d . q), written by the product kernel
   double* rho;    // max_iters + 1 entries
   double* alpha;  // max_iters + 1 entries
-  double* rz;     // multigrid PCG: r_k . z_k, max_iters + 1 entries
+  double* rz;     // multigrid solve: z_k . r_k (the residual update's numerator; PCG's beta), max_iters + 1
   int* flag;      // 2 entries: stop raised, stop iteration
   int* it;        // flexible CG: the device's iteration counter (0 after init, advanced by the test)
   double* last;   // flexible CG: r.r and ||b||^2 of the last stop test

@@ -371,10 +371,17 @@ __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, con
     reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rz1);
     if (blockIdx.x == 0 && threadIdx.x == 0) S.rz[it] = rz1[0];
     if (NP > 0) beta = (float)(rz1[0] / S.rz[it - 1]);
-  } else if (NP > 0) {
-    double zq1[1];
-    reduce_partial_rows(zq, (int)gridDim.x, s_red, zq1);
-    beta = (float)(-zq1[0] / S.rho[it - 1]);
+  } else {
+    if (NP > 0) {
+      double zq1[1];
+      reduce_partial_rows(zq, (int)gridDim.x, s_red, zq1);
+      beta = (float)(-zq1[0] / S.rho[it - 1]);
+    }
+    if (blockIdx.x == 0) {  // z . r once for the update (its numerator), as the update would sum it
+      double zr1[1];
+      reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, zr1);
+      if (threadIdx.x == 0) S.rz[it] = zr1[0];
+    }
   }
   if (NP > 0) dp = Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n;
   float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
@@ -481,9 +488,9 @@ __global__ __launch_bounds__(kTpb) void k_fcg_rupdate(int32_t L, int64_t n, floa
   };
   float r0[4], q0[4];
   if (R.first < R.end) load(R.first, r0, q0);
-  double rows[2];  // d . r, d . q
-  reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rows);
-  const double sigma = rows[1];
+  double dq[1];  // d . q; the numerator z . r was summed once by the product kernel into S.rz[it]
+  reduce_partial_rows(S.pa + kSlotDQ * kCgMaxGroups, (int)gridDim.x, s_red, dq);
+  const double sigma = dq[0];
   if (!(sigma > 0.0)) {  // breakdown, as in k_fcg_update
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       S.flag[1] = it;
@@ -491,7 +498,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_rupdate(int32_t L, int64_t n, floa
     }
     return;
   }
-  const double alpha_d = rows[0] / sigma;
+  const double alpha_d = S.rz[it] / sigma;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     S.rho[it] = sigma;
     S.alpha[it] = alpha_d;
