@@ -20,8 +20,14 @@ namespace {
 constexpr int kH = 2;            // halo
 constexpr int kE = 8 + 2 * kH;   // tile edge
 constexpr int kE3 = kE * kE * kE;
+// The scalar advection's tile has a 1-voxel halo: its two traces and the clamp's neighbours stay
+// inside it while a trace is under one voxel (CFL < 1), and a sample outside reads the field itself
+// (the same values), so the tile only changes where the values come from. At 8 M voxels this took
+// the 5-field kernel from 271 to 196 us (the velocity kernel, not bound by its fill, was slower so).
+constexpr int kSH = 1, kSE = 8 + 2 * kSH, kSE3 = kSE * kSE * kSE;
 
-__device__ __forceinline__ int tix(int x, int y, int z) { return (x * kE + y) * kE + z; }
+template <int E = kE>
+__device__ __forceinline__ int tix(int x, int y, int z) { return (x * E + y) * E + z; }
 
 // Tile cell c -> (27-slot of the leaf holding it) << 9 | its leaf-dense local index, built at
 // compile time so the fill decodes a table word instead of dividing c (the fills were a large
@@ -75,16 +81,20 @@ __device__ inline void fill_tile(float* __restrict__ s, const float* __restrict_
 }
 
 __device__ __forceinline__ float3 f3(float a, float b, float c) { return make_float3(a, b, c); }
-__device__ __forceinline__ float3 tvec(const float* s, int c) { return f3(s[c], s[kE3 + c], s[2 * kE3 + c]); }
+template <int E3 = kE3>
+__device__ __forceinline__ float3 tvec(const float* s, int c) { return f3(s[c], s[E3 + c], s[2 * E3 + c]); }
 
 // tile corner base of a sample position (tile coordinates), or -1 when the 2x2x2 stencil is not
 // inside the tile
+// (H: the tile's halo)
+template <int H = kH>
 __device__ __forceinline__ bool tile_base(const LeafView& V, int32_t i, int32_t j, int32_t k, int& bi, int& bj,
                                           int& bk) {
-  bi = i - (V.ox - kH);
-  bj = j - (V.oy - kH);
-  bk = k - (V.oz - kH);
-  return (unsigned)bi <= (unsigned)(kE - 2) && (unsigned)bj <= (unsigned)(kE - 2) && (unsigned)bk <= (unsigned)(kE - 2);
+  constexpr int E = 8 + 2 * H;
+  bi = i - (V.ox - H);
+  bj = j - (V.oy - H);
+  bk = k - (V.oz - H);
+  return (unsigned)bi <= (unsigned)(E - 2) && (unsigned)bj <= (unsigned)(E - 2) && (unsigned)bk <= (unsigned)(E - 2);
 }
 
 // TrilinearSampler on Vec3f from the tile (same operations as trilinear_vec).
@@ -201,13 +211,14 @@ __device__ inline void interp_weights(float x, float y, float z, int32_t& i0, in
   w[4] = w00 * tz;  w[5] = w10 * tz;  w[6] = w01 * tz;  w[7] = w11 * tz;
 }
 // corner q of the explicit-weight stencil: (i0 + (q&1), j0 + ((q>>1)&1), k0 + (q>>2))
+template <int H>
 __device__ inline bool setup_interp_tile(const LeafView& V, float x, float y, float z, Interp8T& d) {
   int32_t i0, j0, k0;
   interp_weights(x, y, z, i0, j0, k0, d.w);
   int bi, bj, bk;
-  if (tile_base(V, i0, j0, k0, bi, bj, bk)) {
+  if (tile_base<H>(V, i0, j0, k0, bi, bj, bk)) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) d.idx[q] = tix(bi + (q & 1), bj + ((q >> 1) & 1), bk + (q >> 2));
+    for (int q = 0; q < 8; ++q) d.idx[q] = tix<8 + 2 * H>(bi + (q & 1), bj + ((q >> 1) & 1), bk + (q >> 2));
     return true;
   }
 #pragma unroll
@@ -215,7 +226,7 @@ __device__ inline bool setup_interp_tile(const LeafView& V, float x, float y, fl
   return false;
 }
 
-// The velocity tile and all M field tiles are filled in one pass (dynamic LDS: (3 + M) 12^3
+// The velocity tile and all M field tiles are filled in one pass (dynamic LDS: (3 + M) 10^3
 // floats), so the loads of every tile are in flight together and there is one barrier.
 // M = F.m fields (a template argument: every field loop unrolls without run-time checks)
 template <int M>
@@ -232,13 +243,13 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   {
     // advect_scalars reads element 0 on a miss (velocity and fields alike): an absent cell loads
     // element 0 of each plane. Every load of the tile is issued before the first LDS store.
-    constexpr int kIt = (kE3 + 511) / 512;
+    constexpr int kIt = (kSE3 + 511) / 512;
     float v[kIt][3 + M];
 #pragma unroll
     for (int it = 0; it < kIt; ++it) {
-      const int c = min((int)threadIdx.x + it * 512, kE3 - 1);
-      const int tx = c / (kE * kE), ty = (c / kE) % kE, tz = c % kE;
-      const int lx = tx - kH, ly = ty - kH, lz = tz - kH;
+      const int c = min((int)threadIdx.x + it * 512, kSE3 - 1);
+      const int tx = c / (kSE * kSE), ty = (c / kSE) % kSE, tz = c % kSE;
+      const int lx = tx - kSH, ly = ty - kSH, lz = tz - kSH;
       const int32_t nl = s_nb[((lx >> 3) + 1) * 9 + ((ly >> 3) + 1) * 3 + ((lz >> 3) + 1)];
       const int32_t g = nl < 0 ? 0 : nl * kLeafVox + (((lx & 7) << 6) | ((ly & 7) << 3) | (lz & 7));
       v[it][0] = u[g];
@@ -250,9 +261,9 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
 #pragma unroll
     for (int it = 0; it < kIt; ++it) {
       const int c = (int)threadIdx.x + it * 512;
-      if (c < kE3) {
+      if (c < kSE3) {
 #pragma unroll
-        for (int k = 0; k < 3 + M; ++k) s_u[k * kE3 + c] = v[it][k];
+        for (int k = 0; k < 3 + M; ++k) s_u[k * kSE3 + c] = v[it][k];
       }
     }
     __syncthreads();
@@ -261,19 +272,19 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
   const int32_t x = V.ox + lx, y = V.oy + ly, z = V.oz + lz;
   const int32_t idx = l * kLeafVox + n;
-  const int ct = tix(lx + kH, ly + kH, lz + kH);
+  const int ct = tix<kSE>(lx + kSH, ly + kSH, lz + kSH);
   const float px = (float)x, py = (float)y, pz = (float)z;
-  const float3 vc = tvec(s_u, ct);
+  const float3 vc = tvec<kSE3>(s_u, ct);
   float bx = px - sdt * vc.x, by = py - sdt * vc.y, bz = pz - sdt * vc.z;
   if (sdf && trilinear_s(V, sdf, bx, by, bz) < 0.f) { bx = px; by = py; bz = pz; }
   Interp8T bd;
-  const bool bt = setup_interp_tile(V, bx, by, bz, bd);
+  const bool bt = setup_interp_tile<kSH>(V, bx, by, bz, bd);
   float vfx = 0.f, vfy = 0.f, vfz = 0.f;
   // waves whose traces all stay in the tile (the common case) take a path without global reads
   if (__all(bt)) {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const float3 v = tvec(s_u, bd.idx[q]);
+      const float3 v = tvec<kSE3>(s_u, bd.idx[q]);
       vfx = fmaf(bd.w[q], v.x, vfx);
       vfy = fmaf(bd.w[q], v.y, vfy);
       vfz = fmaf(bd.w[q], v.z, vfz);
@@ -281,7 +292,7 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   } else {
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
-      const float3 v = bt ? tvec(s_u, bd.idx[q]) : f3(U.x[bd.idx[q]], U.y[bd.idx[q]], U.z[bd.idx[q]]);
+      const float3 v = bt ? tvec<kSE3>(s_u, bd.idx[q]) : f3(U.x[bd.idx[q]], U.y[bd.idx[q]], U.z[bd.idx[q]]);
       vfx = fmaf(bd.w[q], v.x, vfx);
       vfy = fmaf(bd.w[q], v.y, vfy);
       vfz = fmaf(bd.w[q], v.z, vfz);
@@ -290,13 +301,13 @@ __global__ __launch_bounds__(512) void k_advect_scalars_lds(DevTopo T, const flo
   float fx = bx + sdt * vfx, fy = by + sdt * vfy, fz = bz + sdt * vfz;
   if (sdf && trilinear_s(V, sdf, fx, fy, fz) < 0.f) { fx = bx; fy = by; fz = bz; }
   Interp8T fd;
-  const bool ft = setup_interp_tile(V, fx, fy, fz, fd);
-  const int nct[6] = {ct - kE * kE, ct + kE * kE, ct - kE, ct + kE, ct - 1, ct + 1};
+  const bool ft = setup_interp_tile<kSH>(V, fx, fy, fz, fd);
+  const int nct[6] = {ct - kSE * kSE, ct + kSE * kSE, ct - kSE, ct + kSE, ct - 1, ct + 1};
   const bool wave_tile = __all(bt && ft);
 #pragma unroll
   for (int s = 0; s < M; ++s) {
     const float* __restrict__ d = F.in[s];
-    const float* s_f = s_dyn + (3 + s) * kE3;
+    const float* s_f = s_dyn + (3 + s) * kSE3;
     const float phiOrig = s_f[ct];
     float phiF = 0.f, phiB = 0.f;
     if (wave_tile) {
@@ -418,7 +429,7 @@ void launch_advect_scalars_lds(const DevTopo& T, const float* u, const FieldSet&
       g.in[q] = f.in[b + q];
       g.out[q] = f.out[b + q];
     }
-    const size_t lds = (size_t)(3 + g.m) * kE3 * sizeof(float);
+    const size_t lds = (size_t)(3 + g.m) * kSE3 * sizeof(float);
     const unsigned grid = grid_for_leaves(T.L);
     const float sdt = dt * inv_dx;
     switch (g.m) {
