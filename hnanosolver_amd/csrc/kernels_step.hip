@@ -372,6 +372,9 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
   const int32_t idx = l * kLeafVox + n;
   const FaceRow Fr = face_row(T, l);
   auto at = [](int a, int b, int c) { return (a * 10 + b) * 10 + c; };
+  // the velocity's loads are issued with the pressure's, before the barrier (the compiler does not
+  // move loads across it)
+  const float u0 = u[idx], u1 = u[N + idx], u2 = u[2 * N + idx];
   {
     // the lane's own voxel, and lanes < 384 one face-halo voxel each (absent leaves read 0)
     const float own = p[((lx + ly + lz) & 1) * half + l * kHalfLeaf + cs_local(lx, ly, lz)];
@@ -400,7 +403,7 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
   const float pyp = s_p[c + 10], pym = s_p[c - 10];
   const float pzp = s_p[c + 1], pzm = s_p[c - 1];
   const float s = 0.5f * inv_dx;
-  float3 v = f3(u[idx] - s * (pxp - pxm), u[N + idx] - s * (pyp - pym), u[2 * N + idx] - s * (pzp - pzm));
+  float3 v = f3(u0 - s * (pxp - pxm), u1 - s * (pyp - pym), u2 - s * (pzp - pzm));
   if (sdf) {
     const float sv = sdf[idx];
     if (sv < 0.f) {
