@@ -29,6 +29,10 @@ struct CgState {
   const float* key_p = nullptr;
   uint64_t key_gen = 0;
   int key_depth = 0;
+  // the captured kernels take CgScalars by value: its per-iteration arrays (reallocated when a solve
+  // asks for more iterations than any before) and its tolerance are part of what a graph was made for
+  const double* key_alpha = nullptr;
+  double key_tol2 = -1.0;
   ~CgState() {
     if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
@@ -83,7 +87,7 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
   Multigrid& M = *mg_;
   M.prepare(T, s);
   if (!G.batch_exec || G.key_nbr != T.nbr || G.key_L != T.L || G.key_p != p || G.key_gen != M.generation ||
-      G.key_depth != depth) {
+      G.key_depth != depth || G.key_alpha != S.alpha || G.key_tol2 != S.tol2) {
     if (G.batch_exec) {
       HNS_HIP(hipGraphExecDestroy(G.batch_exec));
       G.batch_exec = nullptr;
@@ -107,6 +111,8 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
     G.key_p = p;
     G.key_gen = M.generation;
     G.key_depth = depth;
+    G.key_alpha = S.alpha;
+    G.key_tol2 = S.tol2;
   }
   HNS_HIP(hipGraphLaunch(G.batch_exec, s));
 }

@@ -210,3 +210,25 @@ def test_mgcg_long_solve_through_replacements(hip):
     assert st.iterations == 72 and st.fallback_iterations == 0
     assert rel < 1e-5
     assert abs(st.rel_residual - rel) <= 1e-2 * rel + 1e-9
+
+
+def test_mgcg_graph_remade_for_new_cap_and_tolerance(hip):
+    """Solves on one topology replay the 4-iteration graphs captured by the first; a later solve with a
+    larger iteration cap than any before (its per-iteration arrays are reallocated) and another
+    tolerance has them made again, and then matches the same solve on a fresh topology bit for bit."""
+    torch = hip
+    sc = scenes.control64()
+    _case("mgcg graph cache across caps and tolerances, control64 N=%d" % sc.n)
+    topo = _topo(sc.coords, sc.voxel_size)
+    div = P.scene_divergence(sc, topo)
+    p1, p2, p3 = torch.empty_like(div), torch.empty_like(div), torch.empty_like(div)
+    st1 = K.pressure_solve(topo, div, p1, sc.voxel_size, solver="mgcg", tol=1e-3, max_iters=40)
+    st2 = K.pressure_solve(topo, div, p2, sc.voxel_size, solver="mgcg", tol=1e-5, max_iters=4000)
+    fresh = _topo(sc.coords, sc.voxel_size)
+    st3 = K.pressure_solve(fresh, div, p3, sc.voxel_size, solver="mgcg", tol=1e-5, max_iters=4000)
+    print("  tol 1e-3: %d iterations; tol 1e-5 after it: %d, on a fresh topology: %d" % (
+        st1.iterations, st2.iterations, st3.iterations))
+    assert st1.iterations >= 5  # the first solve reached the graph batches
+    assert st2.iterations == st3.iterations and bool(torch.equal(p2, p3))
+    r2, d2 = Oracle(sc.coords, np.float64).residual(_dense(torch, topo, p2), _dense(torch, topo, div), sc.voxel_size)
+    assert np.sqrt(r2 / d2) <= 1.01e-5
