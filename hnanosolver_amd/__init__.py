@@ -30,7 +30,7 @@ HNS_OK = 0
 STATUS = {0: "HNS_OK", -1: "HNS_ERR_INVALID_ARGUMENT", -2: "HNS_ERR_EMPTY", -3: "HNS_ERR_NULL_CONTEXT",
           -4: "HNS_ERR_TOPOLOGY", -5: "HNS_ERR_LEAF_ORDER", -6: "HNS_ERR_NO_VELOCITY", -7: "HNS_ERR_HIP",
           -8: "HNS_ERR_OUT_OF_MEMORY", -9: "HNS_ERR_NOT_FOUND", -10: "HNS_ERR_NO_DEVICE",
-          -11: "HNS_ERR_NOT_CONVERGED"}
+          -11: "HNS_ERR_NOT_CONVERGED", -12: "HNS_ERR_NON_FINITE"}
 HNS_FLOAT, HNS_VEC3F = 1, 3
 SOLVERS = {"rbgs": 0, "rbgs_tol": 1, "cg": 2, "pcg_learned": 3, "mgcg": 4}
 KERNEL_CLASSES = ["advect_velocity", "vorticity_forces", "divergence_combustion", "rbgs", "residual", "gradient",

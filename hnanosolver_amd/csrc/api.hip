@@ -159,8 +159,33 @@ void ensure_field_storage(hns_context* C, Field& f) {
   if (!f.buf.p) f.buf.alloc(C->N);
 }
 
-// HNS_Advance body for one substep sequence (HNanoSolver.cu:223-456).
-void advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, const hns_solve_options* opts,
+// Status of one to-tolerance solve (the fixed RBGS count has none): HNS_ERR_NON_FINITE when a
+// residual or ||div|| is not finite, HNS_ERR_NOT_CONVERGED when it ended above tol (iteration cap
+// or update breakdown), else HNS_OK. Sets the message of a failing status.
+hns_status solve_status(const SolveResult& res, int solver, float tol) {
+  if (solver == HNS_SOLVER_RBGS_FIXED) return HNS_OK;
+  if (res.end == kLoopNonFinite || std::isnan(res.rel_residual)) {
+    g_err = "pressure solve met a residual or right-hand side that is not finite";
+    return HNS_ERR_NON_FINITE;
+  }
+  if (!(res.rel_residual <= (double)tol)) {
+    g_err = res.end == kLoopBreakdown
+                ? "pressure solve broke down (d . A d <= 0) at iteration " + std::to_string(res.iterations) +
+                      " before tol"
+                : "pressure solve reached max_iters before tol";
+    return HNS_ERR_NOT_CONVERGED;
+  }
+  return HNS_OK;
+}
+// the worse of two solve statuses (non-finite before not converged before OK)
+hns_status worse(hns_status a, hns_status b) {
+  auto rank = [](hns_status x) { return x == HNS_ERR_NON_FINITE ? 2 : (x == HNS_ERR_NOT_CONVERGED ? 1 : 0); };
+  return rank(b) > rank(a) ? b : a;
+}
+
+// HNS_Advance body for one substep sequence (HNanoSolver.cu:223-456). Returns the worst status of
+// the call's to-tolerance solves (every substep's, the disturbance re-solves' included).
+hns_status advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, const hns_solve_options* opts,
              hns_solve_stats* stats) {
   const float dx = C->voxelSize;
   const float inv_dx = 1.0f / dx;
@@ -197,6 +222,15 @@ void advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, c
     req.pcw = &C->pc->work;
   }
 
+  hns_status st = HNS_OK;
+  std::string msg;
+  auto note = [&](const SolveResult& res) {
+    const hns_status a = solve_status(res, req.solver, req.tol);
+    if (worse(st, a) != st) {
+      st = a;
+      msg = g_err;
+    }
+  };
   for (int sub = 0; sub < SUBS; ++sub) {
     // 1) advect velocity u -> A
     tm.begin(HNS_K_ADVECT_VELOCITY, s);
@@ -260,12 +294,14 @@ void advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, c
     tm.end(s);
     // 4) pressure from p = 0
     SolveResult res = C->solvers.solve(T, C->div.p, C->p.p, iteration, req, tm, s);
+    note(res);
     // 4.5) disturbance: divergence of the buoyant velocity (drops the expansion), max(2, K/2) more
     if (P.disturbanceEnable > 0.0f) {
       tm.begin(HNS_K_DIVERGENCE_COMBUSTION, s);
       launch_divergence(T, ustar, C->div.p, inv_dx, true, nullptr, nullptr, s);
       tm.end(s);
       res = C->solvers.solve(T, C->div.p, C->p.p, std::max(2, iteration / 2), req, tm, s);
+      note(res);
     }
     if (stats) {
       stats->iterations = res.iterations;
@@ -302,6 +338,8 @@ void advance(hns_context* C, int32_t iteration, float dt, const hns_params& P, c
   }
   HNS_HIP(hipStreamSynchronize(s));
   tm.collect();
+  if (st != HNS_OK) g_err = msg;
+  return st;
 }
 
 }  // namespace
@@ -498,12 +536,7 @@ hns_status hns_advance_ex(hns_context* C, int32_t iteration, float dt, const hns
     if (stats && stats->struct_size < sizeof(hns_solve_stats))
       fail(HNS_ERR_INVALID_ARGUMENT, "hns_solve_stats.struct_size too small");
     HNS_HIP(hipSetDevice(C->device));
-    advance(C, iteration, dt, *params, opts, stats);
-    if (stats && opts && opts->solver != HNS_SOLVER_RBGS_FIXED && !(stats->rel_residual <= opts->tol)) {
-      g_err = "pressure solve reached max_iters before tol";
-      return HNS_ERR_NOT_CONVERGED;
-    }
-    return HNS_OK;
+    return advance(C, iteration, dt, *params, opts, stats);
   });
 }
 
@@ -865,11 +898,7 @@ hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, fl
       stats->solve_ms = res.ms;
       stats->fallback_iterations = res.fallback;
     }
-    if (opts && opts->solver != HNS_SOLVER_RBGS_FIXED && !(res.rel_residual <= opts->tol)) {
-      g_err = "pressure solve reached max_iters before tol";
-      return HNS_ERR_NOT_CONVERGED;
-    }
-    return HNS_OK;
+    return opts ? solve_status(res, opts->solver, opts->tol) : HNS_OK;
   });
 }
 
@@ -1102,11 +1131,7 @@ hns_status hnsk_pressure_solve_learned(const hns_topology* topo, hns_precond* p,
       stats->solve_ms = res.ms;
       stats->fallback_iterations = res.fallback;
     }
-    if (!(res.rel_residual <= opts->tol)) {
-      g_err = "pressure solve reached max_iters before tol";
-      return HNS_ERR_NOT_CONVERGED;
-    }
-    return HNS_OK;
+    return solve_status(res, HNS_SOLVER_PCG_LEARNED, opts->tol);
   });
 }
 

@@ -28,6 +28,7 @@ struct DevTopo {
   const int32_t* __restrict__ nbr;
   const unsigned long long* __restrict__ hkeys;
   const int32_t* __restrict__ hvals;
+  uint64_t id;  // host-side identity of the topology build (keys of cached graphs and layouts)
 };
 
 __host__ __device__ inline bool leaf_in_range(int32_t lx, int32_t ly, int32_t lz) {
@@ -51,13 +52,14 @@ __device__ inline int32_t hash_find(const DevTopo& T, int32_t lx, int32_t ly, in
   if (!leaf_in_range(lx, ly, lz)) return -1;
   const uint64_t k = leaf_key(lx, ly, lz);
   uint32_t h = hash_mix(k) & T.hmask;
-  // load factor <= 1/2, so an empty slot ends every probe sequence
-  for (;;) {
+  // load factor <= 1/2, so an empty slot ends every probe sequence long before the bound
+  for (uint32_t probe = 0; probe <= T.hmask; ++probe) {
     const uint64_t kk = T.hkeys[h];
     if (kk == k) return T.hvals[h];
     if (kk == kEmptyKey) return -1;
     h = (h + 1) & T.hmask;
   }
+  return -1;
 }
 
 // Workgroup b is dispatched to XCD b % 8. Give each XCD a contiguous run of leaves so that a

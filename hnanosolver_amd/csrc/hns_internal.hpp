@@ -69,8 +69,10 @@ struct TopoStore {
   DBuf<unsigned long long> hkeys;
   DBuf<int32_t> hvals;
   uint32_t hmask = 0;
+  uint64_t id = 0;  // unique per build in this process (never reused, unlike a freed address)
   DevTopo dev() const {
     DevTopo t;
+    t.id = id;
     t.L = L;
     t.hmask = hmask;
     t.origins = origins.p;

@@ -98,11 +98,11 @@ struct Multigrid {
   DBuf<int32_t> cperm_;  // slot -> leaf (-1: padding)
   DBuf<uint32_t> cnb_;   // three planes of packed 16-bit face-neighbour slots
   int coarse_ch_ = 0;
-  const int32_t* ckey_nbr_ = nullptr;
+  uint64_t ckey_id_ = 0;  // topology the coarse layout was made for
   int32_t ckey_L_ = 0;
   hipStream_t cap_ = nullptr;
   hipGraphExec_t exec_ = nullptr;
-  const int32_t* key_nbr_ = nullptr;
+  uint64_t key_id_ = 0;
   int32_t key_L_ = 0;
   const float* key_r_ = nullptr;
   float* key_z_ = nullptr;

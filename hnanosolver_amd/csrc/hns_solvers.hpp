@@ -3,6 +3,7 @@
 #include <vector>
 
 #include "hns_internal.hpp"
+#include "hns_solve_loop.hpp"
 
 namespace hns {
 
@@ -84,6 +85,7 @@ struct SolveResult {
   int fallback = 0;  // RBGS iterations after a learned PCG that missed tol
   double rel_residual = -1.0;
   double ms = 0.0;
+  LoopEnd end = kLoopStopped;  // how a to-tolerance solve ended (kLoopStopped for the fixed count)
 };
 
 struct CgState;
@@ -120,6 +122,51 @@ class Solvers {
   void mg_batch(const DevTopo& T, float* p, const struct CgScalars& S, int depth, hipStream_t s);
   void mg_iteration(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int it, hipStream_t s,
                     Timing* tm = nullptr);
+
+  // The device operations the loops of hns_solve_loop.hpp drive (one solve's arrays and stream).
+  struct RbgsDev {
+    Solvers* sol;
+    const DevTopo& T;
+    const float* div;
+    float* p;
+    const SolveRequest& r;
+    Timing& tm;
+    hipStream_t s;
+    void sweeps(int first_it, int n);
+    double residual();
+  };
+  struct CgDev {
+    Solvers* sol;
+    const DevTopo& T;
+    const float* div;
+    float* p;
+    const SolveRequest& r;
+    const struct CgScalars& S;
+    Timing& tm;
+    hipStream_t s;
+    void replace_periodic(int it);
+    void iteration(int it);
+    CgReport read();
+    void replace_forced();
+  };
+  struct FcgDev {
+    Solvers* sol;
+    const DevTopo& T;
+    const float* div;
+    float* p;
+    const SolveRequest& r;
+    const struct CgScalars& S;
+    int depth;
+    bool mg;
+    Timing& tm;
+    hipStream_t s;
+    void replace_periodic(int it);
+    bool batchable(int it, int left);
+    void batch(int it);
+    void first_test();
+    void iteration(int it);
+    FcgReport end_batch();
+  };
 };
 
 }  // namespace hns

@@ -1017,14 +1017,14 @@ void Multigrid::ensure(int32_t leaves) {
 }
 
 void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
-  if (ckey_nbr_ == T.nbr && ckey_L_ == T.L) return;
+  if (ckey_id_ == T.id && ckey_L_ == T.L) return;
   if (exec_) {  // a captured V-cycle may refer to the layout about to be replaced
     HNS_HIP(hipGraphExecDestroy(exec_));
     exec_ = nullptr;
   }
   coarse_ch_ = 0;
   ++generation;
-  ckey_nbr_ = T.nbr;
+  ckey_id_ = T.id;
   ckey_L_ = T.L;
   const int32_t L = T.L;
   if (L <= 0 || L > 2 * kCoarseTpb * kCoarseMaxCH) return;
@@ -1187,7 +1187,7 @@ void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStrea
     launch_all(T, r_cs, z_cs, stop, variant, s);
     return;
   }
-  if (!exec_ || key_nbr_ != T.nbr || key_L_ != T.L || key_r_ != r_cs || key_z_ != z_cs || key_stop_ != stop ||
+  if (!exec_ || key_id_ != T.id || key_L_ != T.L || key_r_ != r_cs || key_z_ != z_cs || key_stop_ != stop ||
       key_variant_ != variant) {
     if (exec_) {
       HNS_HIP(hipGraphExecDestroy(exec_));
@@ -1207,7 +1207,7 @@ void Multigrid::apply(const DevTopo& T, const float* r_cs, float* z_cs, hipStrea
     const hipError_t e = hipGraphInstantiate(&exec_, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HNS_HIP(e);
-    key_nbr_ = T.nbr;
+    key_id_ = T.id;
     key_L_ = T.L;
     key_r_ = r_cs;
     key_z_ = z_cs;

@@ -24,14 +24,15 @@ struct CgState {
   // kFcgBatch multigrid-preconditioned iterations captured in one graph, and what it was made for
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t batch_exec = nullptr;
-  const int32_t* key_nbr = nullptr;
+  uint64_t key_topo = 0;
   int32_t key_L = 0;
   const float* key_p = nullptr;
   uint64_t key_gen = 0;
   int key_depth = 0;
-  // the captured kernels take CgScalars by value: its per-iteration arrays (reallocated when a solve
-  // asks for more iterations than any before) and its tolerance are part of what a graph was made for
-  const double* key_alpha = nullptr;
+  // the captured kernels take CgScalars by value: its per-iteration arrays (rho, alpha, rz:
+  // reallocated together when a solve asks for more iterations than any before, which advances
+  // arrays_gen) and its tolerance are part of what a graph was made for
+  uint64_t arrays_gen = 0, key_arrays = 0;
   double key_tol2 = -1.0;
   ~CgState() {
     if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
@@ -86,8 +87,8 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
   CgState& G = *cg_;
   Multigrid& M = *mg_;
   M.prepare(T, s);
-  if (!G.batch_exec || G.key_nbr != T.nbr || G.key_L != T.L || G.key_p != p || G.key_gen != M.generation ||
-      G.key_depth != depth || G.key_alpha != S.alpha || G.key_tol2 != S.tol2) {
+  if (!G.batch_exec || G.key_topo != T.id || G.key_L != T.L || G.key_p != p || G.key_gen != M.generation ||
+      G.key_depth != depth || G.key_arrays != G.arrays_gen || G.key_tol2 != S.tol2) {
     if (G.batch_exec) {
       HNS_HIP(hipGraphExecDestroy(G.batch_exec));
       G.batch_exec = nullptr;
@@ -106,12 +107,12 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
     const hipError_t e = hipGraphInstantiate(&G.batch_exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     HNS_HIP(e);
-    G.key_nbr = T.nbr;
+    G.key_topo = T.id;
     G.key_L = T.L;
     G.key_p = p;
     G.key_gen = M.generation;
     G.key_depth = depth;
-    G.key_alpha = S.alpha;
+    G.key_arrays = G.arrays_gen;
     G.key_tol2 = S.tol2;
   }
   HNS_HIP(hipGraphLaunch(G.batch_exec, s));
@@ -186,6 +187,7 @@ double Solvers::rel_residual(const DevTopo& T, const float* div, const float* p,
   HNS_HIP(hipMemcpyAsync(h_red_, red_.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
   HNS_HIP(hipStreamSynchronize(s));
   const double r2 = h_red_[0], d2 = h_red_[1];
+  if (!std::isfinite(r2) || !std::isfinite(d2)) return NAN;  // a NaN or Inf in p or div
   return d2 > 0.0 ? std::sqrt(r2 / d2) : (r2 > 0.0 ? INFINITY : 0.0);
 }
 
@@ -204,17 +206,12 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
     case HNS_SOLVER_RBGS_TOL: {
       const int chunk = r.check_every > 0 ? r.check_every : 50;
       const int cap = r.max_iters > 0 ? r.max_iters : 100000;
-      int done = 0;
       HNS_HIP(hipMemsetAsync(p, 0, N_ * sizeof(float), s));
-      double rel = rel_residual(T, div, p, r.dx, tm, s);
-      while (rel > (double)r.tol && done < cap) {
-        const int n = std::min(chunk, cap - done);
-        rbgs_iters(T, div, p, done, n, r, tm, s);
-        done += n;
-        rel = rel_residual(T, div, p, r.dx, tm, s);
-      }
-      res.iterations = done;
-      res.rel_residual = rel;
+      RbgsDev d{this, T, div, p, r, tm, s};
+      const LoopResult lr = rbgs_tol_loop(d, (double)r.tol, cap, chunk, 0, rel_residual(T, div, p, r.dx, tm, s),
+                                          &res.rel_residual);
+      res.iterations = lr.iterations;
+      res.end = lr.end;
       break;
     }
     case HNS_SOLVER_CG:
@@ -266,6 +263,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.alpha.alloc((size_t)cap + 1);
     G.rz.alloc((size_t)cap + 1);
     G.cap = cap;
+    ++G.arrays_gen;
   }
   CgScalars S;
   S.pa = G.pa.p;
@@ -306,100 +304,13 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   // computes the same as with batches)
   const int batch0 = tm.enabled ? 1 : (mg ? 2 * kFcgBatch : (pre ? kFcgBatch : 8));
   const int batch_max = tm.enabled ? 1 : (pre ? kFcgBatch : 128);
-  int it = 0, batch = batch0, replaced_at = 0;
-  bool stopped = false;
-  while (it < cap) {
-    int nb = std::min(batch, cap - it);
-    if (pre && it % kFcgBatch != 0) nb = std::min(nb, kFcgBatch - it % kFcgBatch);
-    for (int k = 0; k < nb;) {
-      if (it > 0 && it % kResidualReplace == 0 && replaced_at != it) {
-        tm.begin(HNS_K_OTHER, s);
-        if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushRunning, s);
-        launch_cg_replace(T, div, r.dx, p, G.r.p, S, false, s);
-        if (pre) launch_fcg_test(T.L, false, S, s);  // test the residual just replaced
-        tm.end(s);
-        replaced_at = it;
-      }
-      // (with per-kernel timing on, the iterations are launched one by one so events can bracket them)
-      if (pre && mg && !tm.enabled && it >= depth && it % kFcgBatch == 0 && nb - k >= kFcgBatch) {
-        tm.begin(HNS_K_PRECOND, s);
-        mg_batch(T, p, S, depth, s);
-        tm.end(s);
-        it += kFcgBatch;
-        k += kFcgBatch;
-        continue;
-      }
-      if (pre) {
-        // flexible CG on z = M(r); the stop test of iteration it > 0 ran after the last update
-        if (it == 0) {
-          tm.begin(HNS_K_OTHER, s);
-          launch_fcg_test(T.L, false, S, s);
-          tm.end(s);
-        }
-        if (mg) {
-          mg_->prepare(T, s);
-          mg_iteration(T, p, S, depth, it, s, &tm);
-          ++it;
-          ++k;
-          continue;
-        }
-        tm.begin(HNS_K_PRECOND, s);
-        r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
-        tm.end(s);
-        tm.begin(HNS_K_CG_PRODUCT, s);
-        launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
-        tm.end(s);
-        tm.begin(HNS_K_CG_UPDATE, s);
-        launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, s);
-        launch_fcg_test(T.L, true, S, s);
-        tm.end(s);
-      } else {
-        float* d_in = (it & 1) ? G.d1.p : G.d0.p;
-        float* d_out = (it & 1) ? G.d0.p : G.d1.p;
-        tm.begin(HNS_K_CG_PRODUCT, s);
-        launch_cg_product(T, it, it == 0, G.r.p, d_in, d_out, G.q.p, S, s);
-        tm.end(s);
-        tm.begin(HNS_K_CG_UPDATE, s);
-        launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
-        tm.end(s);
-      }
-      ++it;
-      ++k;
-    }
-    if (pre) {
-      // a stop raised in the batch is re-tested on r = b - A x before the host looks, so one read
-      // decides: stopped, or continue from the device's iteration
-      tm.begin(HNS_K_OTHER, s);
-      if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushStopped, s);
-      launch_fcg_retest(T, div, r.dx, p, G.r.p, S, s);
-      tm.end(s);
-      HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
-      HNS_HIP(hipMemcpyAsync(G.h_last, G.last.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-      HNS_HIP(hipStreamSynchronize(s));
-      if (G.h_flag[0]) {
-        stopped = true;
-        break;
-      }
-      it = G.h_flag[2];
-      batch = batch_max;
-      continue;
-    }
-    HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-    HNS_HIP(hipStreamSynchronize(s));
-    if (G.h_flag[0]) {
-      it = G.h_flag[1];
-      if (it == replaced_at) {
-        stopped = true;
-        break;
-      }
-      tm.begin(HNS_K_OTHER, s);
-      launch_cg_replace(T, div, r.dx, p, G.r.p, S, true, s);
-      tm.end(s);
-      replaced_at = it;
-      batch = batch0;
-      continue;
-    }
-    batch = std::min(batch * 2, batch_max);
+  LoopResult lr;
+  if (pre) {
+    FcgDev d{this, T, div, p, r, S, depth, mg, tm, s};
+    lr = fcg_loop(d, cap, batch0, batch_max, kFcgBatch, kResidualReplace);
+  } else {
+    CgDev d{this, T, div, p, r, S, tm, s};
+    lr = cg_loop(d, cap, batch0, batch_max, kResidualReplace);
   }
   if (S.xdone) {  // x complete before anything reads it (nothing pending after a stop: it was flushed)
     tm.begin(HNS_K_OTHER, s);
@@ -407,25 +318,112 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     tm.end(s);
   }
   SolveResult res;
-  res.iterations = stopped ? G.h_flag[1] : it;
-  if (pre && stopped) {
+  res.iterations = lr.iterations;
+  res.end = lr.end;
+  if (pre && lr.end == kLoopStopped) {
     // the stop stands on r = b - A x evaluated in FP64 (its r.r and ||b||^2 came with the flag)
-    const double rr = G.h_last[0], bb = G.h_last[1];
+    const double rr = lr.rr, bb = lr.bb;
     res.rel_residual = bb > 0.0 ? std::sqrt(rr / bb) : (rr > 0.0 ? INFINITY : 0.0);
   } else {
     res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
   }
+  if (!std::isfinite(res.rel_residual)) res.end = kLoopNonFinite;
   // plain RBGS is the fallback of the learned preconditioner (an untrained network can stall the
-  // solve): continue from the PCG iterate. The multigrid solve stops at max_iters like CG.
-  if (learned && !(res.rel_residual <= (double)r.tol)) {
-    const int chunk = 50, cap_fb = 200000;
-    while (!(res.rel_residual <= (double)r.tol) && res.fallback < cap_fb) {
-      rbgs_iters(T, div, p, 1, chunk, r, tm, s);
-      res.fallback += chunk;
-      res.rel_residual = rel_residual(T, div, p, r.dx, tm, s);
-    }
+  // solve, a breakdown can end it): continue from the PCG iterate while the residual is finite.
+  // The multigrid solve stops at max_iters like CG.
+  if (learned && res.end != kLoopNonFinite && !(res.rel_residual <= (double)r.tol)) {
+    RbgsDev d{this, T, div, p, r, tm, s};
+    const LoopResult fb = rbgs_tol_loop(d, (double)r.tol, 200000, 50, 1, res.rel_residual, &res.rel_residual);
+    res.fallback = fb.iterations;
+    res.end = fb.end;
   }
   return res;
+}
+
+// ---- device operations of the solve loops (hns_solve_loop.hpp) ------------------------------------
+
+void Solvers::RbgsDev::sweeps(int first_it, int n) { sol->rbgs_iters(T, div, p, first_it, n, r, tm, s); }
+double Solvers::RbgsDev::residual() { return sol->rel_residual(T, div, p, r.dx, tm, s); }
+
+void Solvers::CgDev::replace_periodic(int) {
+  tm.begin(HNS_K_OTHER, s);
+  launch_cg_replace(T, div, r.dx, p, sol->cg_->r.p, S, false, s);
+  tm.end(s);
+}
+void Solvers::CgDev::iteration(int it) {
+  CgState& G = *sol->cg_;
+  float* d_in = (it & 1) ? G.d1.p : G.d0.p;
+  float* d_out = (it & 1) ? G.d0.p : G.d1.p;
+  tm.begin(HNS_K_CG_PRODUCT, s);
+  launch_cg_product(T, it, it == 0, G.r.p, d_in, d_out, G.q.p, S, s);
+  tm.end(s);
+  tm.begin(HNS_K_CG_UPDATE, s);
+  launch_cg_update(T.L, it, p, G.r.p, d_out, G.q.p, S, s);
+  tm.end(s);
+}
+CgReport Solvers::CgDev::read() {
+  CgState& G = *sol->cg_;
+  HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HNS_HIP(hipStreamSynchronize(s));
+  return CgReport{G.h_flag[0], G.h_flag[1]};
+}
+void Solvers::CgDev::replace_forced() {
+  tm.begin(HNS_K_OTHER, s);
+  launch_cg_replace(T, div, r.dx, p, sol->cg_->r.p, S, true, s);
+  tm.end(s);
+}
+
+void Solvers::FcgDev::replace_periodic(int) {
+  CgState& G = *sol->cg_;
+  tm.begin(HNS_K_OTHER, s);
+  if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushRunning, s);
+  launch_cg_replace(T, div, r.dx, p, G.r.p, S, false, s);
+  launch_fcg_test(T.L, false, S, s);  // test the residual just replaced
+  tm.end(s);
+}
+// (with per-kernel timing on, the iterations are launched one by one so events can bracket them)
+bool Solvers::FcgDev::batchable(int it, int) { return mg && !tm.enabled && it >= depth; }
+void Solvers::FcgDev::batch(int) {
+  tm.begin(HNS_K_PRECOND, s);
+  sol->mg_batch(T, p, S, depth, s);
+  tm.end(s);
+}
+void Solvers::FcgDev::first_test() {
+  tm.begin(HNS_K_OTHER, s);
+  launch_fcg_test(T.L, false, S, s);
+  tm.end(s);
+}
+// flexible CG on z = M(r); the stop test of iteration it > 0 ran after the last update
+void Solvers::FcgDev::iteration(int it) {
+  CgState& G = *sol->cg_;
+  if (mg) {
+    sol->mg_->prepare(T, s);
+    sol->mg_iteration(T, p, S, depth, it, s, &tm);
+    return;
+  }
+  tm.begin(HNS_K_PRECOND, s);
+  r.pc->forward(T, *r.pcw, G.r.p, G.z.p, false, s);
+  tm.end(s);
+  tm.begin(HNS_K_CG_PRODUCT, s);
+  launch_fcg_direction(T, it, depth, G.r.p, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s);
+  tm.end(s);
+  tm.begin(HNS_K_CG_UPDATE, s);
+  launch_fcg_update(T.L, p, G.r.p, G.dring.p, G.qring.p, S, s);
+  launch_fcg_test(T.L, true, S, s);
+  tm.end(s);
+}
+// a stop raised in the batch is re-tested on r = b - A x before the host looks, so one read
+// decides: stopped, or continue from the device's iteration
+FcgReport Solvers::FcgDev::end_batch() {
+  CgState& G = *sol->cg_;
+  tm.begin(HNS_K_OTHER, s);
+  if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushStopped, s);
+  launch_fcg_retest(T, div, r.dx, p, G.r.p, S, s);
+  tm.end(s);
+  HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
+  HNS_HIP(hipMemcpyAsync(G.h_last, G.last.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  HNS_HIP(hipStreamSynchronize(s));
+  return FcgReport{G.h_flag[0], G.h_flag[1], G.h_flag[2], G.h_last[0], G.h_last[1]};
 }
 
 }  // namespace hns

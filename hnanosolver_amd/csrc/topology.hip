@@ -9,6 +9,7 @@
 // coordinates from (leaf, local offset), so the N x 12 B coordinate array never enters HBM
 // after the build.
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "hns_internal.hpp"
@@ -19,6 +20,7 @@ namespace {
 constexpr int kErrNotLeafDense = 1;
 constexpr int kErrOrder = 2;
 constexpr int kErrRange = 4;
+constexpr int kErrHashFull = 8;
 
 // voxelsToGrid sort key of a leaf origin: root key (PointsToGrid.cuh:596-600) then upper and
 // lower child offsets (NanoVDB.h:3568 CoordToOffset at the two internal levels).
@@ -64,14 +66,15 @@ __global__ void k_check_order(const int4* __restrict__ origins, int32_t L, int* 
 }
 
 __global__ void k_hash_insert(const int4* __restrict__ origins, int32_t L, unsigned long long* __restrict__ keys,
-                              int32_t* __restrict__ vals, uint32_t hmask) {
+                              int32_t* __restrict__ vals, uint32_t hmask, int* __restrict__ err) {
   const int32_t l = blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= L) return;
   const int4 o = origins[l];
   const unsigned long long k = leaf_key(o.x >> 3, o.y >> 3, o.z >> 3);
   uint32_t h = hash_mix(k) & hmask;
-  // keys are unique (strict order was checked first) and the table is at most half full
-  for (;;) {
+  // keys are unique (strict order was checked first) and the table is at most half full, so a free
+  // slot comes within hmask + 1 probes; the bound only turns a broken invariant into an error
+  for (uint32_t probe = 0; probe <= hmask; ++probe) {
     const unsigned long long prev = atomicCAS(&keys[h], (unsigned long long)kEmptyKey, k);
     if (prev == kEmptyKey) {
       vals[h] = l;
@@ -79,6 +82,7 @@ __global__ void k_hash_insert(const int4* __restrict__ origins, int32_t L, unsig
     }
     h = (h + 1) & hmask;
   }
+  atomicOr(err, kErrHashFull);
 }
 
 __global__ void k_neighbours(DevTopo T, int32_t* __restrict__ nbr) {
@@ -101,7 +105,9 @@ std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float
   if (!coords) fail(HNS_ERR_INVALID_ARGUMENT, "topology: coords is null");
   if (n % kLeafVox) fail(HNS_ERR_TOPOLOGY, "topology: N is not a multiple of 512 (payload is not leaf-dense)");
   if (n > (size_t)INT32_MAX) fail(HNS_ERR_INVALID_ARGUMENT, "topology: N must be below 2^31");
+  static std::atomic<uint64_t> next_id{1};
   auto T = std::make_shared<TopoStore>();
+  T->id = next_id++;
   HNS_HIP(hipGetDevice(&T->device));
   T->N = n;
   T->L = (int32_t)(n / kLeafVox);
@@ -132,12 +138,15 @@ std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float
   T->hkeys.alloc(cap);
   T->hvals.alloc(cap);
   HNS_HIP(hipMemsetAsync(T->hkeys.p, 0xFF, cap * sizeof(unsigned long long), s));
-  k_hash_insert<<<(unsigned)((L + 255) / 256), 256, 0, s>>>(T->origins.p, L, T->hkeys.p, T->hvals.p, T->hmask);
+  k_hash_insert<<<(unsigned)((L + 255) / 256), 256, 0, s>>>(T->origins.p, L, T->hkeys.p, T->hvals.p, T->hmask,
+                                                            derr.p);
   HNS_HIP(hipGetLastError());
   T->nbr.alloc((size_t)L * kNbrStride);
   k_neighbours<<<(unsigned)(((size_t)L * kNbrStride + 255) / 256), 256, 0, s>>>(T->dev(), T->nbr.p);
   HNS_HIP(hipGetLastError());
   HNS_HIP(hipStreamSynchronize(s));
+  HNS_HIP(hipMemcpy(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost));
+  if (herr & kErrHashFull) fail(HNS_ERR_TOPOLOGY, "topology: leaf hash table full (internal error)");
   return T;
 }
 

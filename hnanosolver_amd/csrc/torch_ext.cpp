@@ -19,6 +19,17 @@ void check_tensor(const torch::Tensor& t, const char* what) {
               " must be a contiguous float32 device tensor");
 }
 
+// Owns a tape (checkpoints in HBM) for the lifetime of an autograd node: freed by backward, or by
+// the destructor when the node goes without one (outputs dropped before backward).
+struct TapeHolder : torch::CustomClassHolder {
+  hns_diff_tape* tape = nullptr;
+  explicit TapeHolder(hns_diff_tape* t) : tape(t) {}
+  ~TapeHolder() override {
+    if (tape) (void)hns_diff_tape_destroy(&tape);
+  }
+};
+const auto kTapeHolderClass = torch::class_<TapeHolder>("hns_torch", "TapeHolder");
+
 struct DiffStepsFn : public torch::autograd::Function<DiffStepsFn> {
   static torch::autograd::variable_list forward(torch::autograd::AutogradContext* ctx, torch::Tensor vel0,
                                                 torch::Tensor rho0, int64_t topo, int64_t steps, double dt, double dx,
@@ -32,14 +43,16 @@ struct DiffStepsFn : public torch::autograd::Function<DiffStepsFn> {
     check(hnsk_diff_forward(reinterpret_cast<const hns_topology*>(topo), vel0.data_ptr<float>(), rho0.data_ptr<float>(),
                             (int32_t)steps, (float)dt, (float)dx, (int32_t)iterations, (float)omega, (int32_t)every,
                             v.data_ptr<float>(), r.data_ptr<float>(), &tape, reinterpret_cast<void*>(stream)));
-    ctx->saved_data["tape"] = reinterpret_cast<int64_t>(tape);
-    ctx->saved_data["stream"] = stream;
+    auto holder = c10::make_intrusive<TapeHolder>(tape);
+    if (ctx) ctx->saved_data["tape"] = holder;  // else: no gradient wanted, the holder frees it now
+    if (ctx) ctx->saved_data["stream"] = stream;
     return {v, r};
   }
 
   static torch::autograd::variable_list backward(torch::autograd::AutogradContext* ctx,
                                                  torch::autograd::variable_list g) {
-    auto* tape = reinterpret_cast<hns_diff_tape*>(ctx->saved_data["tape"].toInt());
+    auto holder = ctx->saved_data["tape"].toCustomClass<TapeHolder>();
+    hns_diff_tape* tape = holder->tape;
     void* stream = reinterpret_cast<void*>(ctx->saved_data["stream"].toInt());
     TORCH_CHECK(tape != nullptr, "backward called twice on a differentiable step");
     torch::Tensor gv = g[0].defined() ? g[0].contiguous() : torch::Tensor();
@@ -60,8 +73,7 @@ struct DiffStepsFn : public torch::autograd::Function<DiffStepsFn> {
     check(hnsk_diff_backward(tape, gv.defined() ? gv.data_ptr<float>() : nullptr,
                              gr.defined() ? gr.data_ptr<float>() : nullptr, gv0.data_ptr<float>(), gr0.data_ptr<float>(),
                              stream));
-    check(hns_diff_tape_destroy(&tape));
-    ctx->saved_data["tape"] = (int64_t)0;
+    check(hns_diff_tape_destroy(&holder->tape));
     return {gv0, gr0, torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor(),
             torch::Tensor(), torch::Tensor(), torch::Tensor(), torch::Tensor()};
   }
@@ -69,6 +81,11 @@ struct DiffStepsFn : public torch::autograd::Function<DiffStepsFn> {
 
 std::vector<torch::Tensor> diff_steps(torch::Tensor vel0, torch::Tensor rho0, int64_t topo, int64_t steps, double dt,
                                       double dx, int64_t iterations, double omega, int64_t every, int64_t stream) {
+  if (!torch::GradMode::is_enabled() || !(vel0.requires_grad() || rho0.requires_grad())) {
+    // no gradient can be asked for: run the forward and free its tape at once
+    auto out = DiffStepsFn::forward(nullptr, vel0, rho0, topo, steps, dt, dx, iterations, omega, every, stream);
+    return {out[0], out[1]};
+  }
   auto out = DiffStepsFn::apply(vel0, rho0, topo, steps, dt, dx, iterations, omega, every, stream);
   return {out[0], out[1]};
 }

@@ -11,7 +11,7 @@ from typing import Sequence
 
 from . import _check, lib
 
-HNS_ERR_NOT_CONVERGED = -11  # a to-tolerance solve hit max_iters (results still written)
+HNS_ERR_NOT_CONVERGED = -11  # a to-tolerance solve missed tol: iteration cap or breakdown (results written)
 
 
 def _ptr(t):

@@ -44,7 +44,10 @@ typedef int32_t hns_status;
 #define HNS_ERR_OUT_OF_MEMORY (-8)    /* device allocation failed (Utils.cuh:287-296) */
 #define HNS_ERR_NOT_FOUND (-9)        /* a named block is missing */
 #define HNS_ERR_NO_DEVICE (-10)       /* no HIP device is visible */
-#define HNS_ERR_NOT_CONVERGED (-11)   /* a to-tolerance solve hit max_iters (results still written) */
+#define HNS_ERR_NOT_CONVERGED (-11)   /* a to-tolerance solve missed tol: it hit max_iters or its
+                                         update broke down (results still written) */
+#define HNS_ERR_NON_FINITE (-12)      /* a to-tolerance solve met a residual or right-hand side that
+                                         is not finite (results still written) */
 
 /* Message of the last failing call on this thread (never null). */
 const char* hns_last_error(void);

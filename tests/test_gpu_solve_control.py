@@ -1,0 +1,106 @@
+"""GPU checks of how the to-tolerance pressure solves end (the host loops of
+hnanosolver_amd/csrc/hns_solve_loop.hpp on the real kernels): an update breakdown of the learned
+flexible CG ends the solve and hands it to the RBGS fallback, a non-finite right-hand side ends
+every solver with HNS_ERR_NON_FINITE at once, and hns_advance_ex reports a solve that missed tol
+without a stats struct and across substeps. Every condition is set up from the host (weights,
+inputs, iteration caps); every kernel runs to completion."""
+import ctypes as C
+import time
+
+import numpy as np
+import pytest
+
+import hnanosolver_amd as hns
+from hnanosolver_amd import kernels as K
+from hnanosolver_amd import precond as P
+from hnanosolver_amd import scenes
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name):
+    print("CASE", name, flush=True)
+
+
+def _topo(sc):
+    return hns.create_index_grid(hns.GridIndexedData(sc.coords), sc.voxel_size)
+
+
+def _dense(torch, topo, cs):
+    out = torch.empty_like(cs)
+    K.from_colour_split(topo, cs, out)
+    return out.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.timeout(120, method="thread")
+def test_learned_solve_breakdown_falls_back_to_rbgs(hip):
+    """All-zero weights make z = M(r) = 0, so the first update finds d . A d = 0: the flexible-CG
+    loop must end after one batch without progress and plain RBGS must finish the solve from the
+    iterate, to the oracle-checked residual."""
+    torch = hip
+    sc = scenes.smoke64()
+    _case("learned PCG with a zero network, smoke64 N=%d" % sc.n)
+    topo = _topo(sc)
+    div = P.scene_divergence(sc, topo)
+    p = torch.empty_like(div)
+    pc = P.Preconditioner(np.zeros(P.N_PARAMS, np.float32))
+    t0 = time.perf_counter()
+    st = pc.solve(topo, div, p, sc.voxel_size, tol=1e-5, max_iters=5000)
+    dt = time.perf_counter() - t0
+    r2, d2 = Oracle(sc.coords, np.float64).residual(_dense(torch, topo, p), _dense(torch, topo, div), sc.voxel_size)
+    print("  pcg iterations %d, fallback %d RBGS, rel %.2e (oracle-checked %.2e), %.2f s" % (
+        st.iterations, st.fallback_iterations, st.rel_residual, np.sqrt(r2 / d2), dt))
+    assert st.iterations == 0 and st.fallback_iterations > 0
+    assert np.sqrt(r2 / d2) <= 1.01e-5
+    pc.close()
+
+
+@pytest.mark.timeout(120, method="thread")
+@pytest.mark.parametrize("solver", ["rbgs_tol", "cg", "mgcg", "pcg_learned"])
+def test_non_finite_right_hand_side_ends_the_solve(hip, solver):
+    """One NaN in div: the solve ends with HNS_ERR_NON_FINITE instead of running to its cap."""
+    torch = hip
+    sc = scenes.smoke64()
+    _case("NaN right-hand side, %s" % solver)
+    topo = _topo(sc)
+    div = P.scene_divergence(sc, topo)
+    div[12345] = float("nan")
+    p = torch.empty_like(div)
+    t0 = time.perf_counter()
+    with pytest.raises(hns.HNSError) as e:
+        if solver == "pcg_learned":
+            pc = P.Preconditioner()
+            try:
+                pc.solve(topo, div, p, sc.voxel_size, tol=1e-5, max_iters=5000)
+            finally:
+                pc.close()
+        else:
+            K.pressure_solve(topo, div, p, sc.voxel_size, omega=1.5, solver=solver, tol=1e-5, max_iters=5000)
+    dt = time.perf_counter() - t0
+    print("  status %d (%s) after %.2f s" % (e.value.status, e.value, dt))
+    assert e.value.status == -12
+    assert dt < 10.0
+
+
+@pytest.mark.timeout(120, method="thread")
+def test_advance_ex_reports_a_missed_tolerance_without_stats(hip):
+    """hns_advance_ex with a null stats pointer, two substeps and a 3-iteration CG cap returns
+    HNS_ERR_NOT_CONVERGED (every solve of the call is judged, not only the last, stats or not);
+    the same call with a cap that converges returns HNS_OK."""
+    sc = scenes.smoke64()
+    _case("advance_ex NOT_CONVERGED without stats, 2 substeps")
+    d = hns.GridIndexedData(sc.coords)
+    for k, v in sc.fields.items():
+        d.add_value_block(k, v.copy())
+    d.add_value_block("velocity", sc.velocity.copy())
+    topo = hns.create_index_grid(d, sc.voxel_size)
+    ctx = hns.ensure_context(None, d, topo, sc.voxel_size)
+    params = hns.make_params(**dict(sc.params, substeps=2))
+    lib = hns.lib()
+    for cap, want in ((3, -11), (5000, 0)):
+        o = hns.hns_solve_options(C.sizeof(hns.hns_solve_options), hns.SOLVERS["cg"], 1e-5, cap, 0)
+        st = lib.hns_advance_ex(ctx.handle, 0, sc.dt, C.byref(params), C.byref(o), None)
+        print("  cap %d: status %d %s" % (cap, st, lib.hns_last_error().decode()))
+        assert st == want
+    hns.destroy_context(ctx)
