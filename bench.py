@@ -207,7 +207,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="fire256_8m")
-    ap.add_argument("--legs", default="mgcg,cg,pcg_learned,rbgs,rbgs_tol",
+    # pcg_learned is not a default leg: GPU runs of the learned solve at 1.6M and 8M voxels have
+    # stalled (no output, GPU responsive) with the cause not found
+    ap.add_argument("--legs", default="mgcg,cg,rbgs,rbgs_tol",
                     help="solver legs: mgcg, cg, pcg_learned, rbgs (fixed K), rbgs_tol")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
