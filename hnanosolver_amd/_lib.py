@@ -80,6 +80,7 @@ _SIGS = [
     ("hnsk_divergence", C.c_int32, [_VP, _VP, _VP, C.c_float, C.c_int32, _VP, C.c_int32]),
     ("hnsk_rbgs_sweep", C.c_int32, [_VP, _VP, _VP, C.c_float, C.c_int32, C.c_float, _VP, C.c_int32]),
     ("hnsk_residual", C.c_int32, [_VP, _VP, _VP, C.c_float, C.POINTER(C.c_double), C.POINTER(C.c_double), _VP]),
+    ("hnsk_residual_noalloc", C.c_int32, [_VP, _VP, _VP, C.c_float, _VP, _VP]),
     ("hnsk_pressure_solve", C.c_int32, [_VP, _VP, _VP, C.c_float, C.c_float, C.c_int32, C.POINTER(hns_solve_options),
                                         C.POINTER(hns_solve_stats), _VP]),
     ("hnsk_subtract_gradient", C.c_int32, [_VP, _VP, _VP, _VP, C.c_float, _VP, _VP, C.c_int32]),

@@ -867,6 +867,16 @@ hns_status hnsk_residual(const hns_topology* topo, const float* p_cs, const floa
   });
 }
 
+hns_status hnsk_residual_noalloc(const hns_topology* topo, const float* p_cs, const float* div_cs, float dx,
+                                 double* out_dev, void* stream) {
+  return guard([&] {
+    const DevTopo T = topo_of(topo);
+    if (!p_cs || !div_cs || !out_dev) fail(HNS_ERR_INVALID_ARGUMENT, "null argument");
+    launch_residual_noalloc(T, p_cs, div_cs, dx, out_dev, static_cast<hipStream_t>(stream));
+    return HNS_OK;
+  });
+}
+
 hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, float* p_cs, float dx, float omega,
                                int32_t iterations, const hns_solve_options* opts, hns_solve_stats* stats,
                                void* stream) {

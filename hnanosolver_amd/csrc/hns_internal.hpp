@@ -145,5 +145,11 @@ void launch_rbgs_iteration(const DevTopo& T, const float* div_cs, const float* p
 // Writes per-workgroup partial sums (2 doubles per leaf) then reduces into out[0..1].
 void launch_residual(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* partial,
                      double* out, hipStream_t s);
+// The same with the partial sums in the library's static device scratch (no allocation; fails for
+// more than HNS_NOALLOC_MAX_LEAVES leaves). Calls that overlap on one device race on the scratch.
+void launch_residual_noalloc(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* out,
+                             hipStream_t s);
+// Looks up the scratch array's address on the current device once (called when a topology is built).
+void noalloc_setup();
 
 }  // namespace hns

@@ -10,12 +10,20 @@
 // from the same leaf or a face-neighbour leaf. Per sweep and voxel of the domain that is
 // 2 B div + 2 B p + 2 B p written + 2 B of the other colour = 8 B; 16 B per iteration.
 #include <cstdlib>
+#include <mutex>
 
 #include "hns_internal.hpp"
 #include "hns_mg.hpp"
 
 namespace hns {
 namespace {
+
+// Scratch of the allocation-free residual (hnsk_residual_noalloc): the per-leaf partial sums
+// (2 doubles per leaf) of k_residual for up to kNoAllocMaxLeaves leaves, 2 MiB per device, held
+// until the process ends. Host code never passes `scratch` itself to a kernel (in host code the
+// name is only a stand-in for the symbol): it passes the device address hipGetSymbolAddress gave
+// for the current device, looked up once per device.
+static __device__ __align__(256) double scratch[2 * (size_t)HNS_NOALLOC_MAX_LEAVES];
 
 constexpr int kNbXm = 4, kNbXp = 22, kNbYm = 10, kNbYp = 16, kNbZm = 12, kNbZp = 14;  // (dx+1)*9+(dy+1)*3+(dz+1)
 
@@ -302,6 +310,36 @@ void launch_residual(const DevTopo& T, const float* p_cs, const float* div_cs, f
   HNS_HIP(hipGetLastError());
   k_reduce_pairs<<<1, 1024, 0, s>>>(partial, T.L, out);
   HNS_HIP(hipGetLastError());
+}
+
+// Device address of `scratch` on the current device, looked up once per device and kept (each
+// device holds its own copy of the array at its own address). The lookup runs at set-up, when a
+// topology is built on the device (noalloc_setup); the runtime may back the array then.
+static double* scratch_ptr() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static double* ptrs[kMaxDevices] = {};
+  int dev = 0;
+  HNS_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(HNS_ERR_INVALID_ARGUMENT, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    HNS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptrs[dev] = static_cast<double*>(p);
+  }
+  return ptrs[dev];
+}
+
+void noalloc_setup() { (void)scratch_ptr(); }
+
+void launch_residual_noalloc(const DevTopo& T, const float* p_cs, const float* div_cs, float dx, double* out,
+                             hipStream_t s) {
+  if (T.L > HNS_NOALLOC_MAX_LEAVES)
+    fail(HNS_ERR_INVALID_ARGUMENT, "hnsk_residual_noalloc: topology has " + std::to_string(T.L) +
+                                       " leaves, above HNS_NOALLOC_MAX_LEAVES (" +
+                                       std::to_string(HNS_NOALLOC_MAX_LEAVES) + ")");
+  launch_residual(T, p_cs, div_cs, dx, scratch_ptr(), out, s);
 }
 
 }  // namespace hns

@@ -147,6 +147,7 @@ std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float
   HNS_HIP(hipStreamSynchronize(s));
   HNS_HIP(hipMemcpy(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost));
   if (herr & kErrHashFull) fail(HNS_ERR_TOPOLOGY, "topology: leaf hash table full (internal error)");
+  noalloc_setup();  // the allocation-free entry points' scratch address on this device
   return T;
 }
 

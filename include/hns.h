@@ -245,6 +245,18 @@ hns_status hnsk_rbgs_sweep(const hns_topology* topo, const float* div_cs, float*
 /* ||div - L p||^2 and ||div||^2 in FP64 (colour-split p and div); synchronous. */
 hns_status hnsk_residual(const hns_topology* topo, const float* p_cs, const float* div_cs, float dx,
                          double* r2, double* d2, void* stream);
+/* hnsk_residual without allocating. No counterpart in the reference; for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Writes ||div - L p||^2 and ||div||^2 (the same bits as hnsk_residual) to out_dev[0]
+ * and out_dev[1], a caller-owned device buffer, and returns without synchronising `stream`. The
+ * per-leaf partial sums live in a static device array of the library (2 MiB per device, kept until
+ * the process ends; its address is looked up when a topology is built on the device, where the
+ * runtime may back it), so topologies above HNS_NOALLOC_MAX_LEAVES leaves are rejected with
+ * HNS_ERR_INVALID_ARGUMENT. Every call on a device shares that array: calls that overlap on one
+ * device (different streams, or host threads) race and must be ordered by the caller. */
+#define HNS_NOALLOC_MAX_LEAVES 131072
+hns_status hnsk_residual_noalloc(const hns_topology* topo, const float* p_cs, const float* div_cs, float dx,
+                                 double* out_dev, void* stream);
 /* A whole pressure solve from p = 0 with one of the HNS_SOLVER_* methods on colour-split div and
  * p (RBGS uses `iterations` and `omega`; the to-tolerance solvers use opts). Synchronous. */
 hns_status hnsk_pressure_solve(const hns_topology* topo, const float* div_cs, float* p_cs, float dx, float omega,
