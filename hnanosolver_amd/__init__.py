@@ -139,6 +139,12 @@ class Topology:
         _check(lib().hns_topology_neighbours(self._h, out.ctypes.data_as(_lib.c_int32_p), out.size))
         return out
 
+    def builds(self):
+        """(device builds, reuses): calls of create_index_grid that kept this handle count as reuses."""
+        b, r = C.c_int64(), C.c_int64()
+        _check(lib().hns_topology_builds(self._h, C.byref(b), C.byref(r)))
+        return b.value, r.value
+
     def destroy(self):
         if self._h:
             h = C.c_void_p(self._h.value if isinstance(self._h, C.c_void_p) else self._h)
@@ -152,11 +158,18 @@ class Topology:
             pass
 
 
-def create_index_grid(data: GridIndexedData, voxel_size: float) -> Topology:
-    h = C.c_void_p()
+def create_index_grid(data: GridIndexedData, voxel_size: float, topo: Optional[Topology] = None) -> Topology:
+    """CreateIndexGrid. With `topo` (the previous cook's handle) the handle is kept when the payload's
+    leaf origins and voxel size are unchanged, and rebuilt in place otherwise."""
     g = data._c()
+    if topo is None or not topo.handle:
+        h = C.c_void_p()
+        _check(lib().hns_create_index_grid(C.byref(g), float(voxel_size), C.byref(h)))
+        return Topology(h)
+    h = topo.handle if isinstance(topo.handle, C.c_void_p) else C.c_void_p(topo.handle)
     _check(lib().hns_create_index_grid(C.byref(g), float(voxel_size), C.byref(h)))
-    return Topology(h)
+    topo._h = h
+    return topo
 
 
 def leaf_order(origins: np.ndarray) -> np.ndarray:

@@ -20,6 +20,11 @@ class hns_grid_data(C.Structure):
                 ("blocks", C.POINTER(hns_value_block))]
 
 
+class hns_leaf_grid(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("type", C.c_int32), ("role", C.c_int32), ("num_leaves", C.c_size_t),
+                ("origins", c_int32_p), ("values", c_float_p), ("active", C.POINTER(C.c_uint64))]
+
+
 class hns_params(C.Structure):
     """CombustionParams (src/Cuda/Kernels.cuh:5-24), 72 bytes."""
     _fields_ = [("expansionRate", C.c_float), ("temperatureRelease", C.c_float), ("buoyancyStrength", C.c_float),
@@ -53,6 +58,13 @@ _SIGS = [
     ("hns_topology_info", C.c_int32, [_VP, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
     ("hns_topology_neighbours", C.c_int32, [_VP, c_int32_p, C.c_size_t]),
     ("hns_leaf_order", C.c_int32, [c_int32_p, C.c_size_t, C.POINTER(C.c_uint64)]),
+    ("hns_topology_builds", C.c_int32, [_VP, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    ("hns_payload_build", C.c_int32, [C.POINTER(hns_leaf_grid), C.c_size_t, C.c_size_t, C.c_int32, C.POINTER(_VP)]),
+    ("hns_payload_view", C.c_int32, [_VP, C.POINTER(hns_grid_data), c_int32_p]),
+    ("hns_payload_destroy", C.c_int32, [C.POINTER(_VP)]),
+    ("hns_scatter_leaves", C.c_int32, [C.POINTER(hns_grid_data), C.c_char_p, C.POINTER(C.c_uint8), C.c_float,
+                                       c_int32_p, c_float_p, C.POINTER(C.c_uint64), C.c_size_t,
+                                       C.POINTER(C.c_size_t)]),
     ("hns_ensure_context", C.c_int32, [C.POINTER(_VP), C.POINTER(hns_grid_data), _VP, C.c_float, C.c_int32]),
     ("hns_upload_sources", C.c_int32, [_VP, C.POINTER(hns_grid_data)]),
     ("hns_advance", C.c_int32, [_VP, C.c_int32, C.c_float, C.POINTER(hns_params)]),

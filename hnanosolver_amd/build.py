@@ -20,7 +20,7 @@ LIB = os.path.join(PKG, "libhns.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
 SOURCES = ["topology.hip", "kernels_step.hip", "kernels_advect.hip", "kernels_pressure.hip", "kernels_cg.hip", "solvers.hip",
-           "kernels_precond.hip", "precond.hip", "kernels_adjoint.hip", "adjoint.hip", "kernels_mg.hip", "api.hip"]
+           "kernels_precond.hip", "precond.hip", "kernels_adjoint.hip", "adjoint.hip", "kernels_mg.hip", "ingest.hip", "api.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("HNS_OFFLOAD_ARCH", "gfx950")

@@ -21,6 +21,7 @@ using namespace hns;
 
 struct hns_topology {
   std::shared_ptr<TopoStore> s;
+  int64_t builds = 1, reuses = 0;
   // solver scratch of the device-API pressure solves on this topology, made on first use and
   // reused (calls sharing a topology must not overlap)
   mutable std::unique_ptr<Solvers> sol;
@@ -39,6 +40,10 @@ struct hns_precond {
 };
 struct hns_diff_tape {
   std::unique_ptr<DiffTape> t;
+};
+struct hns_payload {
+  Payload* p = nullptr;
+  ~hns_payload() { destroy_payload(p); }
 };
 
 namespace {
@@ -98,6 +103,19 @@ void check_payload(const hns_grid_data* d) {
     if (!b.name) fail(HNS_ERR_INVALID_ARGUMENT, "a value block has no name");
     if (b.type != HNS_FLOAT && b.type != HNS_VEC3F) fail(HNS_ERR_INVALID_ARGUMENT, std::string("block '") + b.name + "' has an unknown type");
   }
+}
+
+// A topology built from the same leaf origins (and voxel size, on the current device) as this
+// payload: the first coordinate of every leaf compared on the host.
+bool same_leaves(const TopoStore& T, const hns_grid_data* d, float voxel_size) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev != T.device) return false;
+  if (T.N != d->size || T.voxelSize != voxel_size || !d->coords) return false;
+  if (T.host_origins.size() != 3 * (size_t)T.L) return false;
+  for (int32_t l = 0; l < T.L; ++l)
+    if (std::memcmp(&T.host_origins[3 * (size_t)l], d->coords + 3 * (size_t)l * kLeafVox, 3 * sizeof(int32_t)))
+      return false;
+  return true;
 }
 
 struct Field {
@@ -363,10 +381,62 @@ hns_status hns_create_index_grid(const hns_grid_data* data, float voxel_size, hn
     if (!data) fail(HNS_ERR_INVALID_ARGUMENT, "CreateIndexGrid: payload is null");
     if (data->size == 0) fail(HNS_ERR_EMPTY, "CreateIndexGrid: N == 0");
     require_device();
+    if (*out && same_leaves(*(*out)->s, data, voxel_size)) {  // unchanged since the last cook: keep it
+      ++(*out)->reuses;
+      return HNS_OK;
+    }
     // built on the null stream: the caller's later work on any blocking stream is ordered after it
     std::shared_ptr<TopoStore> t = build_topology(data->coords, data->size, voxel_size, nullptr);
     if (*out) delete *out;
     *out = new hns_topology{t};
+    return HNS_OK;
+  });
+}
+
+hns_status hns_topology_builds(const hns_topology* topo, int64_t* builds, int64_t* reuses) {
+  return guard([&] {
+    if (!topo) fail(HNS_ERR_INVALID_ARGUMENT, "topology handle is null");
+    if (builds) *builds = topo->builds;
+    if (reuses) *reuses = topo->reuses;
+    return HNS_OK;
+  });
+}
+
+hns_status hns_payload_build(const hns_leaf_grid* grids, size_t num_grids, size_t domain_grid, int32_t padding,
+                             hns_payload** out) {
+  return guard([&] {
+    if (!out) fail(HNS_ERR_INVALID_ARGUMENT, "hns_payload_build: output handle is null");
+    std::unique_ptr<hns_payload> h(new hns_payload());
+    h->p = build_payload(grids, num_grids, domain_grid, padding);
+    if (*out) delete *out;
+    *out = h.release();
+    return HNS_OK;
+  });
+}
+
+hns_status hns_payload_view(hns_payload* payload, hns_grid_data* view, int32_t* pinned) {
+  return guard([&] {
+    if (!payload || !payload->p || !view) fail(HNS_ERR_INVALID_ARGUMENT, "hns_payload_view: null argument");
+    payload_view(payload->p, view, pinned);
+    return HNS_OK;
+  });
+}
+
+hns_status hns_payload_destroy(hns_payload** payload) {
+  return guard([&] {
+    if (payload && *payload) {
+      delete *payload;
+      *payload = nullptr;
+    }
+    return HNS_OK;
+  });
+}
+
+hns_status hns_scatter_leaves(const hns_grid_data* data, const char* name, const uint8_t* mask, float background,
+                              int32_t* origins, float* values, uint64_t* active, size_t capacity,
+                              size_t* num_leaves) {
+  return guard([&] {
+    scatter_leaves(data, name, mask, background, origins, values, active, capacity, num_leaves);
     return HNS_OK;
   });
 }

@@ -5,6 +5,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/hns.h"
 #include "hns_device.hpp"
@@ -70,6 +71,7 @@ struct TopoStore {
   DBuf<int32_t> hvals;
   uint32_t hmask = 0;
   uint64_t id = 0;  // unique per build in this process (never reused, unlike a freed address)
+  std::vector<int32_t> host_origins;  // 3 per leaf: the build's leaf origins, for reuse checks
   DevTopo dev() const {
     DevTopo t;
     t.id = id;
@@ -86,6 +88,14 @@ struct TopoStore {
 // Builds the topology from host coords (3N int32). Throws Error on invalid input.
 std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float voxelSize, hipStream_t s);
 void leaf_order_host(const int32_t* origins, size_t L, uint64_t* perm);
+
+// Host ingestion (ingest.hip).
+struct Payload;
+Payload* build_payload(const hns_leaf_grid* grids, size_t num_grids, size_t domain_grid, int32_t padding);
+void destroy_payload(Payload* p);
+void payload_view(Payload* p, hns_grid_data* view, int32_t* pinned);
+void scatter_leaves(const hns_grid_data* d, const char* name, const uint8_t* mask, float background, int32_t* origins,
+                    float* values, uint64_t* active, size_t capacity, size_t* num_leaves);
 
 // ---- kernel launchers (kernels_step.hip) ------------------------------------------------
 void launch_advect_velocity(const DevTopo& T, const float* u, float* out, float dt, float inv_dx, const float* sdf,

@@ -10,6 +10,7 @@
 // after the build.
 #include <algorithm>
 #include <atomic>
+#include <cstring>
 #include <vector>
 
 #include "hns_internal.hpp"
@@ -148,6 +149,9 @@ std::shared_ptr<TopoStore> build_topology(const int32_t* coords, size_t n, float
   HNS_HIP(hipMemcpy(&herr, derr.p, sizeof(int), hipMemcpyDeviceToHost));
   if (herr & kErrHashFull) fail(HNS_ERR_TOPOLOGY, "topology: leaf hash table full (internal error)");
   noalloc_setup();  // the allocation-free entry points' scratch address on this device
+  T->host_origins.resize(3 * (size_t)L);
+  for (int32_t l = 0; l < L; ++l) std::memcpy(&T->host_origins[3 * (size_t)l], coords + 3 * (size_t)l * kLeafVox,
+                                              3 * sizeof(int32_t));
   return T;
 }
 

@@ -93,6 +93,49 @@ typedef struct hns_params {
   int32_t simReset;
 } hns_params;
 
+/* ---- host ingestion (the SOP's domain and IndexGridBuilder, SOP_HNanoSolver.cpp:276-301,
+ *      src/Utils/GridBuilder.hpp:93-175, :231-251) ------------------------------------------
+ * A source grid as its leaves, the form OpenVDB and NanoVDB hold them in. Tiles are not
+ * represented (the reference's domain ignores them: dilateVoxels(..., IGNORE_TILES) and a
+ * LeafManager over leaves). */
+#define HNS_ROLE_VALUE 0 /* a value grid (feedback or source field) */
+#define HNS_ROLE_SDF 1   /* the collision SDF: its leaves join the domain, missing leaves read 0x01 bytes */
+typedef struct hns_leaf_grid {
+  const char* name;
+  int32_t type;            /* HNS_FLOAT or HNS_VEC3F */
+  int32_t role;            /* HNS_ROLE_VALUE or HNS_ROLE_SDF */
+  size_t num_leaves;
+  const int32_t* origins;  /* 3 int32 per leaf, multiples of 8, no leaf repeated */
+  const float* values;     /* per leaf 512 floats (HNS_FLOAT) or 512 x (x,y,z) (HNS_VEC3F), offset order
+                              x<<6 | y<<3 | z: the leaf buffer */
+  const uint64_t* active;  /* per leaf 8 words, bit (o & 63) of word (o >> 6) = voxel o active; NULL = all */
+} hns_leaf_grid;
+typedef struct hns_payload hns_payload;
+/* The leaf-dense payload of one cook. Domain: the active voxels of grids[domain_grid] (the primary
+ * velocity) dilated by `padding` voxels over the 26-neighbourhood, united with the active voxels of
+ * the first HNS_ROLE_SDF grid; every voxel of every domain leaf, leaves in voxelsToGrid order.
+ * Blocks: one per grid name in the grids' order (a repeated name keeps the first grid, as
+ * GridIndexedData::addValueBlock does); a grid's leaf buffer is copied where it has a leaf at the
+ * domain leaf's origin, else the leaf is 0 (memset(..., 0, ...)), or for the SDF the byte pattern of
+ * memset(..., 1, ...) (every float 0x01010101, about 2.37e-38; GridBuilder.hpp:116). The payload is in
+ * pinned host memory when a HIP device is visible. */
+hns_status hns_payload_build(const hns_leaf_grid* grids, size_t num_grids, size_t domain_grid, int32_t padding,
+                             hns_payload** out);
+/* The payload as an hns_grid_data (pointers into the payload, valid until it is destroyed);
+ * *pinned (nullable) = 1 when it lives in pinned host memory. */
+hns_status hns_payload_view(hns_payload* payload, hns_grid_data* view, int32_t* pinned);
+hns_status hns_payload_destroy(hns_payload** payload);
+/* The output scatter of SOP_HNanoSolver.cpp:345-388 (scatterScalar / scatterVector): the field
+ * `name` of a leaf-dense payload as a grid whose voxels with mask[i] != 0 are set on with their
+ * value (every other voxel at `background`, inactive), returned as leaves: each payload leaf with a
+ * masked voxel, in payload order, gives its origin (3 int32), 512 values (x3 for a Vec3f block)
+ * and 8 active-mask words. mask NULL = all voxels (HNS_GetActiveMask's mask is identically 1).
+ * *num_leaves receives the output leaf count; the arrays (nullable, to query the count) hold
+ * `capacity` leaves. */
+hns_status hns_scatter_leaves(const hns_grid_data* data, const char* name, const uint8_t* mask, float background,
+                              int32_t* origins, float* values, uint64_t* active, size_t capacity,
+                              size_t* num_leaves);
+
 /* ---- topology (replaces nanovdb::GridHandle<nanovdb::cuda::DeviceBuffer>) ------------ */
 typedef struct hns_topology hns_topology;
 
@@ -100,8 +143,12 @@ typedef struct hns_topology hns_topology;
  * src/Cuda/HNanoSolver.cu:498-510. Builds the device-resident leaf topology (leaf origins,
  * 27-entry neighbour table per leaf, leaf hash) from data->coords. Validates that the payload
  * is leaf-dense and in voxelsToGrid order (HNS_ERR_TOPOLOGY / HNS_ERR_LEAF_ORDER otherwise).
- * *out is replaced (an existing handle is released first). */
+ * The reference rebuilds its grid on every cook (SOP_HNanoSolver.cpp:310-317); here an existing
+ * *out built from the same leaf origins, voxel size and device is kept as it is (the payload's
+ * leaf origins are compared on the host), and any other *out is released and replaced. */
 hns_status hns_create_index_grid(const hns_grid_data* data, float voxel_size, hns_topology** out);
+/* Number of device builds behind a handle (1 + the rebuilds avoided = calls that kept it). */
+hns_status hns_topology_builds(const hns_topology* topo, int64_t* builds, int64_t* reuses);
 /* Releases a topology handle (contexts built from it keep their own reference). */
 hns_status hns_topology_destroy(hns_topology** topo);
 /* Number of voxels / leaves described by a handle. */
