@@ -83,6 +83,21 @@ def build_torch_ext() -> str:
     return TORCH_EXT
 
 
+TEST_PROGRAMS = os.path.join(ROOT, "tests", "cpp", "_build")
+
+
+def build_test_programs() -> str:
+    """tests/cpp/adapter_cook.cpp (a SOP-shaped cook through include/hns.hpp) against the in-tree
+    libhns.so, into tests/cpp/_build/ (the GPU tests run it; nothing is compiled on the GPU box)."""
+    os.makedirs(TEST_PROGRAMS, exist_ok=True)
+    exe = os.path.join(TEST_PROGRAMS, "adapter_cook")
+    _run([os.environ.get("CXX", "g++"), "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+          os.path.join(ROOT, "tests", "cpp", "adapter_cook.cpp"), "-o", exe + ".tmp", "-L", PKG, "-l:libhns.so",
+          "-Wl,-rpath,$ORIGIN/../../../hnanosolver_amd"])
+    os.replace(exe + ".tmp", exe)
+    return exe
+
+
 def build_oracle() -> str:
     _run(["make", "-s", "-C", ORACLE_DIR])
     return os.path.join(ORACLE_DIR, "liboracle_hns.so")
@@ -102,6 +117,8 @@ def main(argv=None):
         print("built", build_torch_ext())
     if a.oracle or every:
         print("built", build_oracle())
+    if every:
+        print("built", build_test_programs())
 
 
 if __name__ == "__main__":

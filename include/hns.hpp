@@ -11,6 +11,8 @@
 //   HNS_DestroyContext(void**)                               HNS_DestroyContext(void**)
 //   AdvectIndexGrid / AdvectIndexGridVelocity /              same names; the stream argument is any
 //   Divergence / ProjectNonDivergent (stream&)               pointer-sized stream handle (or nullptr)
+//   IndexGridBuilder(domain, &data) + addGrid + build        hns::Payload(leaf grids, domain index, padding)
+//   scatterScalar / scatterVector (the SOP's output)         hns::ScatterLeaves(data, name, mask, bg, comps)
 //
 // The payload type is any class shaped like HNS::GridIndexedData (src/Utils/GridData.hpp:16-166):
 // size(), pCoords() (3 int32 per voxel), getBlocksOfType<T>() and pValues<T>(name). Its Vec3 type
@@ -24,6 +26,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hns.h"
@@ -102,6 +105,71 @@ struct PayloadView {
   }
   hns_grid_data* get() { return &view; }
 };
+
+// The leaf-dense payload of one cook built by hns_payload_build (the SOP's domain + IndexGridBuilder,
+// SOP_HNanoSolver.cpp:276-301), owned, with HNS::GridIndexedData's interface (size, pCoords,
+// getBlocksOfType<T>, pValues<T>) so the calls below take it like the reference's payload.
+class Payload {
+ public:
+  Payload(const std::vector<hns_leaf_grid>& grids, size_t domain_grid, int padding) {
+    check(hns_payload_build(grids.data(), grids.size(), domain_grid, padding, &p_));
+    int32_t pinned = 0;
+    check(hns_payload_view(p_, &view_, &pinned));
+    pinned_ = pinned != 0;
+  }
+  Payload(const Payload&) = delete;
+  Payload& operator=(const Payload&) = delete;
+  ~Payload() { hns_payload_destroy(&p_); }
+  size_t size() const { return view_.size; }
+  const int32_t* pCoords() const { return view_.coords; }
+  bool pinned() const { return pinned_; }
+  const hns_grid_data& view() const { return view_; }
+  template <class T>
+  std::vector<std::string> getBlocksOfType() const {
+    std::vector<std::string> out;
+    for (size_t i = 0; i < view_.num_blocks; ++i)
+      if (view_.blocks[i].type == type_code<T>()) out.push_back(view_.blocks[i].name);
+    return out;
+  }
+  template <class T>
+  T* pValues(const std::string& name) {
+    for (size_t i = 0; i < view_.num_blocks; ++i)
+      if (view_.blocks[i].type == type_code<T>() && name == view_.blocks[i].name)
+        return static_cast<T*>(view_.blocks[i].values);
+    return nullptr;
+  }
+
+ private:
+  template <class T>
+  static int32_t type_code() {
+    return sizeof(T) == sizeof(Vec3) && !std::is_same<T, float>::value ? HNS_VEC3F : HNS_FLOAT;
+  }
+  hns_payload* p_ = nullptr;
+  hns_grid_data view_{};
+  bool pinned_ = false;
+};
+
+// The output scatter of SOP_HNanoSolver.cpp:345-388 as leaves: origins, values, active masks.
+struct LeafOutput {
+  std::vector<int32_t> origins;
+  std::vector<float> values;
+  std::vector<uint64_t> active;
+  size_t leaves = 0;
+};
+template <class GridData>
+LeafOutput ScatterLeaves(GridData& data, const std::string& name, const std::vector<unsigned char>& mask,
+                         float background, int comps) {
+  PayloadView<GridData> v(data);
+  LeafOutput o;
+  const uint8_t* m = mask.empty() ? nullptr : mask.data();
+  check(hns_scatter_leaves(v.get(), name.c_str(), m, background, nullptr, nullptr, nullptr, 0, &o.leaves));
+  o.origins.resize(3 * o.leaves);
+  o.values.resize((size_t)comps * 512 * o.leaves);
+  o.active.resize(8 * o.leaves);
+  check(hns_scatter_leaves(v.get(), name.c_str(), m, background, o.origins.data(), o.values.data(), o.active.data(),
+                           o.leaves, &o.leaves));
+  return o;
+}
 
 template <class P>
 inline hns_params to_params(const P& p) {

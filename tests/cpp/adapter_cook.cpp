@@ -3,8 +3,9 @@
 // HNS_DownloadForOutput -> HNS_GetActiveMask -> HNS_DestroyContext. The payload class mirrors
 // HNS::GridIndexedData's interface (size, pCoords, getBlocksOfType<T>, pValues<T>).
 //
-//   adapter_cook errors            -> exercises the host-side error paths (no device needed)
-//   adapter_cook cook IN OUT K DT  -> reads coords/density/velocity from IN, writes results to OUT
+//   adapter_cook errors       -> exercises the host-side error paths (no device needed)
+//   adapter_cook cook IN OUT  -> two cooks of the leaf grids in IN (hns::Payload, the SOP's ingestion);
+//                                writes topology build counts and the scattered output leaves to OUT
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -77,41 +78,91 @@ static int errors() {
 }
 
 static void rd(void* p, size_t sz, size_t n, FILE* f) {
-  if (std::fread(p, sz, n, f) != n) throw std::runtime_error("short read");
+  if (n && std::fread(p, sz, n, f) != n) throw std::runtime_error("short read");
+}
+template <class T>
+static T rd1(FILE* f) {
+  T v;
+  rd(&v, sizeof(T), 1, f);
+  return v;
+}
+static void wr(const void* p, size_t sz, size_t n, FILE* f) {
+  if (n && std::fwrite(p, sz, n, f) != n) throw std::runtime_error("short write");
 }
 
-static int cook(const char* in, const char* out, int K, float dt) {
+// Leaf grids of one cook's inputs (what loadGrid hands the SOP, as leaves).
+struct InGrid {
+  std::string name;
+  int32_t type = 0, role = 0;
+  std::vector<int32_t> origins;
+  std::vector<float> values;
+  std::vector<uint64_t> active;
+};
+
+// Two cooks of SOP_HNanoSolver.cpp:205-396 on the same inputs, the context and topology cached
+// between them as the SOP cache keeps them: payload (domain + IndexGridBuilder) -> CreateIndexGrid ->
+// HNS_EnsureContext -> HNS_UploadSources -> HNS_Advance -> HNS_DownloadForOutput ->
+// HNS_GetActiveMask -> scatter of every value grid. The second cook keeps the topology (same
+// leaves) and advances from the state the first one left.
+static int cook(const char* in, const char* out) {
   FILE* f = std::fopen(in, "rb");
   if (!f) return 2;
-  int64_t n = 0;
-  float dx = 0;
-  rd(&n, 8, 1, f);
-  rd(&dx, 4, 1, f);
-  GridIndexedData d;
-  d.coords.resize(n);
-  rd(d.coords.data(), sizeof(Coord), n, f);
-  d.addValueBlock<float>("density");
-  d.addValueBlock<Vec3f>("velocity");
-  rd(d.pValues<float>("density"), 4, n, f);
-  rd(d.pValues<Vec3f>("velocity"), 12, n, f);
+  const float dx = rd1<float>(f), dt = rd1<float>(f);
+  const int32_t K = rd1<int32_t>(f), padding = rd1<int32_t>(f), ng = rd1<int32_t>(f), dom = rd1<int32_t>(f);
+  std::vector<InGrid> in_grids(ng);
+  for (auto& g : in_grids) {
+    g.name.resize(rd1<int32_t>(f));
+    rd(&g.name[0], 1, g.name.size(), f);
+    g.type = rd1<int32_t>(f);
+    g.role = rd1<int32_t>(f);
+    const int64_t L = rd1<int64_t>(f);
+    const int comps = g.type == HNS_VEC3F ? 3 : 1;
+    g.origins.resize(3 * L);
+    g.values.resize((size_t)comps * 512 * L);
+    g.active.resize(8 * L);
+    rd(g.origins.data(), 4, g.origins.size(), f);
+    rd(g.values.data(), 4, g.values.size(), f);
+    rd(g.active.data(), 8, g.active.size(), f);
+  }
   std::fclose(f);
+  std::vector<hns_leaf_grid> grids;
+  for (auto& g : in_grids)
+    grids.push_back(hns_leaf_grid{g.name.c_str(), g.type, g.role, g.origins.size() / 3, g.origins.data(),
+                                  g.values.data(), g.active.data()});
   CombustionParams p{0.1f, 0.5f, 1.0f, 23.0f, 1.0f, 0.5f, 0.0f, 0, 0, 0, 0, 0, 1, 0.0f, 0.0f, 1.0f, 1, 0};
-  hns::TopologyHandle handle;
+  hns::TopologyHandle handle;  // the SOP cache's grid handle and context (SOP_HNanoSolver.hpp:90-98)
   void* ctx = nullptr;
-  CreateIndexGrid(d, handle, dx);
-  HNS_EnsureContext(&ctx, d, handle, dx, false);
-  HNS_UploadSources(ctx, d);
-  HNS_Advance(ctx, K, dt, p);
-  HNS_DownloadForOutput(ctx, d);
-  std::vector<unsigned char> mask;
-  HNS_GetActiveMask(ctx, mask);
-  EXPECT(mask.size() == (size_t)n);
+  FILE* o = std::fopen(out, "wb");
+  for (int c = 0; c < 2; ++c) {
+    hns::Payload data(grids, (size_t)dom, padding);
+    CreateIndexGrid(data, handle, dx);
+    HNS_EnsureContext(&ctx, data, handle, dx, false);
+    HNS_UploadSources(ctx, data);
+    HNS_Advance(ctx, K, dt, p);
+    HNS_DownloadForOutput(ctx, data);
+    std::vector<unsigned char> mask;
+    HNS_GetActiveMask(ctx, mask);
+    EXPECT(mask.size() == data.size());
+    int64_t builds = 0, reuses = 0;
+    hns::check(hns_topology_builds(handle.get(), &builds, &reuses));
+    const int32_t pinned = data.pinned() ? 1 : 0;
+    wr(&builds, 8, 1, o);
+    wr(&reuses, 8, 1, o);
+    wr(&pinned, 4, 1, o);
+    for (auto& g : in_grids) {
+      if (g.role != HNS_ROLE_VALUE) continue;
+      const int comps = g.type == HNS_VEC3F ? 3 : 1;
+      const hns::LeafOutput s = hns::ScatterLeaves(data, g.name, mask, 0.0f, comps);
+      const int64_t L = (int64_t)s.leaves;
+      wr(&L, 8, 1, o);
+      wr(s.origins.data(), 4, s.origins.size(), o);
+      wr(s.values.data(), 4, s.values.size(), o);
+      wr(s.active.data(), 8, s.active.size(), o);
+    }
+  }
   HNS_DestroyContext(&ctx);
   EXPECT(ctx == nullptr);
-  f = std::fopen(out, "wb");
-  std::fwrite(d.pValues<float>("density"), 4, n, f);
-  std::fwrite(d.pValues<Vec3f>("velocity"), 12, n, f);
-  std::fclose(f);
+  std::fclose(o);
   std::printf("cook: %s\n", fails ? "FAILED" : "ok");
   return fails ? 1 : 0;
 }
@@ -119,11 +170,11 @@ static int cook(const char* in, const char* out, int K, float dt) {
 int main(int argc, char** argv) {
   try {
     if (argc >= 2 && std::strcmp(argv[1], "errors") == 0) return errors();
-    if (argc >= 6 && std::strcmp(argv[1], "cook") == 0) return cook(argv[2], argv[3], std::atoi(argv[4]), std::atof(argv[5]));
+    if (argc >= 4 && std::strcmp(argv[1], "cook") == 0) return cook(argv[2], argv[3]);
   } catch (const std::exception& e) {
     std::printf("exception: %s\n", e.what());
     return 3;
   }
-  std::printf("usage: adapter_cook errors | cook IN OUT K DT\n");
+  std::printf("usage: adapter_cook errors | cook IN OUT\n");
   return 2;
 }
