@@ -158,3 +158,42 @@ def test_trilinear_exact_on_affine_fields():
             assert o.trilinear(f, *q) == pytest.approx(float(q @ a + 3.0), abs=1e-5)
             hits += 1
     assert hits > 50
+
+
+NANOVDB_INDEX = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref",
+                             "nanovdb_index")
+
+
+@pytest.mark.skipif(not os.path.exists(NANOVDB_INDEX),
+                    reason="oracle/_ref/nanovdb_index is built only where the reference's NanoVDB headers are")
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_index_matches_nanovdb(seed, tmp_path):
+    """The leaf order and voxel indices against NanoVDB itself: a ValueOnIndex grid built by the
+    reference's vendored NanoVDB (oracle/_ref/nanovdb_index) from the payload's voxels, given in a
+    shuffled order, has its leaves in the order hns_leaf_order (the product's C++) and the oracle's
+    tree traversal give, and every voxel's index (the offset the reference's kernels read, a miss
+    reading 0) is 1 + its position in the leaf-dense payload."""
+    import struct
+    import subprocess
+
+    import hnanosolver_amd as hns
+    from oracle import leaf_order as oracle_order
+    origins = ragged_origins(seed)
+    coords = canonical_coords(origins)
+    shuffled = coords[np.random.default_rng(seed).permutation(len(coords))]
+    with open(tmp_path / "in.bin", "wb") as f:
+        f.write(struct.pack("<q", len(shuffled)) + shuffled.astype("<i4").tobytes())
+    subprocess.run([NANOVDB_INDEX, str(tmp_path / "in.bin"), str(tmp_path / "out.bin")], check=True, timeout=120)
+    raw = open(tmp_path / "out.bin", "rb").read()
+    L = struct.unpack("<q", raw[:8])[0]
+    nano_leaves = np.frombuffer(raw[8:8 + 12 * L], "<i4").reshape(L, 3)
+    idx = np.frombuffer(raw[8 + 12 * L:], "<u8")
+    assert L == len(origins)
+    np.testing.assert_array_equal(nano_leaves, origins[hns.leaf_order(origins)])
+    np.testing.assert_array_equal(nano_leaves, origins[oracle_order(origins)])
+    np.testing.assert_array_equal(nano_leaves, coords.reshape(-1, 512, 3)[:, 0, :])
+    # the index of each shuffled voxel is 1 + its position in the payload
+    pos = {tuple(c): i for i, c in enumerate(coords.tolist())}
+    want = np.array([pos[tuple(c)] + 1 for c in shuffled.tolist()], np.uint64)
+    np.testing.assert_array_equal(idx, want)
+    print("  %d leaves, %d voxels: NanoVDB order and indices equal" % (L, len(coords)))
