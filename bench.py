@@ -38,7 +38,9 @@ KERNEL_BYTES = {
     "cg_update": 24.0,
 }
 KERNEL_NAMES = {"rbgs": "k_rbgs", "cg_product": "k_cg_spmv", "cg_update": "k_cg_update",
-                "mgcg_update": "k_fcg_rupdate"}
+                "mgcg_update": "k_fcg_rupdate",
+                "mg_vcycle": "V-cycle (k_mg0_pre, k_mgl_pre x2, k_mg_coarse, k_mgl_post x2, k_mg0_post_fcg)"}
+TRAFFIC_FILES = {"mg_vcycle": "mg_vcycle"}
 # the residual update of the multigrid solve: read r, q, write r (its CG_UPDATE class; the solve's x
 # update rides on the coarse-level launch of the next V-cycle)
 KERNEL_BYTES["mgcg_update"] = 12.0
@@ -48,6 +50,9 @@ KERNEL_BYTES["mgcg_update"] = 12.0
 # (read z, d_prev, write d, q: 16 B), the residual update (12 B) and the x update (read x, d, write
 # x: 12 B, in the coarse-level launch)
 MGCG_BYTES_PER_ITERATION = 8.5 + 16.5 + 3.0 + 16.0 + 12.0 + 12.0
+# the V-cycle's share (its seven launches, the x update riding on the coarse-level launch included):
+# the dominant work of the multigrid step, reported as the headline roofline
+KERNEL_BYTES["mg_vcycle"] = 8.5 + 16.5 + 3.0 + 12.0
 
 
 def step_bytes(solve_bytes: float, m_scalars: int) -> float:
@@ -72,7 +77,8 @@ def payload(sc):
 
 def read_traffic(kernel, workload):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
-    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (KERNEL_NAMES.get(kernel, kernel), workload))
+    name = TRAFFIC_FILES.get(kernel, KERNEL_NAMES.get(kernel, kernel))
+    path = os.path.join(ROOT, "profiles", "%s_%s_traffic.txt" % (name, workload))
     if not os.path.exists(path):
         return None
     with open(path) as fh:
@@ -82,13 +88,28 @@ def read_traffic(kernel, workload):
     return None
 
 
-def cpu_baseline(sc, solver, iters_per_step, budget_s):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, iters_per_step, budget_s, K):
     """The CPU oracle (a from-scratch restatement of the reference step; the reference has no CPU
-    solver) on the host cores, on a bounded sample of the same workload: one step without the
-    pressure solve, plus a bounded number of solver iterations on the same domain, scaled to the
-    iteration count the GPU step needed per step."""
+    solver) on the host cores, on a bounded sample of the same workload: a step without the
+    pressure solve plus a bounded number of CG iterations on the same domain, scaled to the GPU CG
+    leg's iteration count, at all host threads and at one thread; and the reference's own step
+    (fixed K RBGS iterations) at all threads."""
     import oracle
     o = oracle.Oracle(sc.coords)
+    all_threads = oracle.num_threads()
+    div = np.random.default_rng(0).standard_normal(sc.n).astype(np.float32)
+    omega = float(np.float32(2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(sc.voxel_size)))))
 
     def step_without_solve():
         fields = {k: v.copy() for k, v in sc.fields.items()}
@@ -96,39 +117,47 @@ def cpu_baseline(sc, solver, iters_per_step, budget_s):
         o.advance(sc.voxel_size, 0, sc.dt, sc.params, sc.velocity.copy(), fields)
         return time.perf_counter() - t0
 
-    t_rest = step_without_solve()
-    div = np.random.default_rng(0).standard_normal(sc.n).astype(np.float32)
-    if solver == "cg":
-        run = lambda n: o.cg(div, sc.voxel_size, tol=0.0, max_iters=n)  # noqa: E731
-    else:
-        omega = float(np.float32(2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(sc.voxel_size)))))
-        p = np.zeros(sc.n, np.float32)
-
-        def run(n):
-            q = p
-            for _ in range(n):
-                q = o.rbgs(div, o.rbgs(div, q, sc.voxel_size, 0, omega), sc.voxel_size, 1, omega)
-    t0 = time.perf_counter()
-    run(2)
-    t_probe = (time.perf_counter() - t0) / 2
-    n = int(max(2, min(iters_per_step, (budget_s - t_rest) / max(t_probe, 1e-6))))
-    # the sample (a step without the solve, then n iterations) repeated until about MIN_S of CPU work
-    MIN_S = 10.0
-    rests, iters = [t_rest], []
-    t_start = time.perf_counter()
-    while True:
+    def cg_iter_s(n):
         t0 = time.perf_counter()
-        run(n)
-        iters.append((time.perf_counter() - t0) / n)
-        if time.perf_counter() - t_start + t_rest >= MIN_S or len(iters) >= 20:
-            break
-        rests.append(step_without_solve())
-    t_rest, t_iter = float(np.mean(rests)), float(np.mean(iters))
-    t_step = t_rest + iters_per_step * t_iter
-    return {"value": sc.n / t_step, "unit": "voxel-steps/s", "cores": oracle.num_threads(), "kind": "port",
-            "sample": "%s (N=%d): %d oracle steps without the pressure solve (%.2f s each) and %d rounds of "
-                      "%d %s iterations (%.4f s each), scaled to the GPU's %.0f iterations per step" % (
-                          sc.name, sc.n, len(rests), t_rest, len(iters), n, solver, t_iter, iters_per_step)}
+        o.cg(div, sc.voxel_size, tol=0.0, max_iters=n)
+        return (time.perf_counter() - t0) / n
+
+    def rbgs_iter_s(n):
+        p = np.zeros(sc.n, np.float32)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            p = o.rbgs(div, o.rbgs(div, p, sc.voxel_size, 0, omega), sc.voxel_size, 1, omega)
+        return (time.perf_counter() - t0) / n
+
+    def sample(threads, min_s, budget):
+        oracle.set_threads(threads)
+        t_rest = step_without_solve()
+        probe = cg_iter_s(2)
+        n = int(max(2, min(iters_per_step, (budget - t_rest) / max(probe, 1e-6))))
+        rests, iters, t_start = [t_rest], [], time.perf_counter()
+        while True:
+            iters.append(cg_iter_s(n))
+            if time.perf_counter() - t_start + t_rest >= min_s or len(iters) >= 20:
+                break
+            rests.append(step_without_solve())
+        return float(np.mean(rests)), float(np.mean(iters)), len(rests), n, len(iters)
+
+    rest_a, it_a, nr_a, n_a, ni_a = sample(all_threads, 10.0, budget_s)
+    rest_1, it_1, nr_1, n_1, ni_1 = sample(1, 0.0, budget_s / 2)  # one round at one thread
+    oracle.set_threads(all_threads)
+    rb = rbgs_iter_s(max(2, min(K, 10)))
+    t_all = rest_a + iters_per_step * it_a
+    t_one = rest_1 + iters_per_step * it_1
+    t_ref = rest_a + K * rb
+    return {"value": sc.n / t_all, "unit": "voxel-steps/s", "cores": all_threads, "kind": "port",
+            "cpu_model": cpu_model(), "value_1_thread": sc.n / t_one,
+            "reference_step_value": sc.n / t_ref,
+            "reference_step": "the reference's step (fixed %d RBGS iterations) at %d threads" % (K, all_threads),
+            "sample": "%s (N=%d): the oracle step without the pressure solve (%.2f s at %d threads, %.2f s at 1) and "
+                      "CG iterations (%d rounds of %d at %d threads, %.4f s each; %d at 1 thread, %.4f s each), "
+                      "scaled to the GPU CG leg's %.0f iterations per step; RBGS iterations %.4f s each" % (
+                          sc.name, sc.n, rest_a, all_threads, rest_1, ni_a, n_a, all_threads, it_a, n_1, it_1,
+                          iters_per_step, rb)}
 
 
 def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrument, K):
@@ -253,11 +282,10 @@ def main(argv=None):
     if head == "rbgs":
         kern, solve_b = "rbgs", 2 * KERNEL_BYTES["rbgs"] * K
     elif head == "mgcg":
-        # the solve's HBM-bound kernel that the instrumented steps bracket alone (they launch the
-        # iterations one by one so events bracket it; the timed steps replay graphs): the residual
-        # update. The V-cycle's kernels are timed as one class; the coarse level's one-workgroup
-        # solve takes longest but is bound by LDS latency.
-        kern, timing_class = "mgcg_update", "cg_update"
+        # the dominant work: the V-cycle (its seven kernels, bracketed as one class by HIP events in
+        # the instrumented steps, which launch the iterations one by one; the timed steps replay the
+        # same kernels from graphs). The residual update is reported beside it.
+        kern, timing_class = "mg_vcycle", "precond"
         solve_b = MGCG_BYTES_PER_ITERATION * L["iterations_per_step"]
     else:
         kern = max(("cg_product", "cg_update"), key=lambda k: L["kernels"][k]["ms_per_step"])
@@ -276,6 +304,8 @@ def main(argv=None):
                           "frac": step_b * value / 1e9 / HBM_PEAK_GBS},
         "legs": legs,
     }
+    if head == "mgcg" and "cg_update" in L.get("kernels", {}):
+        out["roofline_mgcg_update"] = roofline(L, "mgcg_update", sc.n, sc.name, "cg_update")
     if "rbgs" in legs and "kernels" in legs["rbgs"]:
         out["roofline_rbgs_sweep"] = roofline(legs["rbgs"], "rbgs", sc.n, sc.name)
     if "rbgs_tol" in legs:
@@ -283,9 +313,10 @@ def main(argv=None):
             out["speedup_%s_vs_plain_rbgs_to_tol" % s] = legs["rbgs_tol"]["ms_per_step"] / legs[s]["ms_per_step"]
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        # the oracle runs CG (it has no multigrid), at the GPU CG leg's iteration count when there is one
-        cpu_iters = legs["cg"]["iterations_per_step"] if "cg" in legs else L["iterations_per_step"]
-        out["cpu_baseline"] = cpu_baseline(sc, "cg" if head != "rbgs" else "rbgs", cpu_iters, a.cpu_budget)
+        # the oracle runs CG (it has no multigrid), at the GPU CG leg's iteration count when there is one;
+        # the GPU legs of the same algorithms (cg, rbgs) are under "legs"
+        cpu_iters = legs["cg"]["iterations_per_step"] if "cg" in legs else 500
+        out["cpu_baseline"] = cpu_baseline(sc, cpu_iters, a.cpu_budget, K)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -220,3 +220,73 @@ def test_cfg5_fire512_adjoint_4_steps(hip):
     print("  directional derivative along a random velocity direction: adjoint %.6e, central difference "
           "(h = %.2f) %.6e, rel %.2e" % (ad, h, fd, rel_u), flush=True)
     assert rel_r <= 1e-3 and rel_u <= 3e-2
+
+
+# ---- config 3 -------------------------------------------------------------------------------------
+
+def test_cfg3_control128_adjoint(hip):
+    """Differentiable density control at its stated size (control128: sphere R = 60 in 128^3,
+    N = 1,177,088; 16 steps of advect + 80 RBGS iterations, checkpoints every 4). The GPU (FP32)
+    gradient against FP64 central differences of the FP64 oracle at the 32 fixture entries
+    (tests/golden/control128_fd.json, written by make_control128_fd.py): at every entry where the
+    quotients at eps and eps/2 agree (no clamp switch or cell crossing within +-eps) it is within 1e-4
+    of their Richardson extrapolation. The whole gradient is compared with the oracle's FP64 tape and
+    with its FP32 tape (the same algorithm in FP32): the distance to the FP64 tape is the FP32
+    algorithm's own, concentrated where FP32 and FP64 forwards take different clamp branches."""
+    import json
+    import os
+    torch = hip
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "control128_fd.json")))
+    sc = scenes.control128()
+    _case("cfg3 control128 N=%d, %d steps, K=%d, checkpoint every 4" % (sc.n, fx["steps"], fx["iterations"]))
+    assert sc.n == fx["voxels"] == 1177088 and fx["complete"]
+    S, K_, dt, dx, om = fx["steps"], fx["iterations"], fx["dt"], fx["voxel_size"], fx["omega"]
+    topo = hns.create_index_grid(hns.GridIndexedData(sc.coords), dx)
+    u0 = torch.from_numpy(np.ascontiguousarray(sc.velocity.T)).cuda()
+    rho0 = torch.from_numpy(sc.fields["density"]).cuda()
+    tg = torch.from_numpy(sc.sources["target_density"]).cuda()
+    vout, rout = torch.empty_like(u0), torch.empty_like(rho0)
+    t0 = time.perf_counter()
+    tape = diff.forward(topo, u0, rho0, S, dt, dx, K_, om, 4, vout, rout)
+    loss = float(((rout.double() - tg.double()) ** 2).sum())
+    gv0, gr0 = torch.empty_like(u0), torch.empty_like(rho0)
+    tape.backward(None, (2.0 * (rout - tg)).contiguous(), gv0, gr0)
+    torch.cuda.synchronize()
+    info = tape.info()
+    tape.close()
+    print("  GPU forward + backward %.2f s; tape %s; loss gpu %.8e fp64 %.8e" % (
+        time.perf_counter() - t0, info, loss, fx["loss_fp64"]), flush=True)
+    assert info["recompute_mismatches"] == 0 and info["checkpoints"] == 4
+    gv = gv0.cpu().numpy().reshape(3, -1).T.astype(np.float64)
+    gr = gr0.cpu().numpy().astype(np.float64)
+    gscale = fx["grad_scale_fp64"]
+    rels, kept = [], 0
+    for e in fx["entries"]:
+        f1, f2 = e["fd_eps"], e["fd_half_eps"]
+        g = gv[e["index"], e["component"]] if e["field"] == "velocity" else gr[e["index"]]
+        smooth = abs(f1 - f2) <= 1e-5 * max(abs(f1), 1e-3 * gscale)
+        fr = (4.0 * f2 - f1) / 3.0
+        rel = abs(g - fr) / max(abs(fr), 1e-2 * gscale)
+        print("  %-8s %7d/%d fd %.9e fd/2 %.9e -> %.9e  gpu %.9e  fp64 tape %.9e  rel %.2e%s" % (
+            e["field"], e["index"], e["component"], f1, f2, fr, g, e["tape_fp64"], rel,
+            "" if smooth else "  (kink inside +-eps: not compared)"))
+        if smooth:
+            kept += 1
+            rels.append(rel)
+    print("  %d of %d entries smooth; rel err vs FP64 FD: median %.2e, max %.2e" % (
+        kept, len(fx["entries"]), np.median(rels), max(rels)), flush=True)
+    assert kept >= 16 and max(rels) <= 1e-4
+    # the whole gradient against the oracle's tapes (FP64, and the same algorithm in FP32)
+    for dtype, bar in ((np.float64, 2e-3), (np.float32, 2e-3)):
+        t1 = time.perf_counter()
+        o = Oracle(sc.coords, dtype)
+        _, tv, trr = o.diff_loss_grad(sc.velocity.astype(dtype), sc.fields["density"].astype(dtype),
+                                      sc.sources["target_density"].astype(dtype), S, dt, dx, K_, om)
+        ev = np.linalg.norm(gv - tv) / np.linalg.norm(tv)
+        er = np.linalg.norm(gr - trr) / np.linalg.norm(trr)
+        d2 = ((gv - tv) ** 2).ravel()
+        top = np.sort(d2)[::-1][:max(1, d2.size // 10000)].sum() / max(d2.sum(), 1e-300)
+        print("  vs the oracle's %s tape (%.0f s): rel L2 d/du0 %.2e, d/drho0 %.2e; share of the squared d/du0 "
+              "difference in the top 0.01%% entries %.2f" % (np.dtype(dtype).name, time.perf_counter() - t1, ev, er,
+                                                             top), flush=True)
+        assert ev <= bar and er <= bar
