@@ -20,6 +20,16 @@ constexpr int kMgLevels = 4;
 struct MgShape {
   int coarse = 12, smooth0 = 1, smooth = 1, depth = 1, pcg = 0;
   float coarse_omega = 1.85f;
+  // The two-level coarse solve (c4 = 1; 0: `coarse` SOR iterations on level 3 alone): c3_pre and
+  // c3_post red-black iterations on level 3 (relaxations c3_omega and c3_omega_post) around c4_iters
+  // SOR iterations (c4_omega) on level 4 (2^3-leaf blocks, Galerkin operator), the restricted
+  // residual scaled by c4_over. HNS_MG_C4, HNS_MG_C3PRE, HNS_MG_C3POST, HNS_MG_C4ITERS,
+  // HNS_MG_C3OMEGA, HNS_MG_C3OMEGAP, HNS_MG_C4OMEGA and HNS_MG_C4OVER (x100) override them. side_groups workgroups carry the deferred
+  // x update beside the one-workgroup coarse solve after side_sleeps * 127 * 64 idle cycles
+  // (HNS_MG_SIDE, HNS_MG_SIDE_SLEEPS; -1: the one-level solve's CH / 2, none for the two-level one).
+  int c4 = 1, c3_pre = 1, c3_post = 2, c4_iters = 8;
+  float c3_omega = 1.0f, c3_omega_post = 1.4f, c4_omega = 1.7f, c4_over = 1.6f;
+  int side_groups = -1, side_sleeps = -1;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
   float over_deep = 1.6f;  // the same below level 1 (restrictions of levels 1 and 2)
   // smoothing relaxation on level 0 and on levels 1-2 (over-relaxed red-black smoothing: 12 -> 10
@@ -79,7 +89,7 @@ struct Multigrid {
   }
   uint64_t generation = 0;  // advanced whenever buffers or the coarse layout change
   // the coarsest level is solved by one workgroup (after prepare for the topology)
-  bool coarse_fused() const { return coarse_ch_ > 0; }
+  bool coarse_fused() const;
   // z = V(r) on colour-split level-0 vectors: approximately A^-1 r for A = 6 I - (face neighbours).
   // graph: replay the captured V-cycle (captured on first use for these pointers); otherwise launch
   // the kernels one by one. Both run the same kernels on the same data. stop: a device flag (null:
@@ -95,9 +105,15 @@ struct Multigrid {
   // colour-sorted slot layout of the coarsest level for its one-workgroup solve, built on the host
   // once per topology (synchronises s); coarse_ch_ = 0 when the leaves do not fit one workgroup
   void prepare_coarse(const DevTopo& T, hipStream_t s);
-  DBuf<int32_t> cperm_;  // slot -> leaf (-1: padding)
+  DBuf<int32_t> cmap_;   // leaf -> slot
   DBuf<uint32_t> cnb_;   // three planes of packed 16-bit face-neighbour slots
   int coarse_ch_ = 0;
+  // level 4 of the two-level coarse solve (c4_ch_ = 0: none; kernels_mg.hip's C4Dev for the arrays)
+  DBuf<uint32_t> c4nb_, c4w_, c4ch3_, c4par3_;
+  DBuf<float> c4invd_, c4b_, c4u_, rho3_;
+  DBuf<int32_t> c4chl_, c4parl_;
+  int c4_ch_ = 0;
+  bool c4_fused_ = false;  // levels 3 and 4 fit the fused two-level kernel
   uint64_t ckey_id_ = 0;  // topology the coarse layout was made for
   int32_t ckey_L_ = 0;
   hipStream_t cap_ = nullptr;

@@ -12,7 +12,9 @@
 // level 0. Smoothing is red-black Gauss-Seidel on each level's colouring by the parity of the global
 // level coordinate (at level 3, the leaf's).
 #include <algorithm>
+#include <array>
 #include <cstdlib>
+#include <unordered_map>
 #include <vector>
 
 #include "hns_cg.hpp"
@@ -38,6 +40,16 @@ const MgShape& mg_shape() {
     m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
     m.over = 0.01f * (float)env("HNS_MG_OVER", 50, 300, (int)(100.0f * m.over + 0.5f));
     m.over_deep = 0.01f * (float)env("HNS_MG_OVER_DEEP", 50, 300, (int)(100.0f * m.over + 0.5f));
+    m.c4 = env("HNS_MG_C4", 0, 1, m.c4);
+    m.c3_pre = env("HNS_MG_C3PRE", 1, 8, m.c3_pre);
+    m.c3_post = env("HNS_MG_C3POST", 0, 8, m.c3_post);
+    m.c4_iters = env("HNS_MG_C4ITERS", 1, 64, m.c4_iters);
+    m.c3_omega = 0.01f * (float)env("HNS_MG_C3OMEGA", 50, 199, (int)(100.0f * m.c3_omega + 0.5f));
+    m.c3_omega_post = 0.01f * (float)env("HNS_MG_C3OMEGAP", 50, 199, (int)(100.0f * m.c3_omega_post + 0.5f));
+    m.c4_omega = 0.01f * (float)env("HNS_MG_C4OMEGA", 50, 199, (int)(100.0f * m.c4_omega + 0.5f));
+    m.c4_over = 0.01f * (float)env("HNS_MG_C4OVER", 50, 300, (int)(100.0f * m.c4_over + 0.5f));
+    m.side_groups = env("HNS_MG_SIDE", 1, 255, m.side_groups);
+    m.side_sleeps = env("HNS_MG_SIDE_SLEEPS", 0, 16, m.side_sleeps);
     return m;
   }();
   return v;
@@ -122,7 +134,11 @@ __global__ __launch_bounds__(kTpb) void k_mg_leaf_sweep(DevTopo T, const float* 
 
 // Level 3 solved by `iters` red-black SOR iterations in one workgroup. The leaves are laid out in
 // colour-sorted slots (Multigrid::prepare_coarse): colour 0 in slots [0, H), colour 1 in [H, 2H),
-// H = kCoarseTpb * CH, unused slots padding; slot 2H is a zero that absent neighbours point at.
+// H = kCoarseTpb * CH, unused slots padding; slot 2H is a zero that absent neighbours point at. b and
+// u are in slot order too (the level-2 passes place each leaf's value at its slot), so the solve's
+// loads and stores are coalesced: from leaf-ordered vectors, gathering b and scattering u through
+// the slot map cost one workgroup ~16 us of single-CU address work at 8 M voxels. Padding slots of
+// u receive values that nothing reads.
 // u lives in LDS by slot; lane t owns slots t + k * kCoarseTpb (k < CH: colour 0, else colour 1)
 // and keeps their scaled right-hand sides, old values and face-neighbour slots (16-bit pairs) in
 // registers, so a colour sweep is 6 * CH LDS reads per lane, a barrier, and no branches (a padding
@@ -137,12 +153,13 @@ constexpr int kCoarseMaxCH = 8;
 constexpr int kCoarseSideGroups = 96;
 
 // Workgroup g of G extra workgroups of the coarse launch: the flexible CG's deferred
-// x += alpha_{k-1} d_{k-1} (k = *it; McgXUpdate) over n floats, k_fcg_update's fmaf, four loads of
-// each vector in flight per lane, after SLEEPS * 127 * 64 idle cycles
-template <int SLEEPS>
-__device__ void coarse_side_xupdate(const McgXUpdate& X, const int* __restrict__ it, int64_t n, int g, int G) {
-#pragma unroll
-  for (int j = 0; j < SLEEPS; ++j) __builtin_amdgcn_s_sleep(127);
+// x += alpha_{k-1} d_{k-1} (k = *it; McgXUpdate) over n floats, k_fcg_update's fmaf, after
+// sleeps * 127 * 64 idle cycles. Four float4 of each vector in flight per lane (n is a multiple of
+// 512; a vector not 16-byte aligned takes the same loop a float at a time): with one float per load
+// the update of the 8 M-voxel solve (96 MB) took ~45 us on 255 CUs and outlasted the coarse solve.
+__device__ void coarse_side_xupdate_n(const McgXUpdate& X, const int* __restrict__ it, int64_t n, int g, int G,
+                                      int sleeps) {
+  for (int j = 0; j < sleeps; ++j) __builtin_amdgcn_s_sleep(127);
   const int32_t k = *it;
   if (k < 1 || *X.xdone >= k) return;
   const float alpha = (float)X.alpha[k - 1];
@@ -150,32 +167,40 @@ __device__ void coarse_side_xupdate(const McgXUpdate& X, const int* __restrict__
   float* __restrict__ x = X.x;
   const int64_t stride = (int64_t)G * kCoarseTpb;
   int64_t i = (int64_t)g * kCoarseTpb + threadIdx.x;
-  for (; i + 3 * stride < n; i += 4 * stride) {
-    const float d0 = d[i], d1 = d[i + stride], d2 = d[i + 2 * stride], d3 = d[i + 3 * stride];
-    const float x0 = x[i], x1 = x[i + stride], x2 = x[i + 2 * stride], x3 = x[i + 3 * stride];
-    x[i] = fmaf(alpha, d0, x0);
-    x[i + stride] = fmaf(alpha, d1, x1);
-    x[i + 2 * stride] = fmaf(alpha, d2, x2);
-    x[i + 3 * stride] = fmaf(alpha, d3, x3);
+  if (((uintptr_t)d | (uintptr_t)x) & 15) {
+    for (; i < n; i += stride) x[i] = fmaf(alpha, d[i], x[i]);
+    return;
   }
-  for (; i < n; i += stride) x[i] = fmaf(alpha, d[i], x[i]);
+  const float4* __restrict__ d4 = reinterpret_cast<const float4*>(d);
+  float4* __restrict__ x4 = reinterpret_cast<float4*>(x);
+  const int64_t n4 = n >> 2;
+  auto upd = [alpha](float4 a, float4 b) {
+    return make_float4(fmaf(alpha, a.x, b.x), fmaf(alpha, a.y, b.y), fmaf(alpha, a.z, b.z), fmaf(alpha, a.w, b.w));
+  };
+  for (; i + 3 * stride < n4; i += 4 * stride) {
+    const float4 a0 = d4[i], a1 = d4[i + stride], a2 = d4[i + 2 * stride], a3 = d4[i + 3 * stride];
+    const float4 b0 = x4[i], b1 = x4[i + stride], b2 = x4[i + 2 * stride], b3 = x4[i + 3 * stride];
+    x4[i] = upd(a0, b0);
+    x4[i + stride] = upd(a1, b1);
+    x4[i + 2 * stride] = upd(a2, b2);
+    x4[i + 3 * stride] = upd(a3, b3);
+  }
+  for (; i < n4; i += stride) x4[i] = upd(d4[i], x4[i]);
 }
 
 // Workgroup 0 solves the coarsest level; workgroups 1.. (launched only with X.x set) apply the
 // deferred x update, on CUs the solve leaves idle.
 template <int CH>
-__global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restrict__ perm,
-                                                          const uint32_t* __restrict__ nbp, const float* __restrict__ b,
+__global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const uint32_t* __restrict__ nbp, const float* __restrict__ b,
                                                           float* __restrict__ u, float inv_s, float omega, int iters,
                                                           const int* __restrict__ stop, McgXUpdate X,
-                                                          const int* __restrict__ it_dev, int64_t n) {
+                                                          const int* __restrict__ it_dev, int64_t n, int sleeps) {
   constexpr int C = 2 * CH, H = kCoarseTpb * CH;
   __shared__ float s_u[2 * H + 1];
   __shared__ float s_bi[2 * H];  // scaled right-hand sides by slot (in LDS to leave registers for loads)
   if (stop && *stop) return;
   if (blockIdx.x > 0) {
-    // the solve's length grows with CH, and so does the sleep
-    coarse_side_xupdate<CH / 2>(X, it_dev, n, (int)blockIdx.x - 1, (int)gridDim.x - 1);
+    coarse_side_xupdate_n(X, it_dev, n, (int)blockIdx.x - 1, (int)gridDim.x - 1, sleeps);
     return;
   }
   const int t = threadIdx.x;
@@ -184,11 +209,10 @@ __global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restr
 #pragma unroll
   for (int k = 0; k < C; ++k) {
     const int j = t + k * kCoarseTpb;
-    const int32_t leaf = perm[j];
     nb[k][0] = nbp[j];          // +x | -x << 16
     nb[k][1] = nbp[2 * H + j];  // +y | -y << 16
     nb[k][2] = nbp[4 * H + j];  // +z | -z << 16
-    s_bi[j] = leaf >= 0 ? b[leaf] * inv_s : 0.f;
+    s_bi[j] = b[j] * inv_s;
     uo[k] = 0.f;
   }
   for (int j = t; j <= 2 * H; j += kCoarseTpb) s_u[j] = 0.f;
@@ -239,18 +263,363 @@ __global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse(const int32_t* __restr
     }
   }
 #pragma unroll
-  for (int k = 0; k < C; ++k) {
-    const int32_t leaf = perm[t + k * kCoarseTpb];
-    if (leaf >= 0) u[leaf] = uo[k];
+  for (int k = 0; k < C; ++k) u[t + k * kCoarseTpb] = uo[k];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Two-level coarse solve. Level 4 groups the leaves in blocks of 2^3 (block coordinate = leaf
+// coordinate >> 1, colour = the block coordinate's parity). Its operator is the Galerkin product
+// P^T A3 P of level 3's A3 = 6 I - (face neighbours) under piecewise-constant P: block B carries
+// D_B = the sum over its leaves of (6 - their face neighbours inside B) on the diagonal and -n_BB'
+// (the leaf face pairs between B and B') towards each face-adjacent block: 24 and -4 for full
+// blocks (4 A3 at the block scale), the exact weights for partial ones. The coarse solve runs nu1
+// red-black Gauss-Seidel iterations (relaxation omega3) on level 3 from zero, restricts the
+// residual (a block's leaves summed, times over4), runs it4 red-black SOR iterations (omega4) on
+// level 4 from zero, adds each block's correction to its leaves and runs nu2 level-3 iterations in
+// the reverse colour order. On the 8 M-voxel fire topology (16320 leaves, 2368 blocks) one
+// iteration before and after with 6 on level 4 leaves a smaller energy-norm error than 12 SOR
+// iterations (1.85) on level 3 alone, with 4 level-3 colour sweeps instead of 24.
+// Level-4 slots are colour-sorted like level 3's (slot t + k * kCoarseTpb of colour k / CH4 for
+// lane t, slot 2 H4 a zero); a lane keeps each slot's packed face-neighbour slots, weights n_BB'
+// (3 bits each in the order +x, -x, +y, -y, +z, -z), 1 / D_B and right-hand side in registers.
+constexpr int kC4MaxCH = 4;       // level-4 slots per lane and colour (the one-workgroup level-4 solve)
+constexpr int kC4MaxFusedCH = 2;  // the same inside the fused two-level kernel (its registers)
+
+struct C4Dev {
+  const uint32_t* nb;    // 3 planes of 2 H4 words: slot across +a | slot across -a << 16
+  const uint32_t* w;     // 2 H4 words: the six face weights
+  const float* invd;     // 2 H4: 1 / D_B (0 for padding slots)
+  const uint32_t* ch3;   // fused: 4 planes of 2 H4 words: level-3 slots of children 2p | 2p + 1 << 16
+  const uint32_t* par3;  // fused: H words: level-4 slot of level-3 slot j | of slot j + H << 16
+  const int32_t* chl;    // launches: 8 planes of 2 H4 leaves (-1: absent), child q = 4 dx + 2 dy + dz
+  const int32_t* parl;   // launches: level-4 slot by leaf
+  float* b4;             // launches: 2 H4 right-hand sides
+  float* u4;             // launches: 2 H4 corrections
+  int it4, nu1, nu2;
+  float omega4, over4, omega3, omega3p;
+};
+
+template <int CH4>
+struct L4Regs {
+  uint32_t nb[2 * CH4][3];
+  uint32_t w[2 * CH4];
+  float invd[2 * CH4], b[2 * CH4], eo[2 * CH4];
+};
+template <int CH4>
+__device__ __forceinline__ void l4_load(const C4Dev& C, L4Regs<CH4>& R) {
+  constexpr int H4 = CH4 * kCoarseTpb;
+#pragma unroll
+  for (int k = 0; k < 2 * CH4; ++k) {
+    const int j = (int)threadIdx.x + k * kCoarseTpb;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) R.nb[k][a] = C.nb[2 * a * H4 + j];
+    R.w[k] = C.w[j];
+    R.invd[k] = C.invd[j];
   }
+}
+// iters (>= 1) red-black SOR iterations on level 4 from zero: e (LDS, by slot) with e[2 H4] = 0
+// read before it is first written only by the colour-1 sweep (the caller zeroes it before the
+// colour-0 sweep's barrier); ends with a barrier, the corrections in e and in R.eo
+template <int CH4>
+__device__ __forceinline__ void l4_sweeps(float* s_e, L4Regs<CH4>& R, int iters, float omega) {
+  const int t = threadIdx.x;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int colour = 0; colour < 2; ++colour) {
+      float v[CH4];
+#pragma unroll
+      for (int kk = 0; kk < CH4; ++kk) {
+        const int k = colour * CH4 + kk;
+        float acc = R.b[k];
+        if (!(it == 0 && colour == 0)) {
+          uint32_t n0 = R.nb[k][0], n1 = R.nb[k][1], n2 = R.nb[k][2];
+          opaque_v(n0);
+          opaque_v(n1);
+          opaque_v(n2);
+          const float a0 = s_e[n0 & 0xffff], a1 = s_e[n0 >> 16], a2 = s_e[n1 & 0xffff];
+          const float a3 = s_e[n1 >> 16], a4 = s_e[n2 & 0xffff], a5 = s_e[n2 >> 16];
+          const uint32_t w = R.w[k];
+          acc = fmaf((float)(w & 7u), a0, acc);
+          acc = fmaf((float)((w >> 3) & 7u), a1, acc);
+          acc = fmaf((float)((w >> 6) & 7u), a2, acc);
+          acc = fmaf((float)((w >> 9) & 7u), a3, acc);
+          acc = fmaf((float)((w >> 12) & 7u), a4, acc);
+          acc = fmaf((float)((w >> 15) & 7u), a5, acc);
+        }
+        const float gs = acc * R.invd[k];
+        v[kk] = it == 0 ? omega * gs : fmaf(omega, gs - R.eo[k], R.eo[k]);
+      }
+#pragma unroll
+      for (int kk = 0; kk < CH4; ++kk) {
+        const int k = colour * CH4 + kk;
+        R.eo[k] = v[kk];
+        s_e[t + k * kCoarseTpb] = v[kk];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Workgroup 0 runs the two-level coarse solve with level 3 (CH slots per lane and colour) and
+// level 4 (CH4) in LDS; workgroups 1.. apply the deferred x update (as k_mg_coarse's). Identical
+// arithmetic, in the same order, to the launches of Multigrid's unfused two-level path. The level-3
+// iterations before and after level 4 run as one loop whose body sweeps colour A, then colour B;
+// the level-4 phase runs inside it after the last pre-smoothing iteration and then swaps A and B
+// (registers and slot bases), so the post-smoothing's reverse colour order reuses the same code: a
+// straight-line version (every sweep inlined where it runs) took ~150 instruction-cache misses per
+// launch against ~70 for the loop of k_mg_coarse, ~14 us more of fixed cost at 8 M voxels.
+template <int CH, int CH4>
+__global__ __launch_bounds__(kCoarseTpb) void k_mg_coarse2(const uint32_t* __restrict__ nbp,
+                                                           const float* __restrict__ b, float* __restrict__ u,
+                                                           float inv_s, C4Dev C, const int* __restrict__ stop,
+                                                           McgXUpdate X, const int* __restrict__ it_dev, int64_t n,
+                                                           int sleeps) {
+  constexpr int H = kCoarseTpb * CH, H4 = kCoarseTpb * CH4;
+  static_assert(H4 <= H, "level 4 lives in level 3's right-hand-side array");
+  __shared__ float s_u[2 * H + 1];
+  __shared__ float s_x[2 * H + 1];  // level-3 right-hand sides, then the residual, then level 4's e
+  if (stop && *stop) return;
+  if (blockIdx.x > 0) {
+    coarse_side_xupdate_n(X, it_dev, n, (int)blockIdx.x - 1, (int)gridDim.x - 1, sleeps);
+    return;
+  }
+  const int t = threadIdx.x;
+  // A: the colour swept first in an iteration (0 before level 4, 1 after), B: the other; lane t
+  // holds slots base + t + kk * kCoarseTpb of each
+  float ua[CH], ub[CH];
+  uint32_t na[CH][3], nb[CH][3];
+  int base_a = 0, base_b = H;
+#pragma unroll
+  for (int kk = 0; kk < CH; ++kk) {
+    const int ja = t + kk * kCoarseTpb, jb = H + ja;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      na[kk][a] = nbp[2 * a * H + ja];
+      nb[kk][a] = nbp[2 * a * H + jb];
+    }
+    s_x[ja] = b[ja] * inv_s;
+    s_x[jb] = b[jb] * inv_s;
+    ua[kk] = 0.f;
+    ub[kk] = 0.f;
+  }
+  for (int j = t; j <= 2 * H; j += kCoarseTpb) s_u[j] = 0.f;
+  if (t == 0) s_x[2 * H] = 0.f;
+  __syncthreads();
+  // one level-3 colour sweep of the slots base + t + kk * kCoarseTpb; noread: the colour-0 sweep
+  // from zero (no neighbour reads); zero_old: u was zero; keep: also leave the residual in s_x (each
+  // lane reads and writes only its own slots of s_x)
+  auto sweep3 = [&](float(&uo)[CH], uint32_t(&nbr)[CH][3], int base, bool noread, bool zero_old, bool keep,
+                    float omega) {
+    constexpr int NG = CH < 2 ? CH : 2;
+    float v[CH];
+#pragma unroll
+    for (int kk = 0; kk < CH; kk += NG) {
+      float a[NG][6], bb[NG];
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        bb[g] = s_x[base + t + (kk + g) * kCoarseTpb];
+        if (!noread) {
+          uint32_t n0 = nbr[kk + g][0], n1 = nbr[kk + g][1], n2 = nbr[kk + g][2];
+          opaque_v(n0);
+          opaque_v(n1);
+          opaque_v(n2);
+          a[g][0] = s_u[n0 & 0xffff];
+          a[g][1] = s_u[n0 >> 16];
+          a[g][2] = s_u[n1 & 0xffff];
+          a[g][3] = s_u[n1 >> 16];
+          a[g][4] = s_u[n2 & 0xffff];
+          a[g][5] = s_u[n2 >> 16];
+        }
+      }
+#pragma unroll
+      for (int g = 0; g < NG; ++g) {
+        const int k = kk + g;
+        float sum = 0.f;
+        if (!noread) sum = a[g][0] + a[g][1] + a[g][2] + a[g][3] + a[g][4] + a[g][5];
+        const float gs = (bb[g] + sum) * (1.0f / 6.0f);
+        v[k] = zero_old ? omega * gs : fmaf(omega, gs - uo[k], uo[k]);
+        if (keep) s_x[base + t + k * kCoarseTpb] = fmaf(-6.0f, v[k], bb[g] + sum);
+      }
+    }
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      uo[kk] = v[kk];
+      s_u[base + t + kk * kCoarseTpb] = v[kk];
+    }
+    __syncthreads();
+  };
+  const int nit = C.nu1 + C.nu2;
+#pragma nounroll
+  for (int it = 0; it < nit; ++it) {
+    const float om = it < C.nu1 ? C.omega3 : C.omega3p;
+    sweep3(ua, na, base_a, it == 0, it == 0, false, om);
+    sweep3(ub, nb, base_b, false, it == 0, it == C.nu1 - 1, om);
+    if (it != C.nu1 - 1) continue;
+    // colour 0's residual (colour 1 left its own in the last sweep)
+    constexpr int NG3 = CH < 2 ? CH : 2;
+#pragma unroll
+    for (int kk = 0; kk < CH; kk += NG3) {
+      float a[NG3][6], bb[NG3];
+#pragma unroll
+      for (int g = 0; g < NG3; ++g) {
+        const int k = kk + g;
+        bb[g] = s_x[t + k * kCoarseTpb];
+        uint32_t n0 = na[k][0], n1 = na[k][1], n2 = na[k][2];
+        opaque_v(n0);
+        opaque_v(n1);
+        opaque_v(n2);
+        a[g][0] = s_u[n0 & 0xffff];
+        a[g][1] = s_u[n0 >> 16];
+        a[g][2] = s_u[n1 & 0xffff];
+        a[g][3] = s_u[n1 >> 16];
+        a[g][4] = s_u[n2 & 0xffff];
+        a[g][5] = s_u[n2 >> 16];
+      }
+#pragma unroll
+      for (int g = 0; g < NG3; ++g) {
+        const int k = kk + g;
+        const float sum = a[g][0] + a[g][1] + a[g][2] + a[g][3] + a[g][4] + a[g][5];
+        s_x[t + k * kCoarseTpb] = fmaf(-6.0f, ua[k], bb[g] + sum);
+      }
+    }
+    // level 4's arrays through pointers made opaque here: loads at fixed addresses would otherwise
+    // be hoisted out of the loop, their registers live through every sweep (spilled)
+    C4Dev Cl = C;
+    opaque_s(Cl.nb);
+    opaque_s(Cl.w);
+    opaque_s(Cl.invd);
+    opaque_s(Cl.ch3);
+    opaque_s(Cl.par3);
+    L4Regs<CH4> R;
+    l4_load(Cl, R);
+    uint32_t ch[2 * CH4][4];
+#pragma unroll
+    for (int k = 0; k < 2 * CH4; ++k)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) ch[k][p] = Cl.ch3[2 * p * H4 + t + k * kCoarseTpb];
+    __syncthreads();
+    // restriction: each block's children in the order q = 4 dx + 2 dy + dz (absent: the zero slot)
+#pragma unroll
+    for (int k = 0; k < 2 * CH4; ++k) {
+      float acc = 0.f;
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        acc += s_x[ch[k][p] & 0xffff];
+        acc += s_x[ch[k][p] >> 16];
+      }
+      R.b[k] = C.over4 * acc;
+    }
+    __syncthreads();  // every residual read before s_x holds level 4
+    if (t == 0) s_x[2 * H4] = 0.f;
+    l4_sweeps<CH4>(s_x, R, C.it4, C.omega4);
+    uint32_t par[CH];
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) par[kk] = Cl.par3[t + kk * kCoarseTpb];
+    // prolongation: each leaf adds its block's correction
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      ua[kk] = s_u[t + kk * kCoarseTpb] + s_x[par[kk] & 0xffff];
+      ub[kk] = s_u[H + t + kk * kCoarseTpb] + s_x[par[kk] >> 16];
+    }
+    // the right-hand sides again, as at the start (reloaded: registers are short here)
+    float ba[CH], bb2[CH];
+    const float* bl = b;
+    opaque_s(bl);
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      ba[kk] = bl[t + kk * kCoarseTpb] * inv_s;
+      bb2[kk] = bl[H + t + kk * kCoarseTpb] * inv_s;
+    }
+    __syncthreads();  // every read of level 4 before s_x holds the right-hand sides again
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      s_u[t + kk * kCoarseTpb] = ua[kk];
+      s_u[H + t + kk * kCoarseTpb] = ub[kk];
+      s_x[t + kk * kCoarseTpb] = ba[kk];
+      s_x[H + t + kk * kCoarseTpb] = bb2[kk];
+    }
+    __syncthreads();
+    // the post-smoothing sweeps colour 1 first
+#pragma unroll
+    for (int kk = 0; kk < CH; ++kk) {
+      const float tu = ua[kk];
+      ua[kk] = ub[kk];
+      ub[kk] = tu;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const uint32_t tn = na[kk][a];
+        na[kk][a] = nb[kk][a];
+        nb[kk][a] = tn;
+      }
+    }
+    base_a = H;
+    base_b = 0;
+  }
+#pragma unroll
+  for (int kk = 0; kk < CH; ++kk) {
+    u[base_a + t + kk * kCoarseTpb] = ua[kk];
+    u[base_b + t + kk * kCoarseTpb] = ub[kk];
+  }
+}
+
+// The two-level solve's launches path (topologies whose level 3 does not fit one workgroup, and the
+// unfused variant): the level-3 residual (b / s + sum) - 6 u by leaf ...
+__global__ __launch_bounds__(kTpb) void k_mg_leaf_residual(DevTopo T, const float* __restrict__ b,
+                                                           const float* __restrict__ u, float* __restrict__ rho,
+                                                           float inv_s, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
+  if (c >= T.L) return;
+  rho[c] = fmaf(-6.0f, u[c], b[c] * inv_s + mg_nsum(T, 0, u, c, 0, 0, 0));
+}
+// ... restricted to level 4 by slot (n4 = 2 H4 slots) ...
+__global__ __launch_bounds__(kTpb) void k_mg4_restrict(C4Dev C, int32_t n4, const float* __restrict__ rho,
+                                                       const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int32_t j = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
+  if (j >= n4) return;
+  float acc = 0.f;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int32_t l = C.chl[(int64_t)q * n4 + j];
+    acc += l < 0 ? 0.f : rho[l];
+  }
+  C.b4[j] = C.over4 * acc;
+}
+// ... level 4 solved by one workgroup ...
+template <int CH4>
+__global__ __launch_bounds__(kCoarseTpb) void k_mg4_solve(C4Dev C, const int* __restrict__ stop) {
+  constexpr int H4 = CH4 * kCoarseTpb;
+  __shared__ float s_e[2 * H4 + 1];
+  if (stop && *stop) return;
+  const int t = threadIdx.x;
+  L4Regs<CH4> R;
+  l4_load(C, R);
+#pragma unroll
+  for (int k = 0; k < 2 * CH4; ++k) R.b[k] = C.b4[t + k * kCoarseTpb];
+  if (t == 0) s_e[2 * H4] = 0.f;
+  l4_sweeps<CH4>(s_e, R, C.it4, C.omega4);
+#pragma unroll
+  for (int k = 0; k < 2 * CH4; ++k) C.u4[t + k * kCoarseTpb] = R.eo[k];
+}
+// ... and its correction added to each leaf
+__global__ __launch_bounds__(kTpb) void k_mg4_prolong(int32_t L, const int32_t* __restrict__ parl,
+                                                      const float* __restrict__ u4, float* __restrict__ u,
+                                                      const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
+  if (c >= L) return;
+  u[c] = u[c] + u4[parl[c]];
 }
 
 // Restriction of the level-lgS residual b - s (6 u - sum) (leaf-dense, lgS = 1 or 2) to level
 // lgS + 1: one lane per fine cell forms its residual in LDS, then one lane per coarse cell sums
-// its 8 children (a workgroup holds whole leaves: 256 is a multiple of S^3).
+// its 8 children (a workgroup holds whole leaves: 256 is a multiple of S^3). map3: where level 3
+// keeps a leaf's value (null: at the leaf index; the one-workgroup coarse solve's slot order).
 __global__ __launch_bounds__(kTpb) void k_mg_restrict(DevTopo T, int lgS, const float* __restrict__ b,
                                                       const float* __restrict__ u, float s, float over,
-                                                      float* __restrict__ bc, const int* __restrict__ stop) {
+                                                      float* __restrict__ bc, const int32_t* __restrict__ map3,
+                                                      const int* __restrict__ stop) {
   __shared__ float s_r[kTpb];
   if (stop && *stop) return;
   const int32_t ncell = T.L << (3 * lgS);
@@ -273,19 +642,20 @@ __global__ __launch_bounds__(kTpb) void k_mg_restrict(DevTopo T, int lgS, const 
     const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z + (k & 1);
     sum += s_r[(lb << (3 * lgS)) | (((x << lgS) | y) << lgS) | z];
   }
-  bc[cc] = over * sum;
+  bc[lgC == 0 && map3 ? map3[cc] : cc] = over * sum;
 }
 
-// u (level lgS, leaf-dense) += its parent's value in uc (level lgS + 1)
+// u (level lgS, leaf-dense) += its parent's value in uc (level lgS + 1; map3 as k_mg_restrict's)
 __global__ __launch_bounds__(kTpb) void k_mg_prolong(int32_t L, int lgS, float* __restrict__ u,
-                                                     const float* __restrict__ uc, const int* __restrict__ stop) {
+                                                     const float* __restrict__ uc, const int32_t* __restrict__ map3,
+                                                     const int* __restrict__ stop) {
   if (stop && *stop) return;
   const int32_t c = (int32_t)(blockIdx.x * kTpb + threadIdx.x);
   if (c >= (L << (3 * lgS))) return;
   const int m = (1 << lgS) - 1, lgC = lgS - 1;
   const int32_t leaf = c >> (3 * lgS);
   const int x = (c >> (2 * lgS)) & m, y = (c >> lgS) & m, z = c & m;
-  u[c] += uc[(leaf << (3 * lgC)) | ((((x >> 1) << lgC) | (y >> 1)) << lgC) | (z >> 1)];
+  u[c] += uc[lgC == 0 && map3 ? map3[leaf] : (leaf << (3 * lgC)) | ((((x >> 1) << lgC) | (y >> 1)) << lgC) | (z >> 1)];
 }
 
 // Level 0 (colour-split): residual b - (6 u - sum) of the leaf's voxel pairs staged in LDS, then
@@ -793,10 +1163,11 @@ __device__ __forceinline__ void ltile_c1(const FaceRow& Fr, int lane, int cx, in
   }
 }
 
+// map3 (LG = 1, restricting to level 3): as k_mg_restrict's
 template <int LG>
 __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __restrict__ b, float* __restrict__ zout,
                                                   float scale, float over, float* __restrict__ bc, float w,
-                                                  const int* __restrict__ stop) {
+                                                  const int32_t* __restrict__ map3, const int* __restrict__ stop) {
   using Tl = LTile<LG>;
   __shared__ float s_b[Tl::LPW][Tl::N];
   __shared__ float s_z[Tl::LPW][Tl::N];
@@ -863,14 +1234,15 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
     const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z0 + (k & 1);
     sum += s_r[(lw << (3 * LG)) | (((x << LG) | y) << LG) | z];
   }
-  bc[((int32_t)l << (3 * lgC)) | lane] = over * sum;
+  bc[LG == 1 && map3 ? map3[l] : ((int32_t)l << (3 * lgC)) | lane] = over * sum;
 }
 
+// map3 (LG = 1, prolonging from level 3): as k_mg_restrict's
 template <int LG>
 __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __restrict__ b,
                                                    const float* __restrict__ zin, const float* __restrict__ uc,
                                                    float* __restrict__ zout, float scale, float w,
-                                                   const int* __restrict__ stop) {
+                                                   const int32_t* __restrict__ map3, const int* __restrict__ stop) {
   using Tl = LTile<LG>;
   __shared__ float s_b[Tl::LPW][Tl::N];
   __shared__ float s_z[Tl::LPW][Tl::N];
@@ -886,7 +1258,8 @@ __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __res
   auto zval = [&](int32_t nl, int vx, int vy, int vz) {  // absent leaves load the own leaf, then read 0
     const int32_t src = nl < 0 ? l : nl;
     const float v = zin[Tl::cell(src, vx, vy, vz)] +
-                    uc[(src << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
+                    uc[LG == 1 && map3 ? map3[src]
+                                       : (src << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
     return nl < 0 ? 0.f : v;
   };
   FaceRow Fr{};
@@ -960,39 +1333,94 @@ void smooth(const DevTopo& T, int lgS, const float* b, float* u, float scale, in
   HNS_HIP(hipGetLastError());
 }
 
-// `iters` red-black SOR iterations on level 3: one workgroup on the colour-sorted slots when
-// prepare_coarse found a layout (ch > 0), else one launch per colour sweep
-void coarse_solve(const DevTopo& T, int ch, const int32_t* perm, const uint32_t* nbp, const float* b, float* u,
-                  float scale, int iters, float omega, const int* stop, bool launches, const McgHook* hook,
-                  hipStream_t s) {
+// The coarsest level's layouts: level 3's colour-sorted slots for one workgroup (ch 0: none) and
+// level 4 of the two-level solve (ch4 0: none; fused4: both fit the fused two-level kernel)
+struct CoarseLayout {
+  int ch;
+  const uint32_t* nbp;
+  int ch4;
+  bool fused4;
+  C4Dev c4;
+  float* rho;
+};
+
+template <int CH>
+void launch_coarse2(int ch4, unsigned grid, const CoarseLayout& Ly, const float* b, float* u, float inv,
+                    const int* stop, const McgXUpdate& X, const int* it_dev, int64_t n, int sleeps, hipStream_t s) {
+  if (ch4 == 1)
+    k_mg_coarse2<CH, 1><<<grid, kCoarseTpb, 0, s>>>(Ly.nbp, b, u, inv, Ly.c4, stop, X, it_dev, n, sleeps);
+  else if constexpr (CH >= 2)
+    k_mg_coarse2<CH, 2><<<grid, kCoarseTpb, 0, s>>>(Ly.nbp, b, u, inv, Ly.c4, stop, X, it_dev, n, sleeps);
+}
+
+// Level 3 solved: by the two-level solve when the shape asks for it and level 4 has a layout, else
+// by `coarse` SOR iterations; in one workgroup when the layouts fit it (and `launches` is not
+// asked), else by launches with the same arithmetic
+void coarse_solve(const DevTopo& T, const CoarseLayout& Ly, const float* b, float* u, float scale, const int* stop,
+                  bool launches, const McgHook* hook, hipStream_t s) {
+  const MgShape& M = mg_shape();
   const float inv = 1.0f / scale;
+  const bool two = M.c4 && Ly.ch4 > 0;
+  const bool fused = !launches && (two ? Ly.fused4 : Ly.ch > 0);
   const McgXUpdate X = hook ? hook->xu : McgXUpdate{nullptr, nullptr, nullptr, nullptr};
-  if (X.x && (ch == 0 || launches))
-    fail(HNS_ERR_INVALID_ARGUMENT, "the deferred x update needs the one-workgroup coarse solve");
+  if (X.x && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the deferred x update needs the one-workgroup coarse solve");
   const int64_t n = hook ? hook->n : 0;
   const int* it_dev = hook ? hook->it : nullptr;
-  const int side =
-      X.x ? (int)std::min<int64_t>(kCoarseSideGroups, (n + 4 * kCoarseTpb - 1) / (4 * kCoarseTpb)) : 0;
-  if (ch == 0 || launches) {
-    for (int k = 0; k < iters; ++k) {
-      smooth(T, 0, b, u, scale, 0, k == 0, omega, stop, s);
-      smooth(T, 0, b, u, scale, 1, k == 0, omega, stop, s);
+  if (!fused) {
+    if (!two) {
+      for (int k = 0; k < M.coarse; ++k) {
+        smooth(T, 0, b, u, scale, 0, k == 0, M.coarse_omega, stop, s);
+        smooth(T, 0, b, u, scale, 1, k == 0, M.coarse_omega, stop, s);
+      }
+      return;
+    }
+    for (int k = 0; k < M.c3_pre; ++k) {
+      smooth(T, 0, b, u, scale, 0, k == 0, M.c3_omega, stop, s);
+      smooth(T, 0, b, u, scale, 1, k == 0, M.c3_omega, stop, s);
+    }
+    k_mg_leaf_residual<<<cells_grid(T.L, 0), kTpb, 0, s>>>(T, b, u, Ly.rho, inv, stop);
+    HNS_HIP(hipGetLastError());
+    const int32_t n4 = 2 * Ly.ch4 * kCoarseTpb;
+    k_mg4_restrict<<<(unsigned)((n4 + kTpb - 1) / kTpb), kTpb, 0, s>>>(Ly.c4, n4, Ly.rho, stop);
+    HNS_HIP(hipGetLastError());
+    switch (Ly.ch4) {
+      case 1: k_mg4_solve<1><<<1, kCoarseTpb, 0, s>>>(Ly.c4, stop); break;
+      case 2: k_mg4_solve<2><<<1, kCoarseTpb, 0, s>>>(Ly.c4, stop); break;
+      default: k_mg4_solve<4><<<1, kCoarseTpb, 0, s>>>(Ly.c4, stop); break;
+    }
+    HNS_HIP(hipGetLastError());
+    k_mg4_prolong<<<cells_grid(T.L, 0), kTpb, 0, s>>>(T.L, Ly.c4.parl, Ly.c4.u4, u, stop);
+    HNS_HIP(hipGetLastError());
+    for (int k = 0; k < M.c3_post; ++k) {
+      smooth(T, 0, b, u, scale, 1, false, M.c3_omega_post, stop, s);
+      smooth(T, 0, b, u, scale, 0, false, M.c3_omega_post, stop, s);
     }
     return;
   }
-  switch (ch) {
-    case 1:
-      k_mg_coarse<1><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
-      break;
-    case 2:
-      k_mg_coarse<2><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
-      break;
-    case 4:
-      k_mg_coarse<4><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
-      break;
-    default:
-      k_mg_coarse<8><<<1 + side, kCoarseTpb, 0, s>>>(perm, nbp, b, u, inv, omega, iters, stop, X, it_dev, n);
-      break;
+  const int max_side = M.side_groups > 0 ? M.side_groups : (two ? 255 : kCoarseSideGroups);
+  const int side = X.x ? (int)std::min<int64_t>(max_side, (n + 4 * kCoarseTpb - 1) / (4 * kCoarseTpb)) : 0;
+  const unsigned grid = 1u + (unsigned)side;
+  if (two) {
+    const int sleeps = M.side_sleeps >= 0 ? M.side_sleeps : 0;
+    switch (Ly.ch) {
+      case 1: launch_coarse2<1>(Ly.ch4, grid, Ly, b, u, inv, stop, X, it_dev, n, sleeps, s); break;
+      case 2: launch_coarse2<2>(Ly.ch4, grid, Ly, b, u, inv, stop, X, it_dev, n, sleeps, s); break;
+      case 4: launch_coarse2<4>(Ly.ch4, grid, Ly, b, u, inv, stop, X, it_dev, n, sleeps, s); break;
+      default: launch_coarse2<8>(Ly.ch4, grid, Ly, b, u, inv, stop, X, it_dev, n, sleeps, s); break;
+    }
+    HNS_HIP(hipGetLastError());
+    return;
+  }
+  const int iters = M.coarse;
+  const float omega = M.coarse_omega;
+  const uint32_t* nbp = Ly.nbp;
+  // the solve's length grows with CH, and so does the sleep
+  const int sl = M.side_sleeps >= 0 ? M.side_sleeps : Ly.ch / 2;
+  switch (Ly.ch) {
+    case 1: k_mg_coarse<1><<<grid, kCoarseTpb, 0, s>>>(nbp, b, u, inv, omega, iters, stop, X, it_dev, n, sl); break;
+    case 2: k_mg_coarse<2><<<grid, kCoarseTpb, 0, s>>>(nbp, b, u, inv, omega, iters, stop, X, it_dev, n, sl); break;
+    case 4: k_mg_coarse<4><<<grid, kCoarseTpb, 0, s>>>(nbp, b, u, inv, omega, iters, stop, X, it_dev, n, sl); break;
+    default: k_mg_coarse<8><<<grid, kCoarseTpb, 0, s>>>(nbp, b, u, inv, omega, iters, stop, X, it_dev, n, sl); break;
   }
   HNS_HIP(hipGetLastError());
 }
@@ -1016,44 +1444,28 @@ void Multigrid::ensure(int32_t leaves) {
   }
 }
 
-void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
-  if (ckey_id_ == T.id && ckey_L_ == T.L) return;
-  if (exec_) {  // a captured V-cycle may refer to the layout about to be replaced
-    HNS_HIP(hipGraphExecDestroy(exec_));
-    exec_ = nullptr;
-  }
-  coarse_ch_ = 0;
-  ++generation;
-  ckey_id_ = T.id;
-  ckey_L_ = T.L;
-  const int32_t L = T.L;
-  if (L <= 0 || L > 2 * kCoarseTpb * kCoarseMaxCH) return;
-  std::vector<int4> org((size_t)L);
-  std::vector<int32_t> nbr((size_t)L * kNbrStride);
-  HNS_HIP(hipStreamSynchronize(s));
-  HNS_HIP(hipMemcpy(org.data(), T.origins, org.size() * sizeof(int4), hipMemcpyDeviceToHost));
-  HNS_HIP(hipMemcpy(nbr.data(), T.nbr, nbr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-  std::vector<int32_t> lv[2];  // the leaves of each colour, in slot order
-  for (int32_t l = 0; l < L; ++l) lv[((org[l].x + org[l].y + org[l].z) >> 3) & 1].push_back(l);
-  const int32_t nmax = (int32_t)std::max(lv[0].size(), lv[1].size());
-  int ch = 1;
-  while (ch < kCoarseMaxCH && ch * kCoarseTpb < nmax) ch *= 2;
-  if (ch * kCoarseTpb < nmax) return;  // colours too unbalanced for one workgroup
-  const int32_t H = ch * kCoarseTpb;
-  static const int kFace[6] = {22, 4, 16, 10, 14, 12};  // +x, -x, +y, -y, +z, -z
-  // A colour's slots hold its leaves sorted lexicographically by origin, the fastest-varying axis
-  // chosen (of x, y, z) for the fewest LDS bank conflicts in the solve's neighbour reads: a wave
-  // reads one neighbour of each of 64 consecutive slots at a time, and such a read takes as many
-  // cycles as the most distinct words falling in one of the 64 four-byte banks (counted here over
-  // the whole solve). At 8 M voxels in a y-cylinder, y fastest counts 18 % fewer such cycles than
-  // the topology's order and took the solve from 53.0 to 51.8 us: it waits more on the latency of
-  // its dependent reads and barriers than on LDS throughput. The order does not change the result:
-  // a colour's update reads only the other colour.
-  std::vector<int32_t> slot((size_t)L), best_slot;
+namespace {
+
+// Colour-sorted slots: the items of colour c (c = 0, 1) take slots [c H, c H + count), sorted
+// lexicographically by coordinate with the fastest-varying axis chosen (of x, y, z) for the fewest
+// LDS bank conflicts in the one-workgroup solve's neighbour reads: a wave reads one neighbour of
+// each of 64 consecutive slots at a time, and such a read takes as many cycles as the most distinct
+// words falling in one of the 64 four-byte banks (counted here over the whole solve). At 8 M voxels
+// in a y-cylinder, y fastest counted 18 % fewer such cycles than the topology's order on level 3
+// and took its solve from 53.0 to 51.8 us: it waits more on the latency of its dependent reads and
+// barriers than on LDS throughput. The order does not change the result: a colour's update reads
+// only the other colour. nb[i][q]: item i's neighbour across face q (+x, -x, +y, -y, +z, -z), -1
+// for none.
+std::vector<int32_t> colour_slots(const std::vector<int4>& xyz, const std::vector<int>& colour,
+                                  const std::vector<std::array<int32_t, 6>>& nb, int32_t H) {
+  const size_t n = xyz.size();
+  std::vector<int32_t> lv[2];
+  for (size_t i = 0; i < n; ++i) lv[colour[i]].push_back((int32_t)i);
+  std::vector<int32_t> slot(n), best;
   int64_t best_cost = -1;
   for (int fast = 0; fast < 3; ++fast) {
     const int s0 = fast == 0 ? 1 : 0, s1 = fast == 2 ? 1 : 2;  // the slower axes, in x, y, z order
-    auto ax = [&](int32_t l, int a) { return a == 0 ? org[l].x : (a == 1 ? org[l].y : org[l].z); };
+    auto ax = [&](int32_t i, int a) { return a == 0 ? xyz[i].x : (a == 1 ? xyz[i].y : xyz[i].z); };
     for (int c = 0; c < 2; ++c) {
       std::sort(lv[c].begin(), lv[c].end(), [&](int32_t p, int32_t q) {
         if (ax(p, s0) != ax(q, s0)) return ax(p, s0) < ax(q, s0);
@@ -1069,7 +1481,7 @@ void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
         const int m = (int)std::min<size_t>(64, lv[c].size() - g);
         for (int q = 0; q < 6; ++q) {
           for (int i = 0; i < 64; ++i) {
-            const int32_t nl = i < m ? nbr[(size_t)lv[c][g + i] * kNbrStride + kFace[q]] : -1;
+            const int32_t nl = i < m ? nb[lv[c][g + i]][q] : -1;
             addr[i] = nl < 0 ? (uint32_t)(2 * H) : (uint32_t)slot[nl];
           }
           std::sort(addr, addr + 64);
@@ -1081,27 +1493,184 @@ void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
       }
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
-      best_slot = slot;
+      best = slot;
     }
   }
-  slot.swap(best_slot);
-  std::vector<int32_t> perm((size_t)2 * H, -1);
-  std::vector<uint32_t> nbp((size_t)6 * H, (uint32_t)(2 * H) | ((uint32_t)(2 * H) << 16));
+  return best;
+}
+
+// slots per lane and colour for a colour's `nmax` items (a power of two up to cap), 0: too many
+int slots_per_lane(size_t nmax, int cap) {
+  int ch = 1;
+  while (ch < cap && (size_t)ch * kCoarseTpb < nmax) ch *= 2;
+  return (size_t)ch * kCoarseTpb < nmax ? 0 : ch;
+}
+
+template <class T>
+void upload(DBuf<T>& d, const std::vector<T>& h) {
+  d.alloc(h.size());
+  HNS_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+bool Multigrid::coarse_fused() const { return mg_shape().c4 && c4_ch_ > 0 ? c4_fused_ : coarse_ch_ > 0; }
+
+void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
+  if (ckey_id_ == T.id && ckey_L_ == T.L) return;
+  if (exec_) {  // a captured V-cycle may refer to the layout about to be replaced
+    HNS_HIP(hipGraphExecDestroy(exec_));
+    exec_ = nullptr;
+  }
+  coarse_ch_ = 0;
+  c4_ch_ = 0;
+  c4_fused_ = false;
+  ++generation;
+  ckey_id_ = T.id;
+  ckey_L_ = T.L;
+  const int32_t L = T.L;
+  if (L <= 0) return;
+  std::vector<int4> org((size_t)L);
+  std::vector<int32_t> nbr((size_t)L * kNbrStride);
+  HNS_HIP(hipStreamSynchronize(s));
+  HNS_HIP(hipMemcpy(org.data(), T.origins, org.size() * sizeof(int4), hipMemcpyDeviceToHost));
+  HNS_HIP(hipMemcpy(nbr.data(), T.nbr, nbr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  static const int kFace[6] = {22, 4, 16, 10, 14, 12};  // +x, -x, +y, -y, +z, -z
+  std::vector<std::array<int32_t, 6>> lnb((size_t)L);
+  std::vector<int> lcol((size_t)L);
   for (int32_t l = 0; l < L; ++l) {
-    const int32_t j = slot[l];
-    perm[j] = l;
-    uint32_t f[6];
-    for (int q = 0; q < 6; ++q) {
-      const int32_t nl = nbr[(size_t)l * kNbrStride + kFace[q]];
-      f[q] = nl < 0 ? (uint32_t)(2 * H) : (uint32_t)slot[nl];
-    }
-    for (int a = 0; a < 3; ++a) nbp[(size_t)2 * a * H + j] = f[2 * a] | (f[2 * a + 1] << 16);
+    for (int q = 0; q < 6; ++q) lnb[l][q] = nbr[(size_t)l * kNbrStride + kFace[q]];
+    lcol[l] = ((org[l].x + org[l].y + org[l].z) >> 3) & 1;
   }
-  cperm_.alloc(perm.size());
-  cnb_.alloc(nbp.size());
-  HNS_HIP(hipMemcpy(cperm_.p, perm.data(), perm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-  HNS_HIP(hipMemcpy(cnb_.p, nbp.data(), nbp.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-  coarse_ch_ = ch;
+
+  // level 3: one workgroup when each colour fits kCoarseMaxCH slots per lane
+  size_t ncol[2] = {0, 0};
+  for (int32_t l = 0; l < L; ++l) ++ncol[lcol[l]];
+  const int ch = slots_per_lane(std::max(ncol[0], ncol[1]), kCoarseMaxCH);
+  const int32_t H = ch * kCoarseTpb;
+  std::vector<int32_t> slot3;
+  if (ch > 0) {
+    slot3 = colour_slots(org, lcol, lnb, H);
+    std::vector<uint32_t> nbp((size_t)6 * H, (uint32_t)(2 * H) | ((uint32_t)(2 * H) << 16));
+    for (int32_t l = 0; l < L; ++l) {
+      const int32_t j = slot3[l];
+      uint32_t f[6];
+      for (int q = 0; q < 6; ++q) f[q] = lnb[l][q] < 0 ? (uint32_t)(2 * H) : (uint32_t)slot3[lnb[l][q]];
+      for (int a = 0; a < 3; ++a) nbp[(size_t)2 * a * H + j] = f[2 * a] | (f[2 * a + 1] << 16);
+    }
+    upload(cmap_, slot3);
+    upload(cnb_, nbp);
+    // level 3's vectors in slot order (2 H entries; padding slots of b stay zero)
+    const size_t n3 = std::max<size_t>((size_t)L, (size_t)2 * H);
+    if (b[kMgLevels - 1].n < n3) {
+      b[kMgLevels - 1].alloc(n3);
+      u[kMgLevels - 1].alloc(n3);
+    }
+    HNS_HIP(hipMemset(b[kMgLevels - 1].p, 0, b[kMgLevels - 1].bytes()));
+    HNS_HIP(hipMemset(u[kMgLevels - 1].p, 0, u[kMgLevels - 1].bytes()));
+    coarse_ch_ = ch;
+  }
+
+  // level 4: blocks of 2^3 leaves (block coordinate = leaf coordinate >> 1), child q = 4 dx + 2 dy + dz
+  std::unordered_map<uint64_t, int32_t> bid;
+  std::vector<int4> bxyz;
+  std::vector<std::array<int32_t, 8>> child;
+  std::vector<int32_t> blk((size_t)L);
+  auto bkey = [](int x, int y, int z) {
+    return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) |
+           (uint64_t)(uint32_t)(z + (1 << 20));
+  };
+  for (int32_t l = 0; l < L; ++l) {
+    const int lx = org[l].x >> 3, ly = org[l].y >> 3, lz = org[l].z >> 3;
+    const int bx = lx >> 1, by = ly >> 1, bz = lz >> 1;
+    auto it = bid.emplace(bkey(bx, by, bz), (int32_t)bxyz.size());
+    if (it.second) {
+      bxyz.push_back(make_int4(bx, by, bz, 0));
+      child.emplace_back();
+      child.back().fill(-1);
+    }
+    blk[l] = it.first->second;
+    child[blk[l]][((lx & 1) << 2) | ((ly & 1) << 1) | (lz & 1)] = l;
+  }
+  const int32_t B = (int32_t)bxyz.size();
+  std::vector<int> bcol((size_t)B);
+  size_t bncol[2] = {0, 0};
+  for (int32_t i = 0; i < B; ++i) ++bncol[bcol[i] = (bxyz[i].x + bxyz[i].y + bxyz[i].z) & 1];
+  const int ch4 = slots_per_lane(std::max(bncol[0], bncol[1]), kC4MaxCH);
+  if (ch4 == 0) return;
+  const int32_t H4 = ch4 * kCoarseTpb;
+  // Galerkin weights: D_B = sum over B's leaves of (6 - their face neighbours inside B); n_BB' = the
+  // leaf face pairs between B and the block across face q
+  std::vector<std::array<int32_t, 6>> bnb((size_t)B);
+  std::vector<std::array<int, 6>> cnt((size_t)B);
+  std::vector<int> D((size_t)B, 0);
+  for (int32_t i = 0; i < B; ++i) {
+    bnb[i].fill(-1);
+    cnt[i].fill(0);
+  }
+  for (int32_t l = 0; l < L; ++l) {
+    const int32_t b = blk[l];
+    int inside = 0;
+    for (int q = 0; q < 6; ++q) {
+      const int32_t nl = lnb[l][q];
+      if (nl < 0) continue;
+      if (blk[nl] == b) {
+        ++inside;
+      } else {
+        ++cnt[b][q];
+        bnb[b][q] = blk[nl];
+      }
+    }
+    D[b] += 6 - inside;
+  }
+  const std::vector<int32_t> slot4 = colour_slots(bxyz, bcol, bnb, H4);
+  const uint32_t z4 = (uint32_t)(2 * H4);
+  std::vector<uint32_t> nb4((size_t)6 * H4, z4 | (z4 << 16)), w4((size_t)2 * H4, 0u);
+  std::vector<float> invd((size_t)2 * H4, 0.f);
+  std::vector<int32_t> chl((size_t)16 * H4, -1), parl((size_t)L);
+  for (int32_t i = 0; i < B; ++i) {
+    const int32_t j = slot4[i];
+    uint32_t f[6], w = 0;
+    for (int q = 0; q < 6; ++q) {
+      f[q] = bnb[i][q] < 0 ? z4 : (uint32_t)slot4[bnb[i][q]];
+      w |= (uint32_t)cnt[i][q] << (3 * q);
+    }
+    for (int a = 0; a < 3; ++a) nb4[(size_t)2 * a * H4 + j] = f[2 * a] | (f[2 * a + 1] << 16);
+    w4[j] = w;
+    invd[j] = 1.0f / (float)D[i];
+    for (int q = 0; q < 8; ++q) chl[(size_t)q * 2 * H4 + j] = child[i][q];
+  }
+  for (int32_t l = 0; l < L; ++l) parl[l] = slot4[blk[l]];
+  upload(c4nb_, nb4);
+  upload(c4w_, w4);
+  upload(c4invd_, invd);
+  upload(c4chl_, chl);
+  upload(c4parl_, parl);
+  c4b_.alloc((size_t)2 * H4);
+  c4u_.alloc((size_t)2 * H4);
+  rho3_.alloc((size_t)L);
+  c4_ch_ = ch4;
+  if (ch > 0 && ch4 <= std::min(ch, kC4MaxFusedCH)) {
+    // the fused kernel's children (level-3 slots, absent: the zero slot 2 H) and parents (level-4
+    // slots, padding: the zero slot 2 H4)
+    const uint32_t z3 = (uint32_t)(2 * H);
+    std::vector<uint32_t> ch3((size_t)8 * H4, z3 | (z3 << 16)), par3((size_t)H, z4 | (z4 << 16));
+    for (int32_t i = 0; i < B; ++i)
+      for (int p = 0; p < 4; ++p) {
+        const int32_t c0 = child[i][2 * p], c1 = child[i][2 * p + 1];
+        const uint32_t s0 = c0 < 0 ? z3 : (uint32_t)slot3[c0], s1 = c1 < 0 ? z3 : (uint32_t)slot3[c1];
+        ch3[(size_t)p * 2 * H4 + slot4[i]] = s0 | (s1 << 16);
+      }
+    for (int32_t l = 0; l < L; ++l) {
+      const int32_t j = slot3[l];
+      const uint32_t p = (uint32_t)slot4[blk[l]];
+      uint32_t& word = par3[j < H ? j : j - H];
+      word = j < H ? (word & 0xffff0000u) | p : (word & 0xffffu) | (p << 16);
+    }
+    upload(c4ch3_, ch3);
+    upload(c4par3_, par3);
+    c4_fused_ = true;
+  }
 }
 
 void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
@@ -1109,6 +1678,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
   const MgShape& M = mg_shape();
   const bool fused = M.smooth0 == 1 && !(variant & kMgUnfused);
   if (hook && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the flexible-CG pass needs the fused level-0 V-cycle");
+  // level 3 in the one-workgroup solve's slot order when that solve runs
+  const int32_t* map3 = !(variant & kMgCoarseLaunches) && coarse_fused() ? cmap_.p : nullptr;
   if (fused) {
     // level 0 pre-smoothing and restriction in one pass, into the scratch level-0 vector
     k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, M.omega0, stop);
@@ -1130,8 +1701,9 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
       if (lgS == 2)
-        k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, stop);
-      else k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, stop);
+        k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, map3, stop);
+      else
+        k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, map3, stop);
       HNS_HIP(hipGetLastError());
       continue;
     }
@@ -1140,25 +1712,28 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
       smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, M.omega_l, stop, s);
     }
     k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, M.over_deep, b[l + 1].p,
-                                                        stop);
+                                                        map3, stop);
     HNS_HIP(hipGetLastError());
   }
-  // level 3 (one cell per leaf): M.coarse over-relaxed red-black iterations
-  coarse_solve(T, coarse_ch_, cperm_.p, cnb_.p, b[kMgLevels - 1].p, u[kMgLevels - 1].p,
-               (float)(1 << (2 * (kMgLevels - 1))), M.coarse, M.coarse_omega, stop, (variant & kMgCoarseLaunches) != 0,
-               hook, s);
+  // level 3 (one cell per leaf): the two-level solve, or M.coarse over-relaxed red-black iterations
+  CoarseLayout Ly{coarse_ch_, cnb_.p, c4_ch_, c4_fused_, C4Dev{}, rho3_.p};
+  Ly.c4 = C4Dev{c4nb_.p, c4w_.p, c4invd_.p, c4ch3_.p, c4par3_.p, c4chl_.p, c4parl_.p, c4b_.p, c4u_.p,
+                M.c4_iters, M.c3_pre, M.c3_post, M.c4_omega, M.c4_over, M.c3_omega, M.c3_omega_post};
+  coarse_solve(T, Ly, b[kMgLevels - 1].p, u[kMgLevels - 1].p, (float)(1 << (2 * (kMgLevels - 1))), stop,
+               (variant & kMgCoarseLaunches) != 0, hook, s);
   // ascend: prolong and post-smooth in the reverse colour order
   for (int l = kMgLevels - 2; l >= 1; --l) {
     const int lgS = 3 - l;
     const float scale = (float)(1 << (2 * l));
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
-      if (lgS == 2) k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, stop);
-      else k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, stop);
+      if (lgS == 2)
+        k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, map3, stop);
+      else k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, map3, stop);
       HNS_HIP(hipGetLastError());
       continue;
     }
-    k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p, stop);
+    k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p, map3, stop);
     HNS_HIP(hipGetLastError());
     for (int k = 0; k < M.smooth; ++k) {
       smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, M.omega_l, stop, s);
