@@ -62,6 +62,9 @@ _SIGS = [
     ("hns_payload_build", C.c_int32, [C.POINTER(hns_leaf_grid), C.c_size_t, C.c_size_t, C.c_int32, C.POINTER(_VP)]),
     ("hns_payload_view", C.c_int32, [_VP, C.POINTER(hns_grid_data), c_int32_p]),
     ("hns_payload_destroy", C.c_int32, [C.POINTER(_VP)]),
+    ("hns_payload_domain_mask", C.c_int32, [_VP, C.POINTER(C.c_uint64), C.c_size_t]),
+    ("hns_write_leaf_grid", C.c_int32, [C.POINTER(hns_grid_data), C.c_char_p, C.POINTER(C.c_uint64), c_int32_p,
+                                        c_float_p, C.POINTER(C.c_uint64)]),
     ("hns_scatter_leaves", C.c_int32, [C.POINTER(hns_grid_data), C.c_char_p, C.POINTER(C.c_uint8), C.c_float,
                                        c_int32_p, c_float_p, C.POINTER(C.c_uint64), C.c_size_t,
                                        C.POINTER(C.c_size_t)]),

@@ -422,6 +422,22 @@ hns_status hns_payload_view(hns_payload* payload, hns_grid_data* view, int32_t* 
   });
 }
 
+hns_status hns_payload_domain_mask(hns_payload* payload, uint64_t* active, size_t num_leaves) {
+  return guard([&] {
+    if (!payload || !payload->p) fail(HNS_ERR_INVALID_ARGUMENT, "hns_payload_domain_mask: null payload");
+    payload_domain_mask(payload->p, active, num_leaves);
+    return HNS_OK;
+  });
+}
+
+hns_status hns_write_leaf_grid(const hns_grid_data* data, const char* name, const uint64_t* domain_active,
+                               int32_t* origins, float* values, uint64_t* active) {
+  return guard([&] {
+    write_leaf_grid(data, name, domain_active, origins, values, active);
+    return HNS_OK;
+  });
+}
+
 hns_status hns_payload_destroy(hns_payload** payload) {
   return guard([&] {
     if (payload && *payload) {

@@ -94,6 +94,9 @@ struct Payload;
 Payload* build_payload(const hns_leaf_grid* grids, size_t num_grids, size_t domain_grid, int32_t padding);
 void destroy_payload(Payload* p);
 void payload_view(Payload* p, hns_grid_data* view, int32_t* pinned);
+void payload_domain_mask(Payload* p, uint64_t* active, size_t num_leaves);
+void write_leaf_grid(const hns_grid_data* d, const char* name, const uint64_t* domain_active, int32_t* origins,
+                     float* values, uint64_t* active);
 void scatter_leaves(const hns_grid_data* d, const char* name, const uint8_t* mask, float background, int32_t* origins,
                     float* values, uint64_t* active, size_t capacity, size_t* num_leaves);
 

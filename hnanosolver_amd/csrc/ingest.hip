@@ -4,7 +4,9 @@
 // with the collision SDF's topology), IndexGridBuilder::computeLeafs/build (GridBuilder.hpp:93-175,
 // :231-251: every voxel of every domain leaf, leaves in tree order, each grid's leaf buffer copied
 // where the grid has a leaf at the domain leaf's origin, else zeros, and for the SDF the byte
-// pattern memset(..., 1, ...) leaves), and the output scatter of SOP_HNanoSolver.cpp:345-388.
+// pattern memset(..., 1, ...) leaves), the output scatter of SOP_HNanoSolver.cpp:345-388, and
+// IndexGridBuilder::writeIndexGrid (GridBuilder.hpp:177-226: a grid with the domain's topology whose
+// leaf buffers are the payload's).
 //
 // Input grids come as their leaves, the form OpenVDB and NanoVDB keep them in: per leaf the origin,
 // the 512-value buffer in offset order (x << 6 | y << 3 | z) and the 512-bit active mask. Tiles are
@@ -116,6 +118,74 @@ void dilated_leaves(const hns_leaf_grid& g, int32_t p, std::unordered_set<uint64
   }
 }
 
+// The domain's active voxels as 512-bit leaf masks (word x, bit y << 3 | z): the active voxels of
+// `g` dilated p times over the 26-neighbourhood (dilateVoxels with NN_FACE_EDGE_VERTEX,
+// SOP_HNanoSolver.cpp:280-287), each time one voxel along x, y and z in turn (the 26-neighbourhood
+// is the product of the three 3-voxel lines), with what leaves a leaf carried into its neighbour.
+struct VoxMasks {
+  struct Leaf {
+    std::array<int32_t, 3> org;
+    std::array<uint64_t, kWordsPerLeaf> m;
+  };
+  std::unordered_map<uint64_t, Leaf> at;
+  void add(const std::array<int32_t, 3>& o, const std::array<uint64_t, kWordsPerLeaf>& m) {
+    bool any = false;
+    for (uint64_t w : m) any |= w != 0;
+    if (!any) return;
+    auto it = at.try_emplace(key_of(o.data()), Leaf{o, {}}).first;
+    for (int w = 0; w < kWordsPerLeaf; ++w) it->second.m[w] |= m[w];
+  }
+  // one voxel along `axis` in both directions
+  void dilate_axis(int axis) {
+    constexpr uint64_t kZ0 = 0x0101010101010101ull, kZ7 = 0x8080808080808080ull;
+    std::vector<std::pair<std::array<int32_t, 3>, std::array<uint64_t, kWordsPerLeaf>>> spill;
+    for (auto& kv : at) {
+      std::array<uint64_t, kWordsPerLeaf>& m = kv.second.m;
+      const std::array<uint64_t, kWordsPerLeaf> src = m;
+      std::array<uint64_t, kWordsPerLeaf> up{}, down{};  // carried to the +axis and -axis leaves
+      for (int x = 0; x < kWordsPerLeaf; ++x) {
+        const uint64_t w = src[x];
+        if (axis == 2) {
+          m[x] |= ((w << 1) & ~kZ0) | ((w >> 1) & ~kZ7);
+          up[x] = (w & kZ7) >> 7;
+          down[x] = (w & kZ0) << 7;
+        } else if (axis == 1) {
+          m[x] |= (w << 8) | (w >> 8);
+          up[x] = w >> 56;
+          down[x] = w << 56;
+        } else {
+          if (x + 1 < kWordsPerLeaf) m[x + 1] |= w;
+          if (x > 0) m[x - 1] |= w;
+        }
+      }
+      if (axis == 0) {
+        up[0] = src[kWordsPerLeaf - 1];
+        down[kWordsPerLeaf - 1] = src[0];
+      }
+      std::array<int32_t, 3> o = kv.second.org;
+      o[axis] += 8;
+      spill.emplace_back(o, up);
+      o[axis] -= 16;
+      spill.emplace_back(o, down);
+    }
+    for (auto& sp : spill) {
+      bool any = false;
+      for (uint64_t w : sp.second) any |= w != 0;
+      if (!any) continue;
+      if (!leaf_in_range(sp.first[0] >> 3, sp.first[1] >> 3, sp.first[2] >> 3))
+        fail(HNS_ERR_TOPOLOGY, "hns_payload_build: the dilated domain leaves [-2^23, 2^23)");
+      add(sp.first, sp.second);
+    }
+  }
+  void add_grid(const hns_leaf_grid& g) {
+    for (size_t l = 0; l < g.num_leaves; ++l) {
+      std::array<uint64_t, kWordsPerLeaf> m;
+      for (int w = 0; w < kWordsPerLeaf; ++w) m[w] = g.active ? g.active[(size_t)kWordsPerLeaf * l + w] : ~uint64_t(0);
+      add({g.origins[3 * l], g.origins[3 * l + 1], g.origins[3 * l + 2]}, m);
+    }
+  }
+};
+
 void check_grid(const hns_leaf_grid& g, size_t i) {
   const std::string tag = "hns_payload_build: grid " + std::to_string(i);
   if (!g.name) fail(HNS_ERR_INVALID_ARGUMENT, tag + " has no name");
@@ -158,6 +228,7 @@ struct Payload {
   std::vector<std::string> names;
   std::vector<HostBlock> values;
   std::vector<hns_value_block> blocks;
+  std::vector<uint64_t> domain;  // the domain's active voxels, 8 words per leaf
 };
 
 Payload* build_payload(const hns_leaf_grid* grids, size_t num_grids, size_t domain_grid, int32_t padding) {
@@ -203,6 +274,26 @@ Payload* build_payload(const hns_leaf_grid* grids, size_t num_grids, size_t doma
       }
   });
 
+  // the domain's active voxels per leaf (the topology writeIndexGrid gives its grids)
+  {
+    VoxMasks vm;
+    vm.add_grid(grids[domain_grid]);
+    for (int32_t k = 0; k < padding; ++k)
+      for (int axis = 0; axis < 3; ++axis) vm.dilate_axis(axis);
+    for (size_t i = 0; i < num_grids; ++i) {
+      if (grids[i].role != HNS_ROLE_SDF) continue;
+      vm.add_grid(grids[i]);
+      break;
+    }
+    if (vm.at.size() != L) fail(HNS_ERR_TOPOLOGY, "hns_payload_build: voxel and leaf dilations disagree");
+    P->domain.resize(L * kWordsPerLeaf);
+    for (size_t l = 0; l < L; ++l) {
+      auto it = vm.at.find(key_of(leaves[l].data()));
+      if (it == vm.at.end()) fail(HNS_ERR_TOPOLOGY, "hns_payload_build: voxel and leaf dilations disagree");
+      std::memcpy(P->domain.data() + l * kWordsPerLeaf, it->second.m.data(), kWordsPerLeaf * sizeof(uint64_t));
+    }
+  }
+
   // value blocks in the grids' order; a repeated name keeps the first grid (GridData.hpp:50-53)
   std::vector<size_t> used;
   for (size_t i = 0; i < num_grids; ++i) {
@@ -242,6 +333,12 @@ Payload* build_payload(const hns_leaf_grid* grids, size_t num_grids, size_t doma
 }
 
 void destroy_payload(Payload* p) { delete p; }
+
+void payload_domain_mask(Payload* P, uint64_t* active, size_t num_leaves) {
+  if (!active) fail(HNS_ERR_INVALID_ARGUMENT, "hns_payload_domain_mask: null output");
+  if (num_leaves != P->L) fail(HNS_ERR_INVALID_ARGUMENT, "hns_payload_domain_mask: num_leaves is not the payload's");
+  std::memcpy(active, P->domain.data(), P->domain.size() * sizeof(uint64_t));
+}
 
 void payload_view(Payload* P, hns_grid_data* v, int32_t* pinned) {
   v->size = P->N;
@@ -290,6 +387,32 @@ void scatter_leaves(const hns_grid_data* d, const char* name, const uint8_t* mas
   *num_leaves = n;
   if (n > capacity && (origins || values || active))
     fail(HNS_ERR_INVALID_ARGUMENT, "hns_scatter_leaves: capacity below the number of output leaves");
+}
+
+// writeIndexGrid (GridBuilder.hpp:177-226): the payload field `name` as a grid of every payload
+// leaf, its buffer the payload's 512 values (memcpy, active or not) and its active voxels the
+// domain's (topologyUnion with the domain grid; null: every voxel).
+void write_leaf_grid(const hns_grid_data* d, const char* name, const uint64_t* domain_active, int32_t* origins,
+                     float* values, uint64_t* active) {
+  if (!d || !d->coords || !name) fail(HNS_ERR_INVALID_ARGUMENT, "hns_write_leaf_grid: null argument");
+  if (d->size % kLeafVox) fail(HNS_ERR_TOPOLOGY, "hns_write_leaf_grid: payload is not leaf-dense");
+  const hns_value_block* blk = nullptr;
+  for (size_t i = 0; i < d->num_blocks && !blk; ++i)
+    if (d->blocks[i].name && std::strcmp(d->blocks[i].name, name) == 0) blk = &d->blocks[i];
+  if (!blk || !blk->values) fail(HNS_ERR_NOT_FOUND, std::string("hns_write_leaf_grid: no block '") + name + "'");
+  const int comps = blk->type == HNS_VEC3F ? 3 : 1;
+  const float* src = static_cast<const float*>(blk->values);
+  const size_t L = d->size / kLeafVox;
+  parallel_ranges(L, [&](size_t b, size_t e) {
+    for (size_t l = b; l < e; ++l) {
+      if (origins) std::memcpy(origins + 3 * l, d->coords + 3 * l * kLeafVox, 3 * sizeof(int32_t));
+      if (values)
+        std::memcpy(values + l * kLeafVox * comps, src + l * kLeafVox * comps, kLeafVox * comps * sizeof(float));
+      if (active)
+        for (int w = 0; w < kWordsPerLeaf; ++w)
+          active[l * kWordsPerLeaf + w] = domain_active ? domain_active[l * kWordsPerLeaf + w] : ~uint64_t(0);
+    }
+  });
 }
 
 }  // namespace hns

@@ -9,6 +9,7 @@ OpenVDB and NanoVDB keep grids in:
     topo = create_index_grid(data, voxel_size, topo)               # kept when the leaves did not change
     ...
     out = scatter_leaves(data, "density", background=0.0)          # LeafGrid of the set-on voxels
+    grid = write_leaf_grid(data, "density", payload.domain_mask()) # writeIndexGrid: domain topology
 
 The payload is built in libhns (C++); it lives in pinned host memory when a device is visible, so the
 context's uploads of it are DMA transfers.
@@ -107,6 +108,14 @@ class Payload:
             d.add_value_block(b.name.decode(), arr)
         self.data = d
 
+    def domain_mask(self) -> np.ndarray:
+        """hns_payload_domain_mask: the domain's active voxels, (L, 8) uint64 in payload leaf order
+        (the dilated domain grid united with the SDF's active voxels)."""
+        L = len(self.data.coords) // 512
+        m = np.zeros((L, 8), np.uint64)
+        _check(lib().hns_payload_domain_mask(self._h, m.ctypes.data_as(C.POINTER(C.c_uint64)), L))
+        return m
+
     def close(self):
         if self._h:
             _check(lib().hns_payload_destroy(C.byref(self._h)))
@@ -148,3 +157,23 @@ def scatter_leaves(data: GridIndexedData, name: str, mask: Optional[np.ndarray] 
                                     origins.ctypes.data_as(_lib.c_int32_p), values.ctypes.data_as(_lib.c_float_p),
                                     active.ctypes.data_as(C.POINTER(C.c_uint64)), L, C.byref(n)))
     return LeafGrid(name, origins, values, active, background=background)
+
+
+def write_leaf_grid(data: GridIndexedData, name: str, domain_active: Optional[np.ndarray] = None) -> LeafGrid:
+    """hns_write_leaf_grid (IndexGridBuilder::writeIndexGrid): the payload field `name` as a LeafGrid
+    of every payload leaf, its buffers the payload's values (active or not) and its active voxels
+    domain_active's ((L, 8) uint64, as Payload.domain_mask gives them; None = every voxel)."""
+    g = data._c()
+    L = len(data.coords) // 512
+    vec3 = data[name].ndim == 2
+    origins = np.zeros((L, 3), np.int32)
+    values = np.zeros((L, 512, 3) if vec3 else (L, 512), np.float32)
+    active = np.zeros((L, 8), np.uint64)
+    dm = None
+    if domain_active is not None:
+        dm = np.ascontiguousarray(domain_active, np.uint64).reshape(L, 8)
+    _check(lib().hns_write_leaf_grid(C.byref(g), name.encode(),
+                                     None if dm is None else dm.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     origins.ctypes.data_as(_lib.c_int32_p), values.ctypes.data_as(_lib.c_float_p),
+                                     active.ctypes.data_as(C.POINTER(C.c_uint64))))
+    return LeafGrid(name, origins, values, active)

@@ -124,6 +124,11 @@ hns_status hns_payload_build(const hns_leaf_grid* grids, size_t num_grids, size_
 /* The payload as an hns_grid_data (pointers into the payload, valid until it is destroyed);
  * *pinned (nullable) = 1 when it lives in pinned host memory. */
 hns_status hns_payload_view(hns_payload* payload, hns_grid_data* view, int32_t* pinned);
+/* The domain's active voxels of each payload leaf, the topology IndexGridBuilder::writeIndexGrid
+ * gives its grids (topologyUnion with the domain grid, GridBuilder.hpp:201-206): the dilated domain
+ * grid's active voxels united with the SDF's, 8 words per leaf in payload order (bit (o & 63) of
+ * word (o >> 6) = voxel o). num_leaves must be the payload's leaf count (view.size / 512). */
+hns_status hns_payload_domain_mask(hns_payload* payload, uint64_t* active, size_t num_leaves);
 hns_status hns_payload_destroy(hns_payload** payload);
 /* The output scatter of SOP_HNanoSolver.cpp:345-388 (scatterScalar / scatterVector): the field
  * `name` of a leaf-dense payload as a grid whose voxels with mask[i] != 0 are set on with their
@@ -135,6 +140,13 @@ hns_status hns_payload_destroy(hns_payload** payload);
 hns_status hns_scatter_leaves(const hns_grid_data* data, const char* name, const uint8_t* mask, float background,
                               int32_t* origins, float* values, uint64_t* active, size_t capacity,
                               size_t* num_leaves);
+/* IndexGridBuilder::writeIndexGrid (GridBuilder.hpp:177-226): the field `name` of a leaf-dense
+ * payload as a grid of every payload leaf (payload order), each leaf's buffer the payload's 512
+ * values (x3 for a Vec3f block; memcpy of the leaf, active or not) and its active voxels
+ * domain_active's (8 words per leaf, as hns_payload_domain_mask gives them; NULL = every voxel).
+ * The arrays (nullable) hold view.size / 512 leaves: 3 int32, 512 (x3) floats and 8 words each. */
+hns_status hns_write_leaf_grid(const hns_grid_data* data, const char* name, const uint64_t* domain_active,
+                               int32_t* origins, float* values, uint64_t* active);
 
 /* ---- topology (replaces nanovdb::GridHandle<nanovdb::cuda::DeviceBuffer>) ------------ */
 typedef struct hns_topology hns_topology;

@@ -13,6 +13,7 @@
 //   Divergence / ProjectNonDivergent (stream&)               pointer-sized stream handle (or nullptr)
 //   IndexGridBuilder(domain, &data) + addGrid + build        hns::Payload(leaf grids, domain index, padding)
 //   scatterScalar / scatterVector (the SOP's output)         hns::ScatterLeaves(data, name, mask, bg, comps)
+//   builder.writeIndexGrid<GridT>(name, voxelSize)           hns::WriteLeafGrid(payload, name, comps)
 //
 // The payload type is any class shaped like HNS::GridIndexedData (src/Utils/GridData.hpp:16-166):
 // size(), pCoords() (3 int32 per voxel), getBlocksOfType<T>() and pValues<T>(name). Its Vec3 type
@@ -124,6 +125,12 @@ class Payload {
   const int32_t* pCoords() const { return view_.coords; }
   bool pinned() const { return pinned_; }
   const hns_grid_data& view() const { return view_; }
+  // the domain's active voxels, 8 words per leaf (hns_payload_domain_mask)
+  std::vector<uint64_t> domainMask() const {
+    std::vector<uint64_t> m(view_.size / 512 * 8);
+    check(hns_payload_domain_mask(p_, m.data(), view_.size / 512));
+    return m;
+  }
   template <class T>
   std::vector<std::string> getBlocksOfType() const {
     std::vector<std::string> out;
@@ -168,6 +175,20 @@ LeafOutput ScatterLeaves(GridData& data, const std::string& name, const std::vec
   o.active.resize(8 * o.leaves);
   check(hns_scatter_leaves(v.get(), name.c_str(), m, background, o.origins.data(), o.values.data(), o.active.data(),
                            o.leaves, &o.leaves));
+  return o;
+}
+
+// IndexGridBuilder::writeIndexGrid (GridBuilder.hpp:177-226) as leaves: every payload leaf with
+// the payload's buffer of `name` and the domain's active voxels
+inline LeafOutput WriteLeafGrid(const Payload& payload, const std::string& name, int comps) {
+  LeafOutput o;
+  o.leaves = payload.size() / 512;
+  const std::vector<uint64_t> dm = payload.domainMask();
+  o.origins.resize(3 * o.leaves);
+  o.values.resize((size_t)comps * 512 * o.leaves);
+  o.active.resize(8 * o.leaves);
+  check(hns_write_leaf_grid(&payload.view(), name.c_str(), dm.data(), o.origins.data(), o.values.data(),
+                            o.active.data()));
   return o;
 }
 

@@ -12,7 +12,10 @@ Only tests/ import this module. It restates, voxel by voxel and for small inputs
     leaf copied from the grid's leaf at the same origin (probeConstLeaf + memcpy of the buffer, active
     or not), else memset 0, and for the SDF memset(..., 1, ...): every byte 0x01;
   - the output scatter of SOP_HNanoSolver.cpp:345-388: setValueOn(coord, value) for each voxel whose
-    mask byte is set, on a grid whose other voxels hold the background.
+    mask byte is set, on a grid whose other voxels hold the background;
+  - IndexGridBuilder::writeIndexGrid (GridBuilder.hpp:177-226): a grid whose topology is the domain
+    grid's (topologyUnion: its active voxels are the domain's) and whose leaf buffers are memcpy'd from
+    the payload, every payload leaf in payload order.
 Grids are dicts {"name", "origins" (L, 3), "values" (L, 512[, 3]), "active" (L, 512) bool, "sdf"}.
 """
 from __future__ import annotations
@@ -36,8 +39,9 @@ def active_voxels(g):
     return out
 
 
-def domain_leaves(grids, domain, padding):
-    """Leaf origins of the dilated domain in tree order."""
+def domain_voxels(grids, domain, padding):
+    """The domain's active voxels: grids[domain]'s dilated by `padding` (every voxel within
+    L-infinity distance `padding`), united with the first SDF grid's."""
     vox = set()
     r = range(-padding, padding + 1)
     for c in active_voxels(grids[domain]):
@@ -47,6 +51,12 @@ def domain_leaves(grids, domain, padding):
         if g.get("sdf"):
             vox.update(active_voxels(g))
             break
+    return vox
+
+
+def domain_leaves(grids, domain, padding):
+    """Leaf origins of the dilated domain in tree order."""
+    vox = domain_voxels(grids, domain, padding)
     leaves = sorted({(x >> 3 << 3, y >> 3 << 3, z >> 3 << 3) for x, y, z in vox})
     org = np.asarray(leaves, np.int32).reshape(-1, 3)
     return org[leaf_order(org)] if len(org) else org
@@ -88,3 +98,13 @@ def scatter(coords, values, mask, background):
             if o not in leaves:
                 leaves.append(o)
     return on, leaves
+
+
+def write_index_grid(coords, values, vox):
+    """(origins, leaf buffers, active (L, 512) bool) of writeIndexGrid: every payload leaf, its 512
+    values copied, a voxel active iff it is a domain voxel (vox: the set of domain_voxels)."""
+    L = len(coords) // 512
+    origins = [tuple(int(v) for v in coords[512 * l]) for l in range(L)]
+    active = np.array([[tuple(int(v) for v in coords[512 * l + o]) in vox for o in range(512)] for l in range(L)])
+    buf = np.asarray(values).reshape((L, 512, 3) if np.asarray(values).ndim == 2 else (L, 512))
+    return origins, buf.copy(), active

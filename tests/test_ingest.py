@@ -1,6 +1,7 @@
-"""CPU checks of the host ingestion (hns_payload_build) and output scatter (hns_scatter_leaves)
-against the brute-force restatement in oracle/ingest.py (the SOP's dilated domain and
-IndexGridBuilder, SOP_HNanoSolver.cpp:276-301, GridBuilder.hpp:93-175; the scatter of
+"""CPU checks of the host ingestion (hns_payload_build), the domain's voxel masks
+(hns_payload_domain_mask), the output scatter (hns_scatter_leaves) and the writeIndexGrid export
+(hns_write_leaf_grid) against the brute-force restatement in oracle/ingest.py (the SOP's dilated
+domain and IndexGridBuilder, SOP_HNanoSolver.cpp:276-301, GridBuilder.hpp:93-226; the scatter of
 SOP_HNanoSolver.cpp:345-388), bit for bit, on small random sparse grids."""
 import numpy as np
 import pytest
@@ -111,3 +112,53 @@ def test_payload_errors():
     with pytest.raises(hns.HNSError) as e:
         ingest.build_payload([empty], 0, 3)
     assert e.value.status == -2
+
+
+def bits_to_bool(words):
+    w = np.asarray(words, np.uint64).reshape(-1, 8)
+    return ((w[:, np.arange(512) >> 6] >> (np.arange(512) & 63).astype(np.uint64)) & np.uint64(1)).astype(bool)
+
+
+@pytest.mark.parametrize("seed,padding", [(0, 0), (1, 1), (2, 2), (3, 9), (4, 17)])
+def test_domain_mask_matches_brute_force(seed, padding):
+    """The voxel-level domain (dilation by unit 26-neighbourhood steps, each as three axis shifts of
+    64-bit words with carries between leaves) against every voxel within L-infinity distance
+    `padding` of an active voxel, plus the SDF's voxels; padding 9 and 17 carry across two and three
+    leaves."""
+    grids, dom = scene(seed)
+    p = ingest.build_payload([to_leafgrid(g) for g in grids], dom, padding)
+    got = bits_to_bool(p.domain_mask())
+    vox = ref.domain_voxels(grids, dom, padding)
+    coords = p.data.coords
+    want = np.array([tuple(int(v) for v in c) in vox for c in coords]).reshape(-1, 512)
+    print("  padding %d: %d leaves, %d domain voxels" % (padding, len(want), int(want.sum())))
+    np.testing.assert_array_equal(got, want)
+    assert want.any(axis=1).all()  # every payload leaf holds a domain voxel
+    p.close()
+
+
+def test_write_leaf_grid_matches_and_round_trips():
+    grids, dom = scene(8)
+    lg = [to_leafgrid(g) for g in grids]
+    p = ingest.build_payload(lg, dom, 2)
+    d = p.data
+    dm = p.domain_mask()
+    vox = ref.domain_voxels(grids, dom, 2)
+    for name in ("density", "vel", "collision_sdf"):
+        out = ingest.write_leaf_grid(d, name, dm)
+        org, buf, act = ref.write_index_grid(d.coords, d[name], vox)
+        assert [tuple(o) for o in out.origins] == org
+        assert out.values.tobytes() == buf.astype(np.float32).tobytes(), name
+        np.testing.assert_array_equal(bits_to_bool(out.active), act)
+    full = ingest.write_leaf_grid(d, "density")
+    assert (full.active == ~np.uint64(0)).all()
+    # the exported grids rebuild the same payload: the velocity's active voxels are the domain
+    vel, dens = ingest.write_leaf_grid(d, "vel", dm), ingest.write_leaf_grid(d, "density", dm)
+    p2 = ingest.build_payload([dens, vel], 1, 0)
+    np.testing.assert_array_equal(p2.data.coords, d.coords)
+    np.testing.assert_array_equal(p2.domain_mask(), dm)
+    for name in ("vel", "density"):
+        assert p2.data[name].tobytes() == d[name].tobytes()
+    p2.close()
+    p.close()
+
