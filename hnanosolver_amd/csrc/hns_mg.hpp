@@ -26,7 +26,8 @@ struct MgShape {
   // residual scaled by c4_over. HNS_MG_C4, HNS_MG_C3PRE, HNS_MG_C3POST, HNS_MG_C4ITERS,
   // HNS_MG_C3OMEGA, HNS_MG_C3OMEGAP, HNS_MG_C4OMEGA and HNS_MG_C4OVER (x100) override them. side_groups workgroups carry the deferred
   // x update beside the one-workgroup coarse solve after side_sleeps * 127 * 64 idle cycles
-  // (HNS_MG_SIDE, HNS_MG_SIDE_SLEEPS; -1: the one-level solve's CH / 2, none for the two-level one).
+  // (HNS_MG_SIDE, HNS_MG_SIDE_SLEEPS; -1: 96 and CH / 2 beside the one-level solve, 128 and 2 beside
+  // the two-level one).
   int c4 = 1, c3_pre = 1, c3_post = 2, c4_iters = 8;
   float c3_omega = 1.0f, c3_omega_post = 1.4f, c4_omega = 1.7f, c4_over = 1.6f;
   int side_groups = -1, side_sleeps = -1;

@@ -147,10 +147,12 @@ __global__ __launch_bounds__(kTpb) void k_mg_leaf_sweep(DevTopo T, const float* 
 constexpr int kCoarseTpb = 1024;
 constexpr int kCoarseMaxCH = 8;
 // at most this many extra workgroups carry the deferred x update beside the coarse solve (each on a
-// CU of its own: it holds the solve's LDS allocation). Measured at 8 M voxels (the solve ~50 us,
-// the update ~100 MB): 96 workgroups that first sleep ~13 us (so the solve's initial loads meet an
-// idle memory system) beat 255 that start at once by 2 % of the fire step.
-constexpr int kCoarseSideGroups = 96;
+// CU of its own: it holds the solve's LDS allocation), after a sleep that lets the solve's initial
+// loads meet an idle memory system. Measured at 8 M voxels (the update 96 MB): beside the one-level
+// solve (~50 us) 96 workgroups sleeping CH / 2 * 127 * 64 cycles (~13 us) beat 255 starting at once
+// by 2 % of the fire step; beside the two-level solve (~31 us) 128 sleeping 2 * 127 * 64 cycles
+// (~7 us) beat 255 starting at once by 1.5 % (128-192 workgroups and 1-2 sleeps within noise).
+constexpr int kCoarseSideGroups = 96, kCoarseSideGroups2 = 128, kCoarseSleeps2 = 2;
 
 // Workgroup g of G extra workgroups of the coarse launch: the flexible CG's deferred
 // x += alpha_{k-1} d_{k-1} (k = *it; McgXUpdate) over n floats, k_fcg_update's fmaf, after
@@ -1397,11 +1399,11 @@ void coarse_solve(const DevTopo& T, const CoarseLayout& Ly, const float* b, floa
     }
     return;
   }
-  const int max_side = M.side_groups > 0 ? M.side_groups : (two ? 255 : kCoarseSideGroups);
+  const int max_side = M.side_groups > 0 ? M.side_groups : (two ? kCoarseSideGroups2 : kCoarseSideGroups);
   const int side = X.x ? (int)std::min<int64_t>(max_side, (n + 4 * kCoarseTpb - 1) / (4 * kCoarseTpb)) : 0;
   const unsigned grid = 1u + (unsigned)side;
   if (two) {
-    const int sleeps = M.side_sleeps >= 0 ? M.side_sleeps : 0;
+    const int sleeps = M.side_sleeps >= 0 ? M.side_sleeps : kCoarseSleeps2;
     switch (Ly.ch) {
       case 1: launch_coarse2<1>(Ly.ch4, grid, Ly, b, u, inv, stop, X, it_dev, n, sleeps, s); break;
       case 2: launch_coarse2<2>(Ly.ch4, grid, Ly, b, u, inv, stop, X, it_dev, n, sleeps, s); break;
