@@ -44,7 +44,8 @@ inline int64_t pc_num_params() { return pc_offsets().param[kPcLayers]; }
 // Activations of one forward pass (kept for the backward pass) and gradient scratch, sized for a
 // topology of L leaves.
 struct PcWork {
-  int32_t L = 0;
+  int32_t L = 0;   // leaves the inference activations hold
+  int32_t Lt = 0;  // leaves the training activations and gradients (h6, gA*, gB*) hold
   DBuf<__bf16> h0, p1, h1, p2, h2, h3, g2, s1, h5, g1, s0, h6;
   DBuf<__bf16> gA0, gB0, gA1, gB1, gA2, gB2;  // gradient ping-pong per level
   DBuf<float> wpart;                        // weight-gradient partials

@@ -18,11 +18,13 @@ bool fused_inference() {
 }
 }
 
+// The inference and the training buffers grow separately, each to the largest topology it has
+// served: a training pass after a larger inference-only pass must not find gradient buffers sized
+// for an earlier, smaller topology.
 void PcWork::ensure(int32_t leaves, bool training) {
-  const bool grow = leaves > L;
-  const int64_t n0 = (int64_t)leaves * 512, n1 = (int64_t)leaves * 64, n2 = (int64_t)leaves * 8;
-  if (grow || !h0.p) {
+  if (leaves > L || !h0.p) {
     L = std::max(L, leaves);
+    const int64_t n0 = (int64_t)L * 512, n1 = (int64_t)L * 64, n2 = (int64_t)L * 8;
     h0.alloc(n0 * 16);
     p1.alloc(n1 * 16);
     h1.alloc(n1 * 32);
@@ -34,10 +36,10 @@ void PcWork::ensure(int32_t leaves, bool training) {
     h5.alloc(n1 * 32);
     g1.alloc(n1 * 16);
     s0.alloc(n0 * 16);
-    h6.reset();
-    gA0.reset();
   }
-  if (training && !h6.p) {
+  if (training && (leaves > Lt || !h6.p)) {
+    Lt = L;
+    const int64_t n0 = (int64_t)Lt * 512, n1 = (int64_t)Lt * 64, n2 = (int64_t)Lt * 8;
     h6.alloc(n0 * 16);
     gA0.alloc(n0 * 16);
     gB0.alloc(n0 * 16);
@@ -102,7 +104,7 @@ void LearnedPrecond::forward(const DevTopo& T, PcWork& W, const float* r_cs, flo
 
 void LearnedPrecond::backward(const DevTopo& T, PcWork& W, const float* r_cs, const float* dz_cs, float* grad,
                               hipStream_t s) {
-  if (!W.h6.p || W.L < T.L) fail(HNS_ERR_INVALID_ARGUMENT, "backward needs a forward pass that kept activations");
+  if (!W.h6.p || W.Lt < T.L) fail(HNS_ERR_INVALID_ARGUMENT, "backward needs a forward pass that kept activations");
   const PcOffsets& O = pc_offsets();
   const __bf16* wb = wbwd_.p;
   const int32_t L = T.L;

@@ -127,6 +127,49 @@ def test_backward_matches_dense_oracle(hip):
     pc.close()
 
 
+def test_training_after_larger_inference_reuses_work_buffers(hip):
+    """One preconditioner serving an inference pass on a large topology, then training passes on a
+    small and a mid-sized one (the training buffers must grow with the largest topology the network
+    has served, not with the first training pass): the mid-sized pass's output and weight gradient
+    are bitwise those of a fresh preconditioner's."""
+    torch = hip
+    big, small, mid = compact_ragged(21, extent=8), compact_ragged(22, extent=4), compact_ragged(23, extent=6)
+    L = [len(c) // 512 for c in (big, small, mid)]
+    _case("precond work buffers: inference %d leaves, then training %d and %d leaves" % tuple(L))
+    assert L[1] < L[2] < L[0]
+    w = P.init_params(11)
+    rng = np.random.default_rng(12)
+
+    def pass_on(pc, c, keep, grad):
+        topo = _topo(c)
+        r_cs = _cs(torch, topo, rng.standard_normal(len(c)).astype(np.float32))
+        z_cs = torch.empty_like(r_cs)
+        pc.apply(topo, r_cs, z_cs, keep=keep)
+        if keep:
+            pc.backward(topo, r_cs, _cs(torch, topo, np.ones(len(c), np.float32)), grad)
+        return topo, r_cs, z_cs
+
+    params_a = torch.from_numpy(w).cuda()
+    grad_a = torch.zeros_like(params_a)
+    pc_a = P.Preconditioner(params_a)
+    pass_on(pc_a, big, False, grad_a)
+    pass_on(pc_a, small, True, grad_a)
+    grad_a.zero_()
+    topo, r_cs, z_a = pass_on(pc_a, mid, True, grad_a)
+    params_b = torch.from_numpy(w).cuda()
+    grad_b = torch.zeros_like(params_b)
+    pc_b = P.Preconditioner(params_b)
+    z_b = torch.empty_like(r_cs)
+    pc_b.apply(topo, r_cs, z_b, keep=True)
+    pc_b.backward(topo, r_cs, _cs(torch, topo, np.ones(len(mid), np.float32)), grad_b)
+    torch.cuda.synchronize()
+    nz, ng = int((z_a != z_b).sum()), int((grad_a != grad_b).sum())
+    print("  differing z %d of %d, differing gradient entries %d of %d" % (nz, z_a.numel(), ng, grad_a.numel()))
+    assert nz == 0 and ng == 0
+    pc_a.close()
+    pc_b.close()
+
+
 def test_poisson_operator_matches_oracle(hip):
     torch = hip
     c = compact_ragged(4)
