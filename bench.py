@@ -99,14 +99,16 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(sc, iters_per_step, budget_s, K):
-    """The CPU oracle (a from-scratch restatement of the reference step; the reference has no CPU
-    solver) on the host cores, on a bounded sample of the same workload: a step without the
-    pressure solve plus a bounded number of CG iterations on the same domain, scaled to the GPU CG
-    leg's iteration count, at all host threads and at one thread; and the reference's own step
-    (fixed K RBGS iterations) at all threads."""
+def cpu_baseline(sc, head, gpu_iters, cg_iters, budget_s, K):
+    """The CPU oracle (a from-scratch restatement of the reference step, plus the library's solvers
+    restated on the host; the reference itself has no CPU solver) on the host cores, on a bounded
+    sample of the same workload: the step without the pressure solve, plus a bounded number of
+    iterations of the headline leg's solver (multigrid-preconditioned CG, or CG) scaled to that leg's
+    iterations per step, at all host threads and at one thread. Beside it: the oracle step with CG at
+    the GPU CG leg's count, and the reference's own step (fixed K RBGS iterations), at all threads."""
     import oracle
     o = oracle.Oracle(sc.coords)
+    mg = oracle.Multigrid(sc.coords) if head == "mgcg" else None
     all_threads = oracle.num_threads()
     div = np.random.default_rng(0).standard_normal(sc.n).astype(np.float32)
     omega = float(np.float32(2.0 / (1.0 + np.sin(np.float32(3.14159) * np.float32(sc.voxel_size)))))
@@ -122,6 +124,11 @@ def cpu_baseline(sc, iters_per_step, budget_s, K):
         o.cg(div, sc.voxel_size, tol=0.0, max_iters=n)
         return (time.perf_counter() - t0) / n
 
+    def mg_iter_s(n):  # tol 0: exactly n iterations (and one closing residual evaluation)
+        t0 = time.perf_counter()
+        mg.solve(div, sc.voxel_size, tol=0.0, max_iters=n)
+        return (time.perf_counter() - t0) / n
+
     def rbgs_iter_s(n):
         p = np.zeros(sc.n, np.float32)
         t0 = time.perf_counter()
@@ -129,35 +136,38 @@ def cpu_baseline(sc, iters_per_step, budget_s, K):
             p = o.rbgs(div, o.rbgs(div, p, sc.voxel_size, 0, omega), sc.voxel_size, 1, omega)
         return (time.perf_counter() - t0) / n
 
-    def sample(threads, min_s, budget):
+    def sample(threads, it_s, iters_per_step, min_s, budget, max_rounds=20):
         oracle.set_threads(threads)
         t_rest = step_without_solve()
-        probe = cg_iter_s(2)
+        probe = it_s(2)
         n = int(max(2, min(iters_per_step, (budget - t_rest) / max(probe, 1e-6))))
         rests, iters, t_start = [t_rest], [], time.perf_counter()
         while True:
-            iters.append(cg_iter_s(n))
-            if time.perf_counter() - t_start + t_rest >= min_s or len(iters) >= 20:
+            iters.append(it_s(n))
+            if time.perf_counter() - t_start + t_rest >= min_s or len(iters) >= max_rounds:
                 break
             rests.append(step_without_solve())
         return float(np.mean(rests)), float(np.mean(iters)), len(rests), n, len(iters)
 
-    rest_a, it_a, nr_a, n_a, ni_a = sample(all_threads, 10.0, budget_s)
-    rest_1, it_1, nr_1, n_1, ni_1 = sample(1, 0.0, budget_s / 2)  # one round at one thread
+    algo, it_s, iters = ("mgcg", mg_iter_s, gpu_iters) if mg is not None else ("cg", cg_iter_s, cg_iters)
+    rest_a, it_a, nr_a, n_a, ni_a = sample(all_threads, it_s, iters, 10.0, budget_s)
+    rest_1, it_1, nr_1, n_1, ni_1 = sample(1, it_s, iters, 0.0, budget_s / 2, max_rounds=1)
     oracle.set_threads(all_threads)
+    cg_a = cg_iter_s(max(2, min(int(cg_iters), 50)))
     rb = rbgs_iter_s(max(2, min(K, 10)))
-    t_all = rest_a + iters_per_step * it_a
-    t_one = rest_1 + iters_per_step * it_1
-    t_ref = rest_a + K * rb
+    t_all = rest_a + iters * it_a
+    t_one = rest_1 + iters * it_1
     return {"value": sc.n / t_all, "unit": "voxel-steps/s", "cores": all_threads, "kind": "port",
-            "cpu_model": cpu_model(), "value_1_thread": sc.n / t_one,
-            "reference_step_value": sc.n / t_ref,
+            "algorithm": algo, "cpu_model": cpu_model(), "value_1_thread": sc.n / t_one,
+            "cg_step_value": sc.n / (rest_a + cg_iters * cg_a),
+            "reference_step_value": sc.n / (rest_a + K * rb),
             "reference_step": "the reference's step (fixed %d RBGS iterations) at %d threads" % (K, all_threads),
             "sample": "%s (N=%d): the oracle step without the pressure solve (%.2f s at %d threads, %.2f s at 1) and "
-                      "CG iterations (%d rounds of %d at %d threads, %.4f s each; %d at 1 thread, %.4f s each), "
-                      "scaled to the GPU CG leg's %.0f iterations per step; RBGS iterations %.4f s each" % (
-                          sc.name, sc.n, rest_a, all_threads, rest_1, ni_a, n_a, all_threads, it_a, n_1, it_1,
-                          iters_per_step, rb)}
+                      "%s iterations (%d rounds of %d at %d threads, %.4f s each; %d at 1 thread, %.4f s each), "
+                      "scaled to the GPU %s leg's %.1f iterations per step; CG iterations %.4f s each (scaled to "
+                      "the GPU CG leg's %.0f for cg_step_value), RBGS iterations %.4f s each" % (
+                          sc.name, sc.n, rest_a, all_threads, rest_1, algo, ni_a, n_a, all_threads, it_a, n_1, it_1,
+                          algo, iters, cg_a, cg_iters, rb)}
 
 
 def run_leg(hns, torch, sc, data, topo, params, solver, steps, warmup, instrument, K):
@@ -313,10 +323,11 @@ def main(argv=None):
             out["speedup_%s_vs_plain_rbgs_to_tol" % s] = legs["rbgs_tol"]["ms_per_step"] / legs[s]["ms_per_step"]
     if not a.no_cpu_baseline:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        # the oracle runs CG (it has no multigrid), at the GPU CG leg's iteration count when there is one;
-        # the GPU legs of the same algorithms (cg, rbgs) are under "legs"
-        cpu_iters = legs["cg"]["iterations_per_step"] if "cg" in legs else 500
-        out["cpu_baseline"] = cpu_baseline(sc, cpu_iters, a.cpu_budget, K)
+        # the oracle runs the headline leg's solver at that leg's iterations per step; the GPU legs of
+        # the other algorithms (cg, rbgs) are under "legs", their CPU counterparts beside the value
+        cg_iters = legs["cg"]["iterations_per_step"] if "cg" in legs else 500
+        gpu_iters = L.get("iterations_per_step", cg_iters) if head in ("mgcg", "cg") else cg_iters
+        out["cpu_baseline"] = cpu_baseline(sc, head, gpu_iters, cg_iters, a.cpu_budget, K)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

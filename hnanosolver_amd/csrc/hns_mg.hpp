@@ -13,21 +13,17 @@ constexpr int kMgLevels = 4;
 // post-smoothing Gauss-Seidel iterations on level 0 and on levels 1-2, and the flexible CG's
 // truncation around the V-cycle, and the scale of every coarse-grid correction (piecewise-constant
 // prolongation under-corrects smooth errors; over-correcting by 1.6 took the 8 M-voxel fire solve
-// from 16 to 12 iterations). Defaults are the measured best at 8 M voxels; HNS_MG_COARSE,
-// HNS_MG_COARSE_OMEGA (x100), HNS_MG_SMOOTH0, HNS_MG_SMOOTH, HNS_MG_DEPTH, HNS_MG_OVER (x100),
-// HNS_MG_OVER_DEEP (x100; default: HNS_MG_OVER), HNS_MG_OMEGA0 and HNS_MG_OMEGAL (x100) and
-// HNS_MG_PCG (Fletcher-Reeves PCG instead of FCG(1)) override them (read once per process).
+// from 16 to 12 iterations). The values are the measured best at 8 M voxels, fixed at build time:
+// they are the library's solver, not per-process settings.
 struct MgShape {
   int coarse = 12, smooth0 = 1, smooth = 1, depth = 1, pcg = 0;
   float coarse_omega = 1.85f;
   // The two-level coarse solve (c4 = 1; 0: `coarse` SOR iterations on level 3 alone): c3_pre and
   // c3_post red-black iterations on level 3 (relaxations c3_omega and c3_omega_post) around c4_iters
   // SOR iterations (c4_omega) on level 4 (2^3-leaf blocks, Galerkin operator), the restricted
-  // residual scaled by c4_over. HNS_MG_C4, HNS_MG_C3PRE, HNS_MG_C3POST, HNS_MG_C4ITERS,
-  // HNS_MG_C3OMEGA, HNS_MG_C3OMEGAP, HNS_MG_C4OMEGA and HNS_MG_C4OVER (x100) override them. side_groups workgroups carry the deferred
-  // x update beside the one-workgroup coarse solve after side_sleeps * 127 * 64 idle cycles
-  // (HNS_MG_SIDE, HNS_MG_SIDE_SLEEPS; -1: 96 and CH / 2 beside the one-level solve, 128 and 2 beside
-  // the two-level one).
+  // residual scaled by c4_over. side_groups workgroups carry the deferred x update beside the
+  // one-workgroup coarse solve after side_sleeps * 127 * 64 idle cycles (-1: 96 and CH / 2 beside
+  // the one-level solve, 128 and 2 beside the two-level one).
   int c4 = 1, c3_pre = 1, c3_post = 2, c4_iters = 8;
   float c3_omega = 1.0f, c3_omega_post = 1.4f, c4_omega = 1.7f, c4_over = 1.6f;
   int side_groups = -1, side_sleeps = -1;

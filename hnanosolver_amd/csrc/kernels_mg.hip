@@ -13,7 +13,6 @@
 // level coordinate (at level 3, the leaf's).
 #include <algorithm>
 #include <array>
-#include <cstdlib>
 #include <unordered_map>
 #include <vector>
 
@@ -23,35 +22,7 @@
 namespace hns {
 
 const MgShape& mg_shape() {
-  static const MgShape v = [] {
-    MgShape m;
-    auto env = [](const char* k, int lo, int hi, int d) {
-      const char* e = std::getenv(k);
-      const int x = e ? std::atoi(e) : d;
-      return x >= lo && x <= hi ? x : d;
-    };
-    m.coarse = env("HNS_MG_COARSE", 1, 256, m.coarse);
-    m.smooth0 = env("HNS_MG_SMOOTH0", 1, 8, m.smooth0);
-    m.smooth = env("HNS_MG_SMOOTH", 1, 8, m.smooth);
-    m.depth = env("HNS_MG_DEPTH", 1, 8, m.depth);
-    m.pcg = env("HNS_MG_PCG", 0, 1, m.pcg);
-    m.omega0 = 0.01f * (float)env("HNS_MG_OMEGA0", 50, 199, (int)(100.0f * m.omega0 + 0.5f));
-    m.omega_l = 0.01f * (float)env("HNS_MG_OMEGAL", 50, 199, (int)(100.0f * m.omega_l + 0.5f));
-    m.coarse_omega = 0.01f * (float)env("HNS_MG_COARSE_OMEGA", 100, 199, (int)(100.0f * m.coarse_omega + 0.5f));
-    m.over = 0.01f * (float)env("HNS_MG_OVER", 50, 300, (int)(100.0f * m.over + 0.5f));
-    m.over_deep = 0.01f * (float)env("HNS_MG_OVER_DEEP", 50, 300, (int)(100.0f * m.over + 0.5f));
-    m.c4 = env("HNS_MG_C4", 0, 1, m.c4);
-    m.c3_pre = env("HNS_MG_C3PRE", 1, 8, m.c3_pre);
-    m.c3_post = env("HNS_MG_C3POST", 0, 8, m.c3_post);
-    m.c4_iters = env("HNS_MG_C4ITERS", 1, 64, m.c4_iters);
-    m.c3_omega = 0.01f * (float)env("HNS_MG_C3OMEGA", 50, 199, (int)(100.0f * m.c3_omega + 0.5f));
-    m.c3_omega_post = 0.01f * (float)env("HNS_MG_C3OMEGAP", 50, 199, (int)(100.0f * m.c3_omega_post + 0.5f));
-    m.c4_omega = 0.01f * (float)env("HNS_MG_C4OMEGA", 50, 199, (int)(100.0f * m.c4_omega + 0.5f));
-    m.c4_over = 0.01f * (float)env("HNS_MG_C4OVER", 50, 300, (int)(100.0f * m.c4_over + 0.5f));
-    m.side_groups = env("HNS_MG_SIDE", 1, 255, m.side_groups);
-    m.side_sleeps = env("HNS_MG_SIDE_SLEEPS", 0, 16, m.side_sleeps);
-    return m;
-  }();
+  static const MgShape v{};
   return v;
 }
 

@@ -79,6 +79,19 @@ def load():
             f("advance").restype = C.c_int
             f("advance").argtypes = [vp, C.c_float, C.c_int, C.c_float, C.POINTER(OracleParams), vp, C.c_int, vp, vp,
                                      vp, C.c_int, vp, vp, vp, vp]
+        _lib.oracle_mg_create.restype = vp
+        _lib.oracle_mg_create.argtypes = [vp, C.c_int64]
+        _lib.oracle_mg_destroy.argtypes = [vp]
+        _lib.oracle_mg_two_level.restype = C.c_int
+        _lib.oracle_mg_two_level.argtypes = [vp]
+        _lib.oracle_mg_blocks.restype = C.c_int64
+        _lib.oracle_mg_blocks.argtypes = [vp]
+        _lib.oracle_mg_vcycle.argtypes = [vp, vp, vp]
+        _lib.oracle_mg_apply_level.argtypes = [vp, C.c_int, vp, C.c_float, vp]
+        _lib.oracle_mg_apply_level4.argtypes = [vp, vp, vp]
+        _lib.oracle_mg_blocks_of.argtypes = [vp, vp, vp]
+        _lib.oracle_mgcg.restype = C.c_int
+        _lib.oracle_mgcg.argtypes = [vp, vp, C.c_float, C.c_double, C.c_int, vp, vp]
     return _lib
 
 
@@ -312,3 +325,64 @@ class Oracle:
         if st != 0:
             raise RuntimeError("oracle advance failed: %d" % st)
         return vel, dict(zip(names, arrs))
+
+
+class Multigrid:
+    """The V-cycle preconditioner of the library's multigrid-preconditioned solve and the flexible
+    CG on it (hns_oracle_mg.cpp), FP32 on leaf-dense level-0 vectors of a payload whose leaves are
+    given by their origins (one per 512 voxels, payload order)."""
+
+    def __init__(self, coords: np.ndarray):
+        c = np.ascontiguousarray(coords, dtype=np.int32).reshape(-1, 3)
+        self.origins = np.ascontiguousarray(c[::512])
+        self.L = len(self.origins)
+        self.n = 512 * self.L
+        self._g = load().oracle_mg_create(self.origins.ctypes.data, self.L)
+
+    def __del__(self):
+        if getattr(self, "_g", None):
+            load().oracle_mg_destroy(self._g)
+            self._g = None
+
+    @property
+    def two_level(self) -> bool:
+        return bool(load().oracle_mg_two_level(self._g))
+
+    @property
+    def num_blocks(self) -> int:
+        return int(load().oracle_mg_blocks(self._g))
+
+    def vcycle(self, r) -> np.ndarray:
+        r = np.ascontiguousarray(r, np.float32)
+        z = np.empty(self.n, np.float32)
+        load().oracle_mg_vcycle(self._g, r.ctypes.data, z.ctypes.data)
+        return z
+
+    def apply_level(self, level: int, u, scale: float) -> np.ndarray:
+        """scale (6 u - face neighbours) on level `level` (0..3, (8 >> level)^3 cells per leaf)."""
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.empty_like(u)
+        load().oracle_mg_apply_level(self._g, int(level), u.ctypes.data, float(scale), out.ctypes.data)
+        return out
+
+    def apply_level4(self, e) -> np.ndarray:
+        e = np.ascontiguousarray(e, np.float64)
+        out = np.empty_like(e)
+        load().oracle_mg_apply_level4(self._g, e.ctypes.data, out.ctypes.data)
+        return out
+
+    def blocks(self):
+        """(level-4 block of each leaf, colour of each block)."""
+        blk = np.empty(self.L, np.int32)
+        col = np.empty(self.num_blocks, np.int32)
+        load().oracle_mg_blocks_of(self._g, blk.ctypes.data, col.ctypes.data)
+        return blk, col
+
+    def solve(self, div, dx: float, tol: float = 1e-5, max_iters: int = 1000):
+        """Flexible CG on the V-cycle for A p = -div dx^2 from p = 0: (p, iterations, rel residual)."""
+        d = np.ascontiguousarray(div, np.float32)
+        p = np.empty(self.n, np.float32)
+        rel = C.c_double()
+        it = load().oracle_mgcg(self._g, d.ctypes.data, float(dx), float(tol), int(max_iters), p.ctypes.data,
+                                C.byref(rel))
+        return p, int(it), float(rel.value)

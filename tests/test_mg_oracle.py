@@ -1,0 +1,174 @@
+"""CPU pins of the multigrid oracle (oracle/hns_oracle_mg.cpp), the checker of the library's
+V-cycle and of its multigrid-preconditioned solve. The reference has no multigrid (it runs a fixed
+red-black Gauss-Seidel count, src/Cuda/HNanoSolver.cu:364-373), so the oracle is pinned by what the
+mathematics fixes:
+- every level operator equals the 7-point operator 6 I - (face neighbours) built by brute force
+  from the level's cell coordinates (exact on integer vectors);
+- the hierarchy is Galerkin: R (6 I - N_l) P = 4 (6 I - N_{l+1}) for levels 0-2, and level 4's
+  weights are P^T A_3 P of the 2^3-leaf blocks (exact on integer vectors);
+- the V-cycle is linear in r and, used as the stationary iteration x <- x + V(b - A x), contracts
+  the energy-norm error of the Poisson problem geometrically;
+- the flexible CG on it reaches a scipy solve of the same Poisson problem."""
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from hnanosolver_amd import scenes
+
+from _helpers import canonical_coords, ragged_origins
+
+
+def _origins(coords):
+    return np.asarray(coords, np.int64)[::512]
+
+
+def level_cells(origins, S):
+    """Cell coordinates of the level with S cells per leaf axis, leaf-dense order."""
+    loc = np.stack(np.meshgrid(np.arange(S), np.arange(S), np.arange(S), indexing="ij"), -1).reshape(-1, 3)
+    return ((np.asarray(origins, np.int64) // 8)[:, None, :] * S + loc[None]).reshape(-1, 3)
+
+
+def seven_point(cells):
+    """6 I - (face neighbours present in the cell set), by coordinate lookup."""
+    n = len(cells)
+    lut = {tuple(c): i for i, c in enumerate(cells.tolist())}
+    rows, cols, vals = list(range(n)), list(range(n)), [6.0] * n
+    for i, c in enumerate(cells.tolist()):
+        for d in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+            j = lut.get((c[0] + d[0], c[1] + d[1], c[2] + d[2]))
+            if j is not None:
+                rows.append(i)
+                cols.append(j)
+                vals.append(-1.0)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+
+def prolongation(L, S):
+    """P from the level with S/2 cells per axis to the one with S (a coarse value to its 8 children)."""
+    Sc = S // 2
+    f = np.arange(L * S ** 3)
+    leaf, c = f // S ** 3, f % S ** 3
+    x, y, z = c // (S * S), (c // S) % S, c % S
+    parent = leaf * Sc ** 3 + ((x // 2) * Sc + y // 2) * Sc + z // 2
+    return sp.csr_matrix((np.ones(len(f)), (f, parent)), shape=(len(f), L * Sc ** 3))
+
+
+def _ragged():
+    return canonical_coords(ragged_origins(3))
+
+
+def test_level_operators_match_brute_force():
+    c = _ragged()
+    g = oracle.Multigrid(c)
+    org = _origins(c)
+    rng = np.random.default_rng(0)
+    for level in range(4):
+        S = 8 >> level
+        A = seven_point(level_cells(org, S))
+        u = rng.integers(-8, 9, A.shape[0]).astype(np.float32)
+        got = g.apply_level(level, u, 1.0)
+        assert np.array_equal(got.astype(np.float64), A @ u.astype(np.float64)), level
+
+
+def test_hierarchy_is_galerkin():
+    c = _ragged()
+    org = _origins(c)
+    L = len(org)
+    rng = np.random.default_rng(1)
+    for level in range(3):
+        S = 8 >> level
+        A_f = seven_point(level_cells(org, S))
+        A_c = seven_point(level_cells(org, S // 2))
+        P = prolongation(L, S)
+        uc = rng.integers(-8, 9, A_c.shape[0]).astype(np.float64)
+        assert np.array_equal(P.T @ (A_f @ (P @ uc)), 4.0 * (A_c @ uc)), level
+
+
+def test_level4_weights_are_the_galerkin_product():
+    c = _ragged()
+    g = oracle.Multigrid(c)
+    org = _origins(c)
+    L = len(org)
+    blk, colour = g.blocks()
+    # the blocks independently: 2^3 groups of leaf coordinates
+    bc = (org // 8) >> 1
+    keys, mine = np.unique(bc, axis=0, return_inverse=True)
+    mine = mine.reshape(-1)
+    assert g.num_blocks == len(keys)
+    # the oracle's numbering is a relabelling of the same partition
+    m = np.full(g.num_blocks, -1)
+    m[blk] = mine
+    assert np.array_equal(m[blk], mine) and len(np.unique(m)) == len(keys)
+    assert np.array_equal(colour, keys[m].sum(1) & 1)
+    A3 = seven_point(level_cells(org, 1))
+    P4 = sp.csr_matrix((np.ones(L), (np.arange(L), blk)), shape=(L, g.num_blocks))
+    rng = np.random.default_rng(2)
+    e = rng.integers(-8, 9, g.num_blocks).astype(np.float64)
+    assert np.array_equal(g.apply_level4(e), P4.T @ (A3 @ (P4 @ e)))
+    assert g.two_level
+
+
+def _poisson(c):
+    A = seven_point(c.astype(np.int64))
+    return A.tocsr()
+
+
+def test_vcycle_is_linear_and_contracts_the_error():
+    c = _ragged()
+    g = oracle.Multigrid(c)
+    A = _poisson(c)
+    rng = np.random.default_rng(3)
+    r1 = rng.standard_normal(len(c)).astype(np.float32)
+    r2 = rng.standard_normal(len(c)).astype(np.float32)
+    lhs = g.vcycle((2.0 * r1 - 0.5 * r2).astype(np.float32)).astype(np.float64)
+    rhs = 2.0 * g.vcycle(r1).astype(np.float64) - 0.5 * g.vcycle(r2).astype(np.float64)
+    assert np.linalg.norm(lhs - rhs) <= 1e-5 * np.linalg.norm(rhs)
+    # stationary iteration on A x = b from 0: the A-norm of the error falls geometrically
+    b = rng.standard_normal(len(c))
+    x_star = spla.spsolve(A.tocsc(), b)
+    x = np.zeros(len(c))
+    errs = []
+    for _ in range(6):
+        x = x + g.vcycle((b - A @ x).astype(np.float32)).astype(np.float64)
+        e = x - x_star
+        errs.append(float(np.sqrt(e @ (A @ e))))
+    e0 = float(np.sqrt(x_star @ (A @ x_star)))
+    rates = [errs[0] / e0] + [b_ / a_ for a_, b_ in zip(errs, errs[1:])]
+    print("energy-norm contraction per V-cycle:", ["%.3f" % r for r in rates])
+    assert max(rates) < 0.5
+
+
+def test_flexible_cg_on_the_vcycle_reaches_the_direct_solution():
+    c = _ragged()
+    g = oracle.Multigrid(c)
+    dx = 0.1
+    rng = np.random.default_rng(4)
+    div = rng.standard_normal(len(c)).astype(np.float32)
+    A = _poisson(c)
+    b = -div.astype(np.float64) * float(np.float32(np.float32(dx) * np.float32(dx)))
+    p_star = spla.spsolve(A.tocsc(), b)
+    p, it, rel = g.solve(div, dx, tol=1e-5)
+    true_rel = np.linalg.norm(b - A @ p.astype(np.float64)) / np.linalg.norm(b)
+    print("mgcg %d iterations, rel %.2e (recomputed %.2e)" % (it, rel, true_rel))
+    assert true_rel <= 1.01e-5 and rel <= 1.01e-5
+    assert it <= 12
+    assert np.linalg.norm(p - p_star) <= 1e-3 * np.linalg.norm(p_star)
+
+
+def test_flexible_cg_on_a_scene():
+    """The smoke scene's projection right-hand side (the oracle's own step up to the divergence):
+    to 1e-5 in a handful of iterations, far fewer than plain CG."""
+    sc = scenes.smoke64()
+    o = oracle.Oracle(sc.coords)
+    vel = o.advect_vector(sc.velocity, sc.dt, 1.0 / sc.voxel_size)
+    div = o.divergence(vel, 1.0 / sc.voxel_size)
+    g = oracle.Multigrid(sc.coords)
+    p, it, rel = g.solve(div, sc.voxel_size, tol=1e-5)
+    _, it_cg, _ = o.cg(div, sc.voxel_size, tol=1e-5)
+    r2, d2 = oracle.Oracle(sc.coords, np.float64).residual(p.astype(np.float64), div.astype(np.float64),
+                                                           sc.voxel_size)
+    print("smoke64: mgcg %d iterations (rel %.2e, FP64 oracle residual %.2e), cg %d" % (
+        it, rel, np.sqrt(r2 / d2), it_cg))
+    assert np.sqrt(r2 / d2) <= 1.01e-5
+    assert 3 * it <= it_cg
