@@ -45,14 +45,14 @@ TRAFFIC_FILES = {"mg_vcycle": "mg_vcycle"}
 # update rides on the coarse-level launch of the next V-cycle)
 KERNEL_BYTES["mgcg_update"] = 12.0
 # Multigrid-preconditioned CG, compulsory bytes per voxel and iteration: the V-cycle's fused level-0
-# passes (pre: read r, write z and the level-1 right-hand side, 8.5 B; post: read z, its parents, r
-# and the previous product q, write z, 16.5 B), the coarser levels (~3 B), direction and product
-# (read z, d_prev, write d, q: 16 B), the residual update (12 B) and the x update (read x, d, write
-# x: 12 B, in the coarse-level launch)
-MGCG_BYTES_PER_ITERATION = 8.5 + 16.5 + 3.0 + 16.0 + 12.0 + 12.0
+# passes (pre: read r, write the level-1 right-hand side, 4.5 B; post: read r, the level-1
+# correction and the previous product q, write z, 12.5 B), the coarser levels (~3 B), direction and
+# product (read z, d_prev, write d, q: 16 B), the residual update (12 B) and the x update (read x,
+# d, write x: 12 B, in the coarse-level launch)
+MGCG_BYTES_PER_ITERATION = 4.5 + 12.5 + 3.0 + 16.0 + 12.0 + 12.0
 # the V-cycle's share (its seven launches, the x update riding on the coarse-level launch included):
 # the dominant work of the multigrid step, reported as the headline roofline
-KERNEL_BYTES["mg_vcycle"] = 8.5 + 16.5 + 3.0 + 12.0
+KERNEL_BYTES["mg_vcycle"] = 4.5 + 12.5 + 3.0 + 12.0
 
 
 def step_bytes(solve_bytes: float, m_scalars: int) -> float:

@@ -29,9 +29,14 @@ struct MgShape {
   int side_groups = -1, side_sleeps = -1;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
   float over_deep = 1.6f;  // the same below level 1 (restrictions of levels 1 and 2)
-  // smoothing relaxation on level 0 and on levels 1-2 (over-relaxed red-black smoothing: 12 -> 10
-  // iterations at 8 M voxels; 10 held for 1.1-1.3 and 1.35-1.4)
-  float omega0 = 1.1f, omega_l = 1.4f;
+  // Level 0 is pre-smoothed by one pointwise (Jacobi) step from zero, z' = jacobi0 r / 6, and
+  // post-smoothed by one red-black Gauss-Seidel iteration (relaxation omega0); levels 1-2 by one
+  // red-black SOR iteration each way (omega_l). Chosen with the CPU oracle's multigrid-preconditioned
+  // CG on the smoke, control and fire scenes (up to 14 M voxels): against red-black pre-smoothing at
+  // 1.1 and levels 1-2 at 1.4 it takes the same or one fewer iteration everywhere (9 instead of 10
+  // at 8 M voxels), and the level-0 pre pass then writes only the level-1 right-hand side.
+  float jacobi0 = 1.8f;
+  float omega0 = 1.0f, omega_l = 1.5f;
 };
 const MgShape& mg_shape();
 
@@ -70,7 +75,6 @@ struct McgHook {
 struct Multigrid {
   int32_t L = 0;
   DBuf<float> b[kMgLevels], u[kMgLevels];  // levels 1..3 (leaf-dense); level 0 uses the caller's vectors
-  DBuf<float> zt;                          // level-0 scratch (colour-split): pre-smoothed z of the fused path
   DBuf<float> ut[kMgLevels];               // levels 1..2: pre-smoothed u of the fused path
   void ensure(int32_t leaves);
   // buffers and coarse layout for topology T (synchronises s when the layout is built)

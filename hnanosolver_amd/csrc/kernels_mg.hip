@@ -811,128 +811,73 @@ __device__ __forceinline__ int32_t face_nl(const FaceRow& F, int f) {
   return f == 0 ? F.xm : f == 1 ? F.xp : f == 2 ? F.ym : f == 3 ? F.yp : f == 4 ? F.zm : F.zp;
 }
 
-// Pre-smoothing and restriction of the V-cycle's level 0 in one pass: from z = 0, one red-black
-// Gauss-Seidel iteration on A z = b (colour 0 then colour 1, A = 6 I - neighbours, the arithmetic of
-// k_rbgs with dx2 = -1, omega = 1), the residual b - A z and its restriction to level 1 (times
-// `over`), z written to zout. The tile holds b (s_b) and z (s_z): z0 is a function of b alone, z1 of
-// the ring's z0, and the core's residual needs z of both colours on the depth-1 slabs, so the same
-// values as the separate sweeps come out bit for bit.
-// The global reads of one leaf's tile (absent leaves read the own leaf, then stage 0), issued a leaf
-// ahead in the persistent passes so their latency overlaps the previous leaf's LDS phases. The
-// staging positions themselves (table words) are the same for every leaf.
-struct Pre0In {
-  int32_t fnl[3], enl;
-  float fv[3], ev, b0, b1;
-};
-__device__ __forceinline__ void mg0_pre_load(const DevTopo& T, int32_t l, const float* __restrict__ b,
-                                             const uint32_t (&fw)[3], uint32_t ew, Pre0In& P) {
-  const FaceRow Fr = face_row(T, l);
-  const int t = threadIdx.x;
-  const uint32_t uh = (uint32_t)(T.L * kHalfLeaf), ul = (uint32_t)l;
-  const int c0 = ((t >> 5) + (t >> 2)) & 1;
-  P.enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    P.fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
-    P.fv[k] = ld32(b, tw_cs(fw[k], P.fnl[k] < 0 ? ul : (uint32_t)P.fnl[k], uh));
-  }
-  P.b0 = ld32(b, (uint32_t)c0 * uh + ul * kHalfLeaf + t);
-  P.b1 = ld32(b, (uint32_t)(c0 ^ 1) * uh + ul * kHalfLeaf + t);
-  P.ev = ld32(b, tw_cs(ew, P.enl < 0 ? ul : (uint32_t)P.enl, uh));
-}
-
-// Pre-smoothing and restriction of the V-cycle's level 0 in one pass per leaf: from z = 0, one
-// red-black Gauss-Seidel iteration on A z = b (colour 0 then colour 1, A = 6 I - neighbours, the
-// arithmetic of k_rbgs with dx2 = -1, omega = 1), the residual b - A z and its restriction to level
-// 1 (times `over`), z written to zout. The tile holds b (s_b) and z (s_z): z0 is a function of b
-// alone, z1 of the ring's z0, and the core's residual needs z of both colours on the depth-1 slabs,
-// so the same values as the separate sweeps come out bit for bit. Persistent: each workgroup walks
-// a strided leaf run.
-__global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b,
-                                                  float* __restrict__ zout, float over, float* __restrict__ b1,
-                                                  float w, const int* __restrict__ stop) {
-  __shared__ float s_b[kT12 * kT12 * kT12];
-  __shared__ float s_z[kT12 * kT12 * kT12];
-  __shared__ float s_r[kLeafVox];
+// Pre-smoothing and restriction of the V-cycle's level 0 in one pass per leaf: the pointwise
+// (Jacobi) pre-smoothing step from z = 0, z' = c r with c = jacobi weight * 0.166666667f, the
+// residual r - A z' (A = 6 I - neighbours, in the order +x, -x, +y, -y, +z, -z) and its
+// restriction to level 1 (times `over`). z' is a function of r alone, so only the level-1
+// right-hand side is written: the post pass forms z' again from r. Each leaf's z' and its six face
+// halos are staged in LDS (pair_tile_halo, double-buffered with the residual tile: two barriers
+// per leaf), persistent over a strided leaf run. The arithmetic of a pointwise scaling followed by
+// k_mg0_restrict, so the unfused V-cycle gives the same bits.
+__global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b, float c, float over,
+                                                  float* __restrict__ b1, const int* __restrict__ stop) {
+  __shared__ float s_t[2][kPairTile];
+  __shared__ float s_r[2][kLeafVox];
   if (stop && *stop) return;
   const LeafRun R = leaf_run(T.L);
-  if (R.first >= R.end) return;
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
-  const float dx2 = -1.0f;
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
-  const uint32_t ew = kTab12.edge[t < 96 ? t : 0];
-  uint32_t fw[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) fw[k] = kTab12.face[t + k * kTpb];
-  auto puti = [&](int i, int col, float bv) {
-    s_b[i] = bv;
-    s_z[i] = col == 0 ? sor_update(0.f, bv, 0.f, dx2, w) : 0.f;
-  };
-  Pre0In cur;
-  mg0_pre_load(T, R.first, b, fw, ew, cur);
-  for (int32_t l = R.first; l < R.end; l += R.step) {
-    Pre0In nxt = cur;
-    if (l + R.step < R.end) mg0_pre_load(T, l + R.step, b, fw, ew, nxt);
+  auto get = [&](int col, int32_t i) { return c * b[col * half + i]; };
+  int buf = 0;
+  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
+    float* St = s_t[buf];
+    float* Sr = s_r[buf];
     const FaceRow Fr = face_row(T, l);
-    {
-      const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
-      puti(i0, c0, cur.b0);
-      puti(i0 + 1, c0 ^ 1, cur.b1);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k) puti(tw_tile(fw[k]), (int)((fw[k] >> 19) & 1u), cur.fnl[k] < 0 ? 0.f : cur.fv[k]);
-    if (t < 96) puti(tw_tile(ew), (int)((ew >> 19) & 1u), cur.enl < 0 ? 0.f : cur.ev);
+    const float r0 = b[c0 * half + l * kHalfLeaf + t], r1 = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
+    pair_tile_halo(Fr, get, St);
+    const float u0 = c * r0, u1 = c * r1;
+    St[c0 * kHalfLeaf + t] = u0;
+    St[(c0 ^ 1) * kHalfLeaf + t] = u1;
     __syncthreads();
-    // colour 1 on the core and the depth-1 slabs (positions in absent leaves stay 0)
-    {
-      const int zc1 = 2 * zz + (c0 ^ 1);  // the lane's colour-1 voxel of the core
-      const int i = t12(x + 2, y + 2, zc1 + 2);
-      s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], 0.f, dx2, w);
-    }
-    if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
-      const int i = (int)kTab12.ring[t];
-      s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], 0.f, dx2, w);
-    }
-    __syncthreads();
-    // residual of the core's voxel pair, restriction, and z of the core
-    {
-      const int i0 = t12(x + 2, y + 2, 2 * zz + 2), i1 = i0 + 1;
-      const float u0 = s_z[i0], u1 = s_z[i1];
-      const float sp0 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 2), sp1 = t12_sum(s_z, x + 2, y + 2, 2 * zz + 3);
-      s_r[(x << 6) | (y << 3) | (2 * zz)] = s_b[i0] - (6.0f * u0 - sp0);
-      s_r[(x << 6) | (y << 3) | (2 * zz + 1)] = s_b[i1] - (6.0f * u1 - sp1);
-      zout[c0 * half + l * kHalfLeaf + t] = u0;
-      zout[(c0 ^ 1) * half + l * kHalfLeaf + t] = u1;
-    }
+    float s0, s1;
+    pair_tile_sums(St, t, u0, u1, s0, s1);
+    Sr[(x << 6) | (y << 3) | (2 * zz)] = r0 - (6.0f * u0 - s0);
+    Sr[(x << 6) | (y << 3) | (2 * zz + 1)] = r1 - (6.0f * u1 - s1);
     __syncthreads();
     if (t < 64) {
       const int X = t >> 4, Y = (t >> 2) & 3, Z = t & 3;
       float sum = 0.f;
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        sum += s_r[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
+        sum += Sr[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
       b1[l * 64 + t] = over * sum;
     }
-    __syncthreads();  // the tile and s_r are rewritten for the next leaf
-    cur = nxt;
   }
 }
 
-// Prolongation and post-smoothing of the V-cycle's level 0 for leaf l: zin + (parent value in u1)
-// staged on the tile, then one red-black iteration on A z = b in the reverse order (colour 1 on the
-// core and the depth-1 slabs, then colour 0 on the core), the core written to zout (distinct from
-// zin: other workgroups read zin's halo). Bitwise the result of k_mg0_prolong and two k_rbgs sweeps.
+// z = c r on a colour-split level-0 vector (the unfused V-cycle's pointwise pre-smoothing)
+__global__ __launch_bounds__(kTpb) void k_mg0_jacobi(int64_t n, const float* __restrict__ r, float c,
+                                                     float* __restrict__ z, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int64_t i = (int64_t)blockIdx.x * kTpb + threadIdx.x;
+  if (i < n) z[i] = c * r[i];
+}
+
+// Prolongation and post-smoothing of the V-cycle's level 0 for leaf l: the pre-smoothed z' = c b
+// plus the parent value in u1 staged on the tile, then one red-black iteration on A z = b in the
+// reverse order (colour 1 on the core and the depth-1 slabs, then colour 0 on the core), the core
+// written to zout. Bitwise the result of k_mg0_jacobi, k_mg0_prolong and two k_rbgs sweeps.
 // mg0_post_load issues the leaf's global reads; mg0_post_compute stages and updates the tile and
 // returns the lane's final colour-0 / colour-1 values (v0, v1) and their right-hand sides (r0, r1).
 struct Post0In {
   int32_t fnl[3], enl;
-  float fz[3], fu[3], fb[3], ez, eu, z0, z1, up, b0, b1;
+  float fu[3], fb[3], eb, eu, up, b0, b1;
 };
 __device__ __forceinline__ void mg0_post_load(const DevTopo& T, int32_t l, const float* __restrict__ b,
-                                              const float* __restrict__ zin, const float* __restrict__ u1,
-                                              const uint32_t (&fw)[3], uint32_t ew, Post0In& P) {
+                                              const float* __restrict__ u1, const uint32_t (&fw)[3], uint32_t ew,
+                                              Post0In& P) {
   const FaceRow Fr = face_row(T, l);
   const int t = threadIdx.x;
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
@@ -943,25 +888,20 @@ __device__ __forceinline__ void mg0_post_load(const DevTopo& T, int32_t l, const
   for (int k = 0; k < 3; ++k) {
     P.fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
     const uint32_t src = P.fnl[k] < 0 ? ul : (uint32_t)P.fnl[k];
-    const uint32_t i = tw_cs(fw[k], src, uh);
-    P.fz[k] = ld32(zin, i);
     P.fu[k] = ld32(u1, tw_parent(fw[k], src));
-    P.fb[k] = ld32(b, i);  // used at depth 1
+    P.fb[k] = ld32(b, tw_cs(fw[k], src, uh));
   }
-  const uint32_t i0g = (uint32_t)c0 * uh + ul * kHalfLeaf + t, i1g = (uint32_t)(c0 ^ 1) * uh + ul * kHalfLeaf + t;
-  P.z0 = ld32(zin, i0g);
-  P.z1 = ld32(zin, i1g);
   P.up = ld32(u1, ul * 64u + (uint32_t)(((x >> 1) << 4) + ((y >> 1) << 2) + zz));  // the pair's parent
-  P.b0 = ld32(b, i0g);
-  P.b1 = ld32(b, i1g);
+  P.b0 = ld32(b, (uint32_t)c0 * uh + ul * kHalfLeaf + t);
+  P.b1 = ld32(b, (uint32_t)(c0 ^ 1) * uh + ul * kHalfLeaf + t);
   const uint32_t esrc = P.enl < 0 ? ul : (uint32_t)P.enl;
-  P.ez = ld32(zin, tw_cs(ew, esrc, uh));
+  P.eb = ld32(b, tw_cs(ew, esrc, uh));
   P.eu = ld32(u1, tw_parent(ew, esrc));
 }
 __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, const Post0In& P,
                                                  const uint32_t (&fw)[3], uint32_t ew, float* __restrict__ zout,
-                                                 float* s_b, float* s_z, float w, float& v0, float& v1, float& r0,
-                                                 float& r1) {
+                                                 float* s_b, float* s_z, float c, float w, float& v0, float& v1,
+                                                 float& r0, float& r1) {
   const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
@@ -970,18 +910,18 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
   const int c0 = (x + y) & 1;
   {
     const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
-    s_z[i0] = P.z0 + P.up;
-    s_z[i0 + 1] = P.z1 + P.up;
+    s_z[i0] = c * P.b0 + P.up;
+    s_z[i0 + 1] = c * P.b1 + P.up;
     s_b[i0] = P.b0;
     s_b[i0 + 1] = P.b1;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int i = tw_tile(fw[k]);
-    s_z[i] = P.fnl[k] < 0 ? 0.f : P.fz[k] + P.fu[k];
+    s_z[i] = P.fnl[k] < 0 ? 0.f : c * P.fb[k] + P.fu[k];
     s_b[i] = P.fnl[k] < 0 ? 0.f : P.fb[k];  // read at depth 1 only
   }
-  if (t < 96) s_z[tw_tile(ew)] = P.enl < 0 ? 0.f : P.ez + P.eu;
+  if (t < 96) s_z[tw_tile(ew)] = P.enl < 0 ? 0.f : c * P.eb + P.eu;
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs
   {
@@ -1013,8 +953,8 @@ __device__ __forceinline__ void lane_tables(uint32_t (&fw)[3], uint32_t& ew) {
 
 // One workgroup per leaf (the V-cycle on its own).
 __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
-                                                   const float* __restrict__ zin, const float* __restrict__ u1,
-                                                   float* __restrict__ zout, float w, const int* __restrict__ stop) {
+                                                   const float* __restrict__ u1, float* __restrict__ zout, float c,
+                                                   float w, const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
   __shared__ float s_z[kT12 * kT12 * kT12];
   if (stop && *stop) return;
@@ -1023,18 +963,17 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
   uint32_t fw[3], ew;
   lane_tables(fw, ew);
   Post0In P;
-  mg0_post_load(T, l, b, zin, u1, fw, ew, P);
+  mg0_post_load(T, l, b, u1, fw, ew, P);
   float v0, v1, r0, r1;
-  mg0_post_compute(T, l, P, fw, ew, zout, s_b, s_z, w, v0, v1, r0, r1);
+  mg0_post_compute(T, l, P, fw, ew, zout, s_b, s_z, c, w, v0, v1, r0, r1);
 }
 
 // Inside flexible CG (truncation 1): the same pass over a strided leaf run per workgroup
 // (cg_groups(L) of them), also leaving the partials of z . r (b = r) and, from iteration 1, of
 // z . q_prev for the direction and step length.
 __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* __restrict__ b,
-                                                       const float* __restrict__ zin, const float* __restrict__ u1,
-                                                       float* __restrict__ zout, McgHook H, float w,
-                                                       const int* __restrict__ stop) {
+                                                       const float* __restrict__ u1, float* __restrict__ zout,
+                                                       McgHook H, float c, float w, const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
   __shared__ float s_z[kT12 * kT12 * kT12];
   __shared__ double s_red[kTpb / 64];
@@ -1048,9 +987,9 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
   lane_tables(fw, ew);
   for (int32_t l = R.first; l < R.end; l += R.step) {
     Post0In cur;  // (loading a leaf ahead here costs more in occupancy than it hides)
-    mg0_post_load(T, l, b, zin, u1, fw, ew, cur);
+    mg0_post_load(T, l, b, u1, fw, ew, cur);
     float v0, v1, r0, r1;
-    mg0_post_compute(T, l, cur, fw, ew, zout, s_b, s_z, w, v0, v1, r0, r1);
+    mg0_post_compute(T, l, cur, fw, ew, zout, s_b, s_z, c, w, v0, v1, r0, r1);
     zr += (double)v0 * (double)r0 + (double)v1 * (double)r1;
     if (H.zq && it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
                               (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
@@ -1408,7 +1347,6 @@ void Multigrid::ensure(int32_t leaves) {
   }
   L = leaves;
   ++generation;
-  zt.alloc((size_t)leaves * kLeafVox);
   for (int l = 1; l < kMgLevels; ++l) {
     const size_t n = (size_t)leaves << (3 * (3 - l));
     b[l].alloc(n);
@@ -1653,16 +1591,17 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
   if (hook && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the flexible-CG pass needs the fused level-0 V-cycle");
   // level 3 in the one-workgroup solve's slot order when that solve runs
   const int32_t* map3 = !(variant & kMgCoarseLaunches) && coarse_fused() ? cmap_.p : nullptr;
+  // level 0's pointwise pre-smoothing step z' = c r from z = 0 (c: the Jacobi weight over the
+  // diagonal, as the red-black sweep's 0.166666667f)
+  const float c0 = M.jacobi0 * 0.166666667f;
   if (fused) {
-    // level 0 pre-smoothing and restriction in one pass, into the scratch level-0 vector
-    k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, M.over, b[1].p, M.omega0, stop);
+    // pre-smoothing and restriction in one pass (z' is formed again by the post pass)
+    k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, c0, M.over, b[1].p, stop);
     HNS_HIP(hipGetLastError());
   } else {
-    // level 0: z = 0, red-black pre-smoothing (Gauss-Seidel on A z = r)
-    for (int k = 0; k < M.smooth0; ++k) {
-      launch_rbgs_rhs(T, r_cs, z_cs, 0, k == 0, M.omega0, stop, s);
-      launch_rbgs_rhs(T, r_cs, z_cs, 1, k == 0, M.omega0, stop, s);
-    }
+    const int64_t n = (int64_t)T.L * kLeafVox;
+    k_mg0_jacobi<<<(unsigned)((n + kTpb - 1) / kTpb), kTpb, 0, s>>>(n, r_cs, c0, z_cs, stop);
+    HNS_HIP(hipGetLastError());
     k_mg0_restrict<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, z_cs, M.over, b[1].p, stop);
     HNS_HIP(hipGetLastError());
   }
@@ -1714,8 +1653,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     }
   }
   if (fused) {
-    if (hook) k_mg0_post_fcg<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, *hook, M.omega0, stop);
-    else k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, zt.p, u[1].p, z_cs, M.omega0, stop);
+    if (hook) k_mg0_post_fcg<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, u[1].p, z_cs, *hook, c0, M.omega0, stop);
+    else k_mg0_post<<<grid_for_leaves(T.L), kTpb, 0, s>>>(T, r_cs, u[1].p, z_cs, c0, M.omega0, stop);
     HNS_HIP(hipGetLastError());
     return;
   }

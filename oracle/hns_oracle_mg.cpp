@@ -16,11 +16,13 @@
 // the Galerkin operator of level 3 under piecewise-constant prolongation: diagonal D_B = the sum over
 // the block's leaves of 6 minus their face neighbours inside the block, and -n_BB' (the leaf face
 // pairs between the blocks) towards each face-adjacent block. Red-black colours are the parity of
-// the level coordinate. The shape: level-0 relaxation 1.1, levels 1-2 1.4, every restricted residual
-// scaled by 1.6, one pre- and one post-smoothing iteration per level (post in the reverse colour
-// order); level 3 by one Gauss-Seidel iteration, the level-4 correction (8 SOR iterations at 1.7 on
-// the residual restricted by 1.6) and two iterations at 1.4 in the reverse order; with more than
-// 4096 blocks of a colour, 12 SOR iterations at 1.85 on level 3 alone.
+// the level coordinate. The shape: level 0 pre-smoothed by one pointwise step from zero,
+// z' = 1.8 r * 0.166666667f, and post-smoothed by one red-black Gauss-Seidel iteration (relaxation
+// 1.0, colour 1 first); levels 1-2 by one red-black SOR iteration each way at 1.5 (post in the
+// reverse colour order); every restricted residual scaled by 1.6; level 3 by one Gauss-Seidel
+// iteration, the level-4 correction (8 SOR iterations at 1.7 on the residual restricted by 1.6) and
+// two iterations at 1.4 in the reverse order; with more than 4096 blocks of a colour, 12 SOR
+// iterations at 1.85 on level 3 alone.
 //
 // Parity pins (tests/test_mg_oracle.py): every level operator against the 7-point operator built by
 // coordinate lookup, the Galerkin identities R A_l P = 4 A_{l+1} and the level-4 weights against
@@ -40,7 +42,7 @@
 namespace oracle_mg {
 
 struct Shape {
-  float omega0 = 1.1f, omega_l = 1.4f, over = 1.6f, over_deep = 1.6f;
+  float jacobi0 = 1.8f, omega0 = 1.0f, omega_l = 1.5f, over = 1.6f, over_deep = 1.6f;
   int c3_pre = 1, c3_post = 2, c4_iters = 8;
   float c3_omega = 1.0f, c3_omega_post = 1.4f, c4_omega = 1.7f, c4_over = 1.6f;
   int coarse = 12;  // level 3 alone (too many blocks for level 4)
@@ -230,8 +232,9 @@ void coarse(MG& g) {
 // z = V(r) on leaf-dense level-0 vectors
 void vcycle(MG& g, const float* r, float* z) {
   const Shape& M = g.M;
-  sweep0(g, r, z, 0, true, M.omega0);
-  sweep0(g, r, z, 1, true, M.omega0);
+  const float c = M.jacobi0 * 0.166666667f;  // pointwise pre-smoothing from zero: z' = c r
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < g.L * 512; ++i) z[i] = c * r[i];
   restrict_l(g, 8, r, z, 1.0f, M.over, g.b[1].data());
   for (int l = 1; l <= 2; ++l) {
     const int S = 8 >> l;
