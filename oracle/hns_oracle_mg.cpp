@@ -17,12 +17,12 @@
 // the block's leaves of 6 minus their face neighbours inside the block, and -n_BB' (the leaf face
 // pairs between the blocks) towards each face-adjacent block. Red-black colours are the parity of
 // the level coordinate. The shape: level 0 pre-smoothed by one pointwise step from zero,
-// z' = 1.8 r * 0.166666667f, and post-smoothed by one red-black Gauss-Seidel iteration (relaxation
+// z' = 1.75 r * 0.166666667f, and post-smoothed by one red-black Gauss-Seidel iteration (relaxation
 // 1.0, colour 1 first); levels 1-2 by one red-black SOR iteration each way at 1.5 (post in the
-// reverse colour order); every restricted residual scaled by 1.6; level 3 by one Gauss-Seidel
-// iteration, the level-4 correction (8 SOR iterations at 1.7 on the residual restricted by 1.6) and
-// two iterations at 1.4 in the reverse order; with more than 4096 blocks of a colour, 12 SOR
-// iterations at 1.85 on level 3 alone.
+// reverse colour order); every restricted residual scaled by 1.6; level 3 by one SOR iteration at
+// 1.1, the level-4 correction (8 SOR iterations at 1.7 on the residual restricted by 1.6) and two
+// iterations at 1.8 in the reverse order; with more than 4096 blocks of a colour, 12 SOR iterations
+// at 1.85 on level 3 alone.
 //
 // Parity pins (tests/test_mg_oracle.py): every level operator against the 7-point operator built by
 // coordinate lookup, the Galerkin identities R A_l P = 4 A_{l+1} and the level-4 weights against
@@ -42,9 +42,9 @@
 namespace oracle_mg {
 
 struct Shape {
-  float jacobi0 = 1.8f, omega0 = 1.0f, omega_l = 1.5f, over = 1.6f, over_deep = 1.6f;
+  float jacobi0 = 1.75f, omega0 = 1.0f, omega_l = 1.5f, over = 1.6f, over_deep = 1.6f;
   int c3_pre = 1, c3_post = 2, c4_iters = 8;
-  float c3_omega = 1.0f, c3_omega_post = 1.4f, c4_omega = 1.7f, c4_over = 1.6f;
+  float c3_omega = 1.1f, c3_omega_post = 1.8f, c4_omega = 1.7f, c4_over = 1.6f;
   int coarse = 12;  // level 3 alone (too many blocks for level 4)
   float coarse_omega = 1.85f;
   int max_blocks_per_colour = 4096;
