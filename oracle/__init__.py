@@ -79,6 +79,9 @@ def load():
             f("advance").restype = C.c_int
             f("advance").argtypes = [vp, C.c_float, C.c_int, C.c_float, C.POINTER(OracleParams), vp, C.c_int, vp, vp,
                                      vp, C.c_int, vp, vp, vp, vp]
+        _lib.oracle_advance_mgcg_f32.restype = C.c_int
+        _lib.oracle_advance_mgcg_f32.argtypes = [vp, C.c_float, C.c_float, C.c_float, C.POINTER(OracleParams), vp,
+                                                 C.c_int, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, C.c_int]
         _lib.oracle_mg_create.restype = vp
         _lib.oracle_mg_create.argtypes = [vp, C.c_int64]
         _lib.oracle_mg_destroy.argtypes = [vp]
@@ -292,10 +295,13 @@ class Oracle:
         return v
 
     def advance(self, voxel_size, iterations, dt, params: Dict[str, float], velocity, fields: Dict[str, np.ndarray],
-                sources: Optional[Dict[str, np.ndarray]] = None, src_vel=None, sdf=None):
+                sources: Optional[Dict[str, np.ndarray]] = None, src_vel=None, sdf=None, mgcg_tol: Optional[float] = None,
+                solve_iterations: Optional[list] = None):
         """HNS_Advance on host copies. `fields` maps state names to arrays (the context's state at call
         start); the combustion fields it lacks are created as the reference's needOrMake does.
-        Returns (velocity, fields) after the call."""
+        Returns (velocity, fields) after the call. With `mgcg_tol` (FP32 only) every pressure solve is
+        the multigrid-preconditioned CG to that relative residual instead of `iterations` RBGS
+        iterations; their iteration counts are appended to `solve_iterations`."""
         P = OracleParams()
         for k, _ in OracleParams._fields_:
             v = params[k]
@@ -319,9 +325,20 @@ class Oracle:
         csa = (C.c_void_p * max(1, len(sarrs)))(*[a.ctypes.data for a in sarrs])
         sv = self._a(src_vel, (self.n, 3)) if src_vel is not None else None
         sd = self._a(sdf) if sdf is not None else None
-        st = self._f("advance")(self._t, voxel_size, int(iterations), dt, C.byref(P), vel.ctypes.data, len(names),
-                                C.addressof(cn), C.addressof(cf), cp.ctypes.data, len(snames), C.addressof(csn),
-                                C.addressof(csa), _p(sv), _p(sd))
+        if mgcg_tol is None:
+            st = self._f("advance")(self._t, voxel_size, int(iterations), dt, C.byref(P), vel.ctypes.data, len(names),
+                                    C.addressof(cn), C.addressof(cf), cp.ctypes.data, len(snames), C.addressof(csn),
+                                    C.addressof(csa), _p(sv), _p(sd))
+        else:
+            if self.dtype != np.float32:
+                raise ValueError("the multigrid-preconditioned step is FP32")
+            its = np.zeros(64, np.int32)
+            st = load().oracle_advance_mgcg_f32(self._t, voxel_size, float(mgcg_tol), dt, C.byref(P), vel.ctypes.data,
+                                                len(names), C.addressof(cn), C.addressof(cf), cp.ctypes.data,
+                                                len(snames), C.addressof(csn), C.addressof(csa), _p(sv), _p(sd),
+                                                its.ctypes.data, len(its))
+            if solve_iterations is not None:
+                solve_iterations.extend(int(v) for v in its[its > 0])
         if st != 0:
             raise RuntimeError("oracle advance failed: %d" % st)
         return vel, dict(zip(names, arrs))

@@ -26,12 +26,20 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <unordered_map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+namespace oracle_mg {
+struct MG;
+MG* create(const int32_t* origins, int64_t L);
+void destroy(MG* g);
+int mgcg(MG& g, const float* div, float dx, double tol, int max_iters, float* p, double* rel);
+}  // namespace oracle_mg
 
 namespace oracle {
 
@@ -968,7 +976,8 @@ static_assert(sizeof(Params) == 72, "CombustionParams layout");
 template <typename R>
 int advance(const Topo& t, float voxelSize, int iterations, float dt, const Params& P, R* velFlat, int nState,
             const char* const* names, R* const* fields, const int32_t* present, int nSrc, const char* const* srcNames,
-            const R* const* srcs, const R* srcVelV3, const R* sdf) {
+            const R* const* srcs, const R* srcVelV3, const R* sdf,
+            const std::function<void(const R* div, R* p)>* solve = nullptr) {
   const int64_t N = t.N;
   if (N == 0) return -2;
   V3<R>* vel = reinterpret_cast<V3<R>*>(velFlat);
@@ -1067,9 +1076,10 @@ int advance(const Topo& t, float voxelSize, int iterations, float dt, const Para
     combustionOxygen(N, fields[idxs[0]], fields[idxs[1]], fields[idxs[2]], divg.data(), fields[idxs[3]],
                      R(P.temperatureRelease), R(P.expansionRate));
     buoyancy(N, adv.data(), fields[idxs[2]], dt_sub, R(P.ambientTemp), R(P.buoyancyStrength));
-    // 4) pressure, p = 0 then K iterations of colour 0 then colour 1
+    // 4) pressure, p = 0 then K iterations of colour 0 then colour 1 (or the given solve)
     std::fill(pres.begin(), pres.end(), R(0));
-    for (int it = 0; it < iterations; ++it) {
+    if (solve) (*solve)(divg.data(), pres.data());
+    for (int it = 0; !solve && it < iterations; ++it) {
       rbgsColour(t, divg.data(), pres.data(), dx, 0, omega);
       rbgsColour(t, divg.data(), pres.data(), dx, 1, omega);
     }
@@ -1077,8 +1087,9 @@ int advance(const Topo& t, float voxelSize, int iterations, float dt, const Para
     if (P.disturbanceEnable > 0.0f) {
       divergence(t, adv.data(), divg.data(), inv_dx);
       std::fill(pres.begin(), pres.end(), R(0));
+      if (solve) (*solve)(divg.data(), pres.data());
       const int it2 = std::max(2, iterations / 2);
-      for (int it = 0; it < it2; ++it) {
+      for (int it = 0; !solve && it < it2; ++it) {
         rbgsColour(t, divg.data(), pres.data(), dx, 0, omega);
         rbgsColour(t, divg.data(), pres.data(), dx, 1, omega);
       }
@@ -1282,6 +1293,32 @@ void oracle_set_num_threads(int n) {
 
 ORACLE_KERNELS(f32, float)
 ORACLE_KERNELS(f64, double)
+
+// HNS_Advance with every pressure solve by the multigrid-preconditioned CG to tol (the library's
+// HNS_SOLVER_MGCG) instead of the fixed RBGS count; iterations[k] receives the k-th solve's count
+// (up to max_solves entries). FP32 only.
+int oracle_advance_mgcg_f32(void* t, float voxelSize, float tol, float dt, const oracle::Params* P, float* vel,
+                            int nState, const char* const* names, float* const* fields, const int32_t* present,
+                            int nSrc, const char* const* srcNames, const float* const* srcs, const float* srcVelV3,
+                            const float* sdf, int32_t* iterations, int max_solves) {
+  const Topo& topo = *static_cast<Topo*>(t);
+  const int64_t L = topo.N / 512;
+  std::vector<int32_t> org(3 * (size_t)L);
+  for (int64_t l = 0; l < L; ++l)
+    for (int a = 0; a < 3; ++a) org[3 * l + a] = topo.coords[3 * (l * 512) + a];
+  oracle_mg::MG* mg = oracle_mg::create(org.data(), L);
+  int k = 0;
+  const std::function<void(const float*, float*)> solve = [&](const float* div, float* p) {
+    double rel = 0.0;
+    const int it = oracle_mg::mgcg(*mg, div, voxelSize, (double)tol, 1000, p, &rel);
+    if (k < max_solves) iterations[k] = it;
+    ++k;
+  };
+  const int st = oracle::advance<float>(topo, voxelSize, 0, dt, *P, vel, nState, names, fields, present, nSrc,
+                                        srcNames, srcs, srcVelV3, sdf, &solve);
+  oracle_mg::destroy(mg);
+  return st;
+}
 ORACLE_ADJOINTS(f32, float)
 ORACLE_ADJOINTS(f64, double)
 

@@ -328,6 +328,8 @@ MG* create(const int32_t* origins, int64_t L) {
   return g;
 }
 
+void destroy(MG* g) { delete g; }
+
 // A d on level 0 (6 d - the face neighbours, zero outside): the library's product order
 void apply_A(const MG& g, const float* d, float* q) {
 #pragma omp parallel for schedule(static)

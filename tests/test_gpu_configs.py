@@ -134,6 +134,38 @@ def test_cfg2_fire256_8m_mgcg_step_is_deterministic(hip):
         np.testing.assert_array_equal(outs[0][k], outs[1][k])
 
 
+@pytest.mark.parametrize("name,calls", [("fire256_cone", 3), ("fire256_8m", 2)])
+def test_cfg2_mgcg_steps_match_the_oracle(hip, name, calls):
+    """The headline step: each call with the pressure solved by the multigrid-preconditioned CG to
+    1e-5, against the oracle's step with its own multigrid-preconditioned CG to 1e-5 from the same
+    state (oracle/hns_oracle_mg.cpp: the V-cycle bitwise the library's, the dot products summed in a
+    different order): every field within TOL_REL and the same iteration count."""
+    sc = _scene(name)
+    _case("cfg2 %s N=%d, %d MGCG steps against the oracle's MGCG steps" % (name, sc.n, calls))
+    data = _payload(sc.coords, sc.velocity, sc.fields)
+    topo = hns.create_index_grid(data, sc.voxel_size)
+    ctx = hns.ensure_context(None, data, topo, sc.voxel_size)
+    params = hns.make_params(**sc.params)
+    o32 = Oracle(sc.coords)
+    vel, f = sc.velocity, dict(sc.fields)
+    worst = 0.0
+    for c in range(calls):
+        st = hns.advance(ctx, 0, sc.dt, params, solver="mgcg", tol=1e-5)
+        out = _out_payload(sc.coords, list(dict.fromkeys(list(f) + ["fuel", "waste", "temperature", "flame"])))
+        hns.download_for_output(ctx, out)
+        gv = out["velocity"].copy()
+        gf = {k: out[k].copy() for k in out.blocks if k != "velocity" and ctx.has_field(k)}
+        its = []
+        ov, of = o32.advance(sc.voxel_size, 0, sc.dt, sc.params, vel, f, mgcg_tol=1e-5, solve_iterations=its)
+        print("  call %d: gpu %d iterations (rel %.3e), oracle %s" % (c, st.iterations, st.rel_residual, its),
+              flush=True)
+        assert its and st.iterations == its[-1]
+        worst = max(worst, _compare("%s#%d" % (name, c), gv, gf, ov, of))
+        vel, f = gv, gf
+    hns.destroy_context(ctx)
+    assert worst <= TOL_REL
+
+
 # ---- config 5 -------------------------------------------------------------------------------------
 
 def test_cfg5_fire512_step_rbgs150(hip):

@@ -172,3 +172,21 @@ def test_flexible_cg_on_a_scene():
         it, rel, np.sqrt(r2 / d2), it_cg))
     assert np.sqrt(r2 / d2) <= 1.01e-5
     assert 3 * it <= it_cg
+
+
+def test_mgcg_step_matches_a_converged_rbgs_step():
+    """The oracle's step with its pressure solves by the multigrid-preconditioned CG (to 1e-6)
+    against the same step with 2000 red-black Gauss-Seidel iterations (the reference's solver run to
+    convergence): every field within 2e-6 of the largest value."""
+    from _helpers import rel_err
+    for name in ("smoke64", "control64"):
+        sc = scenes.SCENES[name]()
+        o = oracle.Oracle(sc.coords)
+        its = []
+        v1, f1 = o.advance(sc.voxel_size, 0, sc.dt, sc.params, sc.velocity, sc.fields, mgcg_tol=1e-6,
+                           solve_iterations=its)
+        v2, f2 = o.advance(sc.voxel_size, 2000, sc.dt, sc.params, sc.velocity, sc.fields)
+        errs = {"velocity": rel_err(v1, v2), **{k: rel_err(f1[k], f2[k]) for k in f1}}
+        print(name, "mgcg iterations", its, {k: "%.1e" % v for k, v in errs.items()})
+        assert 0 < len(its) and max(its) <= 15
+        assert max(errs.values()) <= 2e-6
