@@ -115,6 +115,10 @@ class Solvers {
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
   CgState* cg_ = nullptr;
   struct Multigrid* mg_ = nullptr;
+  // iterations the last multigrid-preconditioned solve on these solvers took to its stop (0: none
+  // yet): the next solve reads its stop flag first right after that many (consecutive steps of a
+  // simulation take about the same count), so no batch runs launches past a predicted stop
+  int mg_last_iters_ = 0;
   void rbgs_iters(const DevTopo& T, const float* div, float* p, int first_it, int n, const SolveRequest& r, Timing& tm,
                   hipStream_t s);
   SolveResult solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,

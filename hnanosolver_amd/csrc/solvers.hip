@@ -302,7 +302,9 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   // (with per-kernel timing on, the flag is read after every iteration, so no launch past the stop
   // returning at once is timed: the timed launches are the ones that do the work, and the device
   // computes the same as with batches)
-  const int batch0 = tm.enabled ? 1 : (mg ? 2 * kFcgBatch : (pre ? kFcgBatch : 8));
+  // (the multigrid solve reads first after the previous solve's count when it has one)
+  const int mg_first = mg_last_iters_ > 0 ? std::max(mg_last_iters_, kFcgBatch) : 2 * kFcgBatch;
+  const int batch0 = tm.enabled ? 1 : (mg ? mg_first : (pre ? kFcgBatch : 8));
   const int batch_max = tm.enabled ? 1 : (pre ? kFcgBatch : 128);
   LoopResult lr;
   if (pre) {
@@ -320,6 +322,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   SolveResult res;
   res.iterations = lr.iterations;
   res.end = lr.end;
+  if (mg) mg_last_iters_ = lr.end == kLoopStopped ? lr.iterations : 0;
   if (pre && lr.end == kLoopStopped) {
     // the stop stands on r = b - A x evaluated in FP64 (its r.r and ||b||^2 came with the flag)
     const double rr = lr.rr, bb = lr.bb;
