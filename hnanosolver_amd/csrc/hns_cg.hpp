@@ -96,6 +96,30 @@ struct CgScalars {
 };
 // pa slots: r.r, and for the flexible CG d.r and d.q
 enum CgSlot { kSlotRR = 0, kSlotDR = 1, kSlotDQ = 2, kCgSlots = 4 };
+
+// Step length of flexible-CG iteration it, alpha = rz[it] / (d_it . q_it), d . q summed from the
+// G partials of pa row kSlotDQ; workgroup 0 records S.rho[it] = d . q and S.alpha[it]. A breakdown
+// (d . q not positive, which only round-off reaches, A being SPD) raises the stop flag at `it`
+// instead, and every workgroup gets false. s_red: kCgTpb / 64 doubles of LDS.
+__device__ inline bool fcg_step_length(const CgScalars& S, int32_t it, int G, double* s_red, float& alpha) {
+  double dq[1];
+  reduce_partial_rows(S.pa + kSlotDQ * kCgMaxGroups, G, s_red, dq);
+  const double sigma = dq[0];
+  if (!(sigma > 0.0)) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      S.flag[1] = it;
+      S.flag[0] = 1;
+    }
+    return false;
+  }
+  const double alpha_d = S.rz[it] / sigma;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S.rho[it] = sigma;
+    S.alpha[it] = alpha_d;
+  }
+  alpha = (float)alpha_d;
+  return true;
+}
 // the recurrence residual is replaced by b - A x every this many iterations
 constexpr int kResidualReplace = 64;
 

@@ -229,6 +229,46 @@ __device__ __forceinline__ void pair_tile_halo(const FaceRow& F, G get, T* S) {
     S[kPairHalo + e] = nl < 0 ? T(0) : get((vx + vy + vz) & 1, nl * kHalfLeaf + cs_local(vx, vy, vz));
   }
 }
+// Where lane t's pair-tile entries of leaf l come from, as colour-split vector indices, for kernels
+// that issue a leaf's loads ahead of the leaf before it. Halo entry t (faces -x, +x, -y, +y, one per
+// wave) and entry t + 256 (faces -z, +z; lanes t < 128) of pair_tile_halo's order come from the
+// face neighbours nb[0], nb[1] (pair_halo_nbrs). ix[0], ix[1] the lane's voxel pair (colour
+// c0 = ((t >> 5) + (t >> 2)) & 1, then the other), ix[2], ix[3] the halo entries; -1 where the
+// neighbour leaf is absent or the lane has no second halo entry.
+struct PairIdx {
+  int32_t ix[4];
+};
+// nb[0], nb[1] of leaf l for the wave of lane t (the face row's entry -x, +x, -y or +y by wave,
+// then -z or +z): scalar loads through the constant address space (the neighbour rows are not
+// written while a kernel reads them), which wait on the LDS counter, not on the vector-memory
+// counter that the prefetched data loads are tracked by
+__device__ __forceinline__ void pair_halo_nbrs(const DevTopo& T, int32_t l, int t, int32_t (&nb)[2]) {
+  typedef const __attribute__((address_space(4))) int32_t* ConstRow;
+  const int f = __builtin_amdgcn_readfirstlane(t >> 6);
+  const ConstRow r = (ConstRow)(T.nbr + kNbrStride * l);
+  nb[0] = r[f == 0 ? 4 : f == 1 ? 22 : f == 2 ? 10 : 16];
+  nb[1] = r[(f & 1) ? 14 : 12];
+}
+__device__ __forceinline__ PairIdx pair_tile_index(int32_t l, const int32_t (&nb)[2], int t, int32_t half) {
+  const int a = (t >> 3) & 7, b = t & 7, f = t >> 6, c0 = ((t >> 5) + (t >> 2)) & 1;
+  PairIdx P;
+  P.ix[0] = c0 * half + l * kHalfLeaf + t;
+  P.ix[1] = (c0 ^ 1) * half + l * kHalfLeaf + t;
+  const int vx = f == 0 ? 7 : (f == 1 ? 0 : a), vy = f < 2 ? a : (f == 2 ? 7 : 0);
+  P.ix[2] = nb[0] < 0 ? -1 : ((vx + vy + b) & 1) * half + nb[0] * kHalfLeaf + cs_local(vx, vy, b);
+  const int vz = (f & 1) ? 0 : 7;  // entry t + 256: face 4 (-z) for t < 64, 5 (+z) for t < 128
+  P.ix[3] = (t >= 128 || nb[1] < 0) ? -1 : ((a + b + vz) & 1) * half + nb[1] * kHalfLeaf + cs_local(a, b, vz);
+  return P;
+}
+// Stores lane t's four pair-tile values (as pair_tile_index orders them) into the tile.
+template <class T>
+__device__ __forceinline__ void pair_tile_put(T* S, int t, T v0, T v1, T h0, T h1) {
+  const int c0 = ((t >> 5) + (t >> 2)) & 1;
+  S[c0 * kHalfLeaf + t] = v0;
+  S[(c0 ^ 1) * kHalfLeaf + t] = v1;
+  S[kPairHalo + t] = h0;
+  if (t < 128) S[kPairHalo + 256 + t] = h1;
+}
 // Same sums as pair_face_sums (same order, same values) from a staged tile.
 template <class T>
 __device__ __forceinline__ void pair_tile_sums(const T* S, int32_t t, T v0, T v1, T& s0, T& s1) {

@@ -93,32 +93,33 @@ struct FcgReport {
 
 // Dev provides
 //   void replace_periodic(int it);     residual replacement before iteration it, and its stop test
-//   bool batchable(int it, int left);  the next `align` iterations can go as one replayed batch
-//   void batch(int it);                `align` iterations from it
+//   int batchable(int it, int left);   how many of the next `left` iterations from it go as one
+//                                      replayed batch (below 2: they are launched one by one)
+//   void batch(int it, int n);         n iterations from it as one batch
 //   void first_test();                 the stop test before iteration 0 (records ||b||^2)
 //   void iteration(int it);            one iteration (preconditioner, direction, product, update,
 //                                      stop test advancing the device counter)
 //   FcgReport end_batch();             re-test of a raised stop, then synchronise and read
-// Batches are aligned to multiples of `align` (a replayed batch starts at such a multiple).
+// Between two reads the loop runs `batch` iterations (batch0 first, then batch_max), cut at the
+// periodic replacement points, so a replayed batch never contains one.
 template <class Dev>
-LoopResult fcg_loop(Dev& d, int cap, int batch0, int batch_max, int align, int replace_every) {
+LoopResult fcg_loop(Dev& d, int cap, int batch0, int batch_max, int replace_every) {
   LoopResult out;
-  align = std::max(1, align);
   int it = 0, batch = std::max(1, batch0), replaced_at = 0;
   while (it < cap) {
     const int start = it;
-    int nb = std::min(batch, cap - it);
-    if (it % align != 0) nb = std::min(nb, align - it % align);
+    const int nb = std::min(batch, cap - it);
     for (int k = 0; k < nb;) {
       if (it == 0) d.first_test();
       if (it > 0 && it % replace_every == 0 && replaced_at != it) {
         d.replace_periodic(it);
         replaced_at = it;
       }
-      if (it % align == 0 && nb - k >= align && d.batchable(it, nb - k)) {
-        d.batch(it);
-        it += align;
-        k += align;
+      const int n = std::min(d.batchable(it, std::min(nb - k, replace_every - it % replace_every)), nb - k);
+      if (n >= 2) {
+        d.batch(it, n);
+        it += n;
+        k += n;
         continue;
       }
       d.iteration(it);

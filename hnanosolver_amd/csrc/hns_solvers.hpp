@@ -113,6 +113,8 @@ class Solvers {
   DBuf<double> partial_, red_;
   double* h_red_ = nullptr;
   hipEvent_t e0_ = nullptr, e1_ = nullptr;
+  bool e1_marked_ = false;  // e1_ was recorded before a stop-flag read this solve waited for
+  void mark_solve_end(const SolveRequest& r, hipStream_t s);
   CgState* cg_ = nullptr;
   struct Multigrid* mg_ = nullptr;
   // iterations the last multigrid-preconditioned solve on these solvers took to its stop (0: none
@@ -123,7 +125,7 @@ class Solvers {
                   hipStream_t s);
   SolveResult solve_cg(const DevTopo& T, const float* div, float* p, const SolveRequest& r, Timing& tm,
                        hipStream_t s);
-  void mg_batch(const DevTopo& T, float* p, const struct CgScalars& S, int depth, hipStream_t s);
+  void mg_batch(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int n, hipStream_t s);
   void mg_iteration(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int it, hipStream_t s,
                     Timing* tm = nullptr);
 
@@ -165,8 +167,8 @@ class Solvers {
     Timing& tm;
     hipStream_t s;
     void replace_periodic(int it);
-    bool batchable(int it, int left);
-    void batch(int it);
+    int batchable(int it, int left);
+    void batch(int it, int n);
     void first_test();
     void iteration(int it);
     FcgReport end_batch();

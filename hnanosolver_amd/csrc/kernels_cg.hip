@@ -353,7 +353,11 @@ __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int64_t n, const f
 // beta = -(z . q_prev) / sigma_prev from the partials the V-cycle's last pass left in zq row 0
 // (NP = 1; NP = 0 at iteration 0: d = z), d = z + beta d_prev formed for the leaf's voxel pairs and
 // its face halos (the same fmaf as k_fcg_dir), q = A d, d and q into their ring slots, partials of
-// d . q. The numerator of alpha, z . r, is also left by the V-cycle (pa row kSlotDR).
+// d . q. The numerator of alpha, z . r, is also left by the V-cycle (pa row kSlotDR). NP = 1 also
+// forms d = z when the device counter is at iteration 0, so one launch serves every iteration. A
+// lane's loads for the next leaf of the run are issued before the current leaf is staged (the first
+// leaf's before the partials reduce), so every workgroup keeps one leaf of loads in flight behind
+// the one it computes.
 template <int NP, bool PCG>
 __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, const float* __restrict__ z,
                                                        float* __restrict__ Dr, float* __restrict__ Qr,
@@ -362,17 +366,49 @@ __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, con
   __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
   const int32_t it = *S.it;
+  const bool chain = NP > 0 && it > 0;  // d = z + beta d_prev; otherwise d = z
+  const float* __restrict__ dp = chain ? Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n : z;
+  float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
+  float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const LeafRun R = leaf_run(T.L);
+  const int32_t half = T.L * kHalfLeaf;
+  const int t = threadIdx.x;
+  // lane t's pair-tile values of one leaf (pair_tile_index order): z, d_prev, and whether each
+  // halo entry exists (an absent one loads element 0 and stages 0)
+  struct Leaf {
+    float z[4], p[4];
+    bool h2, h3;
+  };
+  auto fetch = [&](int32_t l, const int32_t (&nb)[2], Leaf& v) {
+    const PairIdx P = pair_tile_index(l, nb, t, half);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t j = P.ix[k] < 0 ? 0 : P.ix[k];
+      v.z[k] = z[j];
+      if (NP > 0) v.p[k] = dp[j];
+    }
+    v.h2 = P.ix[2] >= 0;
+    v.h3 = P.ix[3] >= 0;
+  };
+  // the halo neighbours of the next leaf (n1) are loaded one leaf earlier still (n2); past the run's
+  // end both stand for leaves of the run, so every load stays in bounds and goes unused
+  Leaf cur, nxt;
+  int32_t n1[2], n2[2];
+  if (R.first < R.end) {
+    pair_halo_nbrs(T, R.first, t, n1);
+    fetch(R.first, n1, cur);
+    pair_halo_nbrs(T, R.first + R.step < R.end ? R.first + R.step : R.first, t, n1);
+  }
   // this iteration's V-cycle has applied the deferred x update of iteration it - 1
   if (S.xdone && blockIdx.x == 0 && threadIdx.x == 0) *S.xdone = it;
   float beta = 0.f;
-  const float* __restrict__ dp = z;
   if (PCG) {
     double rz1[1];
     reduce_partial_rows(S.pa + kSlotDR * kCgMaxGroups, (int)gridDim.x, s_red, rz1);
     if (blockIdx.x == 0 && threadIdx.x == 0) S.rz[it] = rz1[0];
-    if (NP > 0) beta = (float)(rz1[0] / S.rz[it - 1]);
+    if (chain) beta = (float)(rz1[0] / S.rz[it - 1]);
   } else {
-    if (NP > 0) {
+    if (chain) {
       double zq1[1];
       reduce_partial_rows(zq, (int)gridDim.x, s_red, zq1);
       beta = (float)(-zq1[0] / S.rho[it - 1]);
@@ -383,36 +419,37 @@ __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, con
       if (threadIdx.x == 0) S.rz[it] = zr1[0];
     }
   }
-  if (NP > 0) dp = Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n;
-  float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
-  float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
-  const LeafRun R = leaf_run(T.L);
-  const int32_t half = T.L * kHalfLeaf;
-  const int t = threadIdx.x;
-  const int c0 = ((t >> 5) + (t >> 2)) & 1;
-  auto get = [&](int c, int32_t i) {
-    const int32_t j = c * half + i;
-    return NP > 0 ? fmaf(beta, dp[j], z[j]) : z[j];
-  };
   double acc = 0.0;
-  int buf = 0;
-  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
-    float* St = s_tile[buf];
-    const FaceRow Fr = face_row(T, l);
-    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
-    const float d0 = get(c0, l * kHalfLeaf + t), d1 = get(c0 ^ 1, l * kHalfLeaf + t);
-    pair_tile_halo(Fr, get, St);
-    St[c0 * kHalfLeaf + t] = d0;
-    St[(c0 ^ 1) * kHalfLeaf + t] = d1;
+  // one leaf: the loads of the leaf after it (halo neighbours nb_next, loaded one leaf earlier) into
+  // nx and of the halo neighbours of the leaf after that into nb_after, then the leaf itself from c
+  auto leaf = [&](int32_t l, const Leaf& c, Leaf& nx, const int32_t (&nb_next)[2], int32_t (&nb_after)[2],
+                  float* St) {
+    pair_halo_nbrs(T, l + 2 * R.step < R.end ? l + 2 * R.step : l, t, nb_after);
+    fetch(l + R.step < R.end ? l + R.step : l, nb_next, nx);
+    float v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = chain ? fmaf(beta, c.p[k], c.z[k]) : c.z[k];
+    if (!c.h2) v[2] = 0.f;
+    if (!c.h3) v[3] = 0.f;
+    pair_tile_put(St, t, v[0], v[1], v[2], v[3]);
     __syncthreads();
     float s0, s1;
-    pair_tile_sums(St, t, d0, d1, s0, s1);
-    const float q0 = 6.0f * d0 - s0, q1 = 6.0f * d1 - s1;
-    d[i0] = d0;
-    d[i1] = d1;
+    pair_tile_sums(St, t, v[0], v[1], s0, s1);
+    const float q0 = 6.0f * v[0] - s0, q1 = 6.0f * v[1] - s1;
+    const int c0 = ((t >> 5) + (t >> 2)) & 1;
+    const int32_t i0 = c0 * half + l * kHalfLeaf + t, i1 = (c0 ^ 1) * half + l * kHalfLeaf + t;
+    d[i0] = v[0];
+    d[i1] = v[1];
     q[i0] = q0;
     q[i1] = q1;
-    acc += (double)d0 * (double)q0 + (double)d1 * (double)q1;
+    acc += (double)v[0] * (double)q0 + (double)v[1] * (double)q1;
+  };
+  // two leaves per trip, the register sets swapping roles, so no value is copied between them (a
+  // copy would wait for the loads in flight); the tile alternates as in the one-leaf loop
+  for (int32_t l = R.first; l < R.end; l += 2 * R.step) {
+    leaf(l, cur, nxt, n1, n2, s_tile[0]);
+    if (l + R.step >= R.end) break;
+    leaf(l + R.step, nxt, cur, n2, n1, s_tile[1]);
   }
   const double tt = block_sum_all(acc, s_red);
   if (threadIdx.x == 0) S.pa[kSlotDQ * kCgMaxGroups + blockIdx.x] = tt;
@@ -488,22 +525,9 @@ __global__ __launch_bounds__(kTpb) void k_fcg_rupdate(int32_t L, int64_t n, floa
   };
   float r0[4], q0[4];
   if (R.first < R.end) load(R.first, r0, q0);
-  double dq[1];  // d . q; the numerator z . r was summed once by the product kernel into S.rz[it]
-  reduce_partial_rows(S.pa + kSlotDQ * kCgMaxGroups, (int)gridDim.x, s_red, dq);
-  const double sigma = dq[0];
-  if (!(sigma > 0.0)) {  // breakdown, as in k_fcg_update
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      S.flag[1] = it;
-      S.flag[0] = 1;
-    }
-    return;
-  }
-  const double alpha_d = S.rz[it] / sigma;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    S.rho[it] = sigma;
-    S.alpha[it] = alpha_d;
-  }
-  const float alpha = (float)alpha_d;
+  // d . q reduced here; the numerator z . r was summed once by the product kernel into S.rz[it]
+  float alpha;
+  if (!fcg_step_length(S, it, (int)gridDim.x, s_red, alpha)) return;
   double acc = 0.0;
   for (int32_t l = R.first; l < R.end; l += 2 * R.step) {
     float r1[4], q1[4];

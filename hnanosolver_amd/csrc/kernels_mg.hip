@@ -816,9 +816,11 @@ __device__ __forceinline__ int32_t face_nl(const FaceRow& F, int f) {
 // residual r - A z' (A = 6 I - neighbours, in the order +x, -x, +y, -y, +z, -z) and its
 // restriction to level 1 (times `over`). z' is a function of r alone, so only the level-1
 // right-hand side is written: the post pass forms z' again from r. Each leaf's z' and its six face
-// halos are staged in LDS (pair_tile_halo, double-buffered with the residual tile: two barriers
-// per leaf), persistent over a strided leaf run. The arithmetic of a pointwise scaling followed by
-// k_mg0_restrict, so the unfused V-cycle gives the same bits.
+// halos are staged in LDS (the pair tile), then the residual tile: two barriers per leaf,
+// persistent over a strided leaf run. A lane's loads for the next leaf of the run are issued before
+// the current leaf is staged, from halo neighbours loaded one leaf earlier still (pair_halo_nbrs),
+// two leaves per trip with the register sets swapping roles. The arithmetic of a pointwise scaling
+// followed by k_mg0_restrict, so the unfused V-cycle gives the same bits.
 __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b, float c, float over,
                                                   float* __restrict__ b1, const int* __restrict__ stop) {
   __shared__ float s_t[2][kPairTile];
@@ -828,18 +830,26 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
-  const int c0 = (x + y) & 1;
-  auto get = [&](int col, int32_t i) { return c * b[col * half + i]; };
-  int buf = 0;
-  for (int32_t l = R.first; l < R.end; l += R.step, buf ^= 1) {
-    float* St = s_t[buf];
-    float* Sr = s_r[buf];
-    const FaceRow Fr = face_row(T, l);
-    const float r0 = b[c0 * half + l * kHalfLeaf + t], r1 = b[(c0 ^ 1) * half + l * kHalfLeaf + t];
-    pair_tile_halo(Fr, get, St);
+  // lane t's r values of a leaf's pair tile (pair_tile_index order) and whether each halo entry
+  // exists (an absent one loads element 0 and stages 0)
+  struct Leaf {
+    float r[4];
+    bool h2, h3;
+  };
+  auto fetch = [&](int32_t l, const int32_t (&nb)[2], Leaf& v) {
+    const PairIdx P = pair_tile_index(l, nb, t, half);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v.r[k] = b[P.ix[k] < 0 ? 0 : P.ix[k]];
+    v.h2 = P.ix[2] >= 0;
+    v.h3 = P.ix[3] >= 0;
+  };
+  auto leaf = [&](int32_t l, const Leaf& cl, Leaf& nx, const int32_t (&nb_next)[2], int32_t (&nb_after)[2],
+                  float* St, float* Sr) {
+    pair_halo_nbrs(T, l + 2 * R.step < R.end ? l + 2 * R.step : l, t, nb_after);
+    fetch(l + R.step < R.end ? l + R.step : l, nb_next, nx);
+    const float r0 = cl.r[0], r1 = cl.r[1];
     const float u0 = c * r0, u1 = c * r1;
-    St[c0 * kHalfLeaf + t] = u0;
-    St[(c0 ^ 1) * kHalfLeaf + t] = u1;
+    pair_tile_put(St, t, u0, u1, cl.h2 ? c * cl.r[2] : 0.f, cl.h3 ? c * cl.r[3] : 0.f);
     __syncthreads();
     float s0, s1;
     pair_tile_sums(St, t, u0, u1, s0, s1);
@@ -854,6 +864,17 @@ __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __rest
         sum += Sr[((2 * X + (k >> 2)) << 6) | ((2 * Y + ((k >> 1) & 1)) << 3) | (2 * Z + (k & 1))];
       b1[l * 64 + t] = over * sum;
     }
+  };
+  if (R.first >= R.end) return;
+  Leaf A, B;
+  int32_t n1[2], n2[2];
+  pair_halo_nbrs(T, R.first, t, n1);
+  fetch(R.first, n1, A);
+  pair_halo_nbrs(T, R.first + R.step < R.end ? R.first + R.step : R.first, t, n1);
+  for (int32_t l = R.first; l < R.end; l += 2 * R.step) {
+    leaf(l, A, B, n1, n2, s_t[0], s_r[0]);
+    if (l + R.step >= R.end) break;
+    leaf(l + R.step, B, A, n2, n1, s_t[1], s_r[1]);
   }
 }
 
@@ -869,24 +890,49 @@ __global__ __launch_bounds__(kTpb) void k_mg0_jacobi(int64_t n, const float* __r
 // plus the parent value in u1 staged on the tile, then one red-black iteration on A z = b in the
 // reverse order (colour 1 on the core and the depth-1 slabs, then colour 0 on the core), the core
 // written to zout. Bitwise the result of k_mg0_jacobi, k_mg0_prolong and two k_rbgs sweeps.
-// mg0_post_load issues the leaf's global reads; mg0_post_compute stages and updates the tile and
-// returns the lane's final colour-0 / colour-1 values (v0, v1) and their right-hand sides (r0, r1).
-struct Post0In {
-  int32_t fnl[3], enl;
-  float fu[3], fb[3], eb, eu, up, b0, b1;
+// mg0_post_nbrs loads the neighbour leaves the lane's staging reads of a leaf come from (a leaf ahead
+// of its data, so the data loads do not wait on them); mg0_post_load issues the leaf's global reads;
+// mg0_post_compute stages and updates the tile and returns the lane's final colour-0 / colour-1
+// values (v0, v1) and their right-hand sides (r0, r1).
+struct Post0Nbr {
+  int32_t fnl[3];  // the lane's face-slab entries: faces (t >> 7) + 2 k (-x, -y, -z below lane 128)
+  int32_t enl;     // its depth-1 edge entry
+  int32_t rnl;     // lanes t < 192: face t >> 5, whose depth-1 slab positions it updates
 };
+__device__ __forceinline__ void mg0_post_nbrs(const DevTopo& T, int32_t l, uint32_t ew, Post0Nbr& N) {
+  // the face entries are uniform per wave: scalar loads through the constant address space (the
+  // neighbour rows are not written while a V-cycle runs)
+  typedef const __attribute__((address_space(4))) int32_t* ConstRow;
+  const int t = threadIdx.x;
+  const int hi = __builtin_amdgcn_readfirstlane(t >> 7);
+  const ConstRow row = (ConstRow)(T.nbr + kNbrStride * l);
+  N.fnl[0] = row[hi ? 22 : 4];
+  N.fnl[1] = row[hi ? 16 : 10];
+  N.fnl[2] = row[hi ? 14 : 12];
+  const int32_t* r = T.nbr + kNbrStride * l;
+  N.enl = r[(int)(ew >> 26)];
+  // face f = t >> 5 (-x, +x, -y, +y, -z, +z) at row entry 13 -+ 9, 3, 1 (4, 22, 10, 16, 12, 14)
+  const int f = t >> 5, d = f < 2 ? 9 : (f < 4 ? 3 : 1);
+  N.rnl = r[f < 6 ? 13 + ((f & 1) ? d : -d) : 13];
+}
+struct Post0In {
+  int32_t fnl[3], enl, rnl;
+  float fu[3], fb[3], eb, eu, up, b0, b1;
+  float q0, q1;  // Q: the previous product at the lane's pair (z . q_prev)
+};
+template <bool Q>
 __device__ __forceinline__ void mg0_post_load(const DevTopo& T, int32_t l, const float* __restrict__ b,
                                               const float* __restrict__ u1, const uint32_t (&fw)[3], uint32_t ew,
-                                              Post0In& P) {
-  const FaceRow Fr = face_row(T, l);
+                                              const Post0Nbr& N, const float* __restrict__ qp, Post0In& P) {
   const int t = threadIdx.x;
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
   const uint32_t uh = (uint32_t)(T.L * kHalfLeaf), ul = (uint32_t)l;
-  P.enl = T.nbr[kNbrStride * l + (int)(ew >> 26)];
+  P.enl = N.enl;
+  P.rnl = N.rnl;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    P.fnl[k] = face_nl(Fr, __builtin_amdgcn_readfirstlane((t >> 7) + 2 * k));
+    P.fnl[k] = N.fnl[k];
     const uint32_t src = P.fnl[k] < 0 ? ul : (uint32_t)P.fnl[k];
     P.fu[k] = ld32(u1, tw_parent(fw[k], src));
     P.fb[k] = ld32(b, tw_cs(fw[k], src, uh));
@@ -897,12 +943,16 @@ __device__ __forceinline__ void mg0_post_load(const DevTopo& T, int32_t l, const
   const uint32_t esrc = P.enl < 0 ? ul : (uint32_t)P.enl;
   P.eb = ld32(b, tw_cs(ew, esrc, uh));
   P.eu = ld32(u1, tw_parent(ew, esrc));
+  if (Q) {
+    P.q0 = ld32(qp, ul * kHalfLeaf + t);
+    P.q1 = ld32(qp, uh + ul * kHalfLeaf + t);
+  }
 }
 __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, const Post0In& P,
-                                                 const uint32_t (&fw)[3], uint32_t ew, float* __restrict__ zout,
+                                                 const uint32_t (&fw)[3], uint32_t ew, uint32_t rw,
+                                                 float* __restrict__ zout,
                                                  float* s_b, float* s_z, float c, float w, float& v0, float& v1,
                                                  float& r0, float& r1) {
-  const FaceRow Fr = face_row(T, l);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const float dx2 = -1.0f;
@@ -921,7 +971,9 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
     s_z[i] = P.fnl[k] < 0 ? 0.f : c * P.fb[k] + P.fu[k];
     s_b[i] = P.fnl[k] < 0 ? 0.f : P.fb[k];  // read at depth 1 only
   }
-  if (t < 96) s_z[tw_tile(ew)] = P.enl < 0 ? 0.f : c * P.eb + P.eu;
+  // lanes past the 96 edge entries store into the pad slot past the tile (no branch around the
+  // edge loads, so they go out with the others)
+  s_z[t < 96 ? tw_tile(ew) : kT12 * kT12 * kT12] = P.enl < 0 ? 0.f : c * P.eb + P.eu;
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs
   {
@@ -929,8 +981,8 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
     const int i = t12(x + 2, y + 2, zc1 + 2);
     s_z[i] = sor_update(t12_sum(s_z, x + 2, y + 2, zc1 + 2), s_b[i], s_z[i], dx2, w);
   }
-  if (t < 192 && face_nl(Fr, t >> 5) >= 0) {
-    const int i = (int)kTab12.ring[t];
+  if (t < 192 && P.rnl >= 0) {
+    const int i = (int)rw;
     s_z[i] = sor_update(t12_sumi(s_z, i), s_b[i], s_z[i], dx2, w);
   }
   __syncthreads();
@@ -944,9 +996,10 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
   zout[l * kHalfLeaf + t] = v0;         // colour 0 plane: the lane's colour-0 voxel
   zout[half + l * kHalfLeaf + t] = v1;  // colour 1 plane
 }
-__device__ __forceinline__ void lane_tables(uint32_t (&fw)[3], uint32_t& ew) {
+__device__ __forceinline__ void lane_tables(uint32_t (&fw)[3], uint32_t& ew, uint32_t& rw) {
   const int t = threadIdx.x;
   ew = kTab12.edge[t < 96 ? t : 0];
+  rw = kTab12.ring[t < 192 ? t : 0];
 #pragma unroll
   for (int k = 0; k < 3; ++k) fw[k] = kTab12.face[t + k * kTpb];
 }
@@ -956,16 +1009,18 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
                                                    const float* __restrict__ u1, float* __restrict__ zout, float c,
                                                    float w, const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
-  __shared__ float s_z[kT12 * kT12 * kT12];
+  __shared__ float s_z[kT12 * kT12 * kT12 + 1];  // + the pad slot of lanes without an edge entry
   if (stop && *stop) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
-  uint32_t fw[3], ew;
-  lane_tables(fw, ew);
+  uint32_t fw[3], ew, rw;
+  lane_tables(fw, ew, rw);
+  Post0Nbr N;
+  mg0_post_nbrs(T, l, ew, N);
   Post0In P;
-  mg0_post_load(T, l, b, u1, fw, ew, P);
+  mg0_post_load<false>(T, l, b, u1, fw, ew, N, nullptr, P);
   float v0, v1, r0, r1;
-  mg0_post_compute(T, l, P, fw, ew, zout, s_b, s_z, c, w, v0, v1, r0, r1);
+  mg0_post_compute(T, l, P, fw, ew, rw, zout, s_b, s_z, c, w, v0, v1, r0, r1);
 }
 
 // Inside flexible CG (truncation 1): the same pass over a strided leaf run per workgroup
@@ -975,7 +1030,7 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
                                                        const float* __restrict__ u1, float* __restrict__ zout,
                                                        McgHook H, float c, float w, const int* __restrict__ stop) {
   __shared__ float s_b[kT12 * kT12 * kT12];
-  __shared__ float s_z[kT12 * kT12 * kT12];
+  __shared__ float s_z[kT12 * kT12 * kT12 + 1];  // + the pad slot of lanes without an edge entry
   __shared__ double s_red[kTpb / 64];
   if (stop && *stop) return;
   const int32_t it = *H.it;
@@ -983,17 +1038,24 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
   const float* __restrict__ qp = H.Qr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * H.n;
   const LeafRun R = leaf_run(T.L);
   double zr = 0.0, zq = 0.0;
-  uint32_t fw[3], ew;
-  lane_tables(fw, ew);
+  uint32_t fw[3], ew, rw;
+  lane_tables(fw, ew, rw);
+  // the next leaf's neighbour indices are loaded while the current leaf computes, and the current
+  // leaf's reads (the previous product's included) go out together (loading a whole leaf of data
+  // ahead costs more in occupancy than it hides); past the run's end the indices of a leaf of the
+  // run stand in, loaded and unused
+  Post0Nbr nc, nn;
+  if (R.first < R.end) mg0_post_nbrs(T, R.first, ew, nc);
   for (int32_t l = R.first; l < R.end; l += R.step) {
-    Post0In cur;  // (loading a leaf ahead here costs more in occupancy than it hides)
-    mg0_post_load(T, l, b, u1, fw, ew, cur);
+    mg0_post_nbrs(T, l + R.step < R.end ? l + R.step : l, ew, nn);
+    Post0In cur;
+    mg0_post_load<true>(T, l, b, u1, fw, ew, nc, qp, cur);
     float v0, v1, r0, r1;
-    mg0_post_compute(T, l, cur, fw, ew, zout, s_b, s_z, c, w, v0, v1, r0, r1);
+    mg0_post_compute(T, l, cur, fw, ew, rw, zout, s_b, s_z, c, w, v0, v1, r0, r1);
     zr += (double)v0 * (double)r0 + (double)v1 * (double)r1;
-    if (H.zq && it > 0) zq += (double)v0 * (double)qp[l * kHalfLeaf + threadIdx.x] +
-                              (double)v1 * (double)qp[half + l * kHalfLeaf + threadIdx.x];
+    if (H.zq && it > 0) zq += (double)v0 * (double)cur.q0 + (double)v1 * (double)cur.q1;
     __syncthreads();  // the tile is restaged for the next leaf
+    nc = nn;
   }
   const double tr = block_sum_all(zr, s_red);
   const double tq = block_sum_all(zq, s_red);

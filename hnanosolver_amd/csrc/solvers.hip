@@ -17,13 +17,30 @@ struct CgState {
   DBuf<float> r, d0, d1, q, z;
   DBuf<float> dring, qring;  // flexible CG: kFcgDepth directions and products
   DBuf<double> rho, alpha, rz, pa, pb, bb, zq, last;
-  DBuf<int> flag;  // stop raised, stop iteration, device iteration counter, deferred-x-update mark
-  int* h_flag = nullptr;
-  double* h_last = nullptr;
+  // the solve's control words, read back in one copy: ints 0-3 (stop raised, stop iteration,
+  // device iteration counter, deferred-x-update mark), then doubles 2-3 (the last stop test's r . r
+  // and ||b||^2)
+  DBuf<double> ctl;
+  double* h_ctl = nullptr;
+  int* flag() const { return reinterpret_cast<int*>(ctl.p); }
+  const int* h_flag() const { return reinterpret_cast<const int*>(h_ctl); }
   int cap = 0;
-  // kFcgBatch multigrid-preconditioned iterations captured in one graph, and what it was made for
+  // runs of multigrid-preconditioned iterations captured as graphs, one per run length (at most
+  // kBatchGraphs lengths: the first read's prediction, kFcgBatch, the runs cut at replacements),
+  // and what they were made for
+  static constexpr int kBatchGraphs = 6;
   hipStream_t cap_stream = nullptr;
-  hipGraphExec_t batch_exec = nullptr;
+  int batch_len[kBatchGraphs] = {};
+  hipGraphExec_t batch_exec[kBatchGraphs] = {};
+  int batch_next = 0;  // the slot a new length replaces when every slot is taken
+  void drop_batches() {
+    for (int i = 0; i < kBatchGraphs; ++i) {
+      if (batch_exec[i]) (void)hipGraphExecDestroy(batch_exec[i]);
+      batch_exec[i] = nullptr;
+      batch_len[i] = 0;
+    }
+    batch_next = 0;
+  }
   uint64_t key_topo = 0;
   int32_t key_L = 0;
   const float* key_p = nullptr;
@@ -35,10 +52,9 @@ struct CgState {
   uint64_t arrays_gen = 0, key_arrays = 0;
   double key_tol2 = -1.0;
   ~CgState() {
-    if (batch_exec) (void)hipGraphExecDestroy(batch_exec);
+    drop_batches();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
-    if (h_flag) (void)hipHostFree(h_flag);
-    if (h_last) (void)hipHostFree(h_last);
+    if (h_ctl) (void)hipHostFree(h_ctl);
   }
 };
 
@@ -81,32 +97,17 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
   end();
 }
 
-// kFcgBatch flexible-CG iterations on the multigrid V-cycle (iteration >= depth, so every
-// iteration's launches are the same), replayed from one graph made on first use for these buffers
-void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth, hipStream_t s) {
+// n >= 2 flexible-CG iterations on the multigrid V-cycle, replayed from a graph made on first use
+// for these buffers and this run length. Every kernel takes the iteration number from the device
+// counter (the product kernel forms d = z at iteration 0), so one graph serves a run of n from any
+// iteration (from iteration depth on with truncation above 1).
+void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth, int n, hipStream_t s) {
   CgState& G = *cg_;
   Multigrid& M = *mg_;
   M.prepare(T, s);
-  if (!G.batch_exec || G.key_topo != T.id || G.key_L != T.L || G.key_p != p || G.key_gen != M.generation ||
+  if (G.key_topo != T.id || G.key_L != T.L || G.key_p != p || G.key_gen != M.generation ||
       G.key_depth != depth || G.key_arrays != G.arrays_gen || G.key_tol2 != S.tol2) {
-    if (G.batch_exec) {
-      HNS_HIP(hipGraphExecDestroy(G.batch_exec));
-      G.batch_exec = nullptr;
-    }
-    if (!G.cap_stream) HNS_HIP(hipStreamCreateWithFlags(&G.cap_stream, hipStreamNonBlocking));
-    hipGraph_t g = nullptr;
-    HNS_HIP(hipStreamBeginCapture(G.cap_stream, hipStreamCaptureModeThreadLocal));
-    try {
-      for (int k = 0; k < kFcgBatch; ++k) mg_iteration(T, p, S, depth, depth, G.cap_stream);
-    } catch (...) {
-      (void)hipStreamEndCapture(G.cap_stream, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
-    }
-    HNS_HIP(hipStreamEndCapture(G.cap_stream, &g));
-    const hipError_t e = hipGraphInstantiate(&G.batch_exec, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    HNS_HIP(e);
+    G.drop_batches();
     G.key_topo = T.id;
     G.key_L = T.L;
     G.key_p = p;
@@ -115,7 +116,34 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
     G.key_arrays = G.arrays_gen;
     G.key_tol2 = S.tol2;
   }
-  HNS_HIP(hipGraphLaunch(G.batch_exec, s));
+  int slot = -1;
+  for (int i = 0; i < CgState::kBatchGraphs; ++i)
+    if (G.batch_exec[i] && G.batch_len[i] == n) slot = i;
+  if (slot < 0) {
+    slot = G.batch_next;
+    G.batch_next = (G.batch_next + 1) % CgState::kBatchGraphs;
+    if (G.batch_exec[slot]) {
+      HNS_HIP(hipGraphExecDestroy(G.batch_exec[slot]));
+      G.batch_exec[slot] = nullptr;
+      G.batch_len[slot] = 0;
+    }
+    if (!G.cap_stream) HNS_HIP(hipStreamCreateWithFlags(&G.cap_stream, hipStreamNonBlocking));
+    hipGraph_t g = nullptr;
+    HNS_HIP(hipStreamBeginCapture(G.cap_stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int k = 0; k < n; ++k) mg_iteration(T, p, S, depth, depth, G.cap_stream);
+    } catch (...) {
+      (void)hipStreamEndCapture(G.cap_stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HNS_HIP(hipStreamEndCapture(G.cap_stream, &g));
+    const hipError_t e = hipGraphInstantiate(&G.batch_exec[slot], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HNS_HIP(e);
+    G.batch_len[slot] = n;
+  }
+  HNS_HIP(hipGraphLaunch(G.batch_exec[slot], s));
 }
 
 Solvers::Solvers() = default;
@@ -123,6 +151,14 @@ Solvers::Solvers() = default;
 Multigrid& Solvers::multigrid() {
   if (!mg_) mg_ = new Multigrid();
   return *mg_;
+}
+
+// the device time of a to-tolerance solve ends at each stop-flag read (the last one ends it); the
+// read then waits for the stream, so the event is complete when the solve returns
+void Solvers::mark_solve_end(const SolveRequest& r, hipStream_t s) {
+  if (!r.want_residual) return;
+  HNS_HIP(hipEventRecord(e1_, s));
+  e1_marked_ = true;
 }
 
 Solvers::~Solvers() {
@@ -194,6 +230,7 @@ double Solvers::rel_residual(const DevTopo& T, const float* div, const float* p,
 SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, const SolveRequest& r, Timing& tm,
                            hipStream_t s) {
   SolveResult res;
+  e1_marked_ = false;
   if (r.want_residual) HNS_HIP(hipEventRecord(e0_, s));
   switch (r.solver) {
     case HNS_SOLVER_RBGS_FIXED: {
@@ -223,8 +260,12 @@ SolveResult Solvers::solve(const DevTopo& T, const float* div, float* p, int K, 
       fail(HNS_ERR_INVALID_ARGUMENT, "unknown pressure solver");
   }
   if (r.want_residual) {
-    HNS_HIP(hipEventRecord(e1_, s));
-    HNS_HIP(hipEventSynchronize(e1_));
+    // a to-tolerance CG solve's time ends at the stop-flag read that ended it, whose event the
+    // host has already waited for: no second wait on the stream here
+    if (!e1_marked_) {
+      HNS_HIP(hipEventRecord(e1_, s));
+      HNS_HIP(hipEventSynchronize(e1_));
+    }
     float ms = 0.f;
     HNS_HIP(hipEventElapsedTime(&ms, e0_, e1_));
     res.ms = ms;
@@ -253,10 +294,8 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.pa.alloc((size_t)kCgMaxGroups * kCgSlots);
     G.pb.alloc(kCgMaxGroups);
     G.bb.alloc(1);
-    G.flag.alloc(4);
-    G.last.alloc(2);
-    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_flag), 3 * sizeof(int), hipHostMallocDefault));
-    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_last), 2 * sizeof(double), hipHostMallocDefault));
+    G.ctl.alloc(4);
+    HNS_HIP(hipHostMalloc(reinterpret_cast<void**>(&G.h_ctl), 4 * sizeof(double), hipHostMallocDefault));
   }
   if (G.cap < cap) {
     G.rho.alloc((size_t)cap + 1);
@@ -271,16 +310,16 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.rho = G.rho.p;
   S.alpha = G.alpha.p;
   S.rz = G.rz.p;
-  S.flag = G.flag.p;
-  S.it = G.flag.p + 2;
-  S.last = G.last.p;
+  S.flag = G.flag();
+  S.it = G.flag() + 2;
+  S.last = G.ctl.p + 2;
   S.bb = G.bb.p;
   S.tol2 = (double)r.tol * (double)r.tol;
   // FCG(1) on the fused V-cycle with a one-workgroup coarse solve: the x update is deferred into
   // the next V-cycle's coarse-level launch (flushed where x must be complete)
   if (mg && depth == 1 && mg_shape().smooth0 == 1) {
     mg_->prepare(T, s);
-    if (mg_->coarse_fused()) S.xdone = G.flag.p + 3;
+    if (mg_->coarse_fused()) S.xdone = G.flag() + 3;
   }
   tm.begin(HNS_K_OTHER, s);
   launch_cg_init(T.L, div, r.dx, p, G.r.p, S, s);
@@ -295,8 +334,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     G.zq.alloc((size_t)kFcgDepth * kCgMaxGroups);
   }
   // a preconditioned iteration costs far more than a sync: check the stop flag every kFcgBatch
-  // iterations, in batches aligned to multiples of kFcgBatch (the multigrid solve replays whole
-  // aligned batches from one graph)
+  // iterations (the multigrid solve replays each run between two reads from one graph)
   // (the multigrid solve's first read comes after 2 batches: it rarely stops before, and iterations
   // past a stop return at once)
   // (with per-kernel timing on, the flag is read after every iteration, so no launch past the stop
@@ -309,7 +347,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   LoopResult lr;
   if (pre) {
     FcgDev d{this, T, div, p, r, S, depth, mg, tm, s};
-    lr = fcg_loop(d, cap, batch0, batch_max, kFcgBatch, kResidualReplace);
+    lr = fcg_loop(d, cap, batch0, batch_max, kResidualReplace);
   } else {
     CgDev d{this, T, div, p, r, S, tm, s};
     lr = cg_loop(d, cap, batch0, batch_max, kResidualReplace);
@@ -366,9 +404,10 @@ void Solvers::CgDev::iteration(int it) {
 }
 CgReport Solvers::CgDev::read() {
   CgState& G = *sol->cg_;
-  HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+  sol->mark_solve_end(r, s);
+  HNS_HIP(hipMemcpyAsync(G.h_ctl, G.ctl.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   HNS_HIP(hipStreamSynchronize(s));
-  return CgReport{G.h_flag[0], G.h_flag[1]};
+  return CgReport{G.h_flag()[0], G.h_flag()[1]};
 }
 void Solvers::CgDev::replace_forced() {
   tm.begin(HNS_K_OTHER, s);
@@ -385,10 +424,15 @@ void Solvers::FcgDev::replace_periodic(int) {
   tm.end(s);
 }
 // (with per-kernel timing on, the iterations are launched one by one so events can bracket them)
-bool Solvers::FcgDev::batchable(int it, int) { return mg && !tm.enabled && it >= depth; }
-void Solvers::FcgDev::batch(int) {
+// (from iteration 0 on with the fused truncation-1 iteration, whose product kernel forms d = z
+// there; the general FCG(depth) kernels are chosen by the host's count, so from depth on)
+int Solvers::FcgDev::batchable(int it, int left) {
+  const bool any_it = depth == 1 && mg_shape().smooth0 == 1;
+  return mg && !tm.enabled && (any_it || it >= depth) ? left : 0;
+}
+void Solvers::FcgDev::batch(int, int n) {
   tm.begin(HNS_K_PRECOND, s);
-  sol->mg_batch(T, p, S, depth, s);
+  sol->mg_batch(T, p, S, depth, n, s);
   tm.end(s);
 }
 void Solvers::FcgDev::first_test() {
@@ -423,10 +467,11 @@ FcgReport Solvers::FcgDev::end_batch() {
   if (S.xdone) launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushStopped, s);
   launch_fcg_retest(T, div, r.dx, p, G.r.p, S, s);
   tm.end(s);
-  HNS_HIP(hipMemcpyAsync(G.h_flag, G.flag.p, 3 * sizeof(int), hipMemcpyDeviceToHost, s));
-  HNS_HIP(hipMemcpyAsync(G.h_last, G.last.p, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  sol->mark_solve_end(r, s);
+  HNS_HIP(hipMemcpyAsync(G.h_ctl, G.ctl.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
   HNS_HIP(hipStreamSynchronize(s));
-  return FcgReport{G.h_flag[0], G.h_flag[1], G.h_flag[2], G.h_last[0], G.h_last[1]};
+  const int* f = G.h_flag();
+  return FcgReport{f[0], f[1], f[2], G.h_ctl[2], G.h_ctl[3]};
 }
 
 }  // namespace hns

@@ -74,15 +74,20 @@ struct FakeFcg {
       stop_it = it;
     }
   }
-  void replace_periodic(int) {
+  std::string replaced;  // host iterations of the periodic replacements, "64 128 ..."
+  bool batch_over_replacement = false;
+  void replace_periodic(int h) {
     budget.take();
+    replaced += std::to_string(h) + " ";
     if (P.stuck || flag) return;
     rr = P.rr_true(it);
     test(false);
   }
-  bool batchable(int it, int) { return it >= 1; }  // as the solver: never iteration 0
-  void batch(int h) {
-    for (int k = 0; k < 4; ++k) iteration(h + k);
+  int batchable(int, int left) { return left; }  // as the multigrid solve: any run of iterations
+  void batch(int h, int n) {
+    for (int k = 1; k < n; ++k)  // a replacement point strictly inside a replayed batch
+      if ((h + k) % 64 == 0) batch_over_replacement = true;
+    for (int k = 0; k < n; ++k) iteration(h + k);
   }
   void first_test() {
     rr = P.rr_rec(0);
@@ -109,14 +114,17 @@ struct FakeFcg {
   }
 };
 
-LoopResult run_fcg(const Problem& P, int cap, long limit, bool* blown) {
+LoopResult run_fcg(const Problem& P, int cap, long limit, bool* blown, std::string* replaced = nullptr,
+                   int batch0 = 8) {
   FakeFcg d(P, limit);
   LoopResult r;
   try {
-    r = fcg_loop(d, cap, 8, 4, 4, 64);
+    r = fcg_loop(d, cap, batch0, 4, 64);
   } catch (const BudgetBlown&) {
   }
   *blown = d.budget.blown;
+  check(!d.batch_over_replacement, "fcg: no replayed batch contains a replacement point");
+  if (replaced) *replaced = d.replaced;
   return r;
 }
 
@@ -252,6 +260,19 @@ void fcg_cases() {
     P.rr_true = P.rr_rec;
     const LoopResult r = run_fcg(P, 50, 1000, &blown);
     check(!blown && r.end == kLoopCapped && r.iterations == 50, "fcg: capped at 50");
+  }
+  {  // a long solve with first batches of odd lengths: replaced before iterations 64, 128 and 192,
+     // never inside a batch, capped at 200
+    for (int b0 : {4, 9, 13, 63}) {
+      Problem P;
+      P.rr_rec = [](int k) { return 1.0 / (1.0 + k); };
+      P.rr_true = P.rr_rec;
+      std::string replaced;
+      const LoopResult r = run_fcg(P, 200, 10000, &blown, &replaced, b0);
+      check(!blown && r.end == kLoopCapped && r.iterations == 200 && replaced == "64 128 192 ",
+            "fcg: periodic replacements at 64, 128, 192 with first batch " + std::to_string(b0) + " (got " +
+                replaced + ")");
+    }
   }
   {  // zero right-hand side: stops before the first iteration
     Problem P;
