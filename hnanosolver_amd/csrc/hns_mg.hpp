@@ -24,21 +24,24 @@ struct MgShape {
   // residual scaled by c4_over. side_groups workgroups carry the deferred x update beside the
   // one-workgroup coarse solve after side_sleeps * 127 * 64 idle cycles (-1: 96 and CH / 2 beside
   // the one-level solve, 128 and 2 beside the two-level one).
-  int c4 = 1, c3_pre = 1, c3_post = 2, c4_iters = 8;
-  float c3_omega = 1.1f, c3_omega_post = 1.8f, c4_omega = 1.7f, c4_over = 1.6f;
+  int c4 = 1, c3_pre = 1, c3_post = 3, c4_iters = 8;
+  float c3_omega = 1.1f, c3_omega_post = 1.9f, c4_omega = 1.7f, c4_over = 1.6f;
   int side_groups = -1, side_sleeps = -1;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)
-  float over_deep = 1.6f;  // the same below level 1 (restrictions of levels 1 and 2)
+  float over_deep = 1.6f;  // the same for the restriction from level 1 to level 2
+  float over3 = 1.8f;      // and from level 2 to level 3
   // Level 0 is pre-smoothed by one pointwise (Jacobi) step from zero, z' = jacobi0 r / 6, and
   // post-smoothed by one red-black Gauss-Seidel iteration (relaxation omega0); levels 1-2 by one
-  // red-black SOR iteration each way (omega_l). Chosen with the CPU oracle's multigrid-preconditioned
-  // CG on the smoke, control and fire scenes (up to 14 M voxels) and on 14 consecutive steps of the
-  // 8 M-voxel fire scene: against red-black pre-smoothing at 1.1, levels 1-2 at 1.4 and level 3 at
-  // 1.0 / 1.4 it takes the same or one fewer iteration everywhere (9 instead of 10 at 8 M voxels,
-  // the largest residual after 9 iterations over the 14 steps 8.0e-6), and the level-0 pre pass
-  // then writes only the level-1 right-hand side.
+  // red-black SOR iteration (omega_l) on the way down, and level 1 by one, level 2 by post2 on the
+  // way up. Chosen with the CPU oracle's multigrid-preconditioned CG on the smoke, control and fire
+  // scenes (up to 14 M voxels) and on 14 consecutive steps of the 8 M-voxel fire scene: every scene
+  // converges to 1e-5 in 8 iterations (9 with one post iteration on level 2, level 2 restricted by
+  // 1.6 and two level-3 iterations at 1.8 after level 4; the largest residual after 8 iterations over
+  // the 14 steps 8.9e-6, against 3.0e-5), and the level-0 pre pass writes only the level-1
+  // right-hand side.
   float jacobi0 = 1.75f;
   float omega0 = 1.0f, omega_l = 1.5f;
+  int post2 = 2;
 };
 const MgShape& mg_shape();
 

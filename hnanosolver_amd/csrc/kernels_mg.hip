@@ -1211,8 +1211,9 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
   bc[LG == 1 && map3 ? map3[l] : ((int32_t)l << (3 * lgC)) | lane] = over * sum;
 }
 
-// map3 (LG = 1, prolonging from level 3): as k_mg_restrict's
-template <int LG>
+// map3 (LG = 1, prolonging from level 3): as k_mg_restrict's. PRO = false: one more red-black
+// iteration on zin alone (no prolongation; uc and map3 unused), the post-smoothing's next iteration.
+template <int LG, bool PRO = true>
 __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __restrict__ b,
                                                    const float* __restrict__ zin, const float* __restrict__ uc,
                                                    float* __restrict__ zout, float scale, float w,
@@ -1231,9 +1232,10 @@ __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __res
   float* Z = s_z[lw];
   auto zval = [&](int32_t nl, int vx, int vy, int vz) {  // absent leaves load the own leaf, then read 0
     const int32_t src = nl < 0 ? l : nl;
-    const float v = zin[Tl::cell(src, vx, vy, vz)] +
-                    uc[LG == 1 && map3 ? map3[src]
-                                       : (src << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
+    float v = zin[Tl::cell(src, vx, vy, vz)];
+    if (PRO)
+      v = v + uc[LG == 1 && map3 ? map3[src]
+                                 : (src << (3 * lgC)) | ((((vx >> 1) << lgC) | (vy >> 1)) << lgC) | (vz >> 1)];
     return nl < 0 ? 0.f : v;
   };
   FaceRow Fr{};
@@ -1677,7 +1679,7 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
       if (lgS == 2)
         k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, map3, stop);
       else
-        k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, map3, stop);
+        k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over3, b[l + 1].p, M.omega_l, map3, stop);
       HNS_HIP(hipGetLastError());
       continue;
     }
@@ -1685,8 +1687,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
       smooth(T, lgS, b[l].p, u[l].p, scale, 0, k == 0, M.omega_l, stop, s);
       smooth(T, lgS, b[l].p, u[l].p, scale, 1, k == 0, M.omega_l, stop, s);
     }
-    k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, M.over_deep, b[l + 1].p,
-                                                        map3, stop);
+    k_mg_restrict<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T, lgS, b[l].p, u[l].p, scale, l == 2 ? M.over3 : M.over_deep,
+                                                        b[l + 1].p, map3, stop);
     HNS_HIP(hipGetLastError());
   }
   // level 3 (one cell per leaf): the two-level solve, or M.coarse over-relaxed red-black iterations
@@ -1695,21 +1697,37 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
                 M.c4_iters, M.c3_pre, M.c3_post, M.c4_omega, M.c4_over, M.c3_omega, M.c3_omega_post};
   coarse_solve(T, Ly, b[kMgLevels - 1].p, u[kMgLevels - 1].p, (float)(1 << (2 * (kMgLevels - 1))), stop,
                (variant & kMgCoarseLaunches) != 0, hook, s);
-  // ascend: prolong and post-smooth in the reverse colour order
+  // ascend: prolong and post-smooth in the reverse colour order (level 2: M.post2 iterations, the
+  // fused path's later ones alternating between u[2] and ut[2]; uc: the level below's result)
+  const float* uc = u[kMgLevels - 1].p;
   for (int l = kMgLevels - 2; l >= 1; --l) {
     const int lgS = 3 - l;
     const float scale = (float)(1 << (2 * l));
+    const int npost = l == 2 ? M.post2 : M.smooth;
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
-      if (lgS == 2)
-        k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, map3, stop);
-      else k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, u[l + 1].p, u[l].p, scale, M.omega_l, map3, stop);
+      if (lgS == 2) {
+        k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, uc, u[l].p, scale, M.omega_l, map3, stop);
+        HNS_HIP(hipGetLastError());  // (fused_l: M.smooth = 1 post iteration on level 1)
+        uc = u[l].p;
+        continue;
+      }
+      k_mgl_post<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, uc, u[l].p, scale, M.omega_l, map3, stop);
       HNS_HIP(hipGetLastError());
+      float* cur = u[l].p;
+      float* oth = ut[l].p;  // its pre-smoothed values are read by the first post pass only
+      for (int k = 1; k < npost; ++k) {
+        k_mgl_post<1, false><<<g, kTpb, 0, s>>>(T, b[l].p, cur, nullptr, oth, scale, M.omega_l, nullptr, stop);
+        HNS_HIP(hipGetLastError());
+        std::swap(cur, oth);
+      }
+      uc = cur;
       continue;
     }
-    k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, u[l + 1].p, map3, stop);
+    k_mg_prolong<<<cells_grid(T.L, lgS), kTpb, 0, s>>>(T.L, lgS, u[l].p, uc, map3, stop);
     HNS_HIP(hipGetLastError());
-    for (int k = 0; k < M.smooth; ++k) {
+    uc = u[l].p;
+    for (int k = 0; k < npost; ++k) {
       smooth(T, lgS, b[l].p, u[l].p, scale, 1, false, M.omega_l, stop, s);
       smooth(T, lgS, b[l].p, u[l].p, scale, 0, false, M.omega_l, stop, s);
     }

@@ -18,11 +18,12 @@
 // pairs between the blocks) towards each face-adjacent block. Red-black colours are the parity of
 // the level coordinate. The shape: level 0 pre-smoothed by one pointwise step from zero,
 // z' = 1.75 r * 0.166666667f, and post-smoothed by one red-black Gauss-Seidel iteration (relaxation
-// 1.0, colour 1 first); levels 1-2 by one red-black SOR iteration each way at 1.5 (post in the
-// reverse colour order); every restricted residual scaled by 1.6; level 3 by one SOR iteration at
-// 1.1, the level-4 correction (8 SOR iterations at 1.7 on the residual restricted by 1.6) and two
-// iterations at 1.8 in the reverse order; with more than 4096 blocks of a colour, 12 SOR iterations
-// at 1.85 on level 3 alone.
+// 1.0, colour 1 first); levels 1-2 by one red-black SOR iteration at 1.5 on the way down and one
+// (level 1) or two (level 2) on the way up, in the reverse colour order; every restricted residual
+// scaled by 1.6, except level 2's by 1.8; level 3 by one SOR iteration at 1.1, the level-4
+// correction (8 SOR iterations at 1.7 on the residual restricted by 1.6) and three iterations at 1.9
+// in the reverse order; with more than 4096 blocks of a colour, 12 SOR iterations at 1.85 on
+// level 3 alone.
 //
 // Parity pins (tests/test_mg_oracle.py): every level operator against the 7-point operator built by
 // coordinate lookup, the Galerkin identities R A_l P = 4 A_{l+1} and the level-4 weights against
@@ -43,8 +44,10 @@ namespace oracle_mg {
 
 struct Shape {
   float jacobi0 = 1.75f, omega0 = 1.0f, omega_l = 1.5f, over = 1.6f, over_deep = 1.6f;
-  int c3_pre = 1, c3_post = 2, c4_iters = 8;
-  float c3_omega = 1.1f, c3_omega_post = 1.8f, c4_omega = 1.7f, c4_over = 1.6f;
+  int post2 = 2;       // red-black post-smoothing iterations on level 2 (one on level 1)
+  float over3 = 1.8f;  // scale of the level-2 -> 3 restriction (over_deep: level 1 -> 2)
+  int c3_pre = 1, c3_post = 3, c4_iters = 8;
+  float c3_omega = 1.1f, c3_omega_post = 1.9f, c4_omega = 1.7f, c4_over = 1.6f;
   int coarse = 12;  // level 3 alone (too many blocks for level 4)
   float coarse_omega = 1.85f;
   int max_blocks_per_colour = 4096;
@@ -241,15 +244,17 @@ void vcycle(MG& g, const float* r, float* z) {
     const float scale = (float)(1 << (2 * l));
     sweep_l(g, S, g.b[l].data(), g.u[l].data(), 1.0f / scale, 0, true, M.omega_l);
     sweep_l(g, S, g.b[l].data(), g.u[l].data(), 1.0f / scale, 1, true, M.omega_l);
-    restrict_l(g, S, g.b[l].data(), g.u[l].data(), scale, M.over_deep, g.b[l + 1].data());
+    restrict_l(g, S, g.b[l].data(), g.u[l].data(), scale, l == 2 ? M.over3 : M.over_deep, g.b[l + 1].data());
   }
   coarse(g);
   for (int l = 2; l >= 1; --l) {
     const int S = 8 >> l;
     const float scale = (float)(1 << (2 * l));
     prolong_l(g, S, g.u[l].data(), g.u[l + 1].data());
-    sweep_l(g, S, g.b[l].data(), g.u[l].data(), 1.0f / scale, 1, false, M.omega_l);
-    sweep_l(g, S, g.b[l].data(), g.u[l].data(), 1.0f / scale, 0, false, M.omega_l);
+    for (int k = 0; k < (l == 2 ? M.post2 : 1); ++k) {
+      sweep_l(g, S, g.b[l].data(), g.u[l].data(), 1.0f / scale, 1, false, M.omega_l);
+      sweep_l(g, S, g.b[l].data(), g.u[l].data(), 1.0f / scale, 0, false, M.omega_l);
+    }
   }
   prolong_l(g, 8, z, g.u[1].data());
   sweep0(g, r, z, 1, false, M.omega0);

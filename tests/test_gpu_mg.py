@@ -248,8 +248,9 @@ def test_mgcg_long_solve_through_replacements(hip):
 
 
 def test_mgcg_graph_remade_for_new_cap_and_tolerance(hip):
-    """Solves on one topology replay the 4-iteration graphs captured by the first; a later solve with a
-    larger iteration cap than any before (its per-iteration arrays are reallocated) and another
+    """Solves on one topology replay the batch graphs captured by the first (every run of iterations
+    between two stop-flag reads is one graph per run length, from iteration 0 on); a later solve with
+    a larger iteration cap than any before (its per-iteration arrays are reallocated) and another
     tolerance has them made again, and then matches the same solve on a fresh topology bit for bit."""
     torch = hip
     sc = scenes.control64()
@@ -263,7 +264,7 @@ def test_mgcg_graph_remade_for_new_cap_and_tolerance(hip):
     st3 = K.pressure_solve(fresh, div, p3, sc.voxel_size, solver="mgcg", tol=1e-5, max_iters=4000)
     print("  tol 1e-3: %d iterations; tol 1e-5 after it: %d, on a fresh topology: %d" % (
         st1.iterations, st2.iterations, st3.iterations))
-    assert st1.iterations >= 5  # the first solve reached the graph batches
+    assert st1.iterations >= 1  # the first solve ran from a batch graph
     assert st2.iterations == st3.iterations and bool(torch.equal(p2, p3))
     r2, d2 = Oracle(sc.coords, np.float64).residual(_dense(torch, topo, p2), _dense(torch, topo, div), sc.voxel_size)
     assert np.sqrt(r2 / d2) <= 1.01e-5
