@@ -39,20 +39,22 @@ KERNEL_BYTES = {
 }
 KERNEL_NAMES = {"rbgs": "k_rbgs", "cg_product": "k_cg_spmv", "cg_update": "k_cg_update",
                 "mgcg_update": "k_fcg_rupdate",
-                "mg_vcycle": "V-cycle (k_mg0_pre, k_mgl_pre x2, k_mg_coarse2, k_mgl_post x2, k_mg0_post_fcg)"}
+                "mg_vcycle": "V-cycle (k_mg0_pre, k_mgl_pre x2, k_mg_coarse2, k_mgl_post x3, k_mg0_post_fcg)"}
 TRAFFIC_FILES = {"mg_vcycle": "mg_vcycle"}
 # the residual update of the multigrid solve: read r, q, write r (its CG_UPDATE class; the solve's x
 # update rides on the coarse-level launch of the next V-cycle)
 KERNEL_BYTES["mgcg_update"] = 12.0
 # Multigrid-preconditioned CG, compulsory bytes per voxel and iteration: the V-cycle's fused level-0
 # passes (pre: read r, write the level-1 right-hand side, 4.5 B; post: read r, the level-1
-# correction and the previous product q, write z, 12.5 B), the coarser levels (~3 B), direction and
+# correction and the previous product q, write z, 12.5 B), the coarser levels (~3.2 B: levels 1-2
+# read their right-hand sides and pre-smoothed values and write their corrections, level 2 twice on
+# the way up), direction and
 # product (read z, d_prev, write d, q: 16 B), the residual update (12 B) and the x update (read x,
 # d, write x: 12 B, in the coarse-level launch)
-MGCG_BYTES_PER_ITERATION = 4.5 + 12.5 + 3.0 + 16.0 + 12.0 + 12.0
-# the V-cycle's share (its seven launches, the x update riding on the coarse-level launch included):
+MGCG_BYTES_PER_ITERATION = 4.5 + 12.5 + 3.2 + 16.0 + 12.0 + 12.0
+# the V-cycle's share (its eight launches, the x update riding on the coarse-level launch included):
 # the dominant work of the multigrid step, reported as the headline roofline
-KERNEL_BYTES["mg_vcycle"] = 4.5 + 12.5 + 3.0 + 12.0
+KERNEL_BYTES["mg_vcycle"] = 4.5 + 12.5 + 3.2 + 12.0
 
 
 def step_bytes(solve_bytes: float, m_scalars: int) -> float:

@@ -69,6 +69,13 @@ __device__ __forceinline__ int32_t leaf_of_block() {
   return (int32_t)((b & 7u) * (G >> 3) + (b >> 3));
 }
 inline unsigned grid_for_leaves(int32_t L) { return 8u * (unsigned)((L + 7) / 8); }
+// The same for a grid of any size G: workgroup b (on XCD b % 8) gets slot b >> 3 of its XCD's
+// contiguous run of slots (the first G % 8 XCDs one slot more), a bijection of [0, G), so
+// workgroups that handle neighbouring leaves share an L2 and reuse each other's halo cells there.
+__device__ __forceinline__ int32_t xcd_slot() {
+  const uint32_t b = blockIdx.x, G = gridDim.x, x = b & 7u, q = G >> 3, r = G & 7u;
+  return (int32_t)(x * q + (x < r ? x : r) + (b >> 3));
+}
 
 // Strided leaf run of a workgroup of a grid smaller than the leaf count (gridDim.x a multiple of
 // 8): the leaves are cut into 8 contiguous ranges, one per XCD, and the XCD's workgroups

@@ -1148,7 +1148,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
   __shared__ float s_r[kTpb];
   if (stop && *stop) return;
   const int lw = threadIdx.x >> (3 * LG), lane = threadIdx.x & (Tl::C - 1);
-  const int32_t l = (int32_t)blockIdx.x * Tl::LPW + lw;
+  const int32_t l = xcd_slot() * Tl::LPW + lw;
   const bool live = l < T.L;
   const float inv_s = 1.0f / scale;
   const int cx = lane >> (2 * LG), cy = (lane >> LG) & (Tl::S - 1), cz = lane & (Tl::S - 1);
@@ -1223,7 +1223,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_post(DevTopo T, const float* __res
   __shared__ float s_z[Tl::LPW][Tl::N];
   if (stop && *stop) return;
   const int lw = threadIdx.x >> (3 * LG), lane = threadIdx.x & (Tl::C - 1);
-  const int32_t l = (int32_t)blockIdx.x * Tl::LPW + lw;
+  const int32_t l = xcd_slot() * Tl::LPW + lw;
   const bool live = l < T.L;
   const float inv_s = 1.0f / scale;
   const int cx = lane >> (2 * LG), cy = (lane >> LG) & (Tl::S - 1), cz = lane & (Tl::S - 1);
