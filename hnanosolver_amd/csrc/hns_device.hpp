@@ -83,12 +83,14 @@ __device__ __forceinline__ int32_t xcd_slot() {
 struct LeafRun {
   int32_t first, end, step;
 };
-__device__ inline LeafRun leaf_run(int32_t L) {
-  const int32_t b = (int32_t)blockIdx.x, W = (int32_t)(gridDim.x >> 3);
+// (workgroup b of G, G a multiple of 8)
+__device__ inline LeafRun leaf_run(int32_t L, int32_t b, int32_t G) {
+  const int32_t W = G >> 3;
   const int32_t per = (L + 7) / 8;
   const int32_t lo = (b & 7) * per;
   return LeafRun{lo + (b >> 3), min(L, lo + per), W};
 }
+__device__ inline LeafRun leaf_run(int32_t L) { return leaf_run(L, (int32_t)blockIdx.x, (int32_t)gridDim.x); }
 
 // float -> int round-down (Stencils.hpp:28-30, __float2int_rd); far/NaN positions are clamped
 // to a coordinate outside every domain so they read as misses.

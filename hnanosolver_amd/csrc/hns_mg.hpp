@@ -1,6 +1,7 @@
 // Geometric multigrid V-cycle over the leaf hierarchy (kernels_mg.hip): the preconditioner of
 // HNS_SOLVER_MGCG, applied inside the flexible CG of kernels_cg.hip.
 #pragma once
+#include "hns_cg.hpp"
 #include "hns_internal.hpp"
 
 namespace hns {
@@ -88,10 +89,12 @@ struct Multigrid {
     prepare_coarse(T, s);
   }
   // the V-cycle's kernels launched on s (for capture into a caller's graph; prepare first)
-  // hook: also leave the flexible-CG partials (null: none)
+  // hook: also leave the flexible-CG partials (null: none); test: the level-0 pre pass first runs
+  // the flexible CG's stop test that follows the previous update (k_fcg_test's advancing test, on
+  // the same partials, in every workgroup; null: none)
   void enqueue(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, hipStream_t s,
-               const McgHook* hook = nullptr) {
-    launch_all(T, r_cs, z_cs, stop, 0u, s, hook);
+               const McgHook* hook = nullptr, const CgScalars* test = nullptr) {
+    launch_all(T, r_cs, z_cs, stop, 0u, s, hook, test);
   }
   uint64_t generation = 0;  // advanced whenever buffers or the coarse layout change
   // the coarsest level is solved by one workgroup (after prepare for the topology)
@@ -107,7 +110,7 @@ struct Multigrid {
 
  private:
   void launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
-                  hipStream_t s, const McgHook* hook = nullptr);
+                  hipStream_t s, const McgHook* hook = nullptr, const CgScalars* test = nullptr);
   // colour-sorted slot layout of the coarsest level for its one-workgroup solve, built on the host
   // once per topology (synchronises s); coarse_ch_ = 0 when the leaves do not fit one workgroup
   void prepare_coarse(const DevTopo& T, hipStream_t s);

@@ -127,7 +127,7 @@ class Solvers {
                        hipStream_t s);
   void mg_batch(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int n, hipStream_t s);
   void mg_iteration(const DevTopo& T, float* p, const struct CgScalars& S, int depth, int it, hipStream_t s,
-                    Timing* tm = nullptr);
+                    Timing* tm = nullptr, bool fold_test = false, bool own_test = true);
 
   // The device operations the loops of hns_solve_loop.hpp drive (one solve's arrays and stream).
   struct RbgsDev {

@@ -821,12 +821,36 @@ __device__ __forceinline__ int32_t face_nl(const FaceRow& F, int f) {
 // the current leaf is staged, from halo neighbours loaded one leaf earlier still (pair_halo_nbrs),
 // two leaves per trip with the register sets swapping roles. The arithmetic of a pointwise scaling
 // followed by k_mg0_restrict, so the unfused V-cycle gives the same bits.
+// TEST: the grid has 8 more workgroups in front, and workgroup 0 runs the flexible CG's stop test
+// after the previous update beside the pass (k_fcg_test with advance, on the G = cg_groups(L)
+// partials of r . r); the later kernels of the iteration read its verdict. On a stop this pass's
+// output goes unused.
+template <bool TEST>
 __global__ __launch_bounds__(kTpb) void k_mg0_pre(DevTopo T, const float* __restrict__ b, float c, float over,
-                                                  float* __restrict__ b1, const int* __restrict__ stop) {
+                                                  float* __restrict__ b1, const int* __restrict__ stop, CgScalars S) {
   __shared__ float s_t[2][kPairTile];
   __shared__ float s_r[2][kLeafVox];
+  __shared__ double s_red[kTpb / 64];
   if (stop && *stop) return;
-  const LeafRun R = leaf_run(T.L);
+  if (TEST && blockIdx.x < 8) {
+    if (blockIdx.x == 0) {
+      double rr1[1];
+      reduce_partial_rows(S.pa + kSlotRR * kCgMaxGroups, (int)gridDim.x - 8, s_red, rr1);
+      if (threadIdx.x == 0) {
+        const double rr = rr1[0], bb = S.bb[0];
+        const int32_t it = *S.it + 1;
+        S.last[0] = rr;
+        S.last[1] = bb;
+        *S.it = it;
+        if (rr <= S.tol2 * bb) {
+          S.flag[1] = it;
+          S.flag[0] = 1;
+        }
+      }
+    }
+    return;
+  }
+  const LeafRun R = TEST ? leaf_run(T.L, (int32_t)blockIdx.x - 8, (int32_t)gridDim.x - 8) : leaf_run(T.L);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
@@ -1649,10 +1673,12 @@ void Multigrid::prepare_coarse(const DevTopo& T, hipStream_t s) {
 }
 
 void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, const int* stop, unsigned variant,
-                           hipStream_t s, const McgHook* hook) {
+                           hipStream_t s, const McgHook* hook, const CgScalars* test) {
   const MgShape& M = mg_shape();
   const bool fused = M.smooth0 == 1 && !(variant & kMgUnfused);
   if (hook && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the flexible-CG pass needs the fused level-0 V-cycle");
+  if (test && (!hook || stop != test->flag))
+    fail(HNS_ERR_INVALID_ARGUMENT, "the folded stop test needs the flexible-CG pass and the solve's stop flag");
   // level 3 in the one-workgroup solve's slot order when that solve runs
   const int32_t* map3 = !(variant & kMgCoarseLaunches) && coarse_fused() ? cmap_.p : nullptr;
   // level 0's pointwise pre-smoothing step z' = c r from z = 0 (c: the Jacobi weight over the
@@ -1660,7 +1686,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
   const float c0 = M.jacobi0 * 0.166666667f;
   if (fused) {
     // pre-smoothing and restriction in one pass (z' is formed again by the post pass)
-    k_mg0_pre<<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, c0, M.over, b[1].p, stop);
+    if (test) k_mg0_pre<true><<<cg_groups(T.L) + 8, kTpb, 0, s>>>(T, r_cs, c0, M.over, b[1].p, stop, *test);
+    else k_mg0_pre<false><<<cg_groups(T.L), kTpb, 0, s>>>(T, r_cs, c0, M.over, b[1].p, stop, CgScalars{});
     HNS_HIP(hipGetLastError());
   } else {
     const int64_t n = (int64_t)T.L * kLeafVox;

@@ -61,8 +61,11 @@ struct CgState {
 // One flexible-CG iteration on the multigrid V-cycle, launched on s (it: the host's count, which
 // only selects kernels). Truncation 1 (the default): the V-cycle's last pass leaves z . r and
 // z . q_prev, then direction and product in one launch; otherwise the general FCG(depth) kernels.
+// fold_test: the stop test after the previous iteration's update runs inside the V-cycle's first
+// pass (its separate launch was left out); own_test: this iteration's update is followed by its
+// stop test launch (otherwise the next iteration folds it).
 void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int depth, int it, hipStream_t s,
-                           Timing* tm) {
+                           Timing* tm, bool fold_test, bool own_test) {
   CgState& G = *cg_;
   auto begin = [&](int k) {
     if (tm) tm->begin(k, s);
@@ -76,12 +79,13 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
     const McgXUpdate X{S.xdone ? p : nullptr, G.dring.p, S.alpha, S.xdone};
     const McgHook H{S.pa + kSlotDR * kCgMaxGroups, pcg ? nullptr : G.zq.p, G.qring.p, (int64_t)N_, S.it, X};
     begin(HNS_K_PRECOND);
-    mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H);
+    mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H, fold_test ? &S : nullptr);
     end();
     begin(HNS_K_CG_PRODUCT);
     launch_fcg1_dirspmv(T, it == 0, G.z.p, G.dring.p, G.qring.p, G.zq.p, S, s, pcg);
     end();
   } else {
+    if (fold_test || !own_test) fail(HNS_ERR_INVALID_ARGUMENT, "the stop test folds only into the fused V-cycle");
     begin(HNS_K_PRECOND);
     mg_->enqueue(T, G.r.p, G.z.p, S.flag, s);
     end();
@@ -92,9 +96,11 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
   begin(HNS_K_CG_UPDATE);
   launch_fcg_update(T.L, S.xdone ? nullptr : p, G.r.p, G.dring.p, G.qring.p, S, s);
   end();
-  begin(HNS_K_OTHER);
-  launch_fcg_test(T.L, true, S, s);
-  end();
+  if (own_test) {
+    begin(HNS_K_OTHER);
+    launch_fcg_test(T.L, true, S, s);
+    end();
+  }
 }
 
 // n >= 2 flexible-CG iterations on the multigrid V-cycle, replayed from a graph made on first use
@@ -131,7 +137,12 @@ void Solvers::mg_batch(const DevTopo& T, float* p, const CgScalars& S, int depth
     hipGraph_t g = nullptr;
     HNS_HIP(hipStreamBeginCapture(G.cap_stream, hipStreamCaptureModeThreadLocal));
     try {
-      for (int k = 0; k < n; ++k) mg_iteration(T, p, S, depth, depth, G.cap_stream);
+      // with truncation 1, every iteration but the run's first takes the stop test of the update
+      // before it inside its V-cycle's first pass (one launch fewer per iteration; the run's last
+      // update keeps its own test, which the host's read follows)
+      const bool fold = depth == 1 && mg_shape().smooth0 == 1;
+      for (int k = 0; k < n; ++k)
+        mg_iteration(T, p, S, depth, depth, G.cap_stream, nullptr, fold && k > 0, !fold || k == n - 1);
     } catch (...) {
       (void)hipStreamEndCapture(G.cap_stream, &g);
       if (g) (void)hipGraphDestroy(g);
