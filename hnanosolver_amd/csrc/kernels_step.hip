@@ -364,7 +364,9 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
   __shared__ float s_p[10 * 10 * 10];  // p of the leaf (local -1..8 at 0..9) with its six face halos
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
-  const LeafView V = load_leaf(T, l, s_nb);
+  // the neighbour row in LDS (and its barrier) only for the collision path's normal
+  LeafView V{};
+  if (sdf) V = load_leaf(T, l, s_nb);
   const int32_t N = T.L * kLeafVox, half = N >> 1;
   const int32_t n = threadIdx.x;
   const int32_t lx = n >> 6, ly = (n >> 3) & 7, lz = n & 7;
