@@ -25,7 +25,7 @@ struct MgShape {
   // residual scaled by c4_over. side_groups workgroups carry the deferred x update beside the
   // one-workgroup coarse solve after side_sleeps * 127 * 64 idle cycles (-1: 96 and CH / 2 beside
   // the one-level solve, 128 and 2 beside the two-level one).
-  int c4 = 1, c3_pre = 1, c3_post = 3, c4_iters = 8;
+  int c4 = 1, c3_pre = 1, c3_post = 3, c4_iters = 6;
   float c3_omega = 1.1f, c3_omega_post = 1.9f, c4_omega = 1.7f, c4_over = 1.6f;
   int side_groups = -1, side_sleeps = -1;
   float over = 1.6f;  // coarse-correction scale (applied to each restricted residual)

@@ -21,7 +21,7 @@
 // 1.0, colour 1 first); levels 1-2 by one red-black SOR iteration at 1.5 on the way down and one
 // (level 1) or two (level 2) on the way up, in the reverse colour order; every restricted residual
 // scaled by 1.6, except level 2's by 1.8; level 3 by one SOR iteration at 1.1, the level-4
-// correction (8 SOR iterations at 1.7 on the residual restricted by 1.6) and three iterations at 1.9
+// correction (6 SOR iterations at 1.7 on the residual restricted by 1.6) and three iterations at 1.9
 // in the reverse order; with more than 4096 blocks of a colour, 12 SOR iterations at 1.85 on
 // level 3 alone.
 //
@@ -46,7 +46,7 @@ struct Shape {
   float jacobi0 = 1.75f, omega0 = 1.0f, omega_l = 1.5f, over = 1.6f, over_deep = 1.6f;
   int post2 = 2;       // red-black post-smoothing iterations on level 2 (one on level 1)
   float over3 = 1.8f;  // scale of the level-2 -> 3 restriction (over_deep: level 1 -> 2)
-  int c3_pre = 1, c3_post = 3, c4_iters = 8;
+  int c3_pre = 1, c3_post = 3, c4_iters = 6;
   float c3_omega = 1.1f, c3_omega_post = 1.9f, c4_omega = 1.7f, c4_over = 1.6f;
   int coarse = 12;  // level 3 alone (too many blocks for level 4)
   float coarse_omega = 1.85f;
