@@ -1161,6 +1161,281 @@ __device__ __forceinline__ void ltile_c1(const FaceRow& Fr, int lane, int cx, in
   }
 }
 
+// Level 1 (S = 4, one wave per leaf) from compile-time staging tables, as kTab12 serves level 0:
+// the passes were bound by VALU issue (~330 instructions a wave, most of them the generic tile
+// addressing of LTile). Per lane (the leaf's cell lane = (x << 4) | (y << 2) | z):
+//   face[k * 64 + lane]: face-slab entry lane + 64 k of LTile<2>::face (face 2 k + (lane >> 5)):
+//       tile index (bits 0-8), source cell in the neighbour (9-14), its level-2 parent (15-17)
+//   edge[lane]: depth-1 edge entry min(lane, 47) of LTile<2>::edge: the same fields (lanes 48-63
+//       repeat entry 47, so that every lane stores: the same value to the same place)
+//   ring[lane] (lanes < 48): the colour-1 slab position ltile_c1<2> updates, tile index (bits 0-8)
+//       on face lane >> 3
+struct TabL1 {
+  uint32_t face[192];
+  uint32_t edge[64];
+  uint32_t ring[64];
+};
+constexpr uint32_t tl1_word(int tx, int ty, int tz, int vx, int vy, int vz) {
+  return (uint32_t)((tx * 8 + ty) * 8 + tz) | ((uint32_t)((((vx << 2) | vy) << 2) | vz) << 9) |
+         ((uint32_t)((((vx >> 1) << 1) | (vy >> 1)) << 1 | (vz >> 1)) << 15);
+}
+constexpr TabL1 make_tabl1() {
+  TabL1 T{};
+  for (int e = 0; e < 192; ++e) {  // LTile<2>::face
+    const int f = e >> 5, ax = f >> 1, pos = f & 1, d = (e >> 4) & 1, a = (e >> 2) & 3, b = e & 3;
+    const int vd = pos ? d : 3 - d, td = pos ? 6 + d : 1 - d;
+    const int vx = ax == 0 ? vd : a, vy = ax == 0 ? a : (ax == 1 ? vd : b), vz = ax == 2 ? vd : b;
+    const int tx = ax == 0 ? td : a + 2, ty = ax == 0 ? a + 2 : (ax == 1 ? td : b + 2), tz = ax == 2 ? td : b + 2;
+    T.face[e] = tl1_word(tx, ty, tz, vx, vy, vz);
+  }
+  for (int lane = 0; lane < 64; ++lane) {  // LTile<2>::edge
+    const int e = lane < 48 ? lane : 47;
+    const int g = e >> 2, w = e & 3, s0 = (g & 1) ? 1 : -1, s1 = (g & 2) ? 1 : -1;
+    const int d0 = g < 8 ? s0 : 0, d1 = g < 4 ? s1 : (g < 8 ? 0 : s0), d2 = g < 4 ? 0 : s1;
+    const int vx = d0 == 0 ? w : (d0 > 0 ? 0 : 3), vy = d1 == 0 ? w : (d1 > 0 ? 0 : 3), vz = d2 == 0 ? w : (d2 > 0 ? 0 : 3);
+    T.edge[lane] = tl1_word(vx + 2 + 4 * d0, vy + 2 + 4 * d1, vz + 2 + 4 * d2, vx, vy, vz);
+  }
+  for (int lane = 0; lane < 48; ++lane) {  // ltile_c1<2>'s slab entry
+    const int f = lane / 8, j = lane % 8, a = j / 2, bh = j % 2;
+    const int e = (f << 5) | (a << 2) | (2 * bh);
+    const int ax = f >> 1, pos = f & 1, d = (e >> 4) & 1, aa = (e >> 2) & 3, b = e & 3;
+    const int td = pos ? 6 + d : 1 - d;
+    int tx = ax == 0 ? td : aa + 2, ty = ax == 0 ? aa + 2 : (ax == 1 ? td : b + 2), tz = ax == 2 ? td : b + 2;
+    if (((tx + ty + tz) & 1) == 0) {
+      if (f < 4) ++tz;
+      else ++ty;
+    }
+    T.ring[lane] = (uint32_t)((tx * 8 + ty) * 8 + tz);
+  }
+  return T;
+}
+__constant__ TabL1 kTabL1 = make_tabl1();
+
+// face_row of a wave-uniform leaf through scalar loads (the constant address space: the neighbour
+// rows are not written while a kernel reads them)
+__device__ __forceinline__ FaceRow face_row_s(const DevTopo& T, int32_t l) {
+  typedef const __attribute__((address_space(4))) int32_t* ConstRow;
+  const ConstRow r = (ConstRow)(T.nbr + kNbrStride * l);
+  return FaceRow{l, r[4], r[22], r[10], r[16], r[12], r[14]};
+}
+// the 27-slot neighbour index of LTile<2>::edge's entry group g = e >> 2, five bits per group
+constexpr uint64_t make_edge_slots() {
+  uint64_t k = 0;
+  for (int g = 0; g < 12; ++g) {
+    const int s0 = (g & 1) ? 1 : -1, s1 = (g & 2) ? 1 : -1;
+    const int d0 = g < 8 ? s0 : 0, d1 = g < 4 ? s1 : (g < 8 ? 0 : s0), d2 = g < 4 ? 0 : s1;
+    k |= (uint64_t)((d0 + 1) * 9 + (d1 + 1) * 3 + (d2 + 1)) << (5 * g);
+  }
+  return k;
+}
+constexpr uint64_t kEdgeSlots = make_edge_slots();
+__device__ __forceinline__ int edge_slot(int lane) { return (int)((kEdgeSlots >> (5 * (min(lane, 47) >> 2))) & 31u); }
+
+// the six neighbours' sum at tile index i of an 8^3 tile, +x, -x, +y, -y, +z, -z (LTile<2>::nsum)
+__device__ __forceinline__ float tl1_nsum(const float* A, int i) {
+  return A[i + 64] + A[i - 64] + A[i + 8] + A[i - 8] + A[i + 1] + A[i - 1];
+}
+
+// Level 1 leaves per wave: the wave's leaves l0, l0 + 4, ... (l0 = 4 NL slot + wave) have all their
+// loads issued up front, and the wave then stages and smooths them one after the other in its one
+// tile. One leaf per wave takes two rounds of the chip's wave slots at 8M voxels, paying the chain
+// of dependent loads twice; two per wave take one.
+constexpr int kL1Leaves = 2;
+
+// bit f set where face f (-x, +x, -y, +y, -z, +z) of the leaf has a neighbour
+__device__ __forceinline__ uint32_t face_mask(const FaceRow& Fr) {
+  return (uint32_t)(Fr.xm >= 0) | (uint32_t)(Fr.xp >= 0) << 1 | (uint32_t)(Fr.ym >= 0) << 2 |
+         (uint32_t)(Fr.yp >= 0) << 3 | (uint32_t)(Fr.zm >= 0) << 4 | (uint32_t)(Fr.zp >= 0) << 5;
+}
+
+// k_mgl_pre<2> (level 1: one red-black iteration from zero, the residual and its restriction to
+// level 2) with the staging positions from kTabL1: the same values in the same order.
+template <int NL>
+__global__ __launch_bounds__(kTpb) void k_mgl1_pre(DevTopo T, const float* __restrict__ b, float* __restrict__ zout,
+                                                   float scale, float over, float* __restrict__ bc, float w,
+                                                   const int* __restrict__ stop) {
+  __shared__ float s_b[4][512];
+  __shared__ float s_z[4][512];
+  __shared__ float s_r[kTpb];
+  if (stop && *stop) return;
+  const int lw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int32_t l0 = xcd_slot() * (4 * NL) + lw;
+  const float inv_s = 1.0f / scale;
+  const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
+  const int ic = ((cx + 2) * 8 + cy + 2) * 8 + cz + 2;  // the lane's cell in the tile
+  float* B = s_b[lw];
+  float* Z = s_z[lw];
+  uint32_t fw[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) fw[k] = kTabL1.face[lane + 64 * k];
+  const uint32_t ew = kTabL1.edge[lane], rw = kTabL1.ring[lane];
+  const int es = edge_slot(lane);
+  // every load of the wave's leaves issued before the first LDS store, the neighbour indices of
+  // every leaf ahead of every data load: the vector-memory counter completes in order (absent
+  // leaves load the own leaf, then read 0)
+  FaceRow Fr[NL];
+  float fv[NL][3], bv[NL], ev[NL];
+  int32_t nl[NL][3], enl[NL];
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int32_t l = l0 + 4 * n;
+    if (l >= T.L) break;
+    Fr[n] = face_row_s(T, l);
+    enl[n] = T.nbr[kNbrStride * l + es];
+  }
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int32_t l = l0 + 4 * n;
+    if (l >= T.L) break;
+    const bool hi = lane & 32;  // face 2 k + 1 of entry k, else face 2 k
+    nl[n][0] = hi ? Fr[n].xp : Fr[n].xm;
+    nl[n][1] = hi ? Fr[n].yp : Fr[n].ym;
+    nl[n][2] = hi ? Fr[n].zp : Fr[n].zm;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fv[n][k] = b[((nl[n][k] < 0 ? l : nl[n][k]) << 6) | (int32_t)((fw[k] >> 9) & 63u)];
+    }
+    bv[n] = b[(l << 6) | lane];
+    ev[n] = b[((enl[n] < 0 ? l : enl[n]) << 6) | (int32_t)((ew >> 9) & 63u)];
+  }
+  __builtin_amdgcn_sched_barrier(0);  // no use of a loaded value scheduled among the loads
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int32_t l = l0 + 4 * n;
+    const bool live = l < T.L;  // wave-uniform
+    if (n > 0) __syncthreads();  // the tile's previous leaf fully read
+    if (live) {
+      auto put = [&](int i, float v) {
+        B[i] = v;
+        Z[i] = (((i >> 6) + (i >> 3) + i) & 1) == 0 ? w * ((v * inv_s + 0.f) * (1.0f / 6.0f)) : 0.f;
+      };
+      put(ic, bv[n]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) put((int)(fw[k] & 511u), nl[n][k] < 0 ? 0.f : fv[n][k]);
+      put((int)(ew & 511u), enl[n] < 0 ? 0.f : ev[n]);  // lanes 48-63 repeat lane 47's entry
+    }
+    __syncthreads();
+    if (live) {
+      auto update = [&](int i) {
+        const float gs = (B[i] * inv_s + tl1_nsum(Z, i)) * (1.0f / 6.0f);
+        Z[i] = w * gs;
+      };
+      if (((cx + cy + cz) & 1) == 1) update(ic);
+      if (lane < 48 && ((face_mask(Fr[n]) >> (lane >> 3)) & 1u)) update((int)rw);
+    }
+    __syncthreads();
+    if (live) {
+      const float u = Z[ic];
+      s_r[threadIdx.x] = B[ic] - scale * (6.0f * u - tl1_nsum(Z, ic));
+      zout[(l << 6) | lane] = u;
+    }
+    __syncthreads();
+    if (live && lane < 8) {
+      const int X = lane >> 2, Y = (lane >> 1) & 1, Z0 = lane & 1;
+      float sum = 0.f;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z0 + (k & 1);
+        sum += s_r[(lw << 6) | (((x << 2) | y) << 2) | z];
+      }
+      bc[((int32_t)l << 3) | lane] = over * sum;
+    }
+  }
+}
+
+// k_mgl_post<2> (level 1: the prolongation from level 2 and one red-black iteration in the reverse
+// colour order) with the staging positions from kTabL1: the same values in the same order.
+template <int NL>
+__global__ __launch_bounds__(kTpb) void k_mgl1_post(DevTopo T, const float* __restrict__ b,
+                                                    const float* __restrict__ zin, const float* __restrict__ uc,
+                                                    float* __restrict__ zout, float scale, float w,
+                                                    const int* __restrict__ stop) {
+  __shared__ float s_b[4][512];
+  __shared__ float s_z[4][512];
+  if (stop && *stop) return;
+  const int lw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int32_t l0 = xcd_slot() * (4 * NL) + lw;
+  const float inv_s = 1.0f / scale;
+  const int cx = lane >> 4, cy = (lane >> 2) & 3, cz = lane & 3;
+  const int ic = ((cx + 2) * 8 + cy + 2) * 8 + cz + 2;
+  float* B = s_b[lw];
+  float* Z = s_z[lw];
+  uint32_t fw[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) fw[k] = kTabL1.face[lane + 64 * k];
+  const uint32_t ew = kTabL1.edge[lane], rw = kTabL1.ring[lane];
+  const int es = edge_slot(lane);
+  const uint32_t cw = (uint32_t)(((((cx >> 1) << 1) | (cy >> 1)) << 1 | (cz >> 1)) << 15) | ((uint32_t)lane << 9);
+  // the prolonged value of cell `word` in leaf nl of own leaf l (absent leaves load the own leaf,
+  // then read 0)
+  auto zval = [&](int32_t l, int32_t nl, uint32_t word) {
+    const int32_t src = nl < 0 ? l : nl;
+    const float v = zin[(src << 6) | (int32_t)((word >> 9) & 63u)] + uc[(src << 3) | (int32_t)((word >> 15) & 7u)];
+    return nl < 0 ? 0.f : v;
+  };
+  // every load of the wave's leaves issued before the first LDS store, the neighbour indices of
+  // every leaf ahead of every data load
+  FaceRow Fr[NL];
+  float fz[NL][3], fb[NL][3], zc[NL], bc0[NL], ez[NL];
+  int32_t nl[NL][3], enl[NL];
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int32_t l = l0 + 4 * n;
+    if (l >= T.L) break;
+    Fr[n] = face_row_s(T, l);
+    enl[n] = T.nbr[kNbrStride * l + es];
+  }
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int32_t l = l0 + 4 * n;
+    if (l >= T.L) break;
+    const bool hi = lane & 32;  // face 2 k + 1 of entry k, else face 2 k
+    nl[n][0] = hi ? Fr[n].xp : Fr[n].xm;
+    nl[n][1] = hi ? Fr[n].yp : Fr[n].ym;
+    nl[n][2] = hi ? Fr[n].zp : Fr[n].zm;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      fz[n][k] = zval(l, nl[n][k], fw[k]);
+      fb[n][k] = b[((nl[n][k] < 0 ? l : nl[n][k]) << 6) | (int32_t)((fw[k] >> 9) & 63u)];
+    }
+    zc[n] = zval(l, l, cw);
+    bc0[n] = b[(l << 6) | lane];
+    ez[n] = zval(l, enl[n], ew);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  auto sweep = [&](int i) {
+    const float gs = (B[i] * inv_s + tl1_nsum(Z, i)) * (1.0f / 6.0f);
+    Z[i] = fmaf(w, gs - Z[i], Z[i]);
+  };
+#pragma unroll
+  for (int n = 0; n < NL; ++n) {
+    const int32_t l = l0 + 4 * n;
+    const bool live = l < T.L;  // wave-uniform
+    if (n > 0) __syncthreads();  // the tile's previous leaf fully read
+    if (live) {
+      Z[ic] = zc[n];
+      B[ic] = bc0[n];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int j = (int)(fw[k] & 511u);
+        Z[j] = fz[n][k];
+        B[j] = nl[n][k] < 0 ? 0.f : fb[n][k];  // read at depth 1 only
+      }
+      Z[(int)(ew & 511u)] = ez[n];  // lanes 48-63 repeat lane 47's entry
+    }
+    __syncthreads();
+    if (live) {
+      if (((cx + cy + cz) & 1) == 1) sweep(ic);
+      if (lane < 48 && ((face_mask(Fr[n]) >> (lane >> 3)) & 1u)) sweep((int)rw);
+    }
+    __syncthreads();
+    if (live) {
+      if (((cx + cy + cz) & 1) == 0) sweep(ic);
+      zout[(l << 6) | lane] = Z[ic];
+    }
+  }
+}
+
 // map3 (LG = 1, restricting to level 3): as k_mg_restrict's
 template <int LG>
 __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __restrict__ b, float* __restrict__ zout,
@@ -1704,7 +1979,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
       if (lgS == 2)
-        k_mgl_pre<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, map3, stop);
+        k_mgl1_pre<kL1Leaves><<<(unsigned)((T.L + 4 * kL1Leaves - 1) / (4 * kL1Leaves)), kTpb, 0, s>>>(
+            T, b[l].p, ut[l].p, scale, M.over_deep, b[l + 1].p, M.omega_l, stop);
       else
         k_mgl_pre<1><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, scale, M.over3, b[l + 1].p, M.omega_l, map3, stop);
       HNS_HIP(hipGetLastError());
@@ -1734,7 +2010,8 @@ void Multigrid::launch_all(const DevTopo& T, const float* r_cs, float* z_cs, con
     if (fused_l) {
       const unsigned g = (unsigned)(((int64_t)T.L * (1 << (3 * lgS)) + kTpb - 1) / kTpb);
       if (lgS == 2) {
-        k_mgl_post<2><<<g, kTpb, 0, s>>>(T, b[l].p, ut[l].p, uc, u[l].p, scale, M.omega_l, map3, stop);
+        k_mgl1_post<kL1Leaves><<<(unsigned)((T.L + 4 * kL1Leaves - 1) / (4 * kL1Leaves)), kTpb, 0, s>>>(
+            T, b[l].p, ut[l].p, uc, u[l].p, scale, M.omega_l, stop);
         HNS_HIP(hipGetLastError());  // (fused_l: M.smooth = 1 post iteration on level 1)
         uc = u[l].p;
         continue;
