@@ -78,6 +78,8 @@ __device__ inline void block_store_rows(double (&v)[N], double* s_red, double* r
 
 // Device scalars of one solve. rho[k] = r_k . r_k (CG) or sigma_k = d_k . A d_k (flexible CG), alpha[k] the step
 // length of iteration k; flag[0] = stop, flag[1] = iterations done when stopped.
+// the flexible CG's largest truncation + 1: the slots of its direction and product rings
+constexpr int kFcgDepth = 8;
 struct CgScalars {
   double* pa;     // kCgSlots rows of kCgMaxGroups partials (row = slot), written by init/replace/update and FCG
   double* pb;     // one partial per workgroup (d . q), written by the product kernel
@@ -93,6 +95,11 @@ struct CgScalars {
   // iteration k whose V-cycle last applied the pending x += alpha_{k-1} d_{k-1} (so every update
   // before k is in x), or the iteration a flush completed x at
   int* xdone = nullptr;
+  // flexible CG: the slots of the direction and product rings in use (fcg_slot): kFcgDepth for the
+  // general truncation, 2 for the multigrid solve's truncation 1 (the previous pair and the current
+  // one), whose 8 M-voxel iterations then cycle 2 x 2 vectors of 33 MB instead of 2 x 8 and keep
+  // more of their working set in the 256 MB last-level cache
+  int ring = kFcgDepth;
 };
 // pa slots: r.r, and for the flexible CG d.r and d.q
 enum CgSlot { kSlotRR = 0, kSlotDR = 1, kSlotDQ = 2, kCgSlots = 4 };
@@ -148,8 +155,8 @@ void launch_cg_update(int32_t L, int32_t it, float* x, float* r, const float* d,
 void launch_fcg1_dirspmv(const DevTopo& T, bool first, const float* z, float* Dr, float* Qr, const double* zq,
                          const CgScalars& S, hipStream_t s, bool pcg = false);
 // x += alpha d, r -= alpha q. zq: kFcgDepth rows of kCgMaxGroups partials.
-constexpr int kFcgDepth = 8;
-__device__ __forceinline__ int fcg_slot(int32_t i) { return (int)(i % kFcgDepth); }
+// (ring: the slots in use, a divisor of kFcgDepth, so that i + kFcgDepth keeps the slot of i)
+__device__ __forceinline__ int fcg_slot(int32_t i, int ring) { return (int)(i % ring); }
 // iterations between stop-flag reads of a flexible-CG solve (and per replayed multigrid graph)
 constexpr int kFcgBatch = 4;
 // The flexible-CG kernels take the iteration number from *S.it, so one iteration's launches are the

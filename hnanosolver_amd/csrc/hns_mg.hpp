@@ -64,10 +64,11 @@ struct McgXUpdate {
   const float* Dr;
   const double* alpha;
   const int* xdone;
+  int ring;  // the direction ring's slots in use (CgScalars::ring)
 };
 // What the last level-0 pass leaves for flexible CG with truncation 1 (kernels_cg.hip): per
 // workgroup (cg_groups(L) of them) partials of z . r into zr and of z . q_prev into zq, q_prev the
-// product of the previous iteration in the ring Qr (slot (*it - 1) mod kFcgDepth, n floats each).
+// product of the previous iteration in the ring Qr (slot (*it - 1) mod ring, n floats each).
 // xu: the deferred x update, if any.
 struct McgHook {
   double* zr;
@@ -76,6 +77,7 @@ struct McgHook {
   int64_t n;
   const int* it;
   McgXUpdate xu;
+  int ring;  // the product ring's slots in use (CgScalars::ring)
 };
 
 struct Multigrid {

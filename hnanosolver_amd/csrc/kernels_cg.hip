@@ -260,7 +260,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dots(int32_t L, int64_t n, const f
   const int32_t it = *S.it;
   const float* q[kFcgDepth];
 #pragma unroll
-  for (int k = 0; k < kFcgDepth; ++k) q[k] = Qr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
+  for (int k = 0; k < kFcgDepth; ++k) q[k] = Qr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth, S.ring) * n;
   double acc[kFcgDepth];
 #pragma unroll
   for (int k = 0; k < kFcgDepth; ++k) acc[k] = 0.0;
@@ -293,9 +293,9 @@ __global__ __launch_bounds__(kTpb) void k_fcg_dir(int32_t L, int64_t n, const fl
 #pragma unroll
   for (int k = 0; k < NP; ++k) {
     bf[k] = (float)(-zqk[k] / S.rho[it - 1 - k]);
-    dk[k] = Dr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth) * n;
+    dk[k] = Dr + (int64_t)fcg_slot(it - 1 - k + kFcgDepth, S.ring) * n;
   }
-  float* dout = Dr + (int64_t)fcg_slot(it) * n;  // the slot of d_{it-m}, read first by the same lane
+  float* dout = Dr + (int64_t)fcg_slot(it, S.ring) * n;  // the slot of d_{it-m}, read first by the same lane
   const LeafRun R = leaf_run(L);
   const int32_t half = L * kHalfLeaf;
   double acc = 0.0;
@@ -320,8 +320,8 @@ __global__ __launch_bounds__(kTpb) void k_fcg_spmv(DevTopo T, int64_t n, const f
   __shared__ float s_tile[2][kPairTile];
   if (S.flag[0]) return;
   const int32_t it = *S.it;
-  const float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
-  float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const float* __restrict__ d = Dr + (int64_t)fcg_slot(it, S.ring) * n;
+  float* __restrict__ q = Qr + (int64_t)fcg_slot(it, S.ring) * n;
   const LeafRun R = leaf_run(T.L);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
@@ -367,9 +367,9 @@ __global__ __launch_bounds__(kTpb) void k_fcg1_dirspmv(DevTopo T, int64_t n, con
   if (S.flag[0]) return;
   const int32_t it = *S.it;
   const bool chain = NP > 0 && it > 0;  // d = z + beta d_prev; otherwise d = z
-  const float* __restrict__ dp = chain ? Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * n : z;
-  float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
-  float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const float* __restrict__ dp = chain ? Dr + (int64_t)fcg_slot(it - 1 + kFcgDepth, S.ring) * n : z;
+  float* __restrict__ d = Dr + (int64_t)fcg_slot(it, S.ring) * n;
+  float* __restrict__ q = Qr + (int64_t)fcg_slot(it, S.ring) * n;
   const LeafRun R = leaf_run(T.L);
   const int32_t half = T.L * kHalfLeaf;
   const int t = threadIdx.x;
@@ -477,8 +477,8 @@ __global__ __launch_bounds__(kTpb) void k_fcg_update(int32_t L, int64_t n, float
     S.alpha[it] = alpha_d;
   }
   const float alpha = (float)alpha_d;
-  const float* __restrict__ d = Dr + (int64_t)fcg_slot(it) * n;
-  const float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const float* __restrict__ d = Dr + (int64_t)fcg_slot(it, S.ring) * n;
+  const float* __restrict__ q = Qr + (int64_t)fcg_slot(it, S.ring) * n;
   const LeafRun R = leaf_run(L);
   const int32_t half = L * kHalfLeaf;
   double acc = 0.0;
@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_rupdate(int32_t L, int64_t n, floa
   __shared__ double s_red[2 * (kTpb / 64)];
   if (S.flag[0]) return;
   const int32_t it = *S.it;
-  const float* __restrict__ q = Qr + (int64_t)fcg_slot(it) * n;
+  const float* __restrict__ q = Qr + (int64_t)fcg_slot(it, S.ring) * n;
   const LeafRun R = leaf_run(L);
   const int32_t half = L * kHalfLeaf;
   auto at = [&](int32_t leaf, int c) { return c * half + leaf * kHalfLeaf + (int32_t)threadIdx.x; };
@@ -563,7 +563,7 @@ __global__ __launch_bounds__(kTpb) void k_fcg_xflush(int32_t L, int64_t n, float
   const int32_t k = *S.it;
   if (k < 1 || *S.xdone >= k) return;
   const float alpha = (float)S.alpha[k - 1];
-  const float* __restrict__ d = Dr + (int64_t)fcg_slot(k - 1) * n;
+  const float* __restrict__ d = Dr + (int64_t)fcg_slot(k - 1, S.ring) * n;
   const LeafRun R = leaf_run(L);
   const int32_t half = L * kHalfLeaf;
   for (int32_t l = R.first; l < R.end; l += R.step) {

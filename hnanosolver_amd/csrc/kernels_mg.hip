@@ -136,7 +136,7 @@ __device__ void coarse_side_xupdate_n(const McgXUpdate& X, const int* __restrict
   const int32_t k = *it;
   if (k < 1 || *X.xdone >= k) return;
   const float alpha = (float)X.alpha[k - 1];
-  const float* __restrict__ d = X.Dr + (int64_t)fcg_slot(k - 1) * n;
+  const float* __restrict__ d = X.Dr + (int64_t)fcg_slot(k - 1, X.ring) * n;
   float* __restrict__ x = X.x;
   const int64_t stride = (int64_t)G * kCoarseTpb;
   int64_t i = (int64_t)g * kCoarseTpb + threadIdx.x;
@@ -1059,7 +1059,7 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* _
   if (stop && *stop) return;
   const int32_t it = *H.it;
   const int32_t half = T.L * kHalfLeaf;
-  const float* __restrict__ qp = H.Qr + (int64_t)fcg_slot(it - 1 + kFcgDepth) * H.n;
+  const float* __restrict__ qp = H.Qr + (int64_t)fcg_slot(it - 1 + kFcgDepth, H.ring) * H.n;
   const LeafRun R = leaf_run(T.L);
   double zr = 0.0, zq = 0.0;
   uint32_t fw[3], ew, rw;
@@ -1637,7 +1637,7 @@ void coarse_solve(const DevTopo& T, const CoarseLayout& Ly, const float* b, floa
   const float inv = 1.0f / scale;
   const bool two = M.c4 && Ly.ch4 > 0;
   const bool fused = !launches && (two ? Ly.fused4 : Ly.ch > 0);
-  const McgXUpdate X = hook ? hook->xu : McgXUpdate{nullptr, nullptr, nullptr, nullptr};
+  const McgXUpdate X = hook ? hook->xu : McgXUpdate{nullptr, nullptr, nullptr, nullptr, kFcgDepth};
   if (X.x && !fused) fail(HNS_ERR_INVALID_ARGUMENT, "the deferred x update needs the one-workgroup coarse solve");
   const int64_t n = hook ? hook->n : 0;
   const int* it_dev = hook ? hook->it : nullptr;

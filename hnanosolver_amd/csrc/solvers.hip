@@ -76,8 +76,8 @@ void Solvers::mg_iteration(const DevTopo& T, float* p, const CgScalars& S, int d
   if (depth == 1 && mg_shape().smooth0 == 1) {
     const bool pcg = mg_shape().pcg != 0;
     // with S.xdone set, x += alpha d of the previous iteration rides on the coarse-level launch
-    const McgXUpdate X{S.xdone ? p : nullptr, G.dring.p, S.alpha, S.xdone};
-    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, pcg ? nullptr : G.zq.p, G.qring.p, (int64_t)N_, S.it, X};
+    const McgXUpdate X{S.xdone ? p : nullptr, G.dring.p, S.alpha, S.xdone, S.ring};
+    const McgHook H{S.pa + kSlotDR * kCgMaxGroups, pcg ? nullptr : G.zq.p, G.qring.p, (int64_t)N_, S.it, X, S.ring};
     begin(HNS_K_PRECOND);
     mg_->enqueue(T, G.r.p, G.z.p, S.flag, s, &H, fold_test ? &S : nullptr);
     end();
@@ -326,6 +326,7 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   S.last = G.ctl.p + 2;
   S.bb = G.bb.p;
   S.tol2 = (double)r.tol * (double)r.tol;
+  if (mg && depth == 1) S.ring = 2;
   // FCG(1) on the fused V-cycle with a one-workgroup coarse solve: the x update is deferred into
   // the next V-cycle's coarse-level launch (flushed where x must be complete)
   if (mg && depth == 1 && mg_shape().smooth0 == 1) {
