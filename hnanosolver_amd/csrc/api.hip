@@ -310,11 +310,20 @@ hns_status advance(hns_context* C, int32_t iteration, float dt, const hns_params
     tm.begin(HNS_K_DIVERGENCE_COMBUSTION, s);
     launch_divergence(T, cur, C->div.p, inv_dx, true, &ca, ustar, s);
     tm.end(s);
-    // 4) pressure from p = 0
+    // 4) pressure from p = 0. Without the disturbance solve, the projection (5) is handed to the
+    // solve, which launches it gated behind each read of its stop flag (the device goes on into it
+    // while the host reads), and is launched below only when it did not run.
+    const bool disturb = P.disturbanceEnable > 0.0f;
+    if (!disturb)
+      req.on_stop = [&, ustar](const int* gate) {
+        launch_subtract_gradient(T, ustar, C->p.p, C->vel.p, inv_dx, sdf, s, gate);
+        if (sdf) launch_enforce_collision(T, C->vel.p, sdf, dx, s, gate);
+      };
     SolveResult res = C->solvers.solve(T, C->div.p, C->p.p, iteration, req, tm, s);
+    req.on_stop = nullptr;
     note(res);
     // 4.5) disturbance: divergence of the buoyant velocity (drops the expansion), max(2, K/2) more
-    if (P.disturbanceEnable > 0.0f) {
+    if (disturb) {
       tm.begin(HNS_K_DIVERGENCE_COMBUSTION, s);
       launch_divergence(T, ustar, C->div.p, inv_dx, true, nullptr, nullptr, s);
       tm.end(s);
@@ -328,13 +337,15 @@ hns_status advance(hns_context* C, int32_t iteration, float dt, const hns_params
       stats->fallback_iterations = res.fallback;
     }
     // 5) projection into the state velocity, then collision
-    tm.begin(HNS_K_GRADIENT, s);
-    launch_subtract_gradient(T, ustar, C->p.p, C->vel.p, inv_dx, sdf, s);
-    tm.end(s);
-    if (sdf) {
-      tm.begin(HNS_K_OTHER, s);
-      launch_enforce_collision(T, C->vel.p, sdf, dx, s);
+    if (!res.on_stop_ran) {
+      tm.begin(HNS_K_GRADIENT, s);
+      launch_subtract_gradient(T, ustar, C->p.p, C->vel.p, inv_dx, sdf, s);
       tm.end(s);
+      if (sdf) {
+        tm.begin(HNS_K_OTHER, s);
+        launch_enforce_collision(T, C->vel.p, sdf, dx, s);
+        tm.end(s);
+      }
     }
     // 6) advect the snapshot's scalars with the projected velocity (misses read element 0)
     for (size_t b = 0; b < advNames.size(); b += FieldSet::kMax) {

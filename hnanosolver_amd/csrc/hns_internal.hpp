@@ -131,9 +131,11 @@ void launch_divergence(const DevTopo& T, const float* u, float* div, float inv_d
                        const CombustionArgs* comb, float* u_out, hipStream_t s);
 void launch_combustion_buoyancy(size_t N, const CombustionArgs& c, float* div, const float* u, float* u_out,
                                 hipStream_t s);
+// gate (non-null): the kernel runs only while the device int *gate is set
 void launch_subtract_gradient(const DevTopo& T, const float* u, const float* p_cs, float* out, float inv_dx,
-                              const float* sdf, hipStream_t s);
-void launch_enforce_collision(const DevTopo& T, float* u, const float* sdf, float voxelSize, hipStream_t s);
+                              const float* sdf, hipStream_t s, const int* gate = nullptr);
+void launch_enforce_collision(const DevTopo& T, float* u, const float* sdf, float voxelSize, hipStream_t s,
+                              const int* gate = nullptr);
 void launch_to_colour_split(const DevTopo& T, const float* in, float* out, bool inverse, hipStream_t s);
 void launch_aos_soa(size_t n, const float* in, float* out, bool to_soa, hipStream_t s);
 

@@ -1,5 +1,6 @@
 // Pressure-solve driver and per-kernel event timing of a context.
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "hns_internal.hpp"
@@ -79,6 +80,11 @@ struct SolveRequest {
   float omega = 1.f;
   float dx = 1.f;
   bool want_residual = false;
+  // The work that follows a converged multigrid-preconditioned solve, launched on the solve's stream
+  // after each read of the stop flag is enqueued and gated by that device flag (its kernels return
+  // at once unless the read finds the solve stopped); the host then waits for the read alone, so
+  // the device goes on into this work while the host takes the verdict. Empty: none.
+  std::function<void(const int* gate)> on_stop;
 };
 struct SolveResult {
   int iterations = 0;
@@ -86,6 +92,7 @@ struct SolveResult {
   double rel_residual = -1.0;
   double ms = 0.0;
   LoopEnd end = kLoopStopped;  // how a to-tolerance solve ended (kLoopStopped for the fixed count)
+  bool on_stop_ran = false;    // the request's on_stop work ran on the device (the caller skips it)
 };
 
 struct CgState;
@@ -172,6 +179,8 @@ class Solvers {
     void first_test();
     void iteration(int it);
     FcgReport end_batch();
+    // the reads launch the request's on_stop work (the multigrid solve without per-kernel timing)
+    bool continues() const { return mg && !tm.enabled && static_cast<bool>(r.on_stop); }
   };
 };
 

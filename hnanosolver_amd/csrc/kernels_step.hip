@@ -357,11 +357,15 @@ __global__ void k_combustion_buoyancy(int32_t N, CombustionArgs C, float* __rest
 // ---------------------------------------------------------------------------------------
 // subtractPressureGradient, Kernel.cu:495-541 (p colour-split).
 // ---------------------------------------------------------------------------------------
+// gate (non-null): runs only while *gate is set (launched ahead of the host's read of a solve's stop
+// flag; returns at once when the solve has not ended)
 __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const float* __restrict__ u,
                                                            const float* __restrict__ p, float* __restrict__ out,
-                                                           float inv_dx, const float* __restrict__ sdf) {
+                                                           float inv_dx, const float* __restrict__ sdf,
+                                                           const int* __restrict__ gate) {
   __shared__ int32_t s_nb[27];
   __shared__ float s_p[10 * 10 * 10];  // p of the leaf (local -1..8 at 0..9) with its six face halos
+  if (gate && !*gate) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   // the neighbour row in LDS (and its barrier) only for the collision path's normal
@@ -423,8 +427,10 @@ __global__ __launch_bounds__(512) void k_subtract_gradient(DevTopo T, const floa
 // enforceCollisionBoundaries, Kernel.cu:132-158 (in place; each lane reads only the SDF of
 // its neighbours and writes its own velocity).
 __global__ __launch_bounds__(512) void k_enforce_collision(DevTopo T, float* __restrict__ u,
-                                                           const float* __restrict__ sdf, float voxelSize) {
+                                                           const float* __restrict__ sdf, float voxelSize,
+                                                           const int* __restrict__ gate) {
   __shared__ int32_t s_nb[27];
+  if (gate && !*gate) return;  // as k_subtract_gradient's
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
   const LeafView V = load_leaf(T, l, s_nb);
@@ -527,12 +533,13 @@ void launch_combustion_buoyancy(size_t N, const CombustionArgs& c, float* div, c
   HNS_HIP(hipGetLastError());
 }
 void launch_subtract_gradient(const DevTopo& T, const float* u, const float* p_cs, float* out, float inv_dx,
-                              const float* sdf, hipStream_t s) {
-  k_subtract_gradient<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, p_cs, out, inv_dx, sdf);
+                              const float* sdf, hipStream_t s, const int* gate) {
+  k_subtract_gradient<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, p_cs, out, inv_dx, sdf, gate);
   HNS_HIP(hipGetLastError());
 }
-void launch_enforce_collision(const DevTopo& T, float* u, const float* sdf, float voxelSize, hipStream_t s) {
-  k_enforce_collision<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, sdf, voxelSize);
+void launch_enforce_collision(const DevTopo& T, float* u, const float* sdf, float voxelSize, hipStream_t s,
+                              const int* gate) {
+  k_enforce_collision<<<grid_for_leaves(T.L), 512, 0, s>>>(T, u, sdf, voxelSize, gate);
   HNS_HIP(hipGetLastError());
 }
 void launch_to_colour_split(const DevTopo& T, const float* in, float* out, bool inverse, hipStream_t s) {

@@ -30,6 +30,7 @@ struct CgState {
   // and what they were made for
   static constexpr int kBatchGraphs = 6;
   hipStream_t cap_stream = nullptr;
+  hipEvent_t read_ev = nullptr;  // the stop-flag read, for the host to wait on alone (on_stop)
   int batch_len[kBatchGraphs] = {};
   hipGraphExec_t batch_exec[kBatchGraphs] = {};
   int batch_next = 0;  // the slot a new length replaces when every slot is taken
@@ -54,6 +55,7 @@ struct CgState {
   ~CgState() {
     drop_batches();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    if (read_ev) (void)hipEventDestroy(read_ev);
     if (h_ctl) (void)hipHostFree(h_ctl);
   }
 };
@@ -357,9 +359,11 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   const int batch0 = tm.enabled ? 1 : (mg ? mg_first : (pre ? kFcgBatch : 8));
   const int batch_max = tm.enabled ? 1 : (pre ? kFcgBatch : 128);
   LoopResult lr;
+  bool continued = false;  // the reads launched r.on_stop
   if (pre) {
     FcgDev d{this, T, div, p, r, S, depth, mg, tm, s};
     lr = fcg_loop(d, cap, batch0, batch_max, kResidualReplace);
+    continued = d.continues();
   } else {
     CgDev d{this, T, div, p, r, S, tm, s};
     lr = cg_loop(d, cap, batch0, batch_max, kResidualReplace);
@@ -373,6 +377,8 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
   res.iterations = lr.iterations;
   res.end = lr.end;
   if (mg) mg_last_iters_ = lr.end == kLoopStopped ? lr.iterations : 0;
+  // the read that ended the solve found the stop standing, which its on_stop work was gated on
+  res.on_stop_ran = continued && lr.end == kLoopStopped;
   if (pre && lr.end == kLoopStopped) {
     // the stop stands on r = b - A x evaluated in FP64 (its r.r and ||b||^2 came with the flag)
     const double rr = lr.rr, bb = lr.bb;
@@ -481,7 +487,14 @@ FcgReport Solvers::FcgDev::end_batch() {
   tm.end(s);
   sol->mark_solve_end(r, s);
   HNS_HIP(hipMemcpyAsync(G.h_ctl, G.ctl.p, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
-  HNS_HIP(hipStreamSynchronize(s));
+  if (continues()) {
+    if (!G.read_ev) HNS_HIP(hipEventCreateWithFlags(&G.read_ev, hipEventDisableTiming));
+    HNS_HIP(hipEventRecord(G.read_ev, s));
+    r.on_stop(S.flag);
+    HNS_HIP(hipEventSynchronize(G.read_ev));
+  } else {
+    HNS_HIP(hipStreamSynchronize(s));
+  }
   const int* f = G.h_flag();
   return FcgReport{f[0], f[1], f[2], G.h_ctl[2], G.h_ctl[3]};
 }
