@@ -39,7 +39,8 @@ KERNEL_BYTES = {
 }
 KERNEL_NAMES = {"rbgs": "k_rbgs", "cg_product": "k_cg_spmv", "cg_update": "k_cg_update",
                 "mgcg_update": "k_fcg_rupdate",
-                "mg_vcycle": "V-cycle (k_mg0_pre, k_mgl_pre x2, k_mg_coarse2, k_mgl_post x3, k_mg0_post_fcg)"}
+                "mg_vcycle": "V-cycle (k_mg0_pre, k_mgl1_pre, k_mgl_pre, k_mg_coarse2, k_mgl_post x2, k_mgl1_post, "
+                              "k_mg0_post_fcg)"}
 TRAFFIC_FILES = {"mg_vcycle": "mg_vcycle"}
 # the residual update of the multigrid solve: read r, q, write r (its CG_UPDATE class; the solve's x
 # update rides on the coarse-level launch of the next V-cycle)
