@@ -195,6 +195,44 @@ def test_mgcg_step_graph_batches_match_eager_iterations(hip):
         assert n == 0
 
 
+def test_mgcg_step_capped_solve_projects_once(hip):
+    """A fire step whose solve ends at its cap (no stop raised): the projection that each read of the
+    stop flag launches gated on the flag returns at once, and the step launches it after the solve
+    (with the deferred x update flushed), so graph runs (timing off) and eager iterations (timing on,
+    the projection never handed to the solve) leave bitwise the same state."""
+    sc = scenes.control64()
+    _case("mgcg step capped at 6 iterations, graph vs eager, control64 N=%d" % sc.n)
+
+    def run(timing):
+        d = hns.GridIndexedData(sc.coords)
+        for k, v in sc.fields.items():
+            d.add_value_block(k, v.copy())
+        d.add_value_block("velocity", sc.velocity.copy())
+        topo = hns.create_index_grid(d, sc.voxel_size)
+        ctx = hns.ensure_context(None, d, topo, sc.voxel_size)
+        ctx.set_timing(timing)
+        p = hns.make_params(**sc.params)
+        st = [hns.advance(ctx, 0, sc.dt, p, solver="mgcg", tol=1e-12, max_iters=6) for _ in range(2)]
+        out = hns.GridIndexedData(sc.coords)
+        names = [k for k in sc.fields if ctx.has_field(k)]
+        for k in names:
+            out.add_value_block(k, np.zeros(sc.n, np.float32))
+        out.add_value_block("velocity", np.zeros((sc.n, 3), np.float32))
+        hns.download_for_output(ctx, out)
+        hns.destroy_context(ctx)
+        return [x.iterations for x in st], [x.rel_residual for x in st], {k: out[k].copy() for k in names + ["velocity"]}
+
+    its_g, rel_g, a = run(False)
+    its_e, rel_e, b = run(True)
+    print("  iterations per step: graph %s, eager %s; residual %s" % (its_g, its_e, ["%.2e" % r for r in rel_g]))
+    assert its_g == its_e == [6, 6]
+    assert all(r > 1e-12 for r in rel_g)  # the cap, not the tolerance, ended both solves
+    for k in a:
+        n = int(np.sum(a[k] != b[k]))
+        print("  %-12s differing elements %d of %d" % (k, n, a[k].size))
+        assert n == 0
+
+
 def test_mgcg_capped_solves_apply_every_update(hip):
     """Solves ended by max_iters (no stop; the multigrid solve has no fallback): the x update of an
     iteration is applied by the next iteration's coarse-level launch, the last one by the flush after
