@@ -685,13 +685,15 @@ __global__ __launch_bounds__(kTpb) void k_mg0_prolong(int32_t L, float* __restri
 }
 
 // ---- level 0 fused: one workgroup per leaf on a 12^3 LDS tile ------------------------------
-// Tile coordinates 0..11 are the leaf's local coordinates -2..9, index (x * 12 + y) * 12 + z; the
-// colour of a tile position is (x + y + z) & 1 (leaf origins are multiples of 8). A red-black
+// Tile coordinates 0..11 are the leaf's local coordinates -2..9, index x * 145 + y * 12 + z (a plane
+// every 145 floats, not 144: lanes staging a z face span four x planes, which 144 = 16 mod 32 put on
+// the same LDS banks, four-way; 145 spreads them over all 32, and the core's reads, one x plane per
+// half-wave, do not depend on it); the colour of a tile position is (x + y + z) & 1 (leaf origins are multiples of 8). A red-black
 // iteration of the core needs, of the ring around it, the face slabs at depth 1 (both colours) and
 // depth 2 and the depth-1 edges (colour 0 of those suffices; both are staged); corners and deeper
 // edges are never read and are not staged. Values of absent leaves are 0 (the Dirichlet boundary).
-constexpr int kT12 = 12;
-__device__ __forceinline__ int t12(int x, int y, int z) { return (x * kT12 + y) * kT12 + z; }
+constexpr int kT12 = 12, kT12P = 145, kT12Size = kT12P * (kT12 - 1) + kT12 * kT12;  // (tile floats)
+__device__ __forceinline__ int t12(int x, int y, int z) { return x * kT12P + y * kT12 + z; }
 
 struct TilePos {
   int32_t nl;      // leaf holding the position (-1: absent)
@@ -740,7 +742,7 @@ __device__ __forceinline__ float t12_sum(const float* S, int x, int y, int z) { 
          S[t12(x, y, z + 1)] + S[t12(x, y, z - 1)];
 }
 __device__ __forceinline__ float t12_sumi(const float* S, int i) {  // the same sum at tile index i
-  return S[i + kT12 * kT12] + S[i - kT12 * kT12] + S[i + kT12] + S[i - kT12] + S[i + 1] + S[i - 1];
+  return S[i + kT12P] + S[i - kT12P] + S[i + kT12] + S[i - kT12] + S[i + 1] + S[i - 1];
 }
 // colour-1 positions updated ahead of the core's colour 0: the core's (one per lane) and those of
 // the depth-1 face slabs (lanes < 192: face t >> 5, 32 positions each)
@@ -766,7 +768,7 @@ struct Tab12 {
   uint32_t ring[192];
 };
 constexpr uint32_t tab_word(int tx, int ty, int tz, int vx, int vy, int vz) {
-  return (uint32_t)((tx * kT12 + ty) * kT12 + tz) | ((uint32_t)((vx << 5) | (vy << 2) | (vz >> 1)) << 11) |
+  return (uint32_t)(tx * kT12P + ty * kT12 + tz) | ((uint32_t)((vx << 5) | (vy << 2) | (vz >> 1)) << 11) |
          ((uint32_t)((vx + vy + vz) & 1) << 19) | ((uint32_t)(((vx >> 1) << 4) | ((vy >> 1) << 2) | (vz >> 1)) << 20);
 }
 constexpr Tab12 make_tab12() {
@@ -783,7 +785,7 @@ constexpr Tab12 make_tab12() {
       if (((tx + ty + tz) & 1) == 0) {
         if (f < 4) ++rz; else ++ry;
       }
-      T.ring[r] = (uint32_t)((tx * kT12 + ry) * kT12 + rz);
+      T.ring[r] = (uint32_t)(tx * kT12P + ry * kT12 + rz);
     }
   }
   for (int e = 0; e < 96; ++e) {
@@ -797,6 +799,7 @@ constexpr Tab12 make_tab12() {
 }
 __constant__ Tab12 kTab12 = make_tab12();
 // table word -> tile index / colour-split index in leaf nl / its level-1 parent in leaf nl
+static_assert(kT12Size <= 2048, "a tile index must fit the table word's 11 bits");
 __device__ __forceinline__ int tw_tile(uint32_t w) { return (int)(w & 2047u); }
 __device__ __forceinline__ uint32_t tw_cs(uint32_t w, uint32_t nl, uint32_t half) {
   return ((w >> 19) & 1u) * half + nl * (uint32_t)kHalfLeaf + ((w >> 11) & 255u);
@@ -983,11 +986,13 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
   const int x = t >> 5, y = (t >> 2) & 7, zz = t & 3;
   const int c0 = (x + y) & 1;
   {
+    // the pair in colour order (z parity alternating with y across a half-wave: conflict-free banks)
     const int i0 = t12(x + 2, y + 2, 2 * zz + 2);
-    s_z[i0] = c * P.b0 + P.up;
-    s_z[i0 + 1] = c * P.b1 + P.up;
-    s_b[i0] = P.b0;
-    s_b[i0 + 1] = P.b1;
+    const float z0 = c * P.b0 + P.up, z1 = c * P.b1 + P.up;
+    s_z[i0 + c0] = c0 ? z1 : z0;
+    s_z[i0 + (c0 ^ 1)] = c0 ? z0 : z1;
+    s_b[i0 + c0] = c0 ? P.b1 : P.b0;
+    s_b[i0 + (c0 ^ 1)] = c0 ? P.b0 : P.b1;
   }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -997,7 +1002,7 @@ __device__ __forceinline__ void mg0_post_compute(const DevTopo& T, int32_t l, co
   }
   // lanes past the 96 edge entries store into the pad slot past the tile (no branch around the
   // edge loads, so they go out with the others)
-  s_z[t < 96 ? tw_tile(ew) : kT12 * kT12 * kT12] = P.enl < 0 ? 0.f : c * P.eb + P.eu;
+  s_z[t < 96 ? tw_tile(ew) : kT12Size] = P.enl < 0 ? 0.f : c * P.eb + P.eu;
   __syncthreads();
   // colour 1 on the core and the depth-1 slabs
   {
@@ -1032,8 +1037,8 @@ __device__ __forceinline__ void lane_tables(uint32_t (&fw)[3], uint32_t& ew, uin
 __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __restrict__ b,
                                                    const float* __restrict__ u1, float* __restrict__ zout, float c,
                                                    float w, const int* __restrict__ stop) {
-  __shared__ float s_b[kT12 * kT12 * kT12];
-  __shared__ float s_z[kT12 * kT12 * kT12 + 1];  // + the pad slot of lanes without an edge entry
+  __shared__ float s_b[kT12Size];
+  __shared__ float s_z[kT12Size + 1];  // + the pad slot of lanes without an edge entry
   if (stop && *stop) return;
   const int32_t l = leaf_of_block();
   if (l >= T.L) return;
@@ -1053,8 +1058,8 @@ __global__ __launch_bounds__(kTpb) void k_mg0_post(DevTopo T, const float* __res
 __global__ __launch_bounds__(kTpb) void k_mg0_post_fcg(DevTopo T, const float* __restrict__ b,
                                                        const float* __restrict__ u1, float* __restrict__ zout,
                                                        McgHook H, float c, float w, const int* __restrict__ stop) {
-  __shared__ float s_b[kT12 * kT12 * kT12];
-  __shared__ float s_z[kT12 * kT12 * kT12 + 1];  // + the pad slot of lanes without an edge entry
+  __shared__ float s_b[kT12Size];
+  __shared__ float s_z[kT12Size + 1];  // + the pad slot of lanes without an edge entry
   __shared__ double s_red[kTpb / 64];
   if (stop && *stop) return;
   const int32_t it = *H.it;
