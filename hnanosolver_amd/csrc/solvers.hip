@@ -368,7 +368,9 @@ SolveResult Solvers::solve_cg(const DevTopo& T, const float* div, float* p, cons
     CgDev d{this, T, div, p, r, S, tm, s};
     lr = cg_loop(d, cap, batch0, batch_max, kResidualReplace);
   }
-  if (S.xdone) {  // x complete before anything reads it (nothing pending after a stop: it was flushed)
+  // x complete before anything reads it; after a stop nothing is pending (the flush before the
+  // stop's re-test applied it), so the launch is left out there
+  if (S.xdone && lr.end != kLoopStopped) {
     tm.begin(HNS_K_OTHER, s);
     launch_fcg_xflush(T.L, p, G.dring.p, S, kFlushEnd, s);
     tm.end(s);
