@@ -1458,6 +1458,9 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
   const int cx = lane >> (2 * LG), cy = (lane >> LG) & (Tl::S - 1), cz = lane & (Tl::S - 1);
   float* B = s_b[lw];
   float* Z = s_z[lw];
+  // restricting to level 3: the leaf's slot there, loaded with the leaf's data (the store address at
+  // the end would otherwise wait for it after the last barrier)
+  const int32_t slot3 = LG == 1 && map3 && live ? map3[l] : 0;
   FaceRow Fr{};
   if (live) {
     Fr = face_row(T, l);
@@ -1512,7 +1515,7 @@ __global__ __launch_bounds__(kTpb) void k_mgl_pre(DevTopo T, const float* __rest
     const int x = 2 * X + (k >> 2), y = 2 * Y + ((k >> 1) & 1), z = 2 * Z0 + (k & 1);
     sum += s_r[(lw << (3 * LG)) | (((x << LG) | y) << LG) | z];
   }
-  bc[LG == 1 && map3 ? map3[l] : ((int32_t)l << (3 * lgC)) | lane] = over * sum;
+  bc[LG == 1 && map3 ? slot3 : ((int32_t)l << (3 * lgC)) | lane] = over * sum;
 }
 
 // map3 (LG = 1, prolonging from level 3): as k_mg_restrict's. PRO = false: one more red-black
