@@ -1243,8 +1243,9 @@ __device__ __forceinline__ float tl1_nsum(const float* A, int i) {
 
 // Level 1 leaves per wave: the wave's leaves l0, l0 + 4, ... (l0 = 4 NL slot + wave) have all their
 // loads issued up front, and the wave then stages and smooths them one after the other in its one
-// tile. One leaf per wave takes two rounds of the chip's wave slots at 8M voxels, paying the chain
-// of dependent loads twice; two per wave take one.
+// tile. At 8M voxels one leaf per wave asks for twice the chip's 8,192 wave slots, so half the waves
+// wait for the first half to finish and the chain of dependent loads is paid twice; with two leaves
+// per wave every wave is resident at once.
 constexpr int kL1Leaves = 2;
 
 // bit f set where face f (-x, +x, -y, +y, -z, +z) of the leaf has a neighbour
